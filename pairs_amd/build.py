@@ -1,0 +1,49 @@
+"""Build the gfx950 shared library libpairs_hip.so in-tree with hipcc.
+
+Usage: python -m pairs_amd.build   (or pairs_amd.build.build())
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpairs_hip.so")
+ARCH = os.environ.get("PAIRS_HIP_ARCH", "gfx950")
+
+SOURCES = ["md_kernels.hip", "dem_kernels.hip", "batch_kernels.hip", "mlp_kernels.hip", "pairs_hip.cpp",
+           "dem_capi.cpp", "batch_capi.cpp", "mlp_capi.cpp"]
+
+# -ffp-contract=off: every multiply and add rounds separately, as in the
+#   reference's generated CPU code (bit-exact parity of the FP64 paths).
+# -packed-fp32-ops off: no v_pk_{add,mul,fma}_f32 generated from float code.
+# -munsafe-fp-atomics: hardware FP64 atomic add for the half-list forces.
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}", "-ffp-contract=off",
+         "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-command-line-argument"]
+
+
+def hipcc():
+    for c in (os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc"), shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def build(verbose=False):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc()] + FLAGS + ["-I" + os.path.join(HERE, "..", "include"), "-o", tmp]
+    for s in srcs:
+        cmd += (["-x", "hip", s] if s.endswith(".cpp") else [s])
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

@@ -1,0 +1,892 @@
+// C ABI of the LJ molecular-dynamics hot path (include/pairs_hip.h).
+//
+// The context replaces the reference's PairsSimulation object
+// (runtime/pairs.hpp:27-297): it owns every device array, keeps counts on the
+// device so that rebuild kernels chain without host round trips, and replays
+// hipGraphs of the fused per-step kernels between neighbour rebuilds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pairs_hip.h"
+#include "md_kernels.hpp"
+
+using namespace pk;
+
+struct pairs_hip_ctx {
+    pairs_hip_md_config cfg;
+    LJParams prm;
+    Geom geom;
+    DevState d;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool cells_ready = false;
+    std::string err;
+    int req[5] = {0, 0, 0, 0, 0};
+    int h_counters[C_COUNT];
+    int nghost = 0, nsend_all = 0;
+    int ghost_grid = 1;
+    // graphs of one non-rebuild step, by parity of the current buffer
+    bool use_graphs = true;
+    hipGraphExec_t step_graph[2] = {nullptr, nullptr};
+    int graph_ghost_grid[2] = {0, 0};
+    // force-kernel timing
+    bool time_force = false;
+    int time_every = 10;
+    std::vector<hipEvent_t> ev_begin, ev_end;
+    int ev_used = 0;
+    double force_ms_avg = 0.0;
+    int force_launches = 0;
+    double *d_eatom = nullptr;
+    int eatom_cap = 0;
+};
+
+#define HIP_OK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                              \
+            return PAIRS_HIP_ERR_HIP;                                                                  \
+        }                                                                                              \
+    } while (0)
+
+#define CHECK_LAUNCH()                                                                                 \
+    do {                                                                                               \
+        hipError_t e_ = hipGetLastError();                                                             \
+        if (e_ != hipSuccess) {                                                                        \
+            ctx->err = std::string("kernel launch: ") + hipGetErrorString(e_);                         \
+            return PAIRS_HIP_ERR_HIP;                                                                  \
+        }                                                                                              \
+    } while (0)
+
+static void drop_graphs(pairs_hip_ctx *ctx) {
+    for (int p = 0; p < 2; p++) {
+        if (ctx->step_graph[p]) hipGraphExecDestroy(ctx->step_graph[p]);
+        ctx->step_graph[p] = nullptr;
+    }
+}
+
+template <typename T>
+static int dalloc(pairs_hip_ctx *ctx, T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIP_OK(hipMalloc((void **) p, n * sizeof(T)));
+    return 0;
+}
+
+static void dfree(void *p) {
+    if (p) hipFree(p);
+}
+
+static void free_particle_arrays(DevState &d) {
+    dfree(d.P[0]); dfree(d.P[1]); dfree(d.vx); dfree(d.vy); dfree(d.vz); dfree(d.fx); dfree(d.fy); dfree(d.fz);
+    dfree(d.mass); dfree(d.flags); dfree(d.uid); dfree(d.shape); dfree(d.numneighs); dfree(d.particle_cell);
+    dfree(d.slot); dfree(d.sorted_idx); dfree(d.sorted_pos); dfree(d.exchg_flag); dfree(d.tile_cnt);
+    dfree(d.partials);
+    d.P[0] = d.P[1] = nullptr;
+    d.vx = d.vy = d.vz = d.fx = d.fy = d.fz = d.mass = nullptr;
+    d.flags = d.uid = d.shape = d.numneighs = d.particle_cell = d.slot = d.sorted_idx = d.exchg_flag = d.tile_cnt = nullptr;
+    d.sorted_pos = nullptr;
+    d.partials = nullptr;
+}
+
+static int alloc_particle_arrays(pairs_hip_ctx *ctx, DevState &d, int cap) {
+    int r = 0;
+    r |= dalloc(ctx, &d.P[0], cap); r |= dalloc(ctx, &d.P[1], cap);
+    r |= dalloc(ctx, &d.vx, cap); r |= dalloc(ctx, &d.vy, cap); r |= dalloc(ctx, &d.vz, cap);
+    r |= dalloc(ctx, &d.fx, cap); r |= dalloc(ctx, &d.fy, cap); r |= dalloc(ctx, &d.fz, cap);
+    r |= dalloc(ctx, &d.mass, cap); r |= dalloc(ctx, &d.flags, cap); r |= dalloc(ctx, &d.uid, cap);
+    r |= dalloc(ctx, &d.shape, cap); r |= dalloc(ctx, &d.numneighs, cap); r |= dalloc(ctx, &d.particle_cell, cap);
+    r |= dalloc(ctx, &d.slot, cap); r |= dalloc(ctx, &d.sorted_idx, cap); r |= dalloc(ctx, &d.sorted_pos, cap);
+    r |= dalloc(ctx, &d.exchg_flag, cap);
+    r |= dalloc(ctx, &d.tile_cnt, 2 * ((size_t) cap / 256 + 2));
+    r |= dalloc(ctx, &d.partials, 3 * ((size_t) cap / 256 + 2));
+    if (r) return PAIRS_HIP_ERR_HIP;
+    d.cap = cap;
+    return 0;
+}
+
+static int alloc_neighbor_arrays(pairs_hip_ctx *ctx, DevState &d, int ncap) {
+    dfree(d.nl);
+    d.nl = nullptr;
+    d.nl_stride = d.cap;
+    if (dalloc(ctx, &d.nl, (size_t) ncap * d.nl_stride)) return PAIRS_HIP_ERR_HIP;
+    d.ncap = ncap;
+    return 0;
+}
+
+static int alloc_send_arrays(pairs_hip_ctx *ctx, DevState &d, int send_cap) {
+    dfree(d.send_map); dfree(d.send_mult); dfree(d.scratch_i); dfree(d.stage);
+    d.send_map = d.send_mult = d.scratch_i = nullptr;
+    d.stage = nullptr;
+    int r = 0;
+    r |= dalloc(ctx, &d.send_map, send_cap);
+    r |= dalloc(ctx, &d.send_mult, 3 * (size_t) send_cap);
+    r |= dalloc(ctx, &d.scratch_i, 2 * (size_t) send_cap);
+    r |= dalloc(ctx, &d.stage, 10 * (size_t) send_cap);
+    if (r) return PAIRS_HIP_ERR_HIP;
+    d.send_cap = send_cap;
+    return 0;
+}
+
+static int zero_particle_arrays(pairs_hip_ctx *ctx, int from, int to) {
+    DevState &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    const int n = to - from;
+    if (n <= 0) return 0;
+    launch_fill_i32(s, d.flags + from, 0, n);
+    launch_fill_i32(s, d.uid + from, 0, n);
+    launch_fill_i32(s, d.shape + from, SHAPE_POINT_MASS, n);
+    launch_fill_i32(s, d.numneighs + from, 0, n);
+    launch_fill_f64(s, d.mass + from, 1.0, n);
+    for (double *p : {d.vx, d.vy, d.vz, d.fx, d.fy, d.fz}) launch_fill_f64(s, p + from, 0.0, n);
+    launch_fill_f64(s, (double *) (d.P[0] + from), 0.0, 4 * n);
+    launch_fill_f64(s, (double *) (d.P[1] + from), 0.0, 4 * n);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+// build_cell_lists_stencil (src/pairs/sim/cell_lists.py:46-87)
+static int setup_geometry(pairs_hip_ctx *ctx) {
+    Geom &g = ctx->geom;
+    const pairs_hip_md_config &c = ctx->cfg;
+    const double s = c.cell_spacing;
+    int total = 1;
+    for (int d = 0; d < 3; d++) {
+        g.lo[d] = c.box_lo[d];
+        g.hi[d] = c.box_hi[d];
+        g.len[d] = c.box_hi[d] - c.box_lo[d];
+        const double dmax = g.hi[d] + s, dmin = g.lo[d] - s;
+        g.dim_cells[d] = (int) std::ceil((dmax - dmin) / s) + 1;
+        total *= g.dim_cells[d];
+    }
+    g.spacing = s;
+    g.ncells = total + 1;
+    g.nstencil = 0;
+    for (int a = -1; a < 2; a++)
+        for (int b = -1; b < 2; b++)
+            for (int e = -1; e < 2; e++) g.stencil[g.nstencil++] = (a * g.dim_cells[1] + b) * g.dim_cells[2] + e;
+    if (g.ncells + 1 >= ctx->d.ncells_cap) {
+        ctx->req[PAIRS_HIP_OVERFLOW_CELLS] = g.ncells + 2;
+        return PAIRS_HIP_OVERFLOW_CELLS;
+    }
+    ctx->cells_ready = true;
+    return 0;
+}
+
+static int read_counters(pairs_hip_ctx *ctx) {
+    HIP_OK(hipMemcpyAsync(ctx->h_counters, ctx->d.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    ctx->nghost = ctx->h_counters[C_NGHOST];
+    ctx->nsend_all = ctx->h_counters[C_NSEND_ALL];
+    ctx->ghost_grid = std::max(1, std::min((ctx->nsend_all + 255) / 256 + 1, 4096));
+    return 0;
+}
+
+// overflow left by the last device work, as a positive return code
+static int take_overflow(pairs_hip_ctx *ctx) {
+    const int *h = ctx->h_counters;
+    int code = 0;
+    if (h[C_OVF] == OVF_SEND) { ctx->req[PAIRS_HIP_OVERFLOW_SEND] = h[C_REQ_SEND]; code = PAIRS_HIP_OVERFLOW_SEND; }
+    else if (h[C_OVF] == OVF_PART) { ctx->req[PAIRS_HIP_OVERFLOW_PARTICLES] = h[C_REQ_PART]; code = PAIRS_HIP_OVERFLOW_PARTICLES; }
+    else if (h[C_REQ_NEIGH] > 0) { ctx->req[PAIRS_HIP_OVERFLOW_NEIGHBORS] = h[C_REQ_NEIGH]; code = PAIRS_HIP_OVERFLOW_NEIGHBORS; }
+    if (code) {
+        launch_set_counters(ctx->stream, ctx->d.counters, C_OVF, 1, 0);
+        launch_set_counters(ctx->stream, ctx->d.counters, C_REQ_NEIGH, 3, 0);
+    }
+    return code;
+}
+
+extern "C" {
+
+const char *pairs_hip_last_error(const pairs_hip_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pairs_hip_last_required(const pairs_hip_ctx *ctx, int which) {
+    if (!ctx || which < 0 || which > 4) return -1;
+    return ctx->req[which];
+}
+
+int pairs_hip_create(pairs_hip_ctx **out, const pairs_hip_md_config *cfg) {
+    if (!out || !cfg) return PAIRS_HIP_ERR_ARG;
+    *out = nullptr;
+    if (cfg->particle_capacity <= 0 || cfg->neighbor_capacity <= 2 || cfg->send_capacity <= 0 ||
+        cfg->ncells_capacity <= 0 || cfg->ntypes < 1 || cfg->ntypes > 4 || cfg->cell_spacing <= 0.0)
+        return PAIRS_HIP_ERR_ARG;
+    pairs_hip_ctx *ctx = new pairs_hip_ctx();
+    ctx->cfg = *cfg;
+    ctx->cfg.epsilon = nullptr;
+    ctx->cfg.sigma6 = nullptr;
+    LJParams &p = ctx->prm;
+    p.rc2 = cfg->cutoff * cfg->cutoff;
+    p.dt = cfg->dt;
+    p.hdt = cfg->dt * 0.5;
+    p.ntypes = cfg->ntypes;
+    for (int k = 0; k < 16; k++) {
+        p.eps[k] = (k < cfg->ntypes * cfg->ntypes && cfg->epsilon) ? cfg->epsilon[k] : 1.0;
+        p.sig6[k] = (k < cfg->ntypes * cfg->ntypes && cfg->sigma6) ? cfg->sigma6[k] : 1.0;
+    }
+    memset(&ctx->d, 0, sizeof(DevState));
+    auto fail = [&](int code) { pairs_hip_destroy(ctx); return code; };
+    if (cfg->stream) {
+        ctx->stream = (hipStream_t) cfg->stream;
+    } else {
+        if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(PAIRS_HIP_ERR_HIP);
+        ctx->own_stream = true;
+    }
+    DevState &d = ctx->d;
+    // the cell grid is fixed by the box; its arrays are sized to fit it
+    // (the generated code grows ncells_capacity to twice the need on overflow)
+    d.ncells_cap = 1 << 30;
+    if (setup_geometry(ctx) != 0) return fail(PAIRS_HIP_ERR_ARG);
+    d.ncells_cap = std::max(cfg->ncells_capacity, ctx->geom.ncells + 2);
+    if (alloc_particle_arrays(ctx, d, cfg->particle_capacity)) return fail(PAIRS_HIP_ERR_HIP);
+    if (alloc_neighbor_arrays(ctx, d, cfg->neighbor_capacity)) return fail(PAIRS_HIP_ERR_HIP);
+    if (alloc_send_arrays(ctx, d, cfg->send_capacity)) return fail(PAIRS_HIP_ERR_HIP);
+    if (dalloc(ctx, &d.cell_count, (size_t) d.ncells_cap + 1) ||
+        dalloc(ctx, &d.cell_start, (size_t) d.ncells_cap + 1) ||
+        dalloc(ctx, &d.cell_nlocal, (size_t) d.ncells_cap + 1) || dalloc(ctx, &d.counters, C_COUNT) ||
+        dalloc(ctx, &d.thermo, 5 * 4096))
+        return fail(PAIRS_HIP_ERR_HIP);
+    launch_fill_i32(ctx->stream, d.counters, 0, C_COUNT);
+    launch_fill_i32(ctx->stream, d.cell_start, 0, d.ncells_cap + 1);
+    if (zero_particle_arrays(ctx, 0, d.cap)) return fail(PAIRS_HIP_ERR_HIP);
+    d.cur = 0;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(PAIRS_HIP_ERR_HIP);
+    *out = ctx;
+    return 0;
+}
+
+int pairs_hip_destroy(pairs_hip_ctx *ctx) {
+    if (!ctx) return 0;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    drop_graphs(ctx);
+    for (auto e : ctx->ev_begin) hipEventDestroy(e);
+    for (auto e : ctx->ev_end) hipEventDestroy(e);
+    DevState &d = ctx->d;
+    free_particle_arrays(d);
+    dfree(d.nl); dfree(d.send_map); dfree(d.send_mult); dfree(d.scratch_i); dfree(d.stage);
+    dfree(d.cell_count); dfree(d.cell_start); dfree(d.cell_nlocal); dfree(d.counters); dfree(d.thermo);
+    dfree(ctx->d_eatom);
+    if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int pairs_hip_reserve(pairs_hip_ctx *ctx, int particle_capacity, int neighbor_capacity, int send_capacity) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    DevState &d = ctx->d;
+    if (particle_capacity > d.cap) {
+        DevState nd = d;
+        if (alloc_particle_arrays(ctx, nd, particle_capacity)) return PAIRS_HIP_ERR_HIP;
+        const size_t n = (size_t) d.cap;
+        HIP_OK(hipMemcpyAsync(nd.P[0], d.P[0], n * sizeof(double4), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(nd.P[1], d.P[1], n * sizeof(double4), hipMemcpyDeviceToDevice, ctx->stream));
+        double *src_d[7] = {d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass};
+        double *dst_d[7] = {nd.vx, nd.vy, nd.vz, nd.fx, nd.fy, nd.fz, nd.mass};
+        for (int k = 0; k < 7; k++)
+            HIP_OK(hipMemcpyAsync(dst_d[k], src_d[k], n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        int *src_i[3] = {d.flags, d.uid, d.shape};
+        int *dst_i[3] = {nd.flags, nd.uid, nd.shape};
+        for (int k = 0; k < 3; k++)
+            HIP_OK(hipMemcpyAsync(dst_i[k], src_i[k], n * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        free_particle_arrays(d);
+        int *nl = d.nl;
+        d = nd;
+        d.nl = nl;
+        const int old = (int) n;
+        launch_fill_i32(ctx->stream, d.flags + old, 0, d.cap - old);
+        launch_fill_i32(ctx->stream, d.uid + old, 0, d.cap - old);
+        launch_fill_i32(ctx->stream, d.shape + old, SHAPE_POINT_MASS, d.cap - old);
+        launch_fill_f64(ctx->stream, d.mass + old, 1.0, d.cap - old);
+        launch_fill_i32(ctx->stream, d.numneighs, 0, d.cap);
+        int ncap = std::max(neighbor_capacity, d.ncap);
+        if (alloc_neighbor_arrays(ctx, d, ncap)) return PAIRS_HIP_ERR_HIP;
+    } else if (neighbor_capacity > d.ncap) {
+        if (alloc_neighbor_arrays(ctx, d, neighbor_capacity)) return PAIRS_HIP_ERR_HIP;
+        launch_fill_i32(ctx->stream, d.numneighs, 0, d.cap);
+    }
+    if (send_capacity > d.send_cap) {
+        if (alloc_send_arrays(ctx, d, send_capacity)) return PAIRS_HIP_ERR_HIP;
+        // the ghost maps are gone with the old arrays: a rebuild must follow
+        launch_set_counters(ctx->stream, d.counters, C_NSEND_ALL, 1, 0);
+    }
+    CHECK_LAUNCH();
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int pairs_hip_upload(pairs_hip_ctx *ctx, int n, const double *x, const double *v, const double *m, const int *type,
+                     const int *flags, const int *uid) {
+    if (!ctx || n < 0 || !x || !v) return PAIRS_HIP_ERR_ARG;
+    DevState &d = ctx->d;
+    if (n > d.cap) {
+        ctx->req[PAIRS_HIP_OVERFLOW_PARTICLES] = n;
+        return PAIRS_HIP_OVERFLOW_PARTICLES;
+    }
+    drop_graphs(ctx);
+    std::vector<double4> hp(n);
+    std::vector<double> hv[3] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n)};
+    std::vector<double> hm(n);
+    std::vector<int> hf(n), hu(n), hs(n, SHAPE_POINT_MASS);
+    for (int i = 0; i < n; i++) {
+        const int t = type ? type[i] : 0;
+        if (t < 0 || t >= ctx->cfg.ntypes) { ctx->err = "type out of range"; return PAIRS_HIP_ERR_ARG; }
+        hp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], type_bits(t));
+        for (int k = 0; k < 3; k++) hv[k][i] = v[3 * i + k];
+        hm[i] = m ? m[i] : 1.0;
+        hf[i] = flags ? flags[i] : 0;
+        hu[i] = uid ? uid[i] : 0;
+    }
+    d.cur = 0;
+    if (n > 0) {
+        HIP_OK(hipMemcpyAsync(d.P[0], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.P[1], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.vx, hv[0].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.vy, hv[1].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.vz, hv[2].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.mass, hm.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.flags, hf.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.uid, hu.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.shape, hs.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    }
+    launch_fill_f64(ctx->stream, d.fx, 0.0, d.cap);
+    launch_fill_f64(ctx->stream, d.fy, 0.0, d.cap);
+    launch_fill_f64(ctx->stream, d.fz, 0.0, d.cap);
+    launch_fill_i32(ctx->stream, d.counters, 0, C_COUNT);
+    launch_set_counters(ctx->stream, d.counters, C_NLOCAL, 1, n);
+    CHECK_LAUNCH();
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    d.nlocal = n;
+    ctx->nghost = 0;
+    ctx->nsend_all = 0;
+    return 0;
+}
+
+// Park-Miller minimal standard generator without masking (runtime/copper_fcc_lattice.hpp:11-26)
+static double park_miller(int *seed) {
+    const int IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836;
+    const int k = (*seed) / IQ;
+    *seed = IA * (*seed - k * IQ) - IR * k;
+    if (*seed < 0) *seed += IM;
+    return (1.0 / IM) * (*seed);
+}
+
+// pairs::copper_fcc_lattice (runtime/copper_fcc_lattice.hpp:64-145), on the host
+int pairs_hip_copper_fcc_lattice(pairs_hip_ctx *ctx, int nx, int ny, int nz, double rho, int ntypes) {
+    if (!ctx || nx <= 0 || ny <= 0 || nz <= 0 || rho <= 0.0 || ntypes < 1) return PAIRS_HIP_ERR_ARG;
+    const double *lo = ctx->cfg.box_lo, *hi = ctx->cfg.box_hi;
+    const double xprd = hi[0] - lo[0], yprd = hi[1] - lo[1], zprd = hi[2] - lo[2];
+    const double alat = std::pow((4.0 / rho), (1.0 / 3.0));
+    int ilo = (int) (0.0 / (0.5 * alat) - 1), ihi = (int) (xprd / (0.5 * alat) + 1);
+    int jlo = (int) (0.0 / (0.5 * alat) - 1), jhi = (int) (yprd / (0.5 * alat) + 1);
+    int klo = (int) (0.0 / (0.5 * alat) - 1), khi = (int) (zprd / (0.5 * alat) + 1);
+    ilo = std::max(ilo, 0); ihi = std::min(ihi, 2 * nx - 1);
+    jlo = std::max(jlo, 0); jhi = std::min(jhi, 2 * ny - 1);
+    klo = std::max(klo, 0); khi = std::min(khi, 2 * nz - 1);
+    const double SMALL = 1e-5;
+    std::vector<double> x, v, m;
+    std::vector<int> t;
+    int sx = 0, sy = 0, sz = 0, ox = 0, oy = 0, oz = 0;
+    const int sub = 8;
+    while (oz * sub <= khi) {
+        const int k = oz * sub + sz, j = oy * sub + sy, i = ox * sub + sx;
+        if (((i + j + k) % 2 == 0) && i >= ilo && i <= ihi && j >= jlo && j <= jhi && k >= klo && k <= khi) {
+            const double xt = 0.5 * alat * i, yt = 0.5 * alat * j, zt = 0.5 * alat * k;
+            if (xt >= lo[0] && xt < hi[0] - SMALL && yt >= lo[1] && yt < hi[1] - SMALL && zt >= lo[2] &&
+                zt < hi[2] - SMALL) {
+                int n = k * (2 * ny) * (2 * nx) + j * (2 * nx) + i + 1;
+                for (int q = 0; q < 5; q++) park_miller(&n);
+                const double vx = park_miller(&n);
+                for (int q = 0; q < 5; q++) park_miller(&n);
+                const double vy = park_miller(&n);
+                for (int q = 0; q < 5; q++) park_miller(&n);
+                const double vz = park_miller(&n);
+                x.insert(x.end(), {xt, yt, zt});
+                v.insert(v.end(), {vx, vy, vz});
+                m.push_back(1.0);
+                t.push_back(rand() % ntypes);
+            }
+        }
+        sx++;
+        if (sx == sub) { sx = 0; sy++; }
+        if (sy == sub) { sy = 0; sz++; }
+        if (sz == sub) { sz = 0; ox++; }
+        if (ox * sub > ihi) { ox = 0; oy++; }
+        if (oy * sub > jhi) { oy = 0; oz++; }
+    }
+    const int n = (int) m.size();
+    const int r = pairs_hip_upload(ctx, n, x.data(), v.data(), m.data(), t.data(), nullptr, nullptr);
+    return r ? r : n;
+}
+
+static int download_raw(pairs_hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return 0;
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// pairs::adjust_thermo (runtime/thermo.hpp:53-97), on the host in the same order
+int pairs_hip_adjust_thermo(pairs_hip_ctx *ctx, double temp) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    DevState &d = ctx->d;
+    const int n = d.nlocal;
+    if (n <= 1) return PAIRS_HIP_ERR_STATE;
+    std::vector<double> v[3] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n)}, m(n);
+    if (download_raw(ctx, v[0].data(), d.vx, n * sizeof(double)) || download_raw(ctx, v[1].data(), d.vy, n * sizeof(double)) ||
+        download_raw(ctx, v[2].data(), d.vz, n * sizeof(double)) || download_raw(ctx, m.data(), d.mass, n * sizeof(double)))
+        return PAIRS_HIP_ERR_HIP;
+    double vxt = 0.0, vyt = 0.0, vzt = 0.0;
+    for (int i = 0; i < n; i++) { vxt += v[0][i]; vyt += v[1][i]; vzt += v[2][i]; }
+    vxt /= n; vyt /= n; vzt /= n;
+    for (int i = 0; i < n; i++) { v[0][i] -= vxt; v[1][i] -= vyt; v[2][i] -= vzt; }
+    const double dof = (n * 3 - 3);
+    const double t_scale = 1.0 / dof;
+    double t = 0.0;
+    for (int i = 0; i < n; i++) t += m[i] * (v[0][i] * v[0][i] + v[1][i] * v[1][i] + v[2][i] * v[2][i]);
+    t = t * t_scale;
+    const double factor = std::sqrt(temp / t);
+    for (int i = 0; i < n; i++) { v[0][i] *= factor; v[1][i] *= factor; v[2][i] *= factor; }
+    HIP_OK(hipMemcpyAsync(d.vx, v[0].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemcpyAsync(d.vy, v[1].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipMemcpyAsync(d.vz, v[2].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int pairs_hip_count(pairs_hip_ctx *ctx, int what) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+    switch (what) {
+        case PAIRS_HIP_NLOCAL: return ctx->h_counters[C_NLOCAL];
+        case PAIRS_HIP_NGHOST: return ctx->h_counters[C_NGHOST];
+        case PAIRS_HIP_NCELLS: return ctx->geom.ncells;
+        case PAIRS_HIP_NSEND_ALL: return ctx->h_counters[C_NSEND_ALL];
+        case PAIRS_HIP_NEIGHBOR_CAPACITY: return ctx->d.ncap;
+        case PAIRS_HIP_PARTICLE_CAPACITY: return ctx->d.cap;
+        case PAIRS_HIP_SEND_CAPACITY: return ctx->d.send_cap;
+        case PAIRS_HIP_NEIGHBOR_TOTAL: {
+            const int n = ctx->d.nlocal;
+            std::vector<int> nn(n);
+            if (download_raw(ctx, nn.data(), ctx->d.numneighs, n * sizeof(int))) return PAIRS_HIP_ERR_HIP;
+            long long s = 0;
+            for (int x : nn) s += x;
+            return (int) std::min<long long>(s, 0x7fffffff);
+        }
+    }
+    return PAIRS_HIP_ERR_ARG;
+}
+
+int pairs_hip_download(pairs_hip_ctx *ctx, int what, void *host, int count) {
+    if (!ctx || !host || count < 0) return PAIRS_HIP_ERR_ARG;
+    if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+    DevState &d = ctx->d;
+    const int nlocal = ctx->h_counters[C_NLOCAL], nall = nlocal + ctx->h_counters[C_NGHOST];
+    const int nsend = ctx->h_counters[C_NSEND_ALL];
+    auto lim = [&](int avail) { return std::min(count, avail); };
+    switch (what) {
+        case PAIRS_HIP_POSITION:
+        case PAIRS_HIP_TYPE: {
+            const int n = lim(nall);
+            std::vector<double4> p(n);
+            if (download_raw(ctx, p.data(), d.P[d.cur], n * sizeof(double4))) return PAIRS_HIP_ERR_HIP;
+            for (int i = 0; i < n; i++) {
+                if (what == PAIRS_HIP_POSITION) {
+                    double *o = (double *) host + 3 * i;
+                    o[0] = p[i].x; o[1] = p[i].y; o[2] = p[i].z;
+                } else {
+                    ((int *) host)[i] = pos_type(p[i]);
+                }
+            }
+            return n;
+        }
+        case PAIRS_HIP_VELOCITY:
+        case PAIRS_HIP_FORCE: {
+            const int n = lim(what == PAIRS_HIP_FORCE ? nlocal : nall);
+            double *src[3] = {what == PAIRS_HIP_FORCE ? d.fx : d.vx, what == PAIRS_HIP_FORCE ? d.fy : d.vy,
+                              what == PAIRS_HIP_FORCE ? d.fz : d.vz};
+            std::vector<double> c(n);
+            for (int k = 0; k < 3; k++) {
+                if (download_raw(ctx, c.data(), src[k], n * sizeof(double))) return PAIRS_HIP_ERR_HIP;
+                for (int i = 0; i < n; i++) ((double *) host)[3 * i + k] = c[i];
+            }
+            return n;
+        }
+        case PAIRS_HIP_MASS: {
+            const int n = lim(nall);
+            return download_raw(ctx, host, d.mass, n * sizeof(double)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_FLAGS: {
+            const int n = lim(nall);
+            return download_raw(ctx, host, d.flags, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_UID: {
+            const int n = lim(nall);
+            return download_raw(ctx, host, d.uid, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_NUMNEIGHS: {
+            const int n = lim(nlocal);
+            return download_raw(ctx, host, d.numneighs, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_NEIGHBORLISTS: {
+            const int n = lim(nlocal);
+            if (n == 0) return 0;
+            std::vector<int> rows((size_t) d.ncap * n);
+            HIP_OK(hipMemcpy2DAsync(rows.data(), n * sizeof(int), d.nl, (size_t) d.nl_stride * sizeof(int),
+                                    n * sizeof(int), d.ncap, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_OK(hipStreamSynchronize(ctx->stream));
+            int *o = (int *) host;
+            for (int i = 0; i < n; i++)
+                for (int k = 0; k < d.ncap; k++) o[(size_t) i * d.ncap + k] = rows[(size_t) k * n + i];
+            return n;
+        }
+        case PAIRS_HIP_PARTICLE_CELL: {
+            const int n = lim(nall);
+            return download_raw(ctx, host, d.particle_cell, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_CELL_SIZES: {
+            const int nc = lim(ctx->geom.ncells);
+            std::vector<int> st(ctx->geom.ncells + 1);
+            if (download_raw(ctx, st.data(), d.cell_start, (ctx->geom.ncells + 1) * sizeof(int))) return PAIRS_HIP_ERR_HIP;
+            for (int c = 0; c < nc; c++) ((int *) host)[c] = st[c + 1] - st[c];
+            return nc;
+        }
+        case PAIRS_HIP_CELL_PARTICLES: {
+            const int n = lim(nall);
+            return download_raw(ctx, host, d.sorted_idx, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_SEND_MAP: {
+            const int n = lim(nsend);
+            return download_raw(ctx, host, d.send_map, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_SEND_MULT: {
+            const int n = lim(nsend);
+            return download_raw(ctx, host, d.send_mult, 3 * (size_t) n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+        case PAIRS_HIP_NSEND: {
+            const int n = lim(6);
+            memcpy(host, ctx->h_counters + C_NSEND, n * sizeof(int));
+            return n;
+        }
+        case PAIRS_HIP_ATOM_ENERGY: {
+            const int n = lim(std::min(nlocal, ctx->eatom_cap));
+            if (!ctx->d_eatom) return 0;
+            return download_raw(ctx, host, ctx->d_eatom, n * sizeof(double)) ? PAIRS_HIP_ERR_HIP : n;
+        }
+    }
+    return PAIRS_HIP_ERR_ARG;
+}
+
+int pairs_hip_setup_cells(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    return setup_geometry(ctx);
+}
+
+static int finish_module(pairs_hip_ctx *ctx) {
+    CHECK_LAUNCH();
+    if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+    return take_overflow(ctx);
+}
+
+int pairs_hip_exchange(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    launch_exchange(ctx->stream, ctx->d, ctx->geom, ctx->cfg.periodic);
+    return finish_module(ctx);
+}
+
+int pairs_hip_borders(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    launch_borders(ctx->stream, ctx->d, ctx->geom, ctx->cfg.periodic);
+    return finish_module(ctx);
+}
+
+int pairs_hip_sync_ghosts(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    launch_sync_inplace(ctx->stream, ctx->d, ctx->geom, ctx->ghost_grid, 0);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+int pairs_hip_bin_cells(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    if (!ctx->cells_ready) {
+        int r = setup_geometry(ctx);
+        if (r) return r;
+    }
+    launch_bin(ctx->stream, ctx->d, ctx->geom);
+    return finish_module(ctx);
+}
+
+int pairs_hip_build_neighbors(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    const double rl = ctx->cfg.cell_spacing;
+    launch_build_neighbors(ctx->stream, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
+    return finish_module(ctx);
+}
+
+int pairs_hip_lj(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    DevState &d = ctx->d;
+    if (ctx->cfg.half) launch_lj_half(ctx->stream, d, ctx->prm, d.cur, 0);
+    else launch_lj_full(ctx->stream, d, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+int pairs_hip_vv_initial(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    launch_vv_initial(ctx->stream, ctx->d, ctx->prm);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+int pairs_hip_vv_final(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    DevState &d = ctx->d;
+    launch_vv_post(ctx->stream, d, ctx->prm, d.cur, d.cur, 1, 0, 0);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+static int ensure_eatom(pairs_hip_ctx *ctx) {
+    if (ctx->eatom_cap < ctx->d.nlocal) {
+        dfree(ctx->d_eatom);
+        ctx->d_eatom = nullptr;
+        if (dalloc(ctx, &ctx->d_eatom, ctx->d.cap)) return PAIRS_HIP_ERR_HIP;
+        ctx->eatom_cap = ctx->d.cap;
+    }
+    return 0;
+}
+
+int pairs_hip_energy(pairs_hip_ctx *ctx, double *pe, double *virial) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
+    double out[5];
+    launch_energy(ctx->stream, ctx->d, ctx->prm, ctx->cfg.half, ctx->d_eatom);
+    launch_thermo_reduce(ctx->stream, ctx->d, 2, ctx->geom, ctx->d.thermo);
+    CHECK_LAUNCH();
+    if (download_raw(ctx, out, ctx->d.thermo, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;
+    if (pe) *pe = out[2];
+    if (virial) *virial = out[3];
+    return 0;
+}
+
+int pairs_hip_thermo(pairs_hip_ctx *ctx, double out[5]) {
+    if (!ctx || !out) return PAIRS_HIP_ERR_ARG;
+    if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
+    launch_energy(ctx->stream, ctx->d, ctx->prm, ctx->cfg.half, ctx->d_eatom);
+    launch_ke_partial(ctx->stream, ctx->d);
+    launch_thermo_reduce(ctx->stream, ctx->d, 3, ctx->geom, ctx->d.thermo);
+    CHECK_LAUNCH();
+    return download_raw(ctx, out, ctx->d.thermo, 5 * sizeof(double));
+}
+
+int pairs_hip_sync(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 || ts == 0); }
+
+// exchange + borders + cells + neighbour lists, retried with grown capacities
+// on overflow like the generated resize loops (transformations/modules.py:159-205)
+static int rebuild(pairs_hip_ctx *ctx) {
+    if (!ctx->cells_ready) {
+        int r = setup_geometry(ctx);
+        if (r) return r;
+    }
+    for (int attempt = 0; attempt < 8; attempt++) {
+        DevState &d = ctx->d;
+        hipStream_t s = ctx->stream;
+        const double rl = ctx->cfg.cell_spacing;
+        launch_exchange(s, d, ctx->geom, ctx->cfg.periodic);
+        launch_borders(s, d, ctx->geom, ctx->cfg.periodic);
+        launch_bin(s, d, ctx->geom);
+        launch_build_neighbors(s, d, ctx->geom, ctx->cfg.half, rl * rl);
+        CHECK_LAUNCH();
+        if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+        const int code = take_overflow(ctx);
+        if (code == 0) return 0;
+        int pc = 0, nc = 0, sc = 0;
+        if (code == PAIRS_HIP_OVERFLOW_PARTICLES) pc = 2 * ctx->req[code];
+        if (code == PAIRS_HIP_OVERFLOW_NEIGHBORS) nc = 2 * ctx->req[code];
+        if (code == PAIRS_HIP_OVERFLOW_SEND) sc = 2 * ctx->req[code];
+        int r = pairs_hip_reserve(ctx, pc, nc, sc);
+        if (r) return r;
+    }
+    ctx->err = "capacity growth did not converge";
+    return PAIRS_HIP_ERR_STATE;
+}
+
+int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
+    if (!ctx || ts < 0 || ts > nsteps) return PAIRS_HIP_ERR_ARG;
+    DevState &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    if (ts > 0) launch_vv_initial(s, d, ctx->prm);
+    if (every(ts, ctx->cfg.reneighbor_every)) {
+        int r = rebuild(ctx);
+        if (r) return r;
+    } else {
+        launch_sync_inplace(s, d, ctx->geom, ctx->ghost_grid, 0);
+    }
+    if (ctx->cfg.half) launch_lj_half(s, d, ctx->prm, d.cur, 0);
+    else launch_lj_full(s, d, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    if (ts > 0) launch_vv_post(s, d, ctx->prm, d.cur, d.cur, 1, 0, 0);
+    CHECK_LAUNCH();
+    return 0;
+}
+
+// one non-rebuild step of the fused double-buffered form:
+//   sync ghosts of ts into P[cur] (stale multi-hop values from P[other])
+//   force(ts) + final_integrate(ts) + initial_integrate(ts+1) -> P[other]
+static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int reduce, bool timed) {
+    DevState &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    const int other = 1 - cur;
+    launch_sync_pp(s, d, ctx->geom, cur, other, ctx->ghost_grid);
+    if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
+    if (ctx->cfg.half) {
+        launch_lj_half(s, d, ctx->prm, cur, reduce);
+        if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
+        launch_vv_post(s, d, ctx->prm, cur, other, vv2, vv1, reduce);
+    } else {
+        launch_lj_full(s, d, ctx->prm, cur, other, 1, vv2, vv1, reduce);
+        if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
+    }
+}
+
+static int ensure_events(pairs_hip_ctx *ctx, int n) {
+    while ((int) ctx->ev_begin.size() < n) {
+        hipEvent_t a, b;
+        HIP_OK(hipEventCreate(&a));
+        HIP_OK(hipEventCreate(&b));
+        ctx->ev_begin.push_back(a);
+        ctx->ev_end.push_back(b);
+    }
+    return 0;
+}
+
+int pairs_hip_set_graphs(pairs_hip_ctx *ctx, int enable) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    ctx->use_graphs = enable != 0;
+    if (!ctx->use_graphs) drop_graphs(ctx);
+    return 0;
+}
+
+int pairs_hip_set_force_timing(pairs_hip_ctx *ctx, int every_n) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    ctx->time_force = every_n > 0;
+    ctx->time_every = every_n > 0 ? every_n : 10;
+    return 0;
+}
+
+int pairs_hip_force_kernel_time(pairs_hip_ctx *ctx, double *avg_ms, int *launches) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    if (avg_ms) *avg_ms = ctx->force_ms_avg;
+    if (launches) *launches = ctx->force_launches;
+    return 0;
+}
+
+static int graph_step(pairs_hip_ctx *ctx, int cur) {
+    if (!ctx->step_graph[cur] || ctx->graph_ghost_grid[cur] < ctx->ghost_grid) {
+        if (ctx->step_graph[cur]) hipGraphExecDestroy(ctx->step_graph[cur]);
+        ctx->step_graph[cur] = nullptr;
+        // capture with headroom so that later rebuilds with a few more ghosts reuse it
+        const int saved = ctx->ghost_grid;
+        ctx->ghost_grid = saved + saved / 4 + 4;
+        hipGraph_t g;
+        HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        launch_fused_step(ctx, cur, 1, 1, 0, false);
+        HIP_OK(hipStreamEndCapture(ctx->stream, &g));
+        hipError_t e = hipGraphInstantiate(&ctx->step_graph[cur], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        ctx->graph_ghost_grid[cur] = ctx->ghost_grid;
+        ctx->ghost_grid = saved;
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+            ctx->step_graph[cur] = nullptr;
+            return PAIRS_HIP_ERR_HIP;
+        }
+    }
+    HIP_OK(hipGraphLaunch(ctx->step_graph[cur], ctx->stream));
+    return 0;
+}
+
+int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, double *thermo_out, int thermo_capacity) {
+    if (!ctx || ts_begin < 0 || ts_end > nsteps + 1 || ts_begin > ts_end) return PAIRS_HIP_ERR_ARG;
+    DevState &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    int nthermo = 0;
+    bool prev_in_other = false;  // previous ghosts live in the other buffer
+    ctx->ev_used = 0;
+    if (ctx->time_force && ensure_events(ctx, 4096)) return PAIRS_HIP_ERR_HIP;
+    for (int ts = ts_begin; ts < ts_end; ts++) {
+        const bool R = every(ts, ctx->cfg.reneighbor_every);
+        const bool T = every(ts, ctx->cfg.thermo_every);
+        if (ts == ts_begin && ts > 0) launch_vv_initial(s, d, ctx->prm);
+        const int vv2 = ts > 0, vv1 = (ts + 1 < ts_end) && (ts + 1 <= nsteps);
+        const bool timed = ctx->time_force && !R && ((ts % ctx->time_every) == 0) && ctx->ev_used < 4096;
+        if (!R && !T && vv2 && vv1 && prev_in_other && ctx->use_graphs && !timed) {
+            int r = graph_step(ctx, d.cur);
+            if (r) return r;
+            d.cur = 1 - d.cur;
+            continue;
+        }
+        if (R) {
+            int r = rebuild(ctx);
+            if (r) return r;
+            prev_in_other = false;
+        }
+        const int cur = d.cur, other = 1 - cur;
+        if (!R && !prev_in_other) launch_sync_inplace(s, d, ctx->geom, ctx->ghost_grid, 0);
+        if (!R && prev_in_other) {
+            launch_fused_step(ctx, cur, vv2, vv1, T ? 1 : 0, timed);
+        } else {
+            // ghosts already current in P[cur]: force + integrators only
+            if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
+            if (ctx->cfg.half) {
+                launch_lj_half(s, d, ctx->prm, cur, T ? 1 : 0);
+                if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
+                launch_vv_post(s, d, ctx->prm, cur, other, vv2, vv1, T ? 1 : 0);
+            } else {
+                launch_lj_full(s, d, ctx->prm, cur, other, 1, vv2, vv1, T ? 1 : 0);
+                if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
+            }
+        }
+        if (timed) ctx->ev_used++;
+        if (T && nthermo < 4096) {
+            launch_thermo_reduce(s, d, 3, ctx->geom, d.thermo + 5 * nthermo);
+            nthermo++;
+        }
+        CHECK_LAUNCH();
+        if (vv1) { d.cur = other; prev_in_other = true; }
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    if (thermo_out && nthermo > 0) {
+        const int n = std::min(nthermo, thermo_capacity);
+        HIP_OK(hipMemcpy(thermo_out, d.thermo, 5 * n * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (ctx->time_force && ctx->ev_used > 0) {
+        double tot = 0.0;
+        for (int k = 0; k < ctx->ev_used; k++) {
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, ctx->ev_begin[k], ctx->ev_end[k]));
+            tot += ms;
+        }
+        ctx->force_ms_avg = tot / ctx->ev_used;
+        ctx->force_launches = ctx->ev_used;
+    }
+    return nthermo;
+}
+
+}  // extern "C"

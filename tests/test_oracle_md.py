@@ -1,0 +1,224 @@
+"""Pin the CPU oracle of the LJ path (oracle/md_oracle.c).
+
+1. Against the golden fixtures written by tests/golden/make_golden.py from the
+   reference's own generated CPU program (bit-exact: positions, velocities,
+   forces, ghost positions, neighbour lists, cell membership, send maps).
+2. Against what the mathematics fixes, independently of the reference:
+   closed-form two-body LJ force, brute-force neighbour sets over explicit
+   periodic images, Newton's third law, -dU/dx == F, full == half forces,
+   lattice/thermo closed forms.
+"""
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _golden(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+@pytest.mark.parametrize("name,half", [("md_fcc4_full", False), ("md_fcc4_half", True)])
+def test_oracle_matches_reference_program_bit_exact(name, half):
+    z = _golden(name)
+    s = po.fcc_system(4, half=half)
+    assert np.array_equal(s.positions(), z["init_position"])
+    assert np.array_equal(s.velocities(), z["init_velocity"])
+    checked = 0
+    for ts in range(int(z["meta_steps"]) + 1):
+        s.step(ts)
+        assert s.nlocal == z["nlocal_all"][ts] and s.nghost == z["nghost_all"][ts], ts
+        if f"s{ts}_position" in z.files:
+            assert np.array_equal(s.positions(), z[f"s{ts}_position"]), ts
+            assert np.array_equal(s.velocities(), z[f"s{ts}_velocity"]), ts
+            assert np.array_equal(s.forces(), z[f"s{ts}_force"]), ts
+            checked += 1
+        if f"s{ts}_ghost_position" in z.files:
+            assert np.array_equal(s.positions(True)[s.nlocal:], z[f"s{ts}_ghost_position"]), ts
+        if f"s{ts}_numneighs" in z.files:
+            assert np.array_equal(s.numneighs(), z[f"s{ts}_numneighs"])
+            assert np.array_equal(np.concatenate(s.neighbor_lists()), z[f"s{ts}_neighbors"])
+            assert np.array_equal(s.particle_cell(), z[f"s{ts}_particle_cell"])
+            sm, mult = s.send_map()
+            assert np.array_equal(sm, z[f"s{ts}_send_map"])
+            assert np.array_equal(mult, z[f"s{ts}_send_mult"])
+            assert np.array_equal(s.nsend(), z[f"s{ts}_nsend"])
+            sizes, cells = s.cells()
+            assert np.array_equal(sizes, z[f"s{ts}_cell_sizes"])
+            assert np.array_equal(np.concatenate(cells), z[f"s{ts}_cell_particles"])
+        if ts in (0, 99):
+            t, p = s.last_thermo()
+            row = z["thermo_printed"][0 if ts == 0 else 1]
+            # the program prints 6 significant digits
+            assert abs(t - row[0]) <= 5e-6 * abs(row[0]) and abs(p - row[1]) <= 5e-6 * abs(row[1])
+    assert checked == 17
+
+
+def test_fcc_lattice_closed_form():
+    for n in (2, 3, 4):
+        s = po.fcc_system(n)
+        x = s.positions()
+        assert len(x) == 4 * n ** 3
+        a = math.pow(4.0 / 0.8442, 1.0 / 3.0)
+        # every site is 0.5*a*(i,j,k) with i+j+k even
+        ijk = np.rint(x / (0.5 * a)).astype(int)
+        assert np.allclose(ijk * (0.5 * a), x, rtol=0, atol=1e-12)
+        assert np.all(ijk.sum(axis=1) % 2 == 0)
+        assert len({tuple(r) for r in ijk}) == len(x)
+        v = s.velocities()
+        assert np.allclose(v.sum(axis=0), 0.0, atol=1e-12)
+        assert abs(s.temperature() - 1.44) < 1e-12
+
+
+def _lj_force(d, eps=1.0, s6=1.0):
+    r2 = d @ d
+    sr2 = 1.0 / r2
+    sr6 = sr2 ** 3 * s6
+    return d * (48.0 * sr6 * (sr6 - 0.5) * sr2 * eps)
+
+
+def test_two_body_force_closed_form():
+    L = 20.0
+    for r in (0.95, 2.0 ** (1.0 / 6.0), 1.3, 2.2, 2.49, 2.6):
+        s = po.OracleMD([(0.0, L)] * 3, 64, reneighbor_every=1)
+        x = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]])
+        s.set_particles(x, np.zeros_like(x))
+        s.setup_cells()
+        s.rebuild()
+        s.force()
+        f = s.forces()
+        # analytic: F = 24 eps (2 (s/r)^12 - (s/r)^6) / r along the separation
+        fa = 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7) if r < 2.5 else 0.0
+        assert abs(f[0, 0] + fa) <= 1e-12 * max(1.0, abs(fa))
+        assert np.allclose(f[0] + f[1], 0.0, atol=1e-14)
+        if abs(r - 2.0 ** (1.0 / 6.0)) < 1e-15:
+            assert abs(f[0, 0]) < 1e-12
+
+
+def _brute_images(x, L, rlist):
+    """All (i, j, image) pairs with |x_i - (x_j + image L)| < rlist, images in {-1,0,1}^3."""
+    n = len(x)
+    out = set()
+    r2 = rlist * rlist
+    for sh in itertools.product((-1, 0, 1), repeat=3):
+        shift = np.array(sh, float) * L
+        d = x[:, None, :] - (x[None, :, :] + shift)
+        rsq = (d * d).sum(axis=2)
+        ii, jj = np.nonzero(rsq < r2)
+        for i, j in zip(ii, jj):
+            if sh == (0, 0, 0) and i == j:
+                continue
+            out.add((int(i), int(j), sh))
+    return out
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_neighbor_sets_brute_force(half):
+    """Lists built at ts=19 (liquid) equal the brute-force set of (i, root j,
+    periodic image) within the list radius; a half list holds each local pair
+    once (i < j) and each cross-boundary pair from both sides."""
+    s = po.fcc_system(4, half=half)
+    for ts in range(20):
+        s.step(ts)
+    x = s.positions(True)
+    nl = s.nlocal
+    L = po.fcc_box_length(4, 0.8442)
+    sm, mult = s.send_map()
+    root = np.arange(len(x))
+    img = np.zeros((len(x), 3), int)
+    for g in range(s.nghost):
+        src = sm[g]
+        root[nl + g] = root[src]
+        img[nl + g] = img[src] + mult[g]
+    got = [(i, int(root[j]), tuple(int(k) for k in img[j])) for i, lst in enumerate(s.neighbor_lists()) for j in lst]
+    assert len(got) == len(set(got))
+    got = set(got)
+    # ts=19 is a rebuild step; the lists use the positions of that step, which
+    # stay in the locals until the force kernel -- the step has not integrated since
+    want = _brute_images(x[:nl], L, 2.8)
+    if not half:
+        assert got == want
+    else:
+        local = {(i, j, sh) for (i, j, sh) in want if sh == (0, 0, 0) and i < j}
+        cross = {(i, j, sh) for (i, j, sh) in want if sh != (0, 0, 0)}
+        assert got == local | cross
+
+
+def test_neighbor_sets_brute_force_exact_images():
+    """Full list at ts=0 (lattice, rebuild) equals the brute-force image set exactly."""
+    s = po.fcc_system(4)
+    s.step(0)
+    x = s.positions(True)
+    nl = s.nlocal
+    sm, mult = s.send_map()
+    root = np.arange(len(x))
+    for g in range(s.nghost):
+        root[nl + g] = root[sm[g]]
+    L = po.fcc_box_length(4, 0.8442)
+    got = set()
+    for i, lst in enumerate(s.neighbor_lists()):
+        for j in lst:
+            d = x[j] - x[root[j]]
+            sh = tuple(int(round(c / L)) for c in d)
+            got.add((i, int(root[j]), sh))
+    want = _brute_images(x[:nl], L, 2.8)
+    assert got == want
+    assert all(n == 78 for n in s.numneighs())  # FCC shells 12+6+24+12+24 inside 2.8
+
+
+def test_full_and_half_forces_agree():
+    a = po.fcc_system(4)
+    b = po.fcc_system(4, half=True)
+    for ts in range(40):
+        a.step(ts)
+        b.step(ts)
+    fa, fb = a.forces(), b.forces()
+    assert np.allclose(fa, fb, rtol=0, atol=1e-11 * np.abs(fa).max())
+    # Newton's third law over a periodic system
+    assert np.allclose(fa.sum(axis=0), 0.0, atol=1e-10)
+
+
+def test_energy_gradient_is_force():
+    s = po.fcc_system(3)
+    for ts in range(30):
+        s.step(ts)
+    x = s.positions()
+    v = s.velocities()
+    L = po.fcc_box_length(3, 0.8442)
+
+    def pe_at(xx):
+        t = po.OracleMD([(0.0, L)] * 3, 4096, reneighbor_every=1)
+        t.set_particles(xx, v)
+        t.setup_cells()
+        t.rebuild()
+        t.force()
+        return t.energy()[0], t.forces()
+
+    _, f0 = pe_at(x)
+    h = 1e-6
+    for (i, d) in [(0, 0), (5, 1), (17, 2), (60, 0)]:
+        xp = x.copy(); xp[i, d] += h
+        xm = x.copy(); xm[i, d] -= h
+        num = -(pe_at(xp)[0] - pe_at(xm)[0]) / (2 * h)
+        assert abs(num - f0[i, d]) < 1e-6 * max(1.0, abs(f0[i, d]))
+
+
+def test_energy_two_body_closed_form():
+    s = po.OracleMD([(0.0, 20.0)] * 3, 64, reneighbor_every=1)
+    r = 1.2
+    x = np.array([[5.0, 5.0, 5.0], [5.0, 5.0 + r, 5.0]])
+    s.set_particles(x, np.zeros_like(x))
+    s.setup_cells()
+    s.rebuild()
+    s.force()
+    pe, w, e = s.energy()
+    u = 4.0 * (r ** -12 - r ** -6)
+    assert abs(pe - u) < 1e-13 and abs(e.sum() - u) < 1e-13
+    fr = 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7)
+    assert abs(w - r * fr) < 1e-12
