@@ -5,8 +5,10 @@
 // library is compiled with -ffp-contract=off, so each value is rounded exactly
 // as in the reference's generated CPU program.  Integer/index work (cells,
 // ghost maps, neighbour lists) is deterministic and reproduces the CPU
-// program's order: stable compactions by ballot/prefix scan instead of the
-// atomic insertion the generated CUDA code uses.
+// program's order on reference indices: stable compactions by ballot/prefix
+// scan instead of the atomic insertion the generated CUDA code uses.
+// The storage layout (cell-ordered physical slots, reference-index
+// relabeling, segments and their LDS tiles) is described in md_kernels.hpp.
 #include "md_kernels.hpp"
 
 namespace pk {
@@ -19,42 +21,51 @@ __global__ void k_fill_i32(int *__restrict__ p, int v, int n) {
 __global__ void k_fill_f64(double *__restrict__ p, double v, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_identity(int *__restrict__ p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
 __global__ void k_set_counters(int *__restrict__ c, int first, int count, int v) {
-    int t = threadIdx.x;
+    const int t = threadIdx.x;
     if (t < count) c[first + t] = v;
 }
 
-// exclusive scan of one value per thread over a block of BS threads (BS = 64 * nwaves)
+// exclusive scan of one value per thread over a block of BS threads
 template <int BS>
 __device__ __forceinline__ int block_exclusive_scan(int v, int *lds_waves, int *total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        int y = __shfl_up(x, off, 64);
+        const int y = __shfl_up(x, off, 64);
         if (lane >= off) x += y;
     }
     if (lane == 63) lds_waves[wave] = x;
     __syncthreads();
     if (wave == 0) {
-        int w = lane < BS / 64 ? lds_waves[lane] : 0;
+        const int w = lane < BS / 64 ? lds_waves[lane] : 0;
         int s = w;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
-            int y = __shfl_up(s, off, 64);
+            const int y = __shfl_up(s, off, 64);
             if (lane >= off) s += y;
         }
         if (lane < BS / 64) lds_waves[lane] = s - w;
         if (lane == BS / 64 - 1) lds_waves[BS / 64] = s;
     }
     __syncthreads();
-    int r = x - v + lds_waves[wave];
+    const int r = x - v + lds_waves[wave];
     *total = lds_waves[BS / 64];
     __syncthreads();
     return r;
 }
 
-// ---------------------------------------------------------------- cells
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------- geometry
 
 // cell of a position: src/pairs/sim/cell_lists.py:114-126 (generated as
 // (int)((x - (min - s)) / s), clamped to [0, n-1], flattened z fastest, + 1)
@@ -63,7 +74,7 @@ __device__ __forceinline__ int cell_of(double x, double y, double z, const Geom 
     int idx = 0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double q = (xs[d] - (g.lo[d] - g.spacing)) / g.spacing;
+        const double q = (xs[d] - (g.lo[d] - g.spacing)) / g.spacing;
         int c = (int) q >= 0 ? (int) q : 0;
         c = c < g.dim_cells[d] ? c : g.dim_cells[d] - 1;
         idx = d == 0 ? c : idx * g.dim_cells[d] + c;
@@ -71,177 +82,416 @@ __device__ __forceinline__ int cell_of(double x, double y, double z, const Geom 
     return idx + 1;
 }
 
-// BuildCellLists (src/pairs/sim/cell_lists.py:90-134), histogram half:
-// cell id of every local and ghost particle, per-cell counts.
-__global__ __launch_bounds__(256) void k_bin_count(const double4 *__restrict__ P, const int *__restrict__ flags,
-                                                   const int *__restrict__ counters, Geom g,
-                                                   int *__restrict__ particle_cell, int *__restrict__ cell_count,
-                                                   int *__restrict__ slot) {
-    if (counters[C_OVF]) return;
-    const int n = counters[C_NLOCAL] + counters[C_NGHOST];
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        int c = 0;
-        if (!(flags[i] & FLAG_INFINITE)) {
-            double4 p = P[i];
-            c = cell_of(p.x, p.y, p.z, g);
+struct SegInfo {
+    int cx, cy, za, zb;  // column and z range [za, zb) of the home cells
+    int ca, cb;          // flat cell ids of the first home cell and one past the last
+};
+
+__device__ __forceinline__ SegInfo seg_info(int seg, const Geom &g) {
+    SegInfo s;
+    const int chunk = seg % g.nchunks, col = seg / g.nchunks;
+    s.cy = col % g.dim_cells[1];
+    s.cx = col / g.dim_cells[1];
+    s.za = chunk == 0 ? 0 : 1 + chunk * g.zc;
+    s.zb = chunk == g.nchunks - 1 ? g.dim_cells[2] : 1 + (chunk + 1) * g.zc;
+    const int base = (s.cx * g.dim_cells[1] + s.cy) * g.dim_cells[2] + 1;
+    s.ca = base + s.za;
+    s.cb = base + s.zb;
+    return s;
+}
+
+constexpr int NRUNS = 20;
+
+// The 20 physical runs of a segment's tile, in the reference's candidate
+// order of cells: cell 0 (locals, ghosts), then the columns (dx, dy) of the
+// stencil (dx outer, dy inner; src/pairs/sim/cell_lists.py:70-87), each over
+// flat cells [B + za - 1, B + zb] clipped to the grid (a stencil cell of the
+// reference is valid iff 0 < nc < ncells, interaction.py:92-98), locals then
+// ghosts.  Written by threads 0..19; r_off is the exclusive prefix (r_off[20]
+// = tile size).  lo of each column is kept for candidate enumeration.
+__device__ __forceinline__ void seg_runs(const SegInfo &si, const Geom &g, const int *__restrict__ lstart,
+                                         const int *__restrict__ gstart, int nlocal, int *r_start, int *r_len,
+                                         int *r_off, int *col_lo) {
+    const int t = threadIdx.x;
+    if (t < NRUNS) {
+        int start = 0, len = 0;
+        if (t < 2) {
+            if (t == 0) { start = lstart[0]; len = lstart[1] - lstart[0]; }
+            else { start = nlocal + gstart[0]; len = gstart[1] - gstart[0]; }
+        } else {
+            const int col = (t - 2) >> 1, loc = ((t - 2) & 1) == 0;
+            const int dx = col / 3 - 1, dy = col % 3 - 1;
+            const int B = ((si.cx + dx) * g.dim_cells[1] + (si.cy + dy)) * g.dim_cells[2] + 1;
+            int lo = B + si.za - 1, hi = B + si.zb;
+            lo = max(lo, 1);
+            hi = min(hi, g.ncells - 1);
+            if (lo <= hi) {
+                if (loc) { start = lstart[lo]; len = lstart[hi + 1] - lstart[lo]; }
+                else { start = nlocal + gstart[lo]; len = gstart[hi + 1] - gstart[lo]; }
+            }
+            if (loc) col_lo[col] = lo;
         }
-        particle_cell[i] = c;
-        slot[i] = atomicAdd(&cell_count[c], 1);
+        r_start[t] = start;
+        r_len[t] = len;
     }
-}
-
-// exclusive scan of the cell counts -> cell_start[0..ncells]; one block
-__global__ __launch_bounds__(1024) void k_scan_cells(const int *__restrict__ counters, const int *__restrict__ cell_count,
-                                                     int *__restrict__ cell_start, int ncells) {
-    __shared__ int lds[20];
-    if (counters[C_OVF]) return;
-    int carry = 0;
-    for (int base = 0; base < ncells; base += 1024) {
-        int c = base + threadIdx.x;
-        int v = c < ncells ? cell_count[c] : 0;
-        int total;
-        int ex = block_exclusive_scan<1024>(v, lds, &total);
-        if (c < ncells) cell_start[c] = carry + ex;
-        carry += total;
+    __syncthreads();
+    if (t == 0) {
+        int off = 0;
+        for (int r = 0; r < NRUNS; r++) { r_off[r] = off; off += r_len[r]; }
+        r_off[NRUNS] = off;
     }
-    if (threadIdx.x == 0) cell_start[ncells] = carry;
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ counters, const int *__restrict__ particle_cell,
-                                                     const int *__restrict__ slot, const int *__restrict__ cell_start,
-                                                     int *__restrict__ sorted_idx) {
-    if (counters[C_OVF]) return;
-    const int n = counters[C_NLOCAL] + counters[C_NGHOST];
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-        sorted_idx[cell_start[particle_cell[i]] + slot[i]] = i;
+__device__ __forceinline__ int run_of(const int *r_off, int p) {
+    int r = 0;
+#pragma unroll
+    for (int k = 1; k < NRUNS; k++) r += (p >= r_off[k]) ? 1 : 0;
+    return r;
 }
 
-// Order each cell by particle index -- the order in which the CPU program's
-// sequential "atomic" insertion fills cell_particles -- then gather the
-// cell-sorted copy of the positions and count the locals of each cell.
-__global__ __launch_bounds__(256) void k_sort_cells(const int *__restrict__ counters, const int *__restrict__ cell_start,
-                                                    int *__restrict__ sorted_idx, const double4 *__restrict__ P,
-                                                    double4 *__restrict__ sorted_pos, int *__restrict__ cell_nlocal,
-                                                    int ncells) {
+// ---------------------------------------------------------------- cells (binning by counting sort)
+
+// BuildCellLists (src/pairs/sim/cell_lists.py:90-134), histogram half: cell
+// of every local and ghost, per-cell counts of locals and of ghosts.
+__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ X, const double *__restrict__ Y,
+                                                   const double *__restrict__ Z, const int *__restrict__ flags,
+                                                   const int *__restrict__ counters, Geom g,
+                                                   int *__restrict__ cell, int *__restrict__ lcount,
+                                                   int *__restrict__ gcount, int *__restrict__ slot) {
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) {
-        const int b = cell_start[c], e = cell_start[c + 1];
-        for (int k = b + 1; k < e; k++) {  // insertion sort: ~18 entries per cell
-            int v = sorted_idx[k];
+    const int n = nlocal + counters[C_NGHOST];
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
+        const int c = (flags[p] & FLAG_INFINITE) ? 0 : cell_of(X[p], Y[p], Z[p], g);
+        cell[p] = c;
+        slot[p] = atomicAdd(p < nlocal ? &lcount[c] : &gcount[c], 1);
+    }
+}
+
+// exclusive scans of the local and ghost cell counts -> lstart, gstart [0..ncells]
+__global__ __launch_bounds__(1024) void k_scan_cells(const int *__restrict__ counters, const int *__restrict__ lcount,
+                                                     const int *__restrict__ gcount, int *__restrict__ lstart,
+                                                     int *__restrict__ gstart, int ncells) {
+    __shared__ int lds[20];
+    if (counters[C_OVF]) return;
+    for (int which = 0; which < 2; which++) {
+        const int *cnt = which ? gcount : lcount;
+        int *st = which ? gstart : lstart;
+        int carry = 0;
+        for (int base = 0; base < ncells; base += 1024) {
+            const int c = base + threadIdx.x;
+            const int v = c < ncells ? cnt[c] : 0;
+            int total;
+            const int ex = block_exclusive_scan<1024>(v, lds, &total);
+            if (c < ncells) st[c] = carry + ex;
+            carry += total;
+        }
+        if (threadIdx.x == 0) st[ncells] = carry;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ counters, const int *__restrict__ cell,
+                                                     const int *__restrict__ slot, const int *__restrict__ lstart,
+                                                     const int *__restrict__ gstart, int *__restrict__ perm) {
+    if (counters[C_OVF]) return;
+    const int nlocal = counters[C_NLOCAL];
+    const int n = nlocal + counters[C_NGHOST];
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
+        const int c = cell[p];
+        const int np = p < nlocal ? lstart[c] + slot[p] : nlocal + gstart[c] + slot[p];
+        perm[np] = p;
+    }
+}
+
+// Order the locals and the ghosts of every cell by reference index -- the
+// order in which the CPU program's sequential insertion fills cell_particles.
+// One wave per (cell, locals|ghosts) run: a rank sort of up to 64 keys in
+// registers, a serial insertion sort beyond that.
+__global__ __launch_bounds__(256) void k_sort_cells(const int *__restrict__ counters, const int *__restrict__ lstart,
+                                                    const int *__restrict__ gstart, const int *__restrict__ ref,
+                                                    int *__restrict__ perm, int ncells) {
+    if (counters[C_OVF]) return;
+    const int nlocal = counters[C_NLOCAL];
+    const int lane = threadIdx.x & 63;
+    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= 2 * ncells) return;
+    const int c = item >> 1, ghosts = item & 1;
+    const int b = ghosts ? nlocal + gstart[c] : lstart[c];
+    const int e = ghosts ? nlocal + gstart[c + 1] : lstart[c + 1];
+    const int len = e - b;
+    if (len <= 1) return;
+    if (len <= 64) {
+        const int p = lane < len ? perm[b + lane] : 0;
+        const int key = lane < len ? ref[p] : 0x7fffffff;
+        int rank = 0;
+        for (int q = 0; q < len; q++) {
+            const int kq = __shfl(key, q, 64);
+            rank += (kq < key) ? 1 : 0;
+        }
+        if (lane < len) perm[b + rank] = p;
+    } else if (lane == 0) {
+        for (int k = b + 1; k < e; k++) {
+            const int v = perm[k], kv = ref[v];
             int q = k - 1;
-            while (q >= b && sorted_idx[q] > v) { sorted_idx[q + 1] = sorted_idx[q]; q--; }
-            sorted_idx[q + 1] = v;
+            while (q >= b && ref[perm[q]] > kv) { perm[q + 1] = perm[q]; q--; }
+            perm[q + 1] = v;
         }
-        int nl = 0;
-        for (int k = b; k < e; k++) {
-            int i = sorted_idx[k];
-            sorted_pos[k] = P[i];
-            nl += i < nlocal;
-        }
-        cell_nlocal[c] = nl;
+    }
+}
+
+// Move every particle array into cell order (into the spare arrays) and
+// update phys[reference index].
+__global__ __launch_bounds__(256) void k_permute(const int *__restrict__ counters, const int *__restrict__ perm,
+                                                 const double *__restrict__ X, const double *__restrict__ Y,
+                                                 const double *__restrict__ Z, double *__restrict__ Xn,
+                                                 double *__restrict__ Yn, double *__restrict__ Zn,
+                                                 const double *__restrict__ vx, const double *__restrict__ vy,
+                                                 const double *__restrict__ vz, const double *__restrict__ mass,
+                                                 const int *__restrict__ type, const int *__restrict__ flags,
+                                                 const int *__restrict__ uid, const int *__restrict__ shape,
+                                                 const int *__restrict__ ref, const int *__restrict__ cell,
+                                                 double *__restrict__ avx, double *__restrict__ avy,
+                                                 double *__restrict__ avz, double *__restrict__ amass,
+                                                 int *__restrict__ atype, int *__restrict__ aflags,
+                                                 int *__restrict__ auid, int *__restrict__ ashape,
+                                                 int *__restrict__ aref, int *__restrict__ acell,
+                                                 int *__restrict__ phys) {
+    if (counters[C_OVF]) return;
+    const int n = counters[C_NLOCAL] + counters[C_NGHOST];
+    for (int np = blockIdx.x * 256 + threadIdx.x; np < n; np += gridDim.x * 256) {
+        const int p = perm[np];
+        Xn[np] = X[p]; Yn[np] = Y[p]; Zn[np] = Z[p];
+        avx[np] = vx[p]; avy[np] = vy[p]; avz[np] = vz[p]; amass[np] = mass[p];
+        atype[np] = type[p]; aflags[np] = flags[p]; auid[np] = uid[p]; ashape[np] = shape[p];
+        const int r = ref[p];
+        aref[np] = r;
+        acell[np] = cell[p];
+        phys[r] = np;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy_back(const int *__restrict__ counters, const double *__restrict__ avx,
+                                                   const double *__restrict__ avy, const double *__restrict__ avz,
+                                                   const double *__restrict__ amass, const int *__restrict__ atype,
+                                                   const int *__restrict__ aflags, const int *__restrict__ auid,
+                                                   const int *__restrict__ ashape, const int *__restrict__ aref,
+                                                   const int *__restrict__ acell, double *__restrict__ vx,
+                                                   double *__restrict__ vy, double *__restrict__ vz,
+                                                   double *__restrict__ mass, int *__restrict__ type,
+                                                   int *__restrict__ flags, int *__restrict__ uid,
+                                                   int *__restrict__ shape, int *__restrict__ ref,
+                                                   int *__restrict__ cell) {
+    if (counters[C_OVF]) return;
+    const int n = counters[C_NLOCAL] + counters[C_NGHOST];
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
+        vx[p] = avx[p]; vy[p] = avy[p]; vz[p] = avz[p]; mass[p] = amass[p];
+        type[p] = atype[p]; flags[p] = aflags[p]; uid[p] = auid[p]; shape[p] = ashape[p];
+        ref[p] = aref[p]; cell[p] = acell[p];
+    }
+}
+
+// Ghost refresh maps in physical order: the physical slot of each ghost's
+// source and its image multipliers (send_map / send_mult are in reference
+// ghost order: ghost g = reference index nlocal + g).
+__global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ counters, const int *__restrict__ ref,
+                                                   const int *__restrict__ phys, const int *__restrict__ send_map,
+                                                   const int *__restrict__ send_mult, int *__restrict__ sync_src,
+                                                   int *__restrict__ sync_mult) {
+    if (counters[C_OVF]) return;
+    const int nlocal = counters[C_NLOCAL], ng = counters[C_NGHOST];
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < ng; q += gridDim.x * 256) {
+        const int g = ref[nlocal + q] - nlocal;
+        sync_src[q] = phys[send_map[g]];
+        sync_mult[3 * q + 0] = send_mult[3 * g + 0];
+        sync_mult[3 * q + 1] = send_mult[3 * g + 1];
+        sync_mult[3 * q + 2] = send_mult[3 * g + 2];
     }
 }
 
 // ---------------------------------------------------------------- neighbour lists
 
 // BuildNeighborLists (src/pairs/sim/neighbor_lists.py:21-48) over the cell
-// stencil (src/pairs/sim/interaction.py:90-120): candidates are cell 0 then
-// the 27 stencil cells in order, each cell in index order; accepted when
-// j != i (full) or i < j (half, point masses) and rsq < (rc+skin)^2.
+// stencil (src/pairs/sim/interaction.py:90-120): candidates of a particle in
+// cell c are cell 0, then the 27 stencil cells in order, each cell in
+// reference-index order; accepted when j != i (full) or i < j (half, point
+// masses) and rsq < (rc + skin)^2.
 //
-// One workgroup per home cell: the candidate tile (positions + indices) is
-// staged in LDS once and reused by every home particle; each wave takes one
-// home particle at a time, its 64 lanes test 64 candidates, and a ballot with
-// a lane prefix count compacts the accepted ones in candidate order.
-constexpr int NB_TILE = 1536;
-constexpr int NB_HOME = 256;
+// One workgroup per segment.  Its tile (positions + reference indices) is
+// staged in LDS; per home cell the candidate order is materialised once as a
+// list of tile offsets; each wave takes one home particle at a time, its 64
+// lanes test 64 candidates, and a ballot with a lane prefix count compacts
+// the accepted ones in candidate order.  Lists are 16-bit tile offsets,
+// transposed through LDS so each row k is written as one contiguous run.
+constexpr int NB_CAND = 1024;
 
 __global__ __launch_bounds__(256) void k_build_neighbors(const int *__restrict__ counters, Geom g,
-                                                         const int *__restrict__ cell_start,
-                                                         const int *__restrict__ cell_nlocal,
-                                                         const int *__restrict__ sorted_idx,
-                                                         const double4 *__restrict__ sorted_pos,
+                                                         const int *__restrict__ lstart,
+                                                         const int *__restrict__ gstart,
+                                                         const int *__restrict__ lcount,
+                                                         const int *__restrict__ gcount,
+                                                         const double *__restrict__ X, const double *__restrict__ Y,
+                                                         const double *__restrict__ Z, const int *__restrict__ ref,
                                                          const int *__restrict__ flags, int half, double rc2,
-                                                         int *__restrict__ nl, int nl_stride, int ncap,
-                                                         int *__restrict__ numneighs, int *__restrict__ ovf) {
-    __shared__ double tx[NB_TILE], ty[NB_TILE], tz[NB_TILE];
-    __shared__ int tj[NB_TILE];
-    __shared__ int seg_begin[28], seg_off[29];
-    __shared__ int home_cnt[NB_HOME];
+                                                         uint16_t *__restrict__ nl, int nl_stride, int ncap, int hb,
+                                                         int tcap, int *__restrict__ numneighs,
+                                                         int *__restrict__ ctr) {
+    extern __shared__ char smem[];
+    double *tx = (double *) smem;
+    double *ty = tx + tcap;
+    double *tz = ty + tcap;
+    int *tref = (int *) (tz + tcap);
+    uint16_t *cand = (uint16_t *) (tref + tcap);
+    uint16_t *lbuf = cand + NB_CAND;  // [ncap][hb]
+    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
+    __shared__ int ncand_s, home_cnt[256];
+    __shared__ int sub_start[56], sub_len[56], sub_off[57];
     if (counters[C_OVF]) return;
-    const int c = xcd_block(blockIdx.x, gridDim.x);
-    if (c >= g.ncells) return;
-    const int nhome = cell_nlocal[c];
-    if (nhome == 0) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0) {
-        int off = 0;
-        for (int s = 0; s < 28; s++) {
-            int nc = s == 0 ? 0 : c + g.stencil[s - 1];
-            int ok = s == 0 || (s - 1 < g.nstencil && nc > 0 && nc < g.ncells);
-            int len = ok ? cell_start[nc + 1] - cell_start[nc] : 0;
-            seg_begin[s] = ok ? cell_start[nc] : 0;
-            seg_off[s] = off;
-            off += len;
-        }
-        seg_off[28] = off;
+    const int nlocal = counters[C_NLOCAL];
+    const int seg = xcd_block(blockIdx.x, gridDim.x);
+    if (seg >= g.nsegs) return;
+    const SegInfo si = seg_info(seg, g);
+    const int h0 = lstart[si.ca], h1 = lstart[si.cb];
+    if (h1 <= h0) return;
+    seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
+    const int T = r_off[NRUNS];
+    if (T > tcap) {
+        if (threadIdx.x == 0) { atomicMax(&ctr[C_TILE_MAX], T); atomicMax(&ctr[C_OVF], OVF_TILE); }
+        return;
     }
-    __syncthreads();
-    const int total = seg_off[28];
-    const int home0 = cell_start[c];
-    for (int hb = 0; hb < nhome; hb += NB_HOME) {
-        const int nh = min(NB_HOME, nhome - hb);
-        for (int a = threadIdx.x; a < nh; a += 256) home_cnt[a] = 0;
-        for (int c0 = 0; c0 < total; c0 += NB_TILE) {
-            const int len = min(NB_TILE, total - c0);
-            __syncthreads();
-            for (int p = threadIdx.x; p < len; p += 256) {
-                const int q = c0 + p;
-                int s = 0;
-                while (seg_off[s + 1] <= q) s++;
-                const int gi = seg_begin[s] + (q - seg_off[s]);
-                const double4 pp = sorted_pos[gi];
-                tx[p] = pp.x; ty[p] = pp.y; tz[p] = pp.z;
-                tj[p] = sorted_idx[gi];
+    if (threadIdx.x == 0) atomicMax(&ctr[C_TILE_MAX], T);
+    for (int p = threadIdx.x; p < T; p += 256) {
+        const int r = run_of(r_off, p);
+        const int ph = r_start[r] + (p - r_off[r]);
+        tx[p] = X[ph]; ty[p] = Y[ph]; tz[p] = Z[ph];
+        tref[p] = ref[ph];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int cz = si.za; cz < si.zb; cz++) {
+        const int c = si.ca + (cz - si.za);
+        const int nh = lcount[c];
+        __syncthreads();
+        if (nh == 0) continue;
+        // candidate tile offsets of this cell, in reference order: 56 sub-ranges
+        // (cell 0 locals, ghosts; then per stencil column and dz: locals,
+        // ghosts of that cell), one thread each, then a parallel fill
+        if (threadIdx.x < 56) {
+            const int t = threadIdx.x;
+            int st = 0, ln = 0;
+            if (t < 2) {
+                st = r_off[t];
+                ln = r_len[t];
+            } else {
+                const int u = t - 2, col = u / 6, dz = (u % 6) / 2 - 1, loc = (u & 1) == 0;
+                const int dx = col / 3 - 1, dy = col % 3 - 1;
+                const int B = ((si.cx + dx) * g.dim_cells[1] + (si.cy + dy)) * g.dim_cells[2] + 1;
+                const int cc = B + cz + dz;
+                if (cc > 0 && cc < g.ncells) {
+                    const int lo = col_lo[col];
+                    if (loc) { st = r_off[2 + 2 * col] + (lstart[cc] - lstart[lo]); ln = lcount[cc]; }
+                    else { st = r_off[3 + 2 * col] + (gstart[cc] - gstart[lo]); ln = gcount[cc]; }
+                }
             }
-            __syncthreads();
-            for (int a = wave; a < nh; a += 4) {
-                const int i = sorted_idx[home0 + hb + a];
-                if (flags[i] & FLAG_FIXED) continue;
-                const double4 pi = sorted_pos[home0 + hb + a];
-                int cnt = home_cnt[a];
-                for (int q0 = 0; q0 < len; q0 += 64) {
+            sub_start[t] = st;
+            sub_len[t] = ln;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int off = 0;
+            for (int t = 0; t < 56; t++) { sub_off[t] = off; off += sub_len[t]; }
+            sub_off[56] = off;
+            ncand_s = min(off, NB_CAND);
+            if (off > NB_CAND) atomicMax(&ctr[C_OVF], OVF_TILE);
+        }
+        __syncthreads();
+        const int ncand = ncand_s;
+        for (int q = threadIdx.x; q < ncand; q += 256) {
+            int lo_s = 0, hi_s = 55;  // last sub-range with sub_off <= q
+            while (lo_s < hi_s) {
+                const int mid = (lo_s + hi_s + 1) >> 1;
+                if (sub_off[mid] <= q) lo_s = mid; else hi_s = mid - 1;
+            }
+            cand[q] = (uint16_t) (sub_start[lo_s] + (q - sub_off[lo_s]));
+        }
+        __syncthreads();
+        const int cstart = lstart[c];
+        // the home particle's own tile slot: column (0, 0) holds cell c's locals
+        const int own0 = r_off[2 + 2 * 4] + (cstart - lstart[col_lo[4]]);
+        for (int a = wave; a < nh; a += 4) {
+            const int i = cstart + a;           // physical home slot
+            const int hi = i - h0;              // index in the segment's home range
+            const double xi = tx[own0 + a], yi = ty[own0 + a], zi = tz[own0 + a];
+            const int refi = tref[own0 + a];
+            int cnt = 0;
+            if (!(flags[i] & FLAG_FIXED)) {
+                for (int q0 = 0; q0 < ncand; q0 += 64) {
                     const int q = q0 + lane;
                     bool acc = false;
-                    int j = 0;
-                    if (q < len) {
-                        j = tj[q];
-                        const bool keep = half ? (i < j) : (j != i);
-                        const double dx = pi.x - tx[q], dy = pi.y - ty[q], dz = pi.z - tz[q];
+                    int o = 0;
+                    if (q < ncand) {
+                        o = cand[q];
+                        const int refj = tref[o];
+                        const bool keep = half ? (refi < refj) : (refj != refi);
+                        const double dx = xi - tx[o], dy = yi - ty[o], dz = zi - tz[o];
                         double rsq = dx * dx + dy * dy;
                         rsq = rsq + dz * dz;
                         acc = keep && rsq < rc2;
                     }
                     const unsigned long long m = __ballot(acc);
                     const int k = cnt + lane_rank(m);
-                    if (acc && k < ncap) nl[(size_t) k * nl_stride + i] = j;
+                    if (acc && k < ncap) {
+                        if (hi < hb) lbuf[(size_t) k * hb + hi] = (uint16_t) o;
+                        else nl[(size_t) k * nl_stride + i] = (uint16_t) o;
+                    }
                     cnt += __popcll(m);
                 }
-                if (lane == 0) home_cnt[a] = cnt;
+            }
+            if (lane == 0) {
+                int K = cnt;
+                // reference overflow rule (generated resize check): count+1 >= capacity
+                if (K + 1 >= ncap) {
+                    atomicMax(&ctr[C_REQ_NEIGH], K + 2);
+                    K = ncap - 2;
+                }
+                numneighs[i] = K;
+                if (hi < hb) home_cnt[hi] = K;
             }
         }
-        __syncthreads();
-        for (int a = threadIdx.x; a < nh; a += 256) {
-            const int i = sorted_idx[home0 + hb + a];
-            int K = home_cnt[a];
-            if (flags[i] & FLAG_FIXED) K = 0;
-            // reference overflow rule (generated resize check): count+1 >= capacity
-            if (K + 1 >= ncap) {
-                atomicMax(ovf, K + 2);
-                K = ncap - 2;
-            }
-            numneighs[i] = K;
+    }
+    __syncthreads();
+    // write the transposed rows: row k holds entry k of every home particle
+    const int nhome = min(h1 - h0, hb);
+    int kmax = 0;
+    for (int a = 0; a < nhome; a++) kmax = max(kmax, home_cnt[a]);
+    for (int k = 0; k < kmax; k++)
+        for (int a = threadIdx.x; a < nhome; a += 256)
+            if (k < home_cnt[a]) nl[(size_t) k * nl_stride + h0 + a] = lbuf[(size_t) k * hb + a];
+}
+
+// Expand the 16-bit tile-offset lists into reference indices (for download):
+// out[physical local][k] = reference index of neighbour k.
+__global__ __launch_bounds__(256) void k_expand_neighbors(const int *__restrict__ counters, Geom g,
+                                                          const int *__restrict__ lstart,
+                                                          const int *__restrict__ gstart,
+                                                          const int *__restrict__ ref,
+                                                          const uint16_t *__restrict__ nl, int nl_stride,
+                                                          const int *__restrict__ numneighs,
+                                                          int *__restrict__ out, int ncap_out) {
+    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
+    const int nlocal = counters[C_NLOCAL];
+    const int seg = blockIdx.x;
+    if (seg >= g.nsegs) return;
+    const SegInfo si = seg_info(seg, g);
+    const int h0 = lstart[si.ca], h1 = lstart[si.cb];
+    if (h1 <= h0) return;
+    seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
+    for (int i = h0 + threadIdx.x; i < h1; i += 256) {
+        const int n = numneighs[i];
+        for (int k = 0; k < n && k < ncap_out; k++) {
+            const int o = nl[(size_t) k * nl_stride + i];
+            const int r = run_of(r_off, o);
+            out[(size_t) i * ncap_out + k] = ref[r_start[r] + (o - r_off[r])];
         }
     }
 }
@@ -253,122 +503,198 @@ struct LJTab {
     double eps[16], sig6[16];  // ntypes <= 4
 };
 
-__device__ __forceinline__ void lj_pair(const double4 &pi, const double4 &pj, double rc2, const LJTab &tab, int ti,
-                                        double &t0, double &t1, double &t2, double &u, double &w, bool energy,
-                                        double &g0, double &g1, double &g2, bool &inside) {
-    // examples/md.py:5-8 as emitted: delta, rsq = dx*dx + dy*dy + dz*dz,
-    // sr2 = 1/rsq, sr6 = sr2*sr2*sr2*sigma6, s = 48*sr6*(sr6-0.5)*sr2*eps
-    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+// One pair of examples/md.py:5-8 as the reference emits it: delta,
+// rsq = dx*dx + dy*dy + dz*dz, sr2 = 1/rsq, sr6 = sr2*sr2*sr2*sigma6,
+// s = 48*sr6*(sr6-0.5)*sr2*eps, force contribution delta*s.
+template <bool ENERGY>
+__device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double xj, double yj, double zj, double rc2,
+                                        double sg6, double ep, double &g0, double &g1, double &g2, double &u,
+                                        double &w) {
+    const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
     double rsq = dx * dx + dy * dy;
     rsq = rsq + dz * dz;
-    inside = rsq < rc2;
-    if (inside) {
-        double sg6, ep;
-        if (tab.nt == 1) { sg6 = tab.sig6[0]; ep = tab.eps[0]; }
-        else { const int k = ti * tab.nt + pos_type(pj); sg6 = tab.sig6[k]; ep = tab.eps[k]; }
-        const double sr2 = 1.0 / rsq;
-        double sr6 = sr2 * sr2;
-        sr6 = sr6 * sr2;
-        sr6 = sr6 * sg6;
-        double s = 48.0 * sr6;
-        s = s * (sr6 - 0.5);
-        s = s * sr2;
-        s = s * ep;
-        g0 = dx * s; g1 = dy * s; g2 = dz * s;
-        t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
-        if (energy) {
-            u += 4.0 * ep * sr6 * (sr6 - 1.0);
-            w += 48.0 * sr6 * (sr6 - 0.5) * ep;
+    if (!(rsq < rc2)) return false;
+    const double sr2 = 1.0 / rsq;
+    double sr6 = sr2 * sr2;
+    sr6 = sr6 * sr2;
+    sr6 = sr6 * sg6;
+    double s = 48.0 * sr6;
+    s = s * (sr6 - 0.5);
+    s = s * sr2;
+    s = s * ep;
+    g0 = dx * s; g1 = dy * s; g2 = dz * s;
+    if (ENERGY) {
+        u = 4.0 * ep * sr6 * (sr6 - 1.0);
+        w = 48.0 * sr6 * (sr6 - 0.5) * ep;
+    }
+    return true;
+}
+
+constexpr int FB = 128;  // threads per force block (two waves)
+
+// Stage a segment's tile in LDS: positions (+ type, + physical slot).
+template <bool WITH_TYPE, bool WITH_PHYS>
+__device__ __forceinline__ void stage_tile(const int *r_start, const int *r_off, int T, const double *__restrict__ X,
+                                           const double *__restrict__ Y, const double *__restrict__ Z,
+                                           const int *__restrict__ type, double *tx, double *ty, double *tz,
+                                           int *tt, int *tp) {
+    // SPT entries per thread in flight at once: all loads of a batch issue
+    // before the first LDS store waits on them
+    constexpr int SPT = 8;
+    for (int p0 = 0; p0 < T; p0 += SPT * (int) blockDim.x) {
+        double xv[SPT], yv[SPT], zv[SPT];
+        int tv[SPT], pv[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; k++) {
+            const int p = p0 + k * blockDim.x + threadIdx.x;
+            pv[k] = 0;
+            if (p < T) {
+                const int r = run_of(r_off, p);
+                const int ph = r_start[r] + (p - r_off[r]);
+                xv[k] = X[ph]; yv[k] = Y[ph]; zv[k] = Z[ph];
+                if (WITH_TYPE) tv[k] = type[ph];
+                pv[k] = ph;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < SPT; k++) {
+            const int p = p0 + k * blockDim.x + threadIdx.x;
+            if (p < T) {
+                tx[p] = xv[k]; ty[p] = yv[k]; tz[p] = zv[k];
+                if (WITH_TYPE) tt[p] = tv[k];
+                if (WITH_PHYS) tp[p] = pv[k];
+            }
         }
     }
 }
 
-// Full-list LJ force, optionally fused with the integrator around it:
-//   F(ts); [final_integrate(ts)]; [thermo partial]; [initial_integrate(ts+1)]
-// Positions are double-buffered: the kernel reads P (locals + ghosts of ts)
-// and writes the locals of ts+1 into Pn, so no thread ever overwrites a
-// position another thread may still gather.
-// One lane per particle; neighbour indices stream from the SoA list
-// (nl[k*stride + i]: 64 consecutive lanes read 256 contiguous bytes) and the
-// loop is unrolled U-deep so that U index loads and U position gathers are in
-// flight per lane while the partial sums still add up in list order.
-template <int U, bool FUSE, bool RED>
-__global__ __launch_bounds__(256) void k_lj_full(const double4 *__restrict__ P, double4 *__restrict__ Pn,
-                                                 const int *__restrict__ nl, int nl_stride,
-                                                 const int *__restrict__ numneighs, const int *__restrict__ flags,
-                                                 double *__restrict__ fx, double *__restrict__ fy,
-                                                 double *__restrict__ fz, double *__restrict__ vx,
-                                                 double *__restrict__ vy, double *__restrict__ vz,
-                                                 const double *__restrict__ mass, int nlocal, double rc2, LJTab tab,
-                                                 double dt, double hdt, int do_vv2, int do_vv1,
-                                                 double *__restrict__ partials) {
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int i = b * 256 + threadIdx.x;
+// LJ force over the 16-bit tile-offset lists of one segment per block.
+// MODE 0: forces only (lennard_jones with reset_volatile_properties);
+// MODE 1: fused F(ts) + final_integrate(ts) + [thermo] + initial_integrate(ts+1)
+//         writing the next positions into the other buffer;
+// MODE 2: energy and virial only (no writes but the partials).
+// One lane per home particle; the list is walked in groups of U entries to a
+// wave-uniform depth (entries past a lane's own count are masked), the next
+// group's offsets loading while the current one computes; each lane adds its
+// pair forces in list order, so the sum is bit-identical to the reference's.
+template <int U, int NT, int MODE, bool RED>
+__global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters, Geom g,
+                                                const int *__restrict__ lstart, const int *__restrict__ gstart,
+                                                const double *__restrict__ X, const double *__restrict__ Y,
+                                                const double *__restrict__ Z, double *__restrict__ Xn,
+                                                double *__restrict__ Yn, double *__restrict__ Zn,
+                                                const int *__restrict__ type, const uint16_t *__restrict__ nl,
+                                                int nl_stride, const int *__restrict__ numneighs,
+                                                const int *__restrict__ flags, double *__restrict__ fx,
+                                                double *__restrict__ fy, double *__restrict__ fz,
+                                                double *__restrict__ vx, double *__restrict__ vy,
+                                                double *__restrict__ vz, const double *__restrict__ mass,
+                                                double rc2, LJTab tab, double dt, double hdt, int do_vv2, int do_vv1,
+                                                double *__restrict__ partials, int pstride,
+                                                double *__restrict__ eatom, int tcap) {
+    extern __shared__ char smem[];
+    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
+    __shared__ double s_eps[16], s_sig6[16];
+    const int nlocal = counters[C_NLOCAL];
+    const int seg = xcd_block(blockIdx.x, gridDim.x);
+    const SegInfo si = seg_info(seg, g);
+    const int h0 = lstart[si.ca], h1 = lstart[si.cb];
     double ke = 0.0, pe = 0.0, vir = 0.0;
-    if (i < nlocal) {
-        const double4 pi = P[i];
-        const int fl = flags[i];
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-        double u = 0.0, w = 0.0, g0, g1, g2;
-        bool ins;
-        if (!(fl & FLAG_FIXED)) {
-            const int ti = pos_type(pi);
-            const int n = numneighs[i];
-            const int *lp = nl + i;
-            int k = 0;
-            for (; k + U <= n; k += U) {
-                int jj[U];
-                double4 pj[U];
+    if (h1 > h0) {
+        seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
+        const int T = r_off[NRUNS];
+        const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
+        double *tx = (double *) smem;
+        double *ty = tx + T;
+        double *tz = ty + T;
+        int *tt = (int *) (tz + T);
+        if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
+        if (fits) stage_tile<NT != 1, false>(r_start, r_off, T, X, Y, Z, type, tx, ty, tz, tt, nullptr);
+        __syncthreads();
+        const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
+        const int nt = tab.nt;
+        for (int base = h0; fits && base < h1; base += FB) {
+            const int i = base + threadIdx.x;
+            const bool live = i < h1;
+            const int ii = live ? i : h0;
+            const int fl = live ? flags[ii] : FLAG_FIXED;
+            const int n = (live && !(fl & FLAG_FIXED)) ? numneighs[ii] : 0;
+            const int nmax = wave_max(n);
+            const double xi = X[ii], yi = Y[ii], zi = Z[ii];
+            const int ti = NT != 1 ? type[ii] : 0;
+            const uint16_t *lp = nl + ii;
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
+            int on[U];
 #pragma unroll
-                for (int q = 0; q < U; q++) jj[q] = __builtin_nontemporal_load(lp + (size_t) (k + q) * nl_stride);
+            for (int q = 0; q < U; q++) on[q] = q < n ? (int) __builtin_nontemporal_load(lp + (size_t) q * nl_stride) : 0;
+            for (int k = 0; k < nmax; k += U) {
+                double xj[U], yj[U], zj[U];
+                int tj[U];
 #pragma unroll
-                for (int q = 0; q < U; q++) pj[q] = P[jj[q]];
-#pragma unroll
-                for (int q = 0; q < U; q++) lj_pair(pi, pj[q], rc2, tab, ti, t0, t1, t2, u, w, RED, g0, g1, g2, ins);
-            }
-            for (; k < n; k++) {
-                const int j = lp[(size_t) k * nl_stride];
-                lj_pair(pi, P[j], rc2, tab, ti, t0, t1, t2, u, w, RED, g0, g1, g2, ins);
-            }
-        }
-        // reset_volatile_properties then force[i] = force[i] + reduction
-        const double f0 = 0.0 + t0, f1 = 0.0 + t1, f2 = 0.0 + t2;
-        fx[i] = f0; fy[i] = f1; fz[i] = f2;
-        if (RED) { pe = 0.5 * u; vir = 0.5 * w; }
-        if (FUSE) {
-            double4 pn = pi;
-            if (!(fl & FLAG_FIXED)) {
-                const double m = mass[i];
-                double v0 = vx[i], v1 = vy[i], v2 = vz[i];
-                if (do_vv2) {  // final_integrate(ts)
-                    double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
-                    a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
-                    v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
+                for (int q = 0; q < U; q++) {
+                    xj[q] = tx[on[q]]; yj[q] = ty[on[q]]; zj[q] = tz[on[q]];
+                    if (NT != 1) tj[q] = tt[on[q]];
                 }
-                if (RED) ke = m * (v0 * v0 + v1 * v1 + v2 * v2);
-                if (do_vv1) {  // initial_integrate(ts+1)
-                    double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
-                    a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
-                    v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
-                    pn.x = pi.x + dt * v0; pn.y = pi.y + dt * v1; pn.z = pi.z + dt * v2;
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    const int kk = k + U + q;
+                    on[q] = kk < n ? (int) __builtin_nontemporal_load(lp + (size_t) kk * nl_stride) : 0;
                 }
-                vx[i] = v0; vy[i] = v1; vz[i] = v2;
-            } else if (RED) {
-                const double v0 = vx[i], v1 = vy[i], v2 = vz[i];
-                ke = mass[i] * (v0 * v0 + v1 * v1 + v2 * v2);
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    double sg6 = sig0, ep = eps0;
+                    if (NT != 1) { const int kt = ti * nt + tj[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
+                    double g0, g1, g2, uu, ww;
+                    if (k + q < n && lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], rc2, sg6, ep, g0, g1, g2, uu, ww)) {
+                        t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
+                        if (RED) { u += uu; w += ww; }
+                    }
+                }
             }
-            if (do_vv1) Pn[i] = pn;
+            if (live) {
+                if (RED) { pe += 0.5 * u; vir += 0.5 * w; }
+                if (MODE == 2) {
+                    if (eatom) eatom[i] = 0.5 * u;
+                    continue;
+                }
+                // reset_volatile_properties then force[i] = force[i] + reduction
+                const double f0 = 0.0 + t0, f1 = 0.0 + t1, f2 = 0.0 + t2;
+                fx[i] = f0; fy[i] = f1; fz[i] = f2;
+                if (MODE == 1) {
+                    const double m = mass[i];
+                    double v0 = vx[i], v1 = vy[i], v2 = vz[i];
+                    double px = xi, py = yi, pz = zi;
+                    if (!(fl & FLAG_FIXED) && do_vv2) {  // final_integrate(ts)
+                        double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
+                        a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
+                        v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
+                    }
+                    if (RED) ke += m * (v0 * v0 + v1 * v1 + v2 * v2);
+                    if (!(fl & FLAG_FIXED)) {
+                        if (do_vv1) {  // initial_integrate(ts+1)
+                            double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
+                            a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
+                            v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
+                            px = xi + dt * v0; py = yi + dt * v1; pz = zi + dt * v2;
+                        }
+                        vx[i] = v0; vy[i] = v1; vz[i] = v2;
+                    }
+                    if (do_vv1) { Xn[i] = px; Yn[i] = py; Zn[i] = pz; }
+                }
+            }
         }
     }
     if (RED) {
-        __shared__ double red[3][4];
+        __shared__ double red[3][FB / 64];
         const double a = wave_sum(ke), bb = wave_sum(pe), cc = wave_sum(vir);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         if (lane == 0) { red[0][wave] = a; red[1][wave] = bb; red[2][wave] = cc; }
         __syncthreads();
         if (threadIdx.x < 3) {
             const int r = threadIdx.x;
-            partials[(size_t) b * 3 + r] = ((red[r][0] + red[r][1]) + red[r][2]) + red[r][3];
+            double s = 0.0;
+            for (int k = 0; k < FB / 64; k++) s += red[r][k];
+            if (MODE == 1 || r > 0) partials[(size_t) r * pstride + seg] = s;
         }
     }
 }
@@ -376,120 +702,103 @@ __global__ __launch_bounds__(256) void k_lj_full(const double4 *__restrict__ P, 
 // Half-list LJ (compute_half): i accumulates its own sum; the mirror force on a
 // local, non-fixed j is an FP64 atomic subtract (src/pairs/ir/apply.py:111-125).
 // Forces must be zeroed before the launch (reset_volatile_properties).
-template <int U, bool RED>
-__global__ __launch_bounds__(256) void k_lj_half(const double4 *__restrict__ P, const int *__restrict__ nl,
-                                                 int nl_stride, const int *__restrict__ numneighs,
-                                                 const int *__restrict__ flags, double *__restrict__ fx,
-                                                 double *__restrict__ fy, double *__restrict__ fz, int nlocal,
-                                                 double rc2, LJTab tab, double *__restrict__ partials) {
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int i = b * 256 + threadIdx.x;
+template <int U, int NT, bool RED>
+__global__ __launch_bounds__(FB) void k_lj_half(const int *__restrict__ counters, Geom g,
+                                                const int *__restrict__ lstart, const int *__restrict__ gstart,
+                                                const double *__restrict__ X, const double *__restrict__ Y,
+                                                const double *__restrict__ Z, const int *__restrict__ type,
+                                                const uint16_t *__restrict__ nl, int nl_stride,
+                                                const int *__restrict__ numneighs, const int *__restrict__ flags,
+                                                double *__restrict__ fx, double *__restrict__ fy,
+                                                double *__restrict__ fz, double rc2, LJTab tab,
+                                                double *__restrict__ partials, int pstride, int tcap) {
+    extern __shared__ char smem[];
+    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
+    __shared__ double s_eps[16], s_sig6[16];
+    const int nlocal = counters[C_NLOCAL];
+    const int seg = xcd_block(blockIdx.x, gridDim.x);
+    const SegInfo si = seg_info(seg, g);
+    const int h0 = lstart[si.ca], h1 = lstart[si.cb];
     double pe = 0.0, vir = 0.0;
-    if (i < nlocal && !(flags[i] & FLAG_FIXED)) {
-        const double4 pi = P[i];
-        const int ti = pos_type(pi);
-        const int n = numneighs[i];
-        const int *lp = nl + i;
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0, g0, g1, g2;
-        bool ins;
-        int k = 0;
-        for (; k + U <= n; k += U) {
-            int jj[U];
-            double4 pj[U];
+    if (h1 > h0) {
+        seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
+        const int T = r_off[NRUNS];
+        const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
+        double *tx = (double *) smem;
+        double *ty = tx + T;
+        double *tz = ty + T;
+        int *tp = (int *) (tz + T);
+        int *tt = tp + T;
+        if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
+        if (fits) stage_tile<NT != 1, true>(r_start, r_off, T, X, Y, Z, type, tx, ty, tz, tt, tp);
+        __syncthreads();
+        const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
+        const int nt = tab.nt;
+        for (int base = h0; fits && base < h1; base += FB) {
+            const int i = base + threadIdx.x;
+            const bool live = i < h1;
+            const int ii = live ? i : h0;
+            const int n = (live && !(flags[ii] & FLAG_FIXED)) ? numneighs[ii] : 0;
+            const int nmax = wave_max(n);
+            const double xi = X[ii], yi = Y[ii], zi = Z[ii];
+            const int ti = NT != 1 ? type[ii] : 0;
+            const uint16_t *lp = nl + ii;
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
+            int on[U];
 #pragma unroll
-            for (int q = 0; q < U; q++) jj[q] = __builtin_nontemporal_load(lp + (size_t) (k + q) * nl_stride);
+            for (int q = 0; q < U; q++) on[q] = q < n ? (int) __builtin_nontemporal_load(lp + (size_t) q * nl_stride) : 0;
+            for (int k = 0; k < nmax; k += U) {
+                int oc[U];
 #pragma unroll
-            for (int q = 0; q < U; q++) pj[q] = P[jj[q]];
+                for (int q = 0; q < U; q++) oc[q] = on[q];
 #pragma unroll
-            for (int q = 0; q < U; q++) {
-                double uu = 0.0, ww = 0.0;
-                lj_pair(pi, pj[q], rc2, tab, ti, t0, t1, t2, uu, ww, RED, g0, g1, g2, ins);
-                if (ins) {
-                    const int j = jj[q];
-                    const bool loc = j < nlocal;
-                    if (RED) { const double wt = loc ? 1.0 : 0.5; u += wt * uu; w += wt * ww; }
-                    if (loc && !(flags[j] & FLAG_FIXED)) {
-                        atomic_add_f64(fx + j, -g0); atomic_add_f64(fy + j, -g1); atomic_add_f64(fz + j, -g2);
+                for (int q = 0; q < U; q++) {
+                    const int kk = k + U + q;
+                    on[q] = kk < n ? (int) __builtin_nontemporal_load(lp + (size_t) kk * nl_stride) : 0;
+                }
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    const int o = oc[q];
+                    double sg6 = sig0, ep = eps0;
+                    if (NT != 1) { const int kt = ti * nt + tt[o]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
+                    double g0, g1, g2, uu, ww;
+                    if (k + q < n && lj_pair<RED>(xi, yi, zi, tx[o], ty[o], tz[o], rc2, sg6, ep, g0, g1, g2, uu, ww)) {
+                        t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
+                        const int j = tp[o];
+                        const bool loc = j < nlocal;
+                        if (RED) { const double wt = loc ? 1.0 : 0.5; u += wt * uu; w += wt * ww; }
+                        if (loc && !(flags[j] & FLAG_FIXED)) {
+                            atomic_add_f64(fx + j, -g0); atomic_add_f64(fy + j, -g1); atomic_add_f64(fz + j, -g2);
+                        }
                     }
                 }
             }
-        }
-        for (; k < n; k++) {
-            const int j = lp[(size_t) k * nl_stride];
-            double uu = 0.0, ww = 0.0;
-            lj_pair(pi, P[j], rc2, tab, ti, t0, t1, t2, uu, ww, RED, g0, g1, g2, ins);
-            if (ins) {
-                const bool loc = j < nlocal;
-                if (RED) { const double wt = loc ? 1.0 : 0.5; u += wt * uu; w += wt * ww; }
-                if (loc && !(flags[j] & FLAG_FIXED)) {
-                    atomic_add_f64(fx + j, -g0); atomic_add_f64(fy + j, -g1); atomic_add_f64(fz + j, -g2);
-                }
+            if (n > 0) {
+                atomic_add_f64(fx + i, t0); atomic_add_f64(fy + i, t1); atomic_add_f64(fz + i, t2);
+                pe += u; vir += w;
             }
         }
-        atomic_add_f64(fx + i, t0); atomic_add_f64(fy + i, t1); atomic_add_f64(fz + i, t2);
-        pe = u; vir = w;
     }
     if (RED) {
-        __shared__ double red[2][4];
+        __shared__ double red[2][FB / 64];
         const double a = wave_sum(pe), bb = wave_sum(vir);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         if (lane == 0) { red[0][wave] = a; red[1][wave] = bb; }
         __syncthreads();
         if (threadIdx.x < 2) {
             const int r = threadIdx.x;
-            partials[(size_t) b * 3 + 1 + r] = ((red[r][0] + red[r][1]) + red[r][2]) + red[r][3];
+            double s = 0.0;
+            for (int k = 0; k < FB / 64; k++) s += red[r][k];
+            partials[(size_t) (1 + r) * pstride + seg] = s;
         }
-    }
-}
-
-// Potential energy and virial over the current list (no reference counterpart):
-// U = sum 4 eps sr6 (sr6 - 1), W = sum r.F over pairs r < rc, a pair seen from
-// both sides weighted 1/2; e_i = 1/2 sum_j u_ij (per-atom, full lists).
-__global__ __launch_bounds__(256) void k_energy(const double4 *__restrict__ P, const int *__restrict__ nl,
-                                                int nl_stride, const int *__restrict__ numneighs,
-                                                const int *__restrict__ flags, int nlocal, int half, double rc2,
-                                                LJTab tab, double *__restrict__ partials, double *__restrict__ eatom) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    double pe = 0.0, vir = 0.0;
-    if (i < nlocal) {
-        double ei = 0.0;
-        if (!(flags[i] & FLAG_FIXED)) {
-            const double4 pi = P[i];
-            const int ti = pos_type(pi);
-            const int n = numneighs[i];
-            for (int k = 0; k < n; k++) {
-                const int j = nl[(size_t) k * nl_stride + i];
-                const double4 pj = P[j];
-                const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-                const double rsq = dx * dx + dy * dy + dz * dz;
-                if (rsq >= rc2) continue;
-                const int t = tab.nt == 1 ? 0 : ti * tab.nt + pos_type(pj);
-                const double sr2 = 1.0 / rsq, sr6 = sr2 * sr2 * sr2 * tab.sig6[t];
-                const double u = 4.0 * tab.eps[t] * sr6 * (sr6 - 1.0);
-                const double w = 48.0 * sr6 * (sr6 - 0.5) * tab.eps[t];
-                const double wt = (!half || j >= nlocal) ? 0.5 : 1.0;
-                pe += wt * u;
-                vir += wt * w;
-                ei += 0.5 * u;
-            }
-        }
-        if (eatom) eatom[i] = ei;
-    }
-    __shared__ double red[2][4];
-    const double a = wave_sum(pe), bb = wave_sum(vir);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) { red[0][wave] = a; red[1][wave] = bb; }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        const int r = threadIdx.x;
-        partials[(size_t) blockIdx.x * 3 + 1 + r] = ((red[r][0] + red[r][1]) + red[r][2]) + red[r][3];
     }
 }
 
 // ---------------------------------------------------------------- integration
 
 // initial_integrate (examples/md.py:11-13): v += (dt*0.5)*f/m; x += dt*v
-__global__ __launch_bounds__(256) void k_vv_initial(double4 *__restrict__ P, double *__restrict__ vx,
+__global__ __launch_bounds__(256) void k_vv_initial(double *__restrict__ X, double *__restrict__ Y,
+                                                    double *__restrict__ Z, double *__restrict__ vx,
                                                     double *__restrict__ vy, double *__restrict__ vz,
                                                     const double *__restrict__ fx, const double *__restrict__ fy,
                                                     const double *__restrict__ fz, const double *__restrict__ mass,
@@ -501,29 +810,28 @@ __global__ __launch_bounds__(256) void k_vv_initial(double4 *__restrict__ P, dou
     a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
     const double v0 = vx[i] + a0, v1 = vy[i] + a1, v2 = vz[i] + a2;
     vx[i] = v0; vy[i] = v1; vz[i] = v2;
-    double4 p = P[i];
-    p.x = p.x + dt * v0; p.y = p.y + dt * v1; p.z = p.z + dt * v2;
-    P[i] = p;
+    X[i] = X[i] + dt * v0; Y[i] = Y[i] + dt * v1; Z[i] = Z[i] + dt * v2;
 }
 
 // final_integrate (examples/md.py:16-17), optionally followed by the thermo
-// partial and the next step's initial_integrate written into Pn.
+// partial and the next step's initial_integrate written into (Xn, Yn, Zn).
 template <bool RED>
-__global__ __launch_bounds__(256) void k_vv_post(const double4 *__restrict__ P, double4 *__restrict__ Pn,
+__global__ __launch_bounds__(256) void k_vv_post(const double *__restrict__ X, const double *__restrict__ Y,
+                                                 const double *__restrict__ Z, double *__restrict__ Xn,
+                                                 double *__restrict__ Yn, double *__restrict__ Zn,
                                                  double *__restrict__ vx, double *__restrict__ vy,
                                                  double *__restrict__ vz, const double *__restrict__ fx,
                                                  const double *__restrict__ fy, const double *__restrict__ fz,
                                                  const double *__restrict__ mass, const int *__restrict__ flags,
                                                  int nlocal, double dt, double hdt, int do_vv2, int do_vv1,
                                                  double *__restrict__ partials) {
-    const int b = blockIdx.x;
-    const int i = b * 256 + threadIdx.x;
+    const int i = blockIdx.x * 256 + threadIdx.x;
     double ke = 0.0;
     if (i < nlocal) {
         const int fl = flags[i];
         const double m = mass[i];
         double v0 = vx[i], v1 = vy[i], v2 = vz[i];
-        double4 p = P[i];
+        double px = X[i], py = Y[i], pz = Z[i];
         if (!(fl & FLAG_FIXED)) {
             const double f0 = fx[i], f1 = fy[i], f2 = fz[i];
             if (do_vv2) {
@@ -536,20 +844,20 @@ __global__ __launch_bounds__(256) void k_vv_post(const double4 *__restrict__ P, 
                 double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
                 a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
                 v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
-                p.x = p.x + dt * v0; p.y = p.y + dt * v1; p.z = p.z + dt * v2;
+                px = px + dt * v0; py = py + dt * v1; pz = pz + dt * v2;
             }
             vx[i] = v0; vy[i] = v1; vz[i] = v2;
         } else if (RED) {
             ke = m * (v0 * v0 + v1 * v1 + v2 * v2);
         }
-        if (do_vv1) Pn[i] = p;
+        if (do_vv1) { Xn[i] = px; Yn[i] = py; Zn[i] = pz; }
     }
     if (RED) {
         __shared__ double red[4];
         const double a = wave_sum(ke);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
         __syncthreads();
-        if (threadIdx.x == 0) partials[(size_t) b * 3] = ((red[0] + red[1]) + red[2]) + red[3];
+        if (threadIdx.x == 0) partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
     }
 }
 
@@ -567,17 +875,18 @@ __global__ __launch_bounds__(256) void k_ke_partial(const double *__restrict__ v
     const double a = wave_sum(ke);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
-    if (threadIdx.x == 0) partials[(size_t) blockIdx.x * 3] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// fixed-order sum of the per-block partials; thermo as runtime/thermo.hpp:11-51
-__global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict__ partials, int nblocks,
-                                                       int which_mask, int natoms, double lx, double ly, double lz,
-                                                       double *__restrict__ out) {
+// Fixed-order sums of the per-block partials (row 0 kinetic over nb_ke blocks,
+// rows 1-2 energy/virial over nb_f blocks); thermo as runtime/thermo.hpp:11-51.
+__global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict__ partials, int pstride, int nb_ke,
+                                                       int nb_f, int which_mask, int natoms, double lx, double ly,
+                                                       double lz, double *__restrict__ out) {
     __shared__ double red[3][4];
     double s[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblocks; b += 256)
-        for (int r = 0; r < 3; r++) s[r] += partials[(size_t) b * 3 + r];
+    for (int b = threadIdx.x; b < nb_ke; b += 256) s[0] += partials[b];
+    for (int b = threadIdx.x; b < nb_f; b += 256) { s[1] += partials[pstride + b]; s[2] += partials[2 * pstride + b]; }
     for (int r = 0; r < 3; r++) {
         const double a = wave_sum(s[r]);
         if ((threadIdx.x & 63) == 0) red[r][threadIdx.x >> 6] = a;
@@ -593,7 +902,8 @@ __global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict_
         const double p_scale = 1.0 / 3 / lx / ly / lz;
         const double p = (t * dof) * p_scale;
         if (which_mask & 1) { out[0] = t; out[1] = p; }
-        if (which_mask & 2) { out[2] = pe; out[3] = w; out[4] = p + w / (3.0 * lx * ly * lz); }
+        if (which_mask & 2) { out[2] = pe; out[3] = w; }
+        if (which_mask == 3) out[4] = p + w / (3.0 * lx * ly * lz);
     }
 }
 
@@ -603,18 +913,17 @@ __global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict_
 // (src/pairs/sim/domain_partitioning.py:31-65): side 0 "x_d < min_d + offset"
 // (shift +L), side 1 "x_d > max_d - offset" (shift -L), skipping
 // Infinite|Global particles.  offset 0 selects leavers (exchange).
-__device__ __forceinline__ void ghost_sel(const double4 &p, int fl, int dim, const Geom &g, double offset, bool exch,
-                                          bool &s0, bool &s1) {
-    const double x = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
+__device__ __forceinline__ void ghost_sel(double x, int fl, int dim, const Geom &g, bool exch, bool &s0, bool &s1) {
     const bool ok = !(fl & (FLAG_INFINITE | FLAG_GLOBAL));
     if (exch) { s0 = ok && x < g.lo[dim]; s1 = ok && x > g.hi[dim]; }
-    else { s0 = ok && x < g.lo[dim] + offset; s1 = ok && x > g.hi[dim] - offset; }
+    else { s0 = ok && x < g.lo[dim] + g.spacing; s1 = ok && x > g.hi[dim] - g.spacing; }
 }
 
-// pass 1 of the stable two-sided compaction: per-tile counts
-__global__ __launch_bounds__(256) void k_sel_count(const double4 *__restrict__ P, const int *__restrict__ flags,
-                                                   int *__restrict__ counters, Geom g, int dim, int exch,
-                                                   int *__restrict__ tile_cnt, int *__restrict__ exchg_flag) {
+// pass 1 of the stable two-sided compaction over reference indices: per-tile counts
+__global__ __launch_bounds__(256) void k_sel_count(const double *__restrict__ Xd, const int *__restrict__ flags,
+                                                   const int *__restrict__ phys, int *__restrict__ counters, Geom g,
+                                                   int dim, int exch, int *__restrict__ tile_cnt,
+                                                   int *__restrict__ exchg_flag) {
     __shared__ int wc[2][4];
     if (counters[C_OVF]) return;
     const int n = exch ? counters[C_NLOCAL] : counters[C_NLOCAL] + counters[C_NGHOST];
@@ -622,10 +931,13 @@ __global__ __launch_bounds__(256) void k_sel_count(const double4 *__restrict__ P
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_RANGE] = n;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int i = t * 256 + threadIdx.x;
+        const int r = t * 256 + threadIdx.x;
         bool s0 = false, s1 = false;
-        if (i < n) ghost_sel(P[i], flags[i], dim, g, g.spacing, exch, s0, s1);
-        if (exch && i < n) exchg_flag[i] = (s0 || s1) ? 1 : 0;
+        if (r < n) {
+            const int p = phys[r];
+            ghost_sel(Xd[p], flags[p], dim, g, exch, s0, s1);
+            if (exch) exchg_flag[r] = (s0 || s1) ? 1 : 0;
+        }
         const unsigned long long m0 = __ballot(s0), m1 = __ballot(s1);
         if (lane == 0) { wc[0][wave] = __popcll(m0); wc[1][wave] = __popcll(m1); }
         __syncthreads();
@@ -682,28 +994,33 @@ __global__ __launch_bounds__(1024) void k_sel_scan(int *__restrict__ counters, i
     }
 }
 
-// pass 3: stable scatter into the send map; for borders the ghost is created
-// in the same pass (pack + communicateData self-copy + unpack of comm.py:291-366:
-// uid, type, mass, position + shift, velocity, shape; flags are not sent).
-__global__ __launch_bounds__(256) void k_sel_scatter(double4 *__restrict__ P, double *__restrict__ vx,
+// pass 3: stable scatter into the send map (reference indices); for borders
+// the ghost is created in the same pass (pack + communicateData self-copy +
+// unpack of comm.py:291-366: uid, type, mass, position + shift, velocity,
+// shape) at physical slot = reference index nlocal + g.  Ghost flags are not
+// sent by the reference; a ghost slot's flags read 0 (zeroed memory).
+__global__ __launch_bounds__(256) void k_sel_scatter(double *__restrict__ X, double *__restrict__ Y,
+                                                     double *__restrict__ Z, double *__restrict__ vx,
                                                      double *__restrict__ vy, double *__restrict__ vz,
-                                                     double *__restrict__ mass, int *__restrict__ uid,
-                                                     int *__restrict__ shape, const int *__restrict__ flags,
-                                                     const int *__restrict__ counters, Geom g, int dim, int exch,
-                                                     const int *__restrict__ tile_off, int *__restrict__ send_map,
-                                                     int *__restrict__ send_mult) {
+                                                     double *__restrict__ mass, int *__restrict__ type,
+                                                     int *__restrict__ uid, int *__restrict__ shape,
+                                                     int *__restrict__ flags, int *__restrict__ ref,
+                                                     int *__restrict__ phys, const int *__restrict__ counters, Geom g,
+                                                     int dim, int exch, const int *__restrict__ tile_off,
+                                                     int *__restrict__ send_map, int *__restrict__ send_mult) {
     __shared__ int wc[2][5];
     if (counters[C_OVF]) return;
     const int n = counters[C_RANGE];
     const int ntiles = (n + 255) / 256;
     const int nlocal = counters[C_NLOCAL];
-    const int base1 = counters[C_BASE1] - counters[C_BASE0];  // side-1 block offsets are relative
+    const int base1 = counters[C_BASE1] - counters[C_BASE0];
+    const double *Xd = dim == 0 ? X : (dim == 1 ? Y : Z);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int i = t * 256 + threadIdx.x;
+        const int r = t * 256 + threadIdx.x;
         bool s0 = false, s1 = false;
-        double4 p = make_double4(0, 0, 0, 0);
-        if (i < n) { p = P[i]; ghost_sel(p, flags[i], dim, g, g.spacing, exch, s0, s1); }
+        int p = 0;
+        if (r < n) { p = phys[r]; ghost_sel(Xd[p], flags[p], dim, g, exch, s0, s1); }
         const unsigned long long m0 = __ballot(s0), m1 = __ballot(s1);
         if (lane == 0) { wc[0][wave] = __popcll(m0); wc[1][wave] = __popcll(m1); }
         __syncthreads();
@@ -714,65 +1031,53 @@ __global__ __launch_bounds__(256) void k_sel_scatter(double4 *__restrict__ P, do
         __syncthreads();
         if (s0 || s1) {
             const int side = s0 ? 0 : 1;
-            const int r = side == 0 ? lane_rank(m0) : lane_rank(m1);
+            const int rk = side == 0 ? lane_rank(m0) : lane_rank(m1);
             const int pos = counters[C_BASE0] + (side == 0 ? tile_off[2 * t] : base1 + tile_off[2 * t + 1]) +
-                            wc[side][wave] + r;
+                            wc[side][wave] + rk;
             const int mult = side == 0 ? 1 : -1;  // pbc_prev = +1, pbc_next = -1 on one rank
-            send_map[pos] = i;
-            send_mult[3 * pos + 0] = dim == 0 ? mult : 0;
-            send_mult[3 * pos + 1] = dim == 1 ? mult : 0;
-            send_mult[3 * pos + 2] = dim == 2 ? mult : 0;
+            const int mx = dim == 0 ? mult : 0, my = dim == 1 ? mult : 0, mz = dim == 2 ? mult : 0;
+            send_map[pos] = r;
+            send_mult[3 * pos + 0] = mx;
+            send_mult[3 * pos + 1] = my;
+            send_mult[3 * pos + 2] = mz;
             if (!exch) {
-                const int gslot = nlocal + pos;
-                double4 q = p;
-                q.x = p.x + (double) send_mult[3 * pos + 0] * g.len[0];
-                q.y = p.y + (double) send_mult[3 * pos + 1] * g.len[1];
-                q.z = p.z + (double) send_mult[3 * pos + 2] * g.len[2];
-                P[gslot] = q;
-                vx[gslot] = vx[i]; vy[gslot] = vy[i]; vz[gslot] = vz[i];
-                mass[gslot] = mass[i];
-                uid[gslot] = uid[i];
-                shape[gslot] = shape[i];
+                const int gp = nlocal + pos;
+                X[gp] = X[p] + (double) mx * g.len[0];
+                Y[gp] = Y[p] + (double) my * g.len[1];
+                Z[gp] = Z[p] + (double) mz * g.len[2];
+                vx[gp] = vx[p]; vy[gp] = vy[p]; vz[gp] = vz[p];
+                mass[gp] = mass[p]; type[gp] = type[p]; uid[gp] = uid[p]; shape[gp] = shape[p];
+                flags[gp] = 0;
+                ref[gp] = gp;
+                phys[gp] = gp;
             }
         }
         __syncthreads();
     }
 }
 
-// Exchange of one dimension after its send map is built: pack (with shift),
-// remove_exchanged_particles_pt1/pt2 (tail fill, copy guard "src > 0"), unpack
-// appended at nlocal - n (src/pairs/sim/comm.py:100-151, 445-520).  The
-// sequential tail walk of pt1 is the r-th non-exchanged index below nlocal
-// (descending) for the r-th exchanged particle that lies below nlocal - n.
-// One workgroup: a rebuild moves few particles.
-__global__ __launch_bounds__(1024) void k_exchange_fix(double4 *__restrict__ P, double *__restrict__ vx,
-                                                       double *__restrict__ vy, double *__restrict__ vz,
-                                                       double *__restrict__ mass, int *__restrict__ uid,
-                                                       int *__restrict__ shape, int *__restrict__ flags,
-                                                       const int *__restrict__ counters, Geom g,
-                                                       const int *__restrict__ send_map,
-                                                       const int *__restrict__ send_mult,
-                                                       const int *__restrict__ exchg_flag, double *__restrict__ buf,
-                                                       int *__restrict__ tail, int *__restrict__ copy_to) {
+// Exchange of one dimension after its send map is built, as a relabeling:
+// the reference packs the leavers (position + shift), fills their holes from
+// the tail (remove_exchanged_particles_pt1/pt2, copy guard "src > 0") and
+// appends them at nlocal - n (src/pairs/sim/comm.py:100-151, 445-520).  Here
+// the particles stay in their physical slots: leavers get their shift and
+// the reference index the reference would give them, tail particles moved
+// into a hole get that hole's index.  pt1's sequential tail walk gives the
+// r-th exchanged particle below nlocal - n the r-th non-exchanged reference
+// index below nlocal (descending).  One workgroup: a rebuild moves few particles.
+__global__ __launch_bounds__(1024) void k_exchange_relabel(double *__restrict__ X, double *__restrict__ Y,
+                                                           double *__restrict__ Z, int *__restrict__ ref,
+                                                           int *__restrict__ phys, const int *__restrict__ counters,
+                                                           Geom g, const int *__restrict__ send_map,
+                                                           const int *__restrict__ send_mult,
+                                                           const int *__restrict__ exchg_flag, int *__restrict__ tail,
+                                                           int *__restrict__ copy_to, int *__restrict__ p_leaver,
+                                                           int *__restrict__ p_src) {
     __shared__ int lds[20];
     if (counters[C_OVF]) return;
     const int n = counters[C_NSEND_ALL];
     if (n == 0) return;
     const int nlocal = counters[C_NLOCAL];
-    // pack
-    for (int e = threadIdx.x; e < n; e += 1024) {
-        const int m = send_map[e];
-        double4 p = P[m];
-        p.x = p.x + (double) send_mult[3 * e + 0] * g.len[0];
-        p.y = p.y + (double) send_mult[3 * e + 1] * g.len[1];
-        p.z = p.z + (double) send_mult[3 * e + 2] * g.len[2];
-        double *b = buf + (size_t) e * 10;
-        b[0] = p.x; b[1] = p.y; b[2] = p.z; b[3] = p.w;
-        b[4] = vx[m]; b[5] = vy[m]; b[6] = vz[m]; b[7] = mass[m];
-        int *bi = (int *) (b + 8);
-        bi[0] = uid[m]; bi[1] = shape[m]; bi[2] = flags[m]; bi[3] = 0;
-    }
-    // tail list: non-exchanged indices nlocal-1, nlocal-2, ... >= nlocal-n
     int carry = 0;
     for (int t0 = 0; t0 < n; t0 += 1024) {
         const int t = t0 + threadIdx.x;
@@ -794,25 +1099,26 @@ __global__ __launch_bounds__(1024) void k_exchange_fix(double4 *__restrict__ P, 
         carry += total;
     }
     __syncthreads();
-    // pt2: fill the holes from the tail
+    // read the current physical slots before relabeling anything
     for (int e = threadIdx.x; e < n; e += 1024) {
+        p_leaver[e] = phys[send_map[e]];
         const int src = copy_to[e];
-        if (src > 0) {
-            const int dst = send_map[e];
-            P[dst] = P[src];
-            vx[dst] = vx[src]; vy[dst] = vy[src]; vz[dst] = vz[src];
-            mass[dst] = mass[src]; uid[dst] = uid[src]; shape[dst] = shape[src]; flags[dst] = flags[src];
-        }
+        p_src[e] = src > 0 ? phys[src] : -1;
     }
     __syncthreads();
-    // unpack at the end of the locals
     for (int e = threadIdx.x; e < n; e += 1024) {
-        const int dst = nlocal - n + e;
-        const double *b = buf + (size_t) e * 10;
-        P[dst] = make_double4(b[0], b[1], b[2], b[3]);
-        vx[dst] = b[4]; vy[dst] = b[5]; vz[dst] = b[6]; mass[dst] = b[7];
-        const int *bi = (const int *) (b + 8);
-        uid[dst] = bi[0]; shape[dst] = bi[1]; flags[dst] = bi[2];
+        const int ps = p_src[e];
+        if (ps >= 0) { const int dst = send_map[e]; ref[ps] = dst; phys[dst] = ps; }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += 1024) {
+        const int p = p_leaver[e];
+        X[p] = X[p] + (double) send_mult[3 * e + 0] * g.len[0];
+        Y[p] = Y[p] + (double) send_mult[3 * e + 1] * g.len[1];
+        Z[p] = Z[p] + (double) send_mult[3 * e + 2] * g.len[2];
+        const int nr = nlocal - n + e;
+        ref[p] = nr;
+        phys[nr] = p;
     }
 }
 
@@ -820,51 +1126,55 @@ __global__ __launch_bounds__(1024) void k_exchange_fix(double4 *__restrict__ P, 
 // ghosts are packed before any is unpacked, comm.py:45-54): a ghost whose
 // source is local takes the source's current position; a ghost whose source
 // is another ghost takes that ghost's previous value.  Double-buffered form:
-// Pcur holds this step's locals, Pprev the previous step's ghosts.
-__global__ __launch_bounds__(256) void k_sync_pp(double4 *__restrict__ Pcur, const double4 *__restrict__ Pprev,
-                                                 const int *__restrict__ counters, const int *__restrict__ send_map,
-                                                 const int *__restrict__ send_mult, Geom g) {
+// (X, Y, Z)[cur] holds this step's locals, [prev] the previous ghosts.
+__global__ __launch_bounds__(256) void k_sync_pp(double *__restrict__ Xc, double *__restrict__ Yc,
+                                                 double *__restrict__ Zc, const double *__restrict__ Xp,
+                                                 const double *__restrict__ Yp, const double *__restrict__ Zp,
+                                                 const int *__restrict__ counters, const int *__restrict__ sync_src,
+                                                 const int *__restrict__ sync_mult, Geom g) {
     const int n = counters[C_NSEND_ALL];
     const int nlocal = counters[C_NLOCAL];
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
-        const int src = send_map[e];
-        double4 p = src < nlocal ? Pcur[src] : Pprev[src];
-        p.x = p.x + (double) send_mult[3 * e + 0] * g.len[0];
-        p.y = p.y + (double) send_mult[3 * e + 1] * g.len[1];
-        p.z = p.z + (double) send_mult[3 * e + 2] * g.len[2];
-        Pcur[nlocal + e] = p;
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+        const int src = sync_src[q];
+        const bool loc = src < nlocal;
+        const double x = loc ? Xc[src] : Xp[src];
+        const double y = loc ? Yc[src] : Yp[src];
+        const double z = loc ? Zc[src] : Zp[src];
+        Xc[nlocal + q] = x + (double) sync_mult[3 * q + 0] * g.len[0];
+        Yc[nlocal + q] = y + (double) sync_mult[3 * q + 1] * g.len[1];
+        Zc[nlocal + q] = z + (double) sync_mult[3 * q + 2] * g.len[2];
     }
 }
 
-// single-buffer form: pack everything into a staging buffer, then unpack
-__global__ __launch_bounds__(256) void k_sync_pack(const double4 *__restrict__ P, const double *__restrict__ vx,
+// single-buffer form: pack every ghost into a staging buffer, then unpack
+__global__ __launch_bounds__(256) void k_sync_pack(const double *__restrict__ X, const double *__restrict__ Y,
+                                                   const double *__restrict__ Z, const double *__restrict__ vx,
                                                    const double *__restrict__ vy, const double *__restrict__ vz,
-                                                   const int *__restrict__ counters, const int *__restrict__ send_map,
-                                                   const int *__restrict__ send_mult, Geom g,
+                                                   const int *__restrict__ counters, const int *__restrict__ sync_src,
+                                                   const int *__restrict__ sync_mult, Geom g,
                                                    double *__restrict__ stage, int with_vel) {
     const int n = counters[C_NSEND_ALL];
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
-        const int src = send_map[e];
-        double4 p = P[src];
-        double *s = stage + (size_t) e * 8;
-        s[0] = p.x + (double) send_mult[3 * e + 0] * g.len[0];
-        s[1] = p.y + (double) send_mult[3 * e + 1] * g.len[1];
-        s[2] = p.z + (double) send_mult[3 * e + 2] * g.len[2];
-        s[3] = p.w;
-        if (with_vel) { s[4] = vx[src]; s[5] = vy[src]; s[6] = vz[src]; }
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+        const int src = sync_src[q];
+        double *s = stage + (size_t) q * 6;
+        s[0] = X[src] + (double) sync_mult[3 * q + 0] * g.len[0];
+        s[1] = Y[src] + (double) sync_mult[3 * q + 1] * g.len[1];
+        s[2] = Z[src] + (double) sync_mult[3 * q + 2] * g.len[2];
+        if (with_vel) { s[3] = vx[src]; s[4] = vy[src]; s[5] = vz[src]; }
     }
 }
-__global__ __launch_bounds__(256) void k_sync_unpack(double4 *__restrict__ P, double *__restrict__ vx,
+__global__ __launch_bounds__(256) void k_sync_unpack(double *__restrict__ X, double *__restrict__ Y,
+                                                     double *__restrict__ Z, double *__restrict__ vx,
                                                      double *__restrict__ vy, double *__restrict__ vz,
                                                      const int *__restrict__ counters, const double *__restrict__ stage,
                                                      int with_vel) {
     const int n = counters[C_NSEND_ALL];
     const int nlocal = counters[C_NLOCAL];
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
-        const double *s = stage + (size_t) e * 8;
-        const int gs = nlocal + e;
-        P[gs] = make_double4(s[0], s[1], s[2], s[3]);
-        if (with_vel) { vx[gs] = s[4]; vy[gs] = s[5]; vz[gs] = s[6]; }
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
+        const double *s = stage + (size_t) q * 6;
+        const int gs = nlocal + q;
+        X[gs] = s[0]; Y[gs] = s[1]; Z[gs] = s[2];
+        if (with_vel) { vx[gs] = s[3]; vy[gs] = s[4]; vz[gs] = s[5]; }
     }
 }
 
@@ -872,13 +1182,18 @@ __global__ __launch_bounds__(256) void k_sync_unpack(double4 *__restrict__ P, do
 
 void launch_fill_i32(hipStream_t s, int *p, int v, int n) {
     if (n <= 0) return;
-    int blocks = min((n + 255) / 256, 4096);
+    const int blocks = min((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, s, p, v, n);
 }
 void launch_fill_f64(hipStream_t s, double *p, double v, int n) {
     if (n <= 0) return;
-    int blocks = min((n + 255) / 256, 4096);
+    const int blocks = min((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_fill_f64, dim3(blocks), dim3(256), 0, s, p, v, n);
+}
+void launch_identity(hipStream_t s, int *p, int n) {
+    if (n <= 0) return;
+    const int blocks = min((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_identity, dim3(blocks), dim3(256), 0, s, p, n);
 }
 void launch_set_counters(hipStream_t s, int *c, int first, int count, int v) {
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, c, first, count, v);
@@ -886,44 +1201,76 @@ void launch_set_counters(hipStream_t s, int *c, int first, int count, int v) {
 
 static int tile_grid(int cap) { return max(1, min((cap + 255) / 256, 2048)); }
 
-void launch_bin(hipStream_t s, const DevState &d, const Geom &g) {
-    launch_fill_i32(s, d.cell_count, 0, g.ncells + 1);
+void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g) {
+    launch_fill_i32(s, d.lcount, 0, g.ncells + 1);
+    launch_fill_i32(s, d.gcount, 0, g.ncells + 1);
     const int grid = tile_grid(d.cap);
-    hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, d.P[d.cur], d.flags, d.counters, g,
-                       d.particle_cell, d.cell_count, d.slot);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, d.counters, d.cell_count, d.cell_start, g.ncells);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, d.counters, d.particle_cell, d.slot,
-                       d.cell_start, d.sorted_idx);
-    hipLaunchKernelGGL(k_sort_cells, dim3((g.ncells + 255) / 256), dim3(256), 0, s, d.counters, d.cell_start,
-                       d.sorted_idx, d.P[d.cur], d.sorted_pos, d.cell_nlocal, g.ncells);
+    const int c = d.cur, o = 1 - d.cur;
+    hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.flags, d.counters, g,
+                       d.particle_cell, d.lcount, d.gcount, d.slot);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, d.counters, d.lcount, d.gcount, d.lstart, d.gstart,
+                       g.ncells);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(grid), dim3(256), 0, s, d.counters, d.particle_cell, d.slot, d.lstart,
+                       d.gstart, d.perm);
+    hipLaunchKernelGGL(k_sort_cells, dim3((2 * g.ncells + 3) / 4), dim3(256), 0, s, d.counters, d.lstart, d.gstart,
+                       d.ref, d.perm, g.ncells);
+    hipLaunchKernelGGL(k_permute, dim3(grid), dim3(256), 0, s, d.counters, d.perm, d.X[c], d.Y[c], d.Z[c], d.X[o],
+                       d.Y[o], d.Z[o], d.vx, d.vy, d.vz, d.mass, d.type, d.flags, d.uid, d.shape, d.ref,
+                       d.particle_cell, d.alt_vx, d.alt_vy, d.alt_vz, d.alt_mass, d.alt_type, d.alt_flags,
+                       d.alt_uid, d.alt_shape, d.alt_ref, d.alt_cell, d.phys);
+    d.cur = o;
+    // copy the permuted arrays back: array addresses stay fixed, so captured
+    // step graphs remain valid across rebuilds
+    hipLaunchKernelGGL(k_copy_back, dim3(grid), dim3(256), 0, s, d.counters, d.alt_vx, d.alt_vy, d.alt_vz,
+                       d.alt_mass, d.alt_type, d.alt_flags, d.alt_uid, d.alt_shape, d.alt_ref, d.alt_cell, d.vx,
+                       d.vy, d.vz, d.mass, d.type, d.flags, d.uid, d.shape, d.ref, d.particle_cell);
+    hipLaunchKernelGGL(k_sync_maps, dim3(grid), dim3(256), 0, s, d.counters, d.ref, d.phys, d.send_map, d.send_mult,
+                       d.sync_src, d.sync_mult);
 }
 
 void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2) {
-    hipLaunchKernelGGL(k_build_neighbors, dim3(g.ncells), dim3(256), 0, s, d.counters, g, d.cell_start,
-                       d.cell_nlocal, d.sorted_idx, d.sorted_pos, d.flags, half, rc2, d.nl, d.nl_stride, d.ncap,
-                       d.numneighs, d.counters + C_REQ_NEIGH);
+    // LDS budget: tile (28 B/entry) + candidate list + list transposition rows
+    // of hb home particles (home particles past hb write their rows directly)
+    const int budget = d.max_lds - 4096 - NB_CAND * (int) sizeof(uint16_t);
+    int hb = min(256, max(32, (budget / 2) / (2 * d.ncap)));
+    int tcap = min(2048, (budget - 2 * d.ncap * hb) / 28);
+    tcap = max(tcap, 256);
+    const size_t lds = (size_t) tcap * 28 + NB_CAND * sizeof(uint16_t) + (size_t) d.ncap * hb * sizeof(uint16_t);
+    const int c = d.cur;
+    hipLaunchKernelGGL(k_build_neighbors, dim3(g.nsegs), dim3(256), lds, s, d.counters, g, d.lstart, d.gstart,
+                       d.lcount, d.gcount, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, half, rc2, d.nl, d.nl_stride,
+                       d.ncap, hb, tcap, d.numneighs, d.counters);
 }
 
-void launch_select_dim(hipStream_t s, const DevState &d, const Geom &g, int dim, int exch) {
+void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out) {
+    hipLaunchKernelGGL(k_expand_neighbors, dim3(g.nsegs), dim3(256), 0, s, d.counters, g, d.lstart, d.gstart, d.ref,
+                       d.nl, d.nl_stride, d.numneighs, out_ref, ncap_out);
+}
+
+static void launch_select_dim(hipStream_t s, const DevState &d, const Geom &g, int dim, int exch) {
     const int grid = tile_grid(d.cap);
-    double4 *P = d.P[d.cur];
-    hipLaunchKernelGGL(k_sel_count, dim3(grid), dim3(256), 0, s, P, d.flags, d.counters, g, dim, exch, d.tile_cnt,
-                       d.exchg_flag);
+    const int c = d.cur;
+    const double *Xd = dim == 0 ? d.X[c] : (dim == 1 ? d.Y[c] : d.Z[c]);
+    hipLaunchKernelGGL(k_sel_count, dim3(grid), dim3(256), 0, s, Xd, d.flags, d.phys, d.counters, g, dim, exch,
+                       d.tile_cnt, d.exchg_flag);
     hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, s, d.counters, d.tile_cnt, dim, exch, d.send_cap, d.cap);
-    hipLaunchKernelGGL(k_sel_scatter, dim3(grid), dim3(256), 0, s, P, d.vx, d.vy, d.vz, d.mass, d.uid, d.shape,
-                       d.flags, d.counters, g, dim, exch, d.tile_cnt, d.send_map, d.send_mult);
+    hipLaunchKernelGGL(k_sel_scatter, dim3(grid), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.vx, d.vy, d.vz, d.mass,
+                       d.type, d.uid, d.shape, d.flags, d.ref, d.phys, d.counters, g, dim, exch, d.tile_cnt,
+                       d.send_map, d.send_mult);
 }
 
 void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int *periodic) {
+    const int c = d.cur;
     for (int dim = 0; dim < 3; dim++) {
         if (!periodic[dim]) continue;
         // nsend_all = 0; nghost = 0 (comm.py:105-118)
         launch_set_counters(s, d.counters, C_NSEND_ALL, 1, 0);
         launch_set_counters(s, d.counters, C_NGHOST, 1, 0);
         launch_select_dim(s, d, g, dim, 1);
-        hipLaunchKernelGGL(k_exchange_fix, dim3(1), dim3(1024), 0, s, d.P[d.cur], d.vx, d.vy, d.vz, d.mass, d.uid,
-                           d.shape, d.flags, d.counters, g, d.send_map, d.send_mult, d.exchg_flag, d.stage,
-                           d.scratch_i, d.scratch_i + d.send_cap);
+        int *sc = d.scratch_i;
+        hipLaunchKernelGGL(k_exchange_relabel, dim3(1), dim3(1024), 0, s, d.X[c], d.Y[c], d.Z[c], d.ref, d.phys,
+                           d.counters, g, d.send_map, d.send_mult, d.exchg_flag, sc, sc + d.send_cap,
+                           sc + 2 * d.send_cap, sc + 3 * d.send_cap);
     }
 }
 
@@ -938,75 +1285,102 @@ void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *
 }
 
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid) {
-    hipLaunchKernelGGL(k_sync_pp, dim3(grid), dim3(256), 0, s, d.P[cur], d.P[prev], d.counters, d.send_map,
-                       d.send_mult, g);
+    hipLaunchKernelGGL(k_sync_pp, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[prev], d.Y[prev],
+                       d.Z[prev], d.counters, d.sync_src, d.sync_mult, g);
 }
 
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel) {
-    hipLaunchKernelGGL(k_sync_pack, dim3(grid), dim3(256), 0, s, d.P[d.cur], d.vx, d.vy, d.vz, d.counters,
-                       d.send_map, d.send_mult, g, d.stage, with_vel);
-    hipLaunchKernelGGL(k_sync_unpack, dim3(grid), dim3(256), 0, s, d.P[d.cur], d.vx, d.vy, d.vz, d.counters,
-                       d.stage, with_vel);
+    const int c = d.cur;
+    hipLaunchKernelGGL(k_sync_pack, dim3(grid), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.vx, d.vy, d.vz,
+                       d.counters, d.sync_src, d.sync_mult, g, d.stage, with_vel);
+    hipLaunchKernelGGL(k_sync_unpack, dim3(grid), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.vx, d.vy, d.vz,
+                       d.counters, d.stage, with_vel);
 }
 
 static LJTab make_tab(const LJParams &prm) {
     LJTab t;
     t.nt = prm.ntypes;
-    for (int k = 0; k < 16; k++) { t.eps[k] = k < prm.ntypes * prm.ntypes ? prm.eps[k] : 0.0; t.sig6[k] = k < prm.ntypes * prm.ntypes ? prm.sig6[k] : 0.0; }
+    for (int k = 0; k < 16; k++) {
+        t.eps[k] = k < prm.ntypes * prm.ntypes ? prm.eps[k] : 0.0;
+        t.sig6[k] = k < prm.ntypes * prm.ntypes ? prm.sig6[k] : 0.0;
+    }
     return t;
 }
 
 constexpr int LJ_UNROLL = 8;
 
-void launch_lj_full(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int next, int fuse, int do_vv2,
-                    int do_vv1, int reduce) {
-    const int nlocal = d.nlocal;
-    if (nlocal <= 0) return;
-    const int grid = (nlocal + 255) / 256;
-    LJTab tab = make_tab(prm);
-    const double4 *P = d.P[cur];
-    double4 *Pn = d.P[next];
+int force_blocks(const DevState &d, const Geom &g) { return g.nsegs; }
+
+template <int NT, int MODE, bool RED>
+static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
+                           int do_vv2, int do_vv1, double *eatom) {
+    const size_t lds = (size_t) d.tile_cap * (3 * sizeof(double) + (NT != 1 ? sizeof(int) : 0));
+    const LJTab tab = make_tab(prm);
+    hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, d.counters, g,
+                       d.lstart, d.gstart, d.X[cur], d.Y[cur], d.Z[cur], d.X[next], d.Y[next], d.Z[next], d.type,
+                       d.nl, d.nl_stride, d.numneighs, d.flags, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.mass, prm.rc2,
+                       tab, prm.dt, prm.hdt, do_vv2, do_vv1, d.partials, d.pstride, eatom, d.tile_cap);
+}
+
+template <int NT>
+static void lj_full_nt(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
+                       int fuse, int do_vv2, int do_vv1, int reduce) {
     if (fuse) {
-        if (reduce)
-            hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, true, true>), dim3(grid), dim3(256), 0, s, P, Pn, d.nl, d.nl_stride,
-                               d.numneighs, d.flags, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.mass, nlocal, prm.rc2, tab,
-                               prm.dt, prm.hdt, do_vv2, do_vv1, d.partials);
-        else
-            hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, true, false>), dim3(grid), dim3(256), 0, s, P, Pn, d.nl,
-                               d.nl_stride, d.numneighs, d.flags, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.mass, nlocal,
-                               prm.rc2, tab, prm.dt, prm.hdt, do_vv2, do_vv1, d.partials);
+        if (reduce) lj_full_launch<NT, 1, true>(s, d, g, prm, cur, next, do_vv2, do_vv1, nullptr);
+        else lj_full_launch<NT, 1, false>(s, d, g, prm, cur, next, do_vv2, do_vv1, nullptr);
     } else {
-        if (reduce)
-            hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, false, true>), dim3(grid), dim3(256), 0, s, P, Pn, d.nl,
-                               d.nl_stride, d.numneighs, d.flags, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.mass, nlocal,
-                               prm.rc2, tab, prm.dt, prm.hdt, 0, 0, d.partials);
-        else
-            hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, false, false>), dim3(grid), dim3(256), 0, s, P, Pn, d.nl,
-                               d.nl_stride, d.numneighs, d.flags, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.mass, nlocal,
-                               prm.rc2, tab, prm.dt, prm.hdt, 0, 0, d.partials);
+        if (reduce) lj_full_launch<NT, 0, true>(s, d, g, prm, cur, next, 0, 0, nullptr);
+        else lj_full_launch<NT, 0, false>(s, d, g, prm, cur, next, 0, 0, nullptr);
     }
 }
 
-void launch_lj_half(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int reduce) {
-    const int nlocal = d.nlocal;
-    if (nlocal <= 0) return;
-    const int grid = (nlocal + 255) / 256;
-    LJTab tab = make_tab(prm);
-    launch_fill_f64(s, d.fx, 0.0, nlocal);
-    launch_fill_f64(s, d.fy, 0.0, nlocal);
-    launch_fill_f64(s, d.fz, 0.0, nlocal);
+void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
+                    int do_vv2, int do_vv1, int reduce) {
+    if (d.nlocal <= 0) return;
+    if (prm.ntypes == 1) lj_full_nt<1>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
+    else lj_full_nt<0>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
+}
+
+template <int NT>
+static void lj_half_nt(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce) {
+    const size_t lds = (size_t) d.tile_cap * (3 * sizeof(double) + sizeof(int) + (NT != 1 ? sizeof(int) : 0));
+    const LJTab tab = make_tab(prm);
+    const int c = cur;
     if (reduce)
-        hipLaunchKernelGGL((k_lj_half<LJ_UNROLL, true>), dim3(grid), dim3(256), 0, s, d.P[cur], d.nl, d.nl_stride,
-                           d.numneighs, d.flags, d.fx, d.fy, d.fz, nlocal, prm.rc2, tab, d.partials);
+        hipLaunchKernelGGL((k_lj_half<LJ_UNROLL, NT, true>), dim3(g.nsegs), dim3(FB), lds, s, d.counters, g, d.lstart,
+                           d.gstart, d.X[c], d.Y[c], d.Z[c], d.type, d.nl, d.nl_stride, d.numneighs, d.flags, d.fx,
+                           d.fy, d.fz, prm.rc2, tab, d.partials, d.pstride, d.tile_cap);
     else
-        hipLaunchKernelGGL((k_lj_half<LJ_UNROLL, false>), dim3(grid), dim3(256), 0, s, d.P[cur], d.nl, d.nl_stride,
-                           d.numneighs, d.flags, d.fx, d.fy, d.fz, nlocal, prm.rc2, tab, d.partials);
+        hipLaunchKernelGGL((k_lj_half<LJ_UNROLL, NT, false>), dim3(g.nsegs), dim3(FB), lds, s, d.counters, g,
+                           d.lstart, d.gstart, d.X[c], d.Y[c], d.Z[c], d.type, d.nl, d.nl_stride, d.numneighs, d.flags,
+                           d.fx, d.fy, d.fz, prm.rc2, tab, d.partials, d.pstride, d.tile_cap);
+}
+
+void launch_lj_half(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce) {
+    if (d.nlocal <= 0) return;
+    launch_fill_f64(s, d.fx, 0.0, d.nlocal);
+    launch_fill_f64(s, d.fy, 0.0, d.nlocal);
+    launch_fill_f64(s, d.fz, 0.0, d.nlocal);
+    if (prm.ntypes == 1) lj_half_nt<1>(s, d, g, prm, cur, reduce);
+    else lj_half_nt<0>(s, d, g, prm, cur, reduce);
+}
+
+void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom) {
+    if (d.nlocal <= 0) return;
+    if (half) {
+        // the half-list kernel recomputes (and re-adds) forces; energies come from its partials
+        launch_lj_half(s, d, g, prm, d.cur, 1);
+        return;
+    }
+    if (prm.ntypes == 1) lj_full_launch<1, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
+    else lj_full_launch<0, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
 }
 
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm) {
     if (d.nlocal <= 0) return;
-    hipLaunchKernelGGL(k_vv_initial, dim3((d.nlocal + 255) / 256), dim3(256), 0, s, d.P[d.cur], d.vx, d.vy, d.vz,
-                       d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt, prm.hdt);
+    const int c = d.cur;
+    hipLaunchKernelGGL(k_vv_initial, dim3((d.nlocal + 255) / 256), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.vx,
+                       d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt, prm.hdt);
 }
 
 void launch_vv_post(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
@@ -1014,11 +1388,13 @@ void launch_vv_post(hipStream_t s, const DevState &d, const LJParams &prm, int c
     if (d.nlocal <= 0) return;
     const int grid = (d.nlocal + 255) / 256;
     if (reduce)
-        hipLaunchKernelGGL(k_vv_post<true>, dim3(grid), dim3(256), 0, s, d.P[cur], d.P[next], d.vx, d.vy, d.vz, d.fx,
-                           d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt, prm.hdt, do_vv2, do_vv1, d.partials);
+        hipLaunchKernelGGL(k_vv_post<true>, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[next],
+                           d.Y[next], d.Z[next], d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt,
+                           prm.hdt, do_vv2, do_vv1, d.partials);
     else
-        hipLaunchKernelGGL(k_vv_post<false>, dim3(grid), dim3(256), 0, s, d.P[cur], d.P[next], d.vx, d.vy, d.vz, d.fx,
-                           d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt, prm.hdt, do_vv2, do_vv1, d.partials);
+        hipLaunchKernelGGL(k_vv_post<false>, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[next],
+                           d.Y[next], d.Z[next], d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt,
+                           prm.hdt, do_vv2, do_vv1, d.partials);
 }
 
 void launch_ke_partial(hipStream_t s, const DevState &d) {
@@ -1027,17 +1403,9 @@ void launch_ke_partial(hipStream_t s, const DevState &d) {
                        d.nlocal, d.partials);
 }
 
-void launch_energy(hipStream_t s, const DevState &d, const LJParams &prm, int half, double *eatom) {
-    if (d.nlocal <= 0) return;
-    LJTab tab = make_tab(prm);
-    hipLaunchKernelGGL(k_energy, dim3((d.nlocal + 255) / 256), dim3(256), 0, s, d.P[d.cur], d.nl, d.nl_stride,
-                       d.numneighs, d.flags, d.nlocal, half, prm.rc2, tab, d.partials, eatom);
-}
-
-void launch_thermo_reduce(hipStream_t s, const DevState &d, int which_mask, const Geom &g, double *out) {
-    const int nb = (d.nlocal + 255) / 256;
-    hipLaunchKernelGGL(k_thermo_reduce, dim3(1), dim3(256), 0, s, d.partials, nb, which_mask, d.nlocal, g.len[0],
-                       g.len[1], g.len[2], out);
+void launch_thermo_reduce(hipStream_t s, const DevState &d, int nb_ke, int which_mask, const Geom &g, double *out) {
+    hipLaunchKernelGGL(k_thermo_reduce, dim3(1), dim3(256), 0, s, d.partials, d.pstride, nb_ke, g.nsegs, which_mask,
+                       d.nlocal, g.len[0], g.len[1], g.len[2], out);
 }
 
 }  // namespace pk

@@ -1,6 +1,27 @@
 // Device state and launchers of the LJ molecular-dynamics path.
+//
+// Storage layout in HBM.  Particles live in "physical" slots: locals first,
+// sorted by cell, then ghosts, sorted by cell; every per-particle array is SoA
+// (x, y, z, vx, ..., one array per component).  The reference's own particle
+// index ("reference index": the slot the generated CPU program would use) is
+// kept as a relabeling, ref[physical] and phys[reference], updated exactly as
+// the reference's exchange and borders would move particles; every
+// order-dependent rule (exchange tail fill, ghost order, cell order, half-list
+// i < j, neighbour order) is applied to reference indices.  Positions are
+// double-buffered (X[2], Y[2], Z[2]) so that the fused force/integrate kernel
+// never overwrites a position another thread gathers.
+//
+// Work decomposition: a "segment" is ZC consecutive cells of one z-column of
+// the cell grid.  Its home particles are one contiguous range of physical
+// local slots, and the cells any of them can see (cell 0 plus the 3x3 columns
+// around it, z range widened by one) are 20 contiguous physical runs: the
+// segment's tile, staged in LDS once and shared by all its home particles.
+// Neighbour lists hold 16-bit offsets into the tile, in the reference's
+// candidate order.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
+
 #include "common.hpp"
 
 namespace pk {
@@ -21,9 +42,11 @@ enum {
     C_REQ_NEIGH = 13,
     C_REQ_SEND = 14,
     C_REQ_PART = 15,
-    C_COUNT = 16
+    C_TILE_MAX = 16,  // largest segment tile of the current lists
+    C_UNSUPPORTED = 17,
+    C_COUNT = 18
 };
-enum { OVF_NONE = 0, OVF_PART = 1, OVF_NEIGH = 2, OVF_SEND = 3 };
+enum { OVF_NONE = 0, OVF_PART = 1, OVF_NEIGH = 2, OVF_SEND = 3, OVF_TILE = 4 };
 
 // cell grid and domain, passed by value
 struct Geom {
@@ -33,6 +56,9 @@ struct Geom {
     int ncells;
     int nstencil;
     int stencil[27];
+    int zc;        // cells per segment along z
+    int nchunks;   // segments per z-column
+    int nsegs;     // dim_cells[0] * dim_cells[1] * nchunks
 };
 
 struct LJParams {
@@ -43,42 +69,56 @@ struct LJParams {
 
 struct DevState {
     int cap, ncap, send_cap, ncells_cap, nl_stride;
-    int nlocal;  // host mirror (constant between uploads on one domain)
-    int cur;     // which position buffer holds the current state
-    double4 *P[2];
+    int nlocal;   // host mirror (constant between uploads on one domain)
+    int cur;      // which position buffer holds the current state
+    int tile_cap; // LDS tile entries the force kernels may stage
+    int pstride;  // row stride of the partials array
+    int max_lds;  // shared memory a workgroup may allocate
+    double *X[2], *Y[2], *Z[2];
     double *vx, *vy, *vz, *fx, *fy, *fz, *mass;
-    int *flags, *uid, *shape;
-    int *numneighs, *nl;
-    int *particle_cell, *slot, *sorted_idx;
-    double4 *sorted_pos;
-    int *cell_count, *cell_start, *cell_nlocal;
+    int *type, *flags, *uid, *shape, *ref;
+    int *phys;    // physical slot of each reference index
+    // spare arrays the rebuild permutes into (swapped with the live ones)
+    double *alt_vx, *alt_vy, *alt_vz, *alt_mass;
+    int *alt_type, *alt_flags, *alt_uid, *alt_shape, *alt_ref, *alt_cell;
+    int *numneighs;
+    uint16_t *nl;  // nl[k * nl_stride + physical local] = tile offset
+    int *particle_cell, *slot, *perm;
+    int *lcount, *gcount, *lstart, *gstart;  // per-cell counts and CSR starts (locals / ghosts)
     int *send_map, *send_mult, *exchg_flag;
+    int *sync_src;       // per ghost (physical order): physical source slot
+    int *sync_mult;      // per ghost (physical order): 3 image multipliers
     int *tile_cnt;
     int *counters;
-    int *scratch_i;     // 2 * send_cap
-    double *stage;      // max(send_cap * 10, ...) doubles
-    double *partials;   // 3 per force block
-    double *thermo;     // 5 per thermo record
+    int *scratch_i;      // 4 * send_cap
+    double *stage;       // 10 * send_cap doubles
+    double *partials;    // 3 per force block
+    double *thermo;      // 5 per thermo record
 };
 
 void launch_fill_i32(hipStream_t s, int *p, int v, int n);
 void launch_fill_f64(hipStream_t s, double *p, double v, int n);
 void launch_set_counters(hipStream_t s, int *c, int first, int count, int v);
-void launch_bin(hipStream_t s, const DevState &d, const Geom &g);
-void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2);
-void launch_select_dim(hipStream_t s, const DevState &d, const Geom &g, int dim, int exch);
+void launch_identity(hipStream_t s, int *p, int n);
 void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int *periodic);
 void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *periodic);
+// bins locals and ghosts into cells and permutes every particle array into
+// cell order (swapping d's spare arrays in), then builds the ghost sync maps
+void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g);
+void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2);
+void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out);
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid);
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel);
-void launch_lj_full(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int next, int fuse, int do_vv2,
-                    int do_vv1, int reduce);
-void launch_lj_half(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int reduce);
+// mode: 0 force only, 1 force + integrators (fused), 2 energy only (no writes)
+void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
+                    int do_vv2, int do_vv1, int reduce);
+void launch_lj_half(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce);
+void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom);
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm);
 void launch_vv_post(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
                     int reduce);
 void launch_ke_partial(hipStream_t s, const DevState &d);
-void launch_energy(hipStream_t s, const DevState &d, const LJParams &prm, int half, double *eatom);
-void launch_thermo_reduce(hipStream_t s, const DevState &d, int which_mask, const Geom &g, double *out);
+void launch_thermo_reduce(hipStream_t s, const DevState &d, int nblocks, int which_mask, const Geom &g, double *out);
+int force_blocks(const DevState &d, const Geom &g);
 
 }  // namespace pk

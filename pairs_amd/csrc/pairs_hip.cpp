@@ -4,6 +4,9 @@
 // (runtime/pairs.hpp:27-297): it owns every device array, keeps counts on the
 // device so that rebuild kernels chain without host round trips, and replays
 // hipGraphs of the fused per-step kernels between neighbour rebuilds.
+// Device storage is in cell-ordered physical slots (md_kernels.hpp); every
+// upload and download goes through the reference-index relabeling, so the
+// ABI speaks the reference's particle indices only.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,16 +29,18 @@ struct pairs_hip_ctx {
     DevState d;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    bool cells_ready = false;
     std::string err;
     int req[5] = {0, 0, 0, 0, 0};
     int h_counters[C_COUNT];
     int nghost = 0, nsend_all = 0;
     int ghost_grid = 1;
+    int max_lds = 65536;       // shared memory a workgroup may allocate
+    int zc_override = 0;
     // graphs of one non-rebuild step, by parity of the current buffer
     bool use_graphs = true;
     hipGraphExec_t step_graph[2] = {nullptr, nullptr};
     int graph_ghost_grid[2] = {0, 0};
+    int graph_tile_cap[2] = {0, 0};
     // force-kernel timing
     bool time_force = false;
     int time_every = 10;
@@ -45,6 +50,8 @@ struct pairs_hip_ctx {
     int force_launches = 0;
     double *d_eatom = nullptr;
     int eatom_cap = 0;
+    int *d_expand = nullptr;
+    size_t expand_cap = 0;
 };
 
 #define HIP_OK(expr)                                                                                   \
@@ -84,31 +91,36 @@ static void dfree(void *p) {
     if (p) hipFree(p);
 }
 
+static size_t pstride_of(const DevState &d, const Geom &g) {
+    return (size_t) std::max(g.dim_cells[0] * g.dim_cells[1] * g.dim_cells[2], d.cap / 256 + 1) + 8;
+}
+
 static void free_particle_arrays(DevState &d) {
-    dfree(d.P[0]); dfree(d.P[1]); dfree(d.vx); dfree(d.vy); dfree(d.vz); dfree(d.fx); dfree(d.fy); dfree(d.fz);
-    dfree(d.mass); dfree(d.flags); dfree(d.uid); dfree(d.shape); dfree(d.numneighs); dfree(d.particle_cell);
-    dfree(d.slot); dfree(d.sorted_idx); dfree(d.sorted_pos); dfree(d.exchg_flag); dfree(d.tile_cnt);
-    dfree(d.partials);
-    d.P[0] = d.P[1] = nullptr;
-    d.vx = d.vy = d.vz = d.fx = d.fy = d.fz = d.mass = nullptr;
-    d.flags = d.uid = d.shape = d.numneighs = d.particle_cell = d.slot = d.sorted_idx = d.exchg_flag = d.tile_cnt = nullptr;
-    d.sorted_pos = nullptr;
-    d.partials = nullptr;
+    for (int b = 0; b < 2; b++) { dfree(d.X[b]); dfree(d.Y[b]); dfree(d.Z[b]); d.X[b] = d.Y[b] = d.Z[b] = nullptr; }
+    double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz,
+                     &d.alt_mass, &d.partials};
+    for (auto p : dd) { dfree(*p); *p = nullptr; }
+    int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
+                  &d.alt_shape, &d.alt_ref, &d.alt_cell, &d.numneighs, &d.particle_cell, &d.slot, &d.perm,
+                  &d.exchg_flag, &d.tile_cnt, &d.sync_src, &d.sync_mult};
+    for (auto p : ii) { dfree(*p); *p = nullptr; }
 }
 
 static int alloc_particle_arrays(pairs_hip_ctx *ctx, DevState &d, int cap) {
     int r = 0;
-    r |= dalloc(ctx, &d.P[0], cap); r |= dalloc(ctx, &d.P[1], cap);
-    r |= dalloc(ctx, &d.vx, cap); r |= dalloc(ctx, &d.vy, cap); r |= dalloc(ctx, &d.vz, cap);
-    r |= dalloc(ctx, &d.fx, cap); r |= dalloc(ctx, &d.fy, cap); r |= dalloc(ctx, &d.fz, cap);
-    r |= dalloc(ctx, &d.mass, cap); r |= dalloc(ctx, &d.flags, cap); r |= dalloc(ctx, &d.uid, cap);
-    r |= dalloc(ctx, &d.shape, cap); r |= dalloc(ctx, &d.numneighs, cap); r |= dalloc(ctx, &d.particle_cell, cap);
-    r |= dalloc(ctx, &d.slot, cap); r |= dalloc(ctx, &d.sorted_idx, cap); r |= dalloc(ctx, &d.sorted_pos, cap);
-    r |= dalloc(ctx, &d.exchg_flag, cap);
+    for (int b = 0; b < 2; b++) { r |= dalloc(ctx, &d.X[b], cap); r |= dalloc(ctx, &d.Y[b], cap); r |= dalloc(ctx, &d.Z[b], cap); }
+    double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz, &d.alt_mass};
+    for (auto p : dd) r |= dalloc(ctx, p, cap);
+    int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
+                  &d.alt_shape, &d.alt_ref, &d.alt_cell, &d.numneighs, &d.particle_cell, &d.slot, &d.perm,
+                  &d.exchg_flag, &d.sync_src};
+    for (auto p : ii) r |= dalloc(ctx, p, cap);
+    r |= dalloc(ctx, &d.sync_mult, 3 * (size_t) cap);
     r |= dalloc(ctx, &d.tile_cnt, 2 * ((size_t) cap / 256 + 2));
-    r |= dalloc(ctx, &d.partials, 3 * ((size_t) cap / 256 + 2));
-    if (r) return PAIRS_HIP_ERR_HIP;
     d.cap = cap;
+    d.pstride = (int) pstride_of(d, ctx->geom);
+    r |= dalloc(ctx, &d.partials, 3 * (size_t) d.pstride);
+    if (r) return PAIRS_HIP_ERR_HIP;
     return 0;
 }
 
@@ -128,28 +140,44 @@ static int alloc_send_arrays(pairs_hip_ctx *ctx, DevState &d, int send_cap) {
     int r = 0;
     r |= dalloc(ctx, &d.send_map, send_cap);
     r |= dalloc(ctx, &d.send_mult, 3 * (size_t) send_cap);
-    r |= dalloc(ctx, &d.scratch_i, 2 * (size_t) send_cap);
+    r |= dalloc(ctx, &d.scratch_i, 4 * (size_t) send_cap);
     r |= dalloc(ctx, &d.stage, 10 * (size_t) send_cap);
     if (r) return PAIRS_HIP_ERR_HIP;
     d.send_cap = send_cap;
     return 0;
 }
 
-static int zero_particle_arrays(pairs_hip_ctx *ctx, int from, int to) {
+static int zero_particle_arrays(pairs_hip_ctx *ctx) {
     DevState &d = ctx->d;
     hipStream_t s = ctx->stream;
-    const int n = to - from;
-    if (n <= 0) return 0;
-    launch_fill_i32(s, d.flags + from, 0, n);
-    launch_fill_i32(s, d.uid + from, 0, n);
-    launch_fill_i32(s, d.shape + from, SHAPE_POINT_MASS, n);
-    launch_fill_i32(s, d.numneighs + from, 0, n);
-    launch_fill_f64(s, d.mass + from, 1.0, n);
-    for (double *p : {d.vx, d.vy, d.vz, d.fx, d.fy, d.fz}) launch_fill_f64(s, p + from, 0.0, n);
-    launch_fill_f64(s, (double *) (d.P[0] + from), 0.0, 4 * n);
-    launch_fill_f64(s, (double *) (d.P[1] + from), 0.0, 4 * n);
+    const int n = d.cap;
+    for (int *p : {d.flags, d.uid, d.numneighs, d.type, d.alt_flags, d.alt_uid, d.alt_type}) launch_fill_i32(s, p, 0, n);
+    launch_fill_i32(s, d.shape, SHAPE_POINT_MASS, n);
+    launch_fill_i32(s, d.alt_shape, SHAPE_POINT_MASS, n);
+    launch_fill_f64(s, d.mass, 1.0, n);
+    launch_fill_f64(s, d.alt_mass, 1.0, n);
+    for (double *p : {d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.alt_vx, d.alt_vy, d.alt_vz}) launch_fill_f64(s, p, 0.0, n);
+    for (int b = 0; b < 2; b++) for (double *p : {d.X[b], d.Y[b], d.Z[b]}) launch_fill_f64(s, p, 0.0, n);
+    launch_identity(s, d.ref, n);
+    launch_identity(s, d.phys, n);
     CHECK_LAUNCH();
     return 0;
+}
+
+// segments: zc cells of one z-column, sized for about 112 home particles per
+// 128-thread block (override: PAIRS_HIP_ZC)
+static void setup_segments(pairs_hip_ctx *ctx, int zc) {
+    Geom &g = ctx->geom;
+    if (zc <= 0) {
+        const double vol = g.len[0] * g.len[1] * g.len[2];
+        const double s3 = g.spacing * g.spacing * g.spacing;
+        const double per_cell = ctx->d.nlocal > 0 && vol > 0 ? ctx->d.nlocal * s3 / vol : 16.0;
+        zc = (int) std::lround(112.0 / std::max(per_cell, 1e-3));
+    }
+    zc = std::max(1, std::min(zc, std::max(1, g.dim_cells[2] - 1)));
+    g.zc = zc;
+    g.nchunks = std::max(1, (g.dim_cells[2] - 1 + zc - 1) / zc);
+    g.nsegs = g.dim_cells[0] * g.dim_cells[1] * g.nchunks;
 }
 
 // build_cell_lists_stencil (src/pairs/sim/cell_lists.py:46-87)
@@ -172,11 +200,7 @@ static int setup_geometry(pairs_hip_ctx *ctx) {
     for (int a = -1; a < 2; a++)
         for (int b = -1; b < 2; b++)
             for (int e = -1; e < 2; e++) g.stencil[g.nstencil++] = (a * g.dim_cells[1] + b) * g.dim_cells[2] + e;
-    if (g.ncells + 1 >= ctx->d.ncells_cap) {
-        ctx->req[PAIRS_HIP_OVERFLOW_CELLS] = g.ncells + 2;
-        return PAIRS_HIP_OVERFLOW_CELLS;
-    }
-    ctx->cells_ready = true;
+    setup_segments(ctx, ctx->zc_override);
     return 0;
 }
 
@@ -195,12 +219,38 @@ static int take_overflow(pairs_hip_ctx *ctx) {
     int code = 0;
     if (h[C_OVF] == OVF_SEND) { ctx->req[PAIRS_HIP_OVERFLOW_SEND] = h[C_REQ_SEND]; code = PAIRS_HIP_OVERFLOW_SEND; }
     else if (h[C_OVF] == OVF_PART) { ctx->req[PAIRS_HIP_OVERFLOW_PARTICLES] = h[C_REQ_PART]; code = PAIRS_HIP_OVERFLOW_PARTICLES; }
+    else if (h[C_OVF] == OVF_TILE) { code = PAIRS_HIP_OVERFLOW_CELLS; ctx->req[PAIRS_HIP_OVERFLOW_CELLS] = h[C_TILE_MAX]; }
     else if (h[C_REQ_NEIGH] > 0) { ctx->req[PAIRS_HIP_OVERFLOW_NEIGHBORS] = h[C_REQ_NEIGH]; code = PAIRS_HIP_OVERFLOW_NEIGHBORS; }
     if (code) {
         launch_set_counters(ctx->stream, ctx->d.counters, C_OVF, 1, 0);
         launch_set_counters(ctx->stream, ctx->d.counters, C_REQ_NEIGH, 3, 0);
     }
     return code;
+}
+
+static int download_raw(pairs_hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return 0;
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// physical slot of every reference index in [0, n)
+static int host_phys(pairs_hip_ctx *ctx, int n, std::vector<int> &phys) {
+    phys.resize(n);
+    return download_raw(ctx, phys.data(), ctx->d.phys, n * sizeof(int));
+}
+
+template <typename T>
+static int ref_order(pairs_hip_ctx *ctx, const T *dev, int n, const std::vector<int> &phys, T *out, int stride,
+                     int comp) {
+    if (n <= 0) return 0;
+    int maxp = 0;
+    for (int r = 0; r < n; r++) maxp = std::max(maxp, phys[r]);
+    std::vector<T> tmp(maxp + 1);
+    if (download_raw(ctx, tmp.data(), dev, (maxp + 1) * sizeof(T))) return PAIRS_HIP_ERR_HIP;
+    for (int r = 0; r < n; r++) out[(size_t) r * stride + comp] = tmp[phys[r]];
+    return 0;
 }
 
 extern "C" {
@@ -231,6 +281,7 @@ int pairs_hip_create(pairs_hip_ctx **out, const pairs_hip_md_config *cfg) {
         p.eps[k] = (k < cfg->ntypes * cfg->ntypes && cfg->epsilon) ? cfg->epsilon[k] : 1.0;
         p.sig6[k] = (k < cfg->ntypes * cfg->ntypes && cfg->sigma6) ? cfg->sigma6[k] : 1.0;
     }
+    if (const char *z = getenv("PAIRS_HIP_ZC")) ctx->zc_override = atoi(z);
     memset(&ctx->d, 0, sizeof(DevState));
     auto fail = [&](int code) { pairs_hip_destroy(ctx); return code; };
     if (cfg->stream) {
@@ -239,24 +290,28 @@ int pairs_hip_create(pairs_hip_ctx **out, const pairs_hip_md_config *cfg) {
         if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(PAIRS_HIP_ERR_HIP);
         ctx->own_stream = true;
     }
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+        ctx->max_lds = lds;
     DevState &d = ctx->d;
+    d.max_lds = ctx->max_lds;
+    setup_geometry(ctx);
     // the cell grid is fixed by the box; its arrays are sized to fit it
     // (the generated code grows ncells_capacity to twice the need on overflow)
-    d.ncells_cap = 1 << 30;
-    if (setup_geometry(ctx) != 0) return fail(PAIRS_HIP_ERR_ARG);
     d.ncells_cap = std::max(cfg->ncells_capacity, ctx->geom.ncells + 2);
     if (alloc_particle_arrays(ctx, d, cfg->particle_capacity)) return fail(PAIRS_HIP_ERR_HIP);
     if (alloc_neighbor_arrays(ctx, d, cfg->neighbor_capacity)) return fail(PAIRS_HIP_ERR_HIP);
     if (alloc_send_arrays(ctx, d, cfg->send_capacity)) return fail(PAIRS_HIP_ERR_HIP);
-    if (dalloc(ctx, &d.cell_count, (size_t) d.ncells_cap + 1) ||
-        dalloc(ctx, &d.cell_start, (size_t) d.ncells_cap + 1) ||
-        dalloc(ctx, &d.cell_nlocal, (size_t) d.ncells_cap + 1) || dalloc(ctx, &d.counters, C_COUNT) ||
-        dalloc(ctx, &d.thermo, 5 * 4096))
+    if (dalloc(ctx, &d.lcount, (size_t) d.ncells_cap + 1) || dalloc(ctx, &d.gcount, (size_t) d.ncells_cap + 1) ||
+        dalloc(ctx, &d.lstart, (size_t) d.ncells_cap + 1) || dalloc(ctx, &d.gstart, (size_t) d.ncells_cap + 1) ||
+        dalloc(ctx, &d.counters, C_COUNT) || dalloc(ctx, &d.thermo, 5 * 4096))
         return fail(PAIRS_HIP_ERR_HIP);
     launch_fill_i32(ctx->stream, d.counters, 0, C_COUNT);
-    launch_fill_i32(ctx->stream, d.cell_start, 0, d.ncells_cap + 1);
-    if (zero_particle_arrays(ctx, 0, d.cap)) return fail(PAIRS_HIP_ERR_HIP);
+    for (int *q : {d.lcount, d.gcount, d.lstart, d.gstart}) launch_fill_i32(ctx->stream, q, 0, d.ncells_cap + 1);
+    if (zero_particle_arrays(ctx)) return fail(PAIRS_HIP_ERR_HIP);
     d.cur = 0;
+    d.tile_cap = 64;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(PAIRS_HIP_ERR_HIP);
     *out = ctx;
     return 0;
@@ -271,10 +326,39 @@ int pairs_hip_destroy(pairs_hip_ctx *ctx) {
     DevState &d = ctx->d;
     free_particle_arrays(d);
     dfree(d.nl); dfree(d.send_map); dfree(d.send_mult); dfree(d.scratch_i); dfree(d.stage);
-    dfree(d.cell_count); dfree(d.cell_start); dfree(d.cell_nlocal); dfree(d.counters); dfree(d.thermo);
-    dfree(ctx->d_eatom);
+    dfree(d.lcount); dfree(d.gcount); dfree(d.lstart); dfree(d.gstart); dfree(d.counters); dfree(d.thermo);
+    dfree(ctx->d_eatom); dfree(ctx->d_expand);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
+    return 0;
+}
+
+// Download the locals (reference order) needed to rebuild state elsewhere.
+struct HostLocals {
+    int n = 0;
+    std::vector<double> x, v, m;
+    std::vector<int> t, f, u;
+};
+
+static int fetch_locals(pairs_hip_ctx *ctx, HostLocals &h) {
+    DevState &d = ctx->d;
+    if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+    const int n = ctx->h_counters[C_NLOCAL];
+    h.n = n;
+    std::vector<int> phys;
+    if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+    h.x.resize(3 * (size_t) n); h.v.resize(3 * (size_t) n); h.m.resize(n); h.t.resize(n); h.f.resize(n); h.u.resize(n);
+    const int c = d.cur;
+    const double *pos[3] = {d.X[c], d.Y[c], d.Z[c]}, *vel[3] = {d.vx, d.vy, d.vz};
+    for (int k = 0; k < 3; k++) {
+        if (ref_order(ctx, pos[k], n, phys, h.x.data(), 3, k)) return PAIRS_HIP_ERR_HIP;
+        if (ref_order(ctx, vel[k], n, phys, h.v.data(), 3, k)) return PAIRS_HIP_ERR_HIP;
+    }
+    if (ref_order(ctx, (const double *) d.mass, n, phys, h.m.data(), 1, 0) ||
+        ref_order(ctx, (const int *) d.type, n, phys, h.t.data(), 1, 0) ||
+        ref_order(ctx, (const int *) d.flags, n, phys, h.f.data(), 1, 0) ||
+        ref_order(ctx, (const int *) d.uid, n, phys, h.u.data(), 1, 0))
+        return PAIRS_HIP_ERR_HIP;
     return 0;
 }
 
@@ -284,32 +368,16 @@ int pairs_hip_reserve(pairs_hip_ctx *ctx, int particle_capacity, int neighbor_ca
     drop_graphs(ctx);
     DevState &d = ctx->d;
     if (particle_capacity > d.cap) {
-        DevState nd = d;
-        if (alloc_particle_arrays(ctx, nd, particle_capacity)) return PAIRS_HIP_ERR_HIP;
-        const size_t n = (size_t) d.cap;
-        HIP_OK(hipMemcpyAsync(nd.P[0], d.P[0], n * sizeof(double4), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_OK(hipMemcpyAsync(nd.P[1], d.P[1], n * sizeof(double4), hipMemcpyDeviceToDevice, ctx->stream));
-        double *src_d[7] = {d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass};
-        double *dst_d[7] = {nd.vx, nd.vy, nd.vz, nd.fx, nd.fy, nd.fz, nd.mass};
-        for (int k = 0; k < 7; k++)
-            HIP_OK(hipMemcpyAsync(dst_d[k], src_d[k], n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        int *src_i[3] = {d.flags, d.uid, d.shape};
-        int *dst_i[3] = {nd.flags, nd.uid, nd.shape};
-        for (int k = 0; k < 3; k++)
-            HIP_OK(hipMemcpyAsync(dst_i[k], src_i[k], n * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_OK(hipStreamSynchronize(ctx->stream));
+        // grow by re-uploading the locals in reference order (ghosts, cells and
+        // lists are rebuilt by the next rebuild anyway)
+        HostLocals h;
+        if (fetch_locals(ctx, h)) return PAIRS_HIP_ERR_HIP;
         free_particle_arrays(d);
-        int *nl = d.nl;
-        d = nd;
-        d.nl = nl;
-        const int old = (int) n;
-        launch_fill_i32(ctx->stream, d.flags + old, 0, d.cap - old);
-        launch_fill_i32(ctx->stream, d.uid + old, 0, d.cap - old);
-        launch_fill_i32(ctx->stream, d.shape + old, SHAPE_POINT_MASS, d.cap - old);
-        launch_fill_f64(ctx->stream, d.mass + old, 1.0, d.cap - old);
-        launch_fill_i32(ctx->stream, d.numneighs, 0, d.cap);
-        int ncap = std::max(neighbor_capacity, d.ncap);
-        if (alloc_neighbor_arrays(ctx, d, ncap)) return PAIRS_HIP_ERR_HIP;
+        if (alloc_particle_arrays(ctx, d, particle_capacity)) return PAIRS_HIP_ERR_HIP;
+        if (zero_particle_arrays(ctx)) return PAIRS_HIP_ERR_HIP;
+        if (alloc_neighbor_arrays(ctx, d, std::max(neighbor_capacity, d.ncap))) return PAIRS_HIP_ERR_HIP;
+        int r = pairs_hip_upload(ctx, h.n, h.x.data(), h.v.data(), h.m.data(), h.t.data(), h.f.data(), h.u.data());
+        if (r) return r;
     } else if (neighbor_capacity > d.ncap) {
         if (alloc_neighbor_arrays(ctx, d, neighbor_capacity)) return PAIRS_HIP_ERR_HIP;
         launch_fill_i32(ctx->stream, d.numneighs, 0, d.cap);
@@ -318,6 +386,7 @@ int pairs_hip_reserve(pairs_hip_ctx *ctx, int particle_capacity, int neighbor_ca
         if (alloc_send_arrays(ctx, d, send_capacity)) return PAIRS_HIP_ERR_HIP;
         // the ghost maps are gone with the old arrays: a rebuild must follow
         launch_set_counters(ctx->stream, d.counters, C_NSEND_ALL, 1, 0);
+        launch_set_counters(ctx->stream, d.counters, C_NGHOST, 1, 0);
     }
     CHECK_LAUNCH();
     HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -333,31 +402,37 @@ int pairs_hip_upload(pairs_hip_ctx *ctx, int n, const double *x, const double *v
         return PAIRS_HIP_OVERFLOW_PARTICLES;
     }
     drop_graphs(ctx);
-    std::vector<double4> hp(n);
-    std::vector<double> hv[3] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n)};
+    std::vector<double> hx[3], hv[3];
+    for (int k = 0; k < 3; k++) { hx[k].resize(n); hv[k].resize(n); }
     std::vector<double> hm(n);
-    std::vector<int> hf(n), hu(n), hs(n, SHAPE_POINT_MASS);
+    std::vector<int> ht(n), hf(n), hu(n), hs(n, SHAPE_POINT_MASS);
     for (int i = 0; i < n; i++) {
         const int t = type ? type[i] : 0;
         if (t < 0 || t >= ctx->cfg.ntypes) { ctx->err = "type out of range"; return PAIRS_HIP_ERR_ARG; }
-        hp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], type_bits(t));
-        for (int k = 0; k < 3; k++) hv[k][i] = v[3 * i + k];
+        for (int k = 0; k < 3; k++) { hx[k][i] = x[3 * i + k]; hv[k][i] = v[3 * i + k]; }
         hm[i] = m ? m[i] : 1.0;
+        ht[i] = t;
         hf[i] = flags ? flags[i] : 0;
         hu[i] = uid ? uid[i] : 0;
     }
     d.cur = 0;
     if (n > 0) {
-        HIP_OK(hipMemcpyAsync(d.P[0], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
-        HIP_OK(hipMemcpyAsync(d.P[1], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
+        for (int b = 0; b < 2; b++) {
+            HIP_OK(hipMemcpyAsync(d.X[b], hx[0].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            HIP_OK(hipMemcpyAsync(d.Y[b], hx[1].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            HIP_OK(hipMemcpyAsync(d.Z[b], hx[2].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
         HIP_OK(hipMemcpyAsync(d.vx, hv[0].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         HIP_OK(hipMemcpyAsync(d.vy, hv[1].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         HIP_OK(hipMemcpyAsync(d.vz, hv[2].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         HIP_OK(hipMemcpyAsync(d.mass, hm.data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIP_OK(hipMemcpyAsync(d.type, ht.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         HIP_OK(hipMemcpyAsync(d.flags, hf.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         HIP_OK(hipMemcpyAsync(d.uid, hu.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
         HIP_OK(hipMemcpyAsync(d.shape, hs.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     }
+    launch_identity(ctx->stream, d.ref, d.cap);
+    launch_identity(ctx->stream, d.phys, d.cap);
     launch_fill_f64(ctx->stream, d.fx, 0.0, d.cap);
     launch_fill_f64(ctx->stream, d.fy, 0.0, d.cap);
     launch_fill_f64(ctx->stream, d.fz, 0.0, d.cap);
@@ -368,6 +443,7 @@ int pairs_hip_upload(pairs_hip_ctx *ctx, int n, const double *x, const double *v
     d.nlocal = n;
     ctx->nghost = 0;
     ctx->nsend_all = 0;
+    setup_segments(ctx, ctx->zc_override);
     return 0;
 }
 
@@ -428,37 +504,37 @@ int pairs_hip_copper_fcc_lattice(pairs_hip_ctx *ctx, int nx, int ny, int nz, dou
     return r ? r : n;
 }
 
-static int download_raw(pairs_hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
-    if (bytes == 0) return 0;
-    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(hipStreamSynchronize(ctx->stream));
-    return 0;
-}
-
 // pairs::adjust_thermo (runtime/thermo.hpp:53-97), on the host in the same order
 int pairs_hip_adjust_thermo(pairs_hip_ctx *ctx, double temp) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     DevState &d = ctx->d;
-    const int n = d.nlocal;
+    HostLocals h;
+    if (fetch_locals(ctx, h)) return PAIRS_HIP_ERR_HIP;
+    const int n = h.n;
     if (n <= 1) return PAIRS_HIP_ERR_STATE;
-    std::vector<double> v[3] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n)}, m(n);
-    if (download_raw(ctx, v[0].data(), d.vx, n * sizeof(double)) || download_raw(ctx, v[1].data(), d.vy, n * sizeof(double)) ||
-        download_raw(ctx, v[2].data(), d.vz, n * sizeof(double)) || download_raw(ctx, m.data(), d.mass, n * sizeof(double)))
-        return PAIRS_HIP_ERR_HIP;
+    std::vector<double> &v = h.v;
     double vxt = 0.0, vyt = 0.0, vzt = 0.0;
-    for (int i = 0; i < n; i++) { vxt += v[0][i]; vyt += v[1][i]; vzt += v[2][i]; }
+    for (int i = 0; i < n; i++) { vxt += v[3 * i]; vyt += v[3 * i + 1]; vzt += v[3 * i + 2]; }
     vxt /= n; vyt /= n; vzt /= n;
-    for (int i = 0; i < n; i++) { v[0][i] -= vxt; v[1][i] -= vyt; v[2][i] -= vzt; }
+    for (int i = 0; i < n; i++) { v[3 * i] -= vxt; v[3 * i + 1] -= vyt; v[3 * i + 2] -= vzt; }
     const double dof = (n * 3 - 3);
     const double t_scale = 1.0 / dof;
     double t = 0.0;
-    for (int i = 0; i < n; i++) t += m[i] * (v[0][i] * v[0][i] + v[1][i] * v[1][i] + v[2][i] * v[2][i]);
+    for (int i = 0; i < n; i++)
+        t += h.m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
     t = t * t_scale;
     const double factor = std::sqrt(temp / t);
-    for (int i = 0; i < n; i++) { v[0][i] *= factor; v[1][i] *= factor; v[2][i] *= factor; }
-    HIP_OK(hipMemcpyAsync(d.vx, v[0].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipMemcpyAsync(d.vy, v[1].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIP_OK(hipMemcpyAsync(d.vz, v[2].data(), n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    for (int i = 0; i < 3 * n; i++) v[i] *= factor;
+    std::vector<int> phys;
+    if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+    std::vector<double> comp[3];
+    for (int k = 0; k < 3; k++) {
+        comp[k].resize(d.cap);
+        double *dev = k == 0 ? d.vx : (k == 1 ? d.vy : d.vz);
+        if (download_raw(ctx, comp[k].data(), dev, d.cap * sizeof(double))) return PAIRS_HIP_ERR_HIP;
+        for (int r = 0; r < n; r++) comp[k][phys[r]] = v[3 * r + k];
+        HIP_OK(hipMemcpyAsync(dev, comp[k].data(), d.cap * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
     HIP_OK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -493,76 +569,84 @@ int pairs_hip_download(pairs_hip_ctx *ctx, int what, void *host, int count) {
     const int nlocal = ctx->h_counters[C_NLOCAL], nall = nlocal + ctx->h_counters[C_NGHOST];
     const int nsend = ctx->h_counters[C_NSEND_ALL];
     auto lim = [&](int avail) { return std::min(count, avail); };
+    std::vector<int> phys;
+    const int c = d.cur;
     switch (what) {
         case PAIRS_HIP_POSITION:
-        case PAIRS_HIP_TYPE: {
-            const int n = lim(nall);
-            std::vector<double4> p(n);
-            if (download_raw(ctx, p.data(), d.P[d.cur], n * sizeof(double4))) return PAIRS_HIP_ERR_HIP;
-            for (int i = 0; i < n; i++) {
-                if (what == PAIRS_HIP_POSITION) {
-                    double *o = (double *) host + 3 * i;
-                    o[0] = p[i].x; o[1] = p[i].y; o[2] = p[i].z;
-                } else {
-                    ((int *) host)[i] = pos_type(p[i]);
-                }
-            }
-            return n;
-        }
         case PAIRS_HIP_VELOCITY:
         case PAIRS_HIP_FORCE: {
             const int n = lim(what == PAIRS_HIP_FORCE ? nlocal : nall);
-            double *src[3] = {what == PAIRS_HIP_FORCE ? d.fx : d.vx, what == PAIRS_HIP_FORCE ? d.fy : d.vy,
-                              what == PAIRS_HIP_FORCE ? d.fz : d.vz};
-            std::vector<double> c(n);
-            for (int k = 0; k < 3; k++) {
-                if (download_raw(ctx, c.data(), src[k], n * sizeof(double))) return PAIRS_HIP_ERR_HIP;
-                for (int i = 0; i < n; i++) ((double *) host)[3 * i + k] = c[i];
-            }
+            if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+            const double *src[3];
+            if (what == PAIRS_HIP_POSITION) { src[0] = d.X[c]; src[1] = d.Y[c]; src[2] = d.Z[c]; }
+            else if (what == PAIRS_HIP_VELOCITY) { src[0] = d.vx; src[1] = d.vy; src[2] = d.vz; }
+            else { src[0] = d.fx; src[1] = d.fy; src[2] = d.fz; }
+            for (int k = 0; k < 3; k++)
+                if (ref_order(ctx, src[k], n, phys, (double *) host, 3, k)) return PAIRS_HIP_ERR_HIP;
             return n;
         }
         case PAIRS_HIP_MASS: {
             const int n = lim(nall);
-            return download_raw(ctx, host, d.mass, n * sizeof(double)) ? PAIRS_HIP_ERR_HIP : n;
+            if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+            return ref_order(ctx, (const double *) d.mass, n, phys, (double *) host, 1, 0) ? PAIRS_HIP_ERR_HIP : n;
         }
-        case PAIRS_HIP_FLAGS: {
+        case PAIRS_HIP_TYPE:
+        case PAIRS_HIP_FLAGS:
+        case PAIRS_HIP_UID:
+        case PAIRS_HIP_PARTICLE_CELL: {
             const int n = lim(nall);
-            return download_raw(ctx, host, d.flags, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
-        }
-        case PAIRS_HIP_UID: {
-            const int n = lim(nall);
-            return download_raw(ctx, host, d.uid, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+            if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+            const int *src = what == PAIRS_HIP_TYPE ? d.type
+                             : what == PAIRS_HIP_FLAGS ? d.flags
+                             : what == PAIRS_HIP_UID ? d.uid : d.particle_cell;
+            return ref_order(ctx, src, n, phys, (int *) host, 1, 0) ? PAIRS_HIP_ERR_HIP : n;
         }
         case PAIRS_HIP_NUMNEIGHS: {
             const int n = lim(nlocal);
-            return download_raw(ctx, host, d.numneighs, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+            if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+            return ref_order(ctx, (const int *) d.numneighs, n, phys, (int *) host, 1, 0) ? PAIRS_HIP_ERR_HIP : n;
         }
         case PAIRS_HIP_NEIGHBORLISTS: {
             const int n = lim(nlocal);
             if (n == 0) return 0;
-            std::vector<int> rows((size_t) d.ncap * n);
-            HIP_OK(hipMemcpy2DAsync(rows.data(), n * sizeof(int), d.nl, (size_t) d.nl_stride * sizeof(int),
-                                    n * sizeof(int), d.ncap, hipMemcpyDeviceToHost, ctx->stream));
-            HIP_OK(hipStreamSynchronize(ctx->stream));
+            const size_t need = (size_t) nlocal * d.ncap;
+            if (ctx->expand_cap < need) {
+                dfree(ctx->d_expand);
+                ctx->d_expand = nullptr;
+                if (dalloc(ctx, &ctx->d_expand, need)) return PAIRS_HIP_ERR_HIP;
+                ctx->expand_cap = need;
+            }
+            launch_fill_i32(ctx->stream, ctx->d_expand, -1, (int) need);
+            launch_expand_neighbors(ctx->stream, d, ctx->geom, ctx->d_expand, d.ncap);
+            CHECK_LAUNCH();
+            std::vector<int> rows(need);
+            if (download_raw(ctx, rows.data(), ctx->d_expand, need * sizeof(int))) return PAIRS_HIP_ERR_HIP;
+            if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
             int *o = (int *) host;
-            for (int i = 0; i < n; i++)
-                for (int k = 0; k < d.ncap; k++) o[(size_t) i * d.ncap + k] = rows[(size_t) k * n + i];
+            for (int r = 0; r < n; r++)
+                memcpy(o + (size_t) r * d.ncap, rows.data() + (size_t) phys[r] * d.ncap, d.ncap * sizeof(int));
             return n;
         }
-        case PAIRS_HIP_PARTICLE_CELL: {
-            const int n = lim(nall);
-            return download_raw(ctx, host, d.particle_cell, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
-        }
-        case PAIRS_HIP_CELL_SIZES: {
-            const int nc = lim(ctx->geom.ncells);
-            std::vector<int> st(ctx->geom.ncells + 1);
-            if (download_raw(ctx, st.data(), d.cell_start, (ctx->geom.ncells + 1) * sizeof(int))) return PAIRS_HIP_ERR_HIP;
-            for (int c = 0; c < nc; c++) ((int *) host)[c] = st[c + 1] - st[c];
-            return nc;
-        }
+        case PAIRS_HIP_CELL_SIZES:
         case PAIRS_HIP_CELL_PARTICLES: {
-            const int n = lim(nall);
-            return download_raw(ctx, host, d.sorted_idx, n * sizeof(int)) ? PAIRS_HIP_ERR_HIP : n;
+            const int nc = ctx->geom.ncells;
+            std::vector<int> ls(nc + 1), gs(nc + 1), ref(nall);
+            if (download_raw(ctx, ls.data(), d.lstart, (nc + 1) * sizeof(int)) ||
+                download_raw(ctx, gs.data(), d.gstart, (nc + 1) * sizeof(int)) ||
+                download_raw(ctx, ref.data(), d.ref, nall * sizeof(int)))
+                return PAIRS_HIP_ERR_HIP;
+            if (what == PAIRS_HIP_CELL_SIZES) {
+                const int m = lim(nc);
+                for (int cc = 0; cc < m; cc++) ((int *) host)[cc] = (ls[cc + 1] - ls[cc]) + (gs[cc + 1] - gs[cc]);
+                return m;
+            }
+            const int m = lim(nall);
+            int k = 0;
+            for (int cc = 0; cc < nc && k < m; cc++) {
+                for (int p = ls[cc]; p < ls[cc + 1] && k < m; p++) ((int *) host)[k++] = ref[p];
+                for (int p = nlocal + gs[cc]; p < nlocal + gs[cc + 1] && k < m; p++) ((int *) host)[k++] = ref[p];
+            }
+            return k;
         }
         case PAIRS_HIP_SEND_MAP: {
             const int n = lim(nsend);
@@ -579,8 +663,9 @@ int pairs_hip_download(pairs_hip_ctx *ctx, int what, void *host, int count) {
         }
         case PAIRS_HIP_ATOM_ENERGY: {
             const int n = lim(std::min(nlocal, ctx->eatom_cap));
-            if (!ctx->d_eatom) return 0;
-            return download_raw(ctx, host, ctx->d_eatom, n * sizeof(double)) ? PAIRS_HIP_ERR_HIP : n;
+            if (!ctx->d_eatom || n == 0) return 0;
+            if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+            return ref_order(ctx, (const double *) ctx->d_eatom, n, phys, (double *) host, 1, 0) ? PAIRS_HIP_ERR_HIP : n;
         }
     }
     return PAIRS_HIP_ERR_ARG;
@@ -599,12 +684,14 @@ static int finish_module(pairs_hip_ctx *ctx) {
 
 int pairs_hip_exchange(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
+    drop_graphs(ctx);
     launch_exchange(ctx->stream, ctx->d, ctx->geom, ctx->cfg.periodic);
     return finish_module(ctx);
 }
 
 int pairs_hip_borders(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
+    drop_graphs(ctx);
     launch_borders(ctx->stream, ctx->d, ctx->geom, ctx->cfg.periodic);
     return finish_module(ctx);
 }
@@ -618,26 +705,53 @@ int pairs_hip_sync_ghosts(pairs_hip_ctx *ctx) {
 
 int pairs_hip_bin_cells(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
-    if (!ctx->cells_ready) {
-        int r = setup_geometry(ctx);
-        if (r) return r;
-    }
-    launch_bin(ctx->stream, ctx->d, ctx->geom);
+    drop_graphs(ctx);
+    launch_bin_sort(ctx->stream, ctx->d, ctx->geom);
     return finish_module(ctx);
+}
+
+// the build kernel's LDS: tile (28 B/entry) + candidates + list transposition
+static void set_tile_cap(pairs_hip_ctx *ctx) {
+    const int tmax = ctx->h_counters[C_TILE_MAX];
+    int cap = tmax + tmax / 8 + 64;
+    cap = (cap + 63) / 64 * 64;
+    const int lim = ctx->max_lds / 32;  // force kernels stage up to 32 B per tile entry
+    ctx->d.tile_cap = std::min(std::max(cap, ctx->d.tile_cap), lim);
+}
+
+static int build_neighbors(pairs_hip_ctx *ctx) {
+    const double rl = ctx->cfg.cell_spacing;
+    for (int attempt = 0; attempt < 64; attempt++) {
+        launch_set_counters(ctx->stream, ctx->d.counters, C_TILE_MAX, 1, 0);
+        launch_build_neighbors(ctx->stream, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
+        CHECK_LAUNCH();
+        if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+        if (ctx->h_counters[C_OVF] == OVF_TILE) {
+            // tile too large for one workgroup: fewer cells per segment
+            launch_set_counters(ctx->stream, ctx->d.counters, C_OVF, 1, 0);
+            if (ctx->geom.zc <= 1) { ctx->err = "cell tile exceeds the workgroup's LDS"; return PAIRS_HIP_ERR_STATE; }
+            drop_graphs(ctx);
+            setup_segments(ctx, ctx->geom.zc - 1);
+            continue;
+        }
+        set_tile_cap(ctx);
+        return take_overflow(ctx);
+    }
+    ctx->err = "neighbour build did not converge";
+    return PAIRS_HIP_ERR_STATE;
 }
 
 int pairs_hip_build_neighbors(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
-    const double rl = ctx->cfg.cell_spacing;
-    launch_build_neighbors(ctx->stream, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
-    return finish_module(ctx);
+    drop_graphs(ctx);
+    return build_neighbors(ctx);
 }
 
 int pairs_hip_lj(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     DevState &d = ctx->d;
-    if (ctx->cfg.half) launch_lj_half(ctx->stream, d, ctx->prm, d.cur, 0);
-    else launch_lj_full(ctx->stream, d, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    if (ctx->cfg.half) launch_lj_half(ctx->stream, d, ctx->geom, ctx->prm, d.cur, 0);
+    else launch_lj_full(ctx->stream, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
     CHECK_LAUNCH();
     return 0;
 }
@@ -658,7 +772,7 @@ int pairs_hip_vv_final(pairs_hip_ctx *ctx) {
 }
 
 static int ensure_eatom(pairs_hip_ctx *ctx) {
-    if (ctx->eatom_cap < ctx->d.nlocal) {
+    if (ctx->eatom_cap < ctx->d.cap) {
         dfree(ctx->d_eatom);
         ctx->d_eatom = nullptr;
         if (dalloc(ctx, &ctx->d_eatom, ctx->d.cap)) return PAIRS_HIP_ERR_HIP;
@@ -671,8 +785,8 @@ int pairs_hip_energy(pairs_hip_ctx *ctx, double *pe, double *virial) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
     double out[5];
-    launch_energy(ctx->stream, ctx->d, ctx->prm, ctx->cfg.half, ctx->d_eatom);
-    launch_thermo_reduce(ctx->stream, ctx->d, 2, ctx->geom, ctx->d.thermo);
+    launch_energy(ctx->stream, ctx->d, ctx->geom, ctx->prm, ctx->cfg.half, ctx->d_eatom);
+    launch_thermo_reduce(ctx->stream, ctx->d, 0, 2, ctx->geom, ctx->d.thermo);
     CHECK_LAUNCH();
     if (download_raw(ctx, out, ctx->d.thermo, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;
     if (pe) *pe = out[2];
@@ -683,9 +797,9 @@ int pairs_hip_energy(pairs_hip_ctx *ctx, double *pe, double *virial) {
 int pairs_hip_thermo(pairs_hip_ctx *ctx, double out[5]) {
     if (!ctx || !out) return PAIRS_HIP_ERR_ARG;
     if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
-    launch_energy(ctx->stream, ctx->d, ctx->prm, ctx->cfg.half, ctx->d_eatom);
+    launch_energy(ctx->stream, ctx->d, ctx->geom, ctx->prm, ctx->cfg.half, ctx->d_eatom);
     launch_ke_partial(ctx->stream, ctx->d);
-    launch_thermo_reduce(ctx->stream, ctx->d, 3, ctx->geom, ctx->d.thermo);
+    launch_thermo_reduce(ctx->stream, ctx->d, (ctx->d.nlocal + 255) / 256, 3, ctx->geom, ctx->d.thermo);
     CHECK_LAUNCH();
     return download_raw(ctx, out, ctx->d.thermo, 5 * sizeof(double));
 }
@@ -701,22 +815,18 @@ static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 |
 // exchange + borders + cells + neighbour lists, retried with grown capacities
 // on overflow like the generated resize loops (transformations/modules.py:159-205)
 static int rebuild(pairs_hip_ctx *ctx) {
-    if (!ctx->cells_ready) {
-        int r = setup_geometry(ctx);
-        if (r) return r;
-    }
     for (int attempt = 0; attempt < 8; attempt++) {
-        DevState &d = ctx->d;
         hipStream_t s = ctx->stream;
-        const double rl = ctx->cfg.cell_spacing;
-        launch_exchange(s, d, ctx->geom, ctx->cfg.periodic);
-        launch_borders(s, d, ctx->geom, ctx->cfg.periodic);
-        launch_bin(s, d, ctx->geom);
-        launch_build_neighbors(s, d, ctx->geom, ctx->cfg.half, rl * rl);
+        launch_exchange(s, ctx->d, ctx->geom, ctx->cfg.periodic);
+        launch_borders(s, ctx->d, ctx->geom, ctx->cfg.periodic);
+        launch_bin_sort(s, ctx->d, ctx->geom);
         CHECK_LAUNCH();
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
-        const int code = take_overflow(ctx);
-        if (code == 0) return 0;
+        int code = take_overflow(ctx);
+        if (code == 0) {
+            code = build_neighbors(ctx);
+            if (code <= 0) return code;
+        }
         int pc = 0, nc = 0, sc = 0;
         if (code == PAIRS_HIP_OVERFLOW_PARTICLES) pc = 2 * ctx->req[code];
         if (code == PAIRS_HIP_OVERFLOW_NEIGHBORS) nc = 2 * ctx->req[code];
@@ -734,21 +844,22 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
     hipStream_t s = ctx->stream;
     if (ts > 0) launch_vv_initial(s, d, ctx->prm);
     if (every(ts, ctx->cfg.reneighbor_every)) {
+        drop_graphs(ctx);
         int r = rebuild(ctx);
         if (r) return r;
     } else {
         launch_sync_inplace(s, d, ctx->geom, ctx->ghost_grid, 0);
     }
-    if (ctx->cfg.half) launch_lj_half(s, d, ctx->prm, d.cur, 0);
-    else launch_lj_full(s, d, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    if (ctx->cfg.half) launch_lj_half(s, d, ctx->geom, ctx->prm, d.cur, 0);
+    else launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
     if (ts > 0) launch_vv_post(s, d, ctx->prm, d.cur, d.cur, 1, 0, 0);
     CHECK_LAUNCH();
     return 0;
 }
 
 // one non-rebuild step of the fused double-buffered form:
-//   sync ghosts of ts into P[cur] (stale multi-hop values from P[other])
-//   force(ts) + final_integrate(ts) + initial_integrate(ts+1) -> P[other]
+//   sync ghosts of ts into buffer cur (stale multi-hop values from the other)
+//   force(ts) + final_integrate(ts) + initial_integrate(ts+1) -> other buffer
 static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int reduce, bool timed) {
     DevState &d = ctx->d;
     hipStream_t s = ctx->stream;
@@ -756,11 +867,11 @@ static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int
     launch_sync_pp(s, d, ctx->geom, cur, other, ctx->ghost_grid);
     if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
     if (ctx->cfg.half) {
-        launch_lj_half(s, d, ctx->prm, cur, reduce);
+        launch_lj_half(s, d, ctx->geom, ctx->prm, cur, reduce);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
         launch_vv_post(s, d, ctx->prm, cur, other, vv2, vv1, reduce);
     } else {
-        launch_lj_full(s, d, ctx->prm, cur, other, 1, vv2, vv1, reduce);
+        launch_lj_full(s, d, ctx->geom, ctx->prm, cur, other, 1, vv2, vv1, reduce);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
     }
 }
@@ -798,7 +909,8 @@ int pairs_hip_force_kernel_time(pairs_hip_ctx *ctx, double *avg_ms, int *launche
 }
 
 static int graph_step(pairs_hip_ctx *ctx, int cur) {
-    if (!ctx->step_graph[cur] || ctx->graph_ghost_grid[cur] < ctx->ghost_grid) {
+    if (!ctx->step_graph[cur] || ctx->graph_ghost_grid[cur] < ctx->ghost_grid ||
+        ctx->graph_tile_cap[cur] != ctx->d.tile_cap) {
         if (ctx->step_graph[cur]) hipGraphExecDestroy(ctx->step_graph[cur]);
         ctx->step_graph[cur] = nullptr;
         // capture with headroom so that later rebuilds with a few more ghosts reuse it
@@ -811,6 +923,7 @@ static int graph_step(pairs_hip_ctx *ctx, int cur) {
         hipError_t e = hipGraphInstantiate(&ctx->step_graph[cur], g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         ctx->graph_ghost_grid[cur] = ctx->ghost_grid;
+        ctx->graph_tile_cap[cur] = ctx->d.tile_cap;
         ctx->ghost_grid = saved;
         if (e != hipSuccess) {
             ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
@@ -852,20 +965,21 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         if (!R && prev_in_other) {
             launch_fused_step(ctx, cur, vv2, vv1, T ? 1 : 0, timed);
         } else {
-            // ghosts already current in P[cur]: force + integrators only
+            // ghosts already current in buffer cur: force + integrators only
             if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
             if (ctx->cfg.half) {
-                launch_lj_half(s, d, ctx->prm, cur, T ? 1 : 0);
+                launch_lj_half(s, d, ctx->geom, ctx->prm, cur, T ? 1 : 0);
                 if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
                 launch_vv_post(s, d, ctx->prm, cur, other, vv2, vv1, T ? 1 : 0);
             } else {
-                launch_lj_full(s, d, ctx->prm, cur, other, 1, vv2, vv1, T ? 1 : 0);
+                launch_lj_full(s, d, ctx->geom, ctx->prm, cur, other, 1, vv2, vv1, T ? 1 : 0);
                 if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
             }
         }
         if (timed) ctx->ev_used++;
         if (T && nthermo < 4096) {
-            launch_thermo_reduce(s, d, 3, ctx->geom, d.thermo + 5 * nthermo);
+            const int nb_ke = ctx->cfg.half ? (d.nlocal + 255) / 256 : ctx->geom.nsegs;
+            launch_thermo_reduce(s, d, nb_ke, 3, ctx->geom, d.thermo + 5 * nthermo);
             nthermo++;
         }
         CHECK_LAUNCH();

@@ -237,3 +237,34 @@ def main():
 
 if __name__ == "__main__":
     main()
+
+
+# ---------------------------------------------------------------- DEM
+DEM_STEPS = 200
+
+
+def specialise_dem(src, steps):
+    subs = [
+        (r"^domainSize_SI = \[[^\]]*\]$", "domainSize_SI = [0.04, 0.015, 0.04]"),
+        (r"^timeSteps = \d+$", f"timeSteps = {steps}"),
+        (r"^visSpacing = \d+$", "visSpacing = 1000000"),
+    ]
+    for pat, rep in subs:
+        src, n = re.subn(pat, rep, src, count=1, flags=re.M)
+        assert n == 1, f"pattern not found in dem.py: {pat}"
+    return src
+
+
+def generate_dem(workdir, steps):
+    src = open(os.path.join(REF, "examples/dem.py")).read()
+    script = specialise_dem(src, steps)
+    os.makedirs(os.path.join(workdir, "data"), exist_ok=True)
+    os.makedirs(os.path.join(workdir, "output"), exist_ok=True)
+    shutil.copy(os.path.join(REF, "data/planes.input"), os.path.join(workdir, "data/planes.input"))
+    path = os.path.join(workdir, "dem_golden.py")
+    with open(path, "w") as f:
+        f.write(script)
+    env = dict(os.environ)
+    env["PYTHONPATH"] = os.path.join(REF, "src")
+    subprocess.run([sys.executable, path, "cpu"], cwd=workdir, env=env, check=True, stdout=subprocess.DEVNULL)
+    return os.path.join(workdir, "dem.cpp")
