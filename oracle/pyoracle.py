@@ -232,3 +232,158 @@ def fcc_system(n, rho=0.8442, temp=1.44, *, half=False, cap=None, ntypes=1, **kw
     sim.adjust_thermo(temp)
     sim.setup_cells()
     return sim
+
+
+# ---------------------------------------------------------------- DEM oracle
+
+DEM_X, DEM_V, DEM_W, DEM_F, DEM_T, DEM_R, DEM_M, DEM_Q, DEM_ROT, DEM_UID, DEM_FLAGS, DEM_TYPE, \
+    DEM_NCONT, DEM_CLIST, DEM_STICK, DEM_TSD, DEM_IVM, DEM_SHAPE, DEM_NORMAL, DEM_INVI, DEM_CUSED = range(21)
+
+
+def _dem_lib():
+    L = lib()
+    if getattr(L, "_dem_ready", False):
+        return L
+    L.odem_create.restype = ctypes.c_void_p
+    L.odem_create.argtypes = [ctypes.c_int] * 5 + [_D, _D, _I] + [ctypes.c_double] * 8 + [ctypes.c_int, _D, _D]
+    L.odem_destroy.argtypes = [ctypes.c_void_p]
+    L.odem_sc_grid.restype = ctypes.c_int
+    L.odem_sc_grid.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 9 + [ctypes.c_int]
+    L.odem_add_halfspace.restype = ctypes.c_int
+    L.odem_add_halfspace.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _D,
+                                     ctypes.c_int]
+    for fn in ("odem_setup", "odem_setup_cells"):
+        getattr(L, fn).argtypes = [ctypes.c_void_p]
+    L.odem_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.odem_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.odem_ptr.restype = ctypes.c_void_p
+    L.odem_ptr.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.odem_int.restype = ctypes.c_int
+    L.odem_int.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.odem_add_sphere.restype = ctypes.c_int
+    L.odem_add_sphere.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                  _D, _D]
+    L.odem_mt_next.restype = ctypes.c_uint32
+    L.odem_mt_next.argtypes = [ctypes.c_void_p]
+    L._dem_ready = True
+    return L
+
+
+# examples/dem.py:113-140 (the reference's own DEM configuration)
+DEM_REF = dict(diameter=0.0029, gravity=9.81, rho_f=1000.0, rho_p=2550.0, spacing=0.005, v0=1.0, dt=5e-5,
+               mu=0.5, restitution=0.1, t_c=5e-4, poisson=0.22)
+
+
+def dem_params(cfg=DEM_REF):
+    kappa = 2.0 * (1.0 - cfg["poisson"]) / (2.0 - cfg["poisson"])
+    return dict(kappa=kappa, ln_e=math.log(cfg["restitution"]), cell=1.01 * (cfg["diameter"] * 1.1))
+
+
+class OracleDEM:
+    """The CPU oracle's DEM system (spheres + half-spaces, contact history)."""
+
+    def __init__(self, domain, periodic=(1, 1, 0), cap=4096, ccap=20, cellcap=64, send_cap=4096,
+                 ncells_cap=200000, cfg=DEM_REF, ntypes=1, mu_s=None, mu_d=None):
+        L = _dem_lib()
+        p = dem_params(cfg)
+        lo = (ctypes.c_double * 3)(0.0, 0.0, 0.0)
+        hi = (ctypes.c_double * 3)(*domain)
+        per = (ctypes.c_int * 3)(*periodic)
+        nt2 = ntypes * ntypes
+        ms = (ctypes.c_double * nt2)(*(mu_s if mu_s is not None else [0.0] * nt2))
+        md = (ctypes.c_double * nt2)(*(mu_d if mu_d is not None else [cfg["mu"]] * nt2))
+        self.cfg, self.cell, self.ntypes = cfg, p["cell"], ntypes
+        self.h = L.odem_create(cap, ccap, cellcap, send_cap, ncells_cap, lo, hi, per, p["cell"], cfg["dt"],
+                               cfg["rho_p"], cfg["rho_f"], cfg["gravity"], p["kappa"], p["ln_e"], cfg["t_c"],
+                               ntypes, ms, md)
+        self.L = L
+        self.cap, self.ccap = cap, ccap
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.odem_destroy(self.h)
+            self.h = None
+
+    def sc_grid(self, domain, spacing=None, diameter=None, v0=None):
+        c = self.cfg
+        d = diameter if diameter is not None else c["diameter"]
+        return self.L.odem_sc_grid(self.h, domain[0], domain[1], domain[2], spacing or c["spacing"], d, d * 0.9,
+                                   d * 1.1, c["v0"] if v0 is None else v0, c["rho_p"], self.ntypes)
+
+    def add_halfspace(self, uid, pos, normal, flags=13, type_=0, mass=1.0):
+        p = (ctypes.c_double * 3)(*pos)
+        n = (ctypes.c_double * 3)(*normal)
+        return self.L.odem_add_halfspace(self.h, uid, type_, mass, p, n, flags)
+
+    def add_sphere(self, uid, radius, pos, vel, density=None, type_=0):
+        p = (ctypes.c_double * 3)(*pos)
+        v = (ctypes.c_double * 3)(*vel)
+        return self.L.odem_add_sphere(self.h, uid, type_, radius, self.cfg["rho_p"] if density is None else density,
+                                      p, v)
+
+    def add_reference_planes(self):
+        """data/planes.input of the reference: z = 0 facing up, z = 0.2 facing down."""
+        self.add_halfspace(100000, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+        self.add_halfspace(100001, (0.8, 0.015, 0.2), (0.0, 0.0, -1.0))
+
+    def setup(self):
+        self.L.odem_setup(self.h)
+        self.L.odem_setup_cells(self.h)
+
+    def step(self, ts):
+        self.L.odem_step(self.h, ts)
+
+    def run(self, a, b):
+        self.L.odem_run(self.h, a, b)
+
+    @property
+    def nlocal(self):
+        return self.L.odem_int(self.h, 0)
+
+    @property
+    def nghost(self):
+        return self.L.odem_int(self.h, 1)
+
+    @property
+    def overflow(self):
+        return self.L.odem_int(self.h, 3)
+
+    def _arr(self, which, ctype, n, cols=None):
+        ptr = self.L.odem_ptr(self.h, which)
+        a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n * (cols or 1),)).copy()
+        return a.reshape(n, cols) if cols else a
+
+    def get(self, name, ghosts=False):
+        n = self.nlocal + (self.nghost if ghosts else 0)
+        spec = {"position": (DEM_X, ctypes.c_double, 3), "linear_velocity": (DEM_V, ctypes.c_double, 3),
+                "angular_velocity": (DEM_W, ctypes.c_double, 3), "force": (DEM_F, ctypes.c_double, 3),
+                "torque": (DEM_T, ctypes.c_double, 3), "radius": (DEM_R, ctypes.c_double, None),
+                "mass": (DEM_M, ctypes.c_double, None), "rotation_quat": (DEM_Q, ctypes.c_double, 4),
+                "rotation_matrix": (DEM_ROT, ctypes.c_double, 9), "uid": (DEM_UID, ctypes.c_int, None),
+                "flags": (DEM_FLAGS, ctypes.c_int, None), "type": (DEM_TYPE, ctypes.c_int, None),
+                "shape": (DEM_SHAPE, ctypes.c_int, None), "normal": (DEM_NORMAL, ctypes.c_double, 3),
+                "inv_inertia": (DEM_INVI, ctypes.c_double, 9), "num_contacts": (DEM_NCONT, ctypes.c_int, None)}
+        which, ct, cols = spec[name]
+        return self._arr(which, ct, n, cols)
+
+    def contacts(self):
+        """per local particle: list of (uid, is_sticking, tsd[3], impact) in slot order"""
+        n, c = self.nlocal, self.ccap
+        nc = self._arr(DEM_NCONT, ctypes.c_int, n)
+        cl = self._arr(DEM_CLIST, ctypes.c_int, n, c)
+        st = self._arr(DEM_STICK, ctypes.c_int, n, c)
+        tsd = self._arr(DEM_TSD, ctypes.c_double, n * c, 3).reshape(n, c, 3)
+        iv = self._arr(DEM_IVM, ctypes.c_double, n, c)
+        return nc, cl, st, tsd, iv
+
+    def mt_next(self):
+        return self.L.odem_mt_next(self.h)
+
+
+def dem_reference_system(domain=(0.04, 0.015, 0.04), **kw):
+    """The reference's examples/dem.py setup at a given domain size."""
+    o = OracleDEM(domain, **kw)
+    o.sc_grid(domain)
+    o.add_reference_planes()
+    o.setup()
+    return o

@@ -56,4 +56,46 @@ inline void dump_md(const char *dir, int ts, int nlocal, int nghost, int nsend_a
     fclose(f);
 }
 
+
+// DEM record (magic 0x4D454400 'DEM'):
+//   int32[8] header: magic, ts, nlocal, nghost, contact_capacity, 0, 0, 0
+//   f64 position[nall*3], linear_velocity[nall*3], angular_velocity[nall*3]
+//   f64 force[nlocal*3], torque[nlocal*3], radius[nall], mass[nall]
+//   f64 rotation_quat[nlocal*4], rotation_matrix[nlocal*9]
+//   i32 uid[nall], flags[nall], type[nall]
+//   i32 num_contacts[nlocal], contact_lists[nlocal*cap], is_sticking[nlocal*cap]
+//   f64 tangential_spring_displacement[nlocal*cap*3], impact_velocity_magnitude[nlocal*cap]
+inline void dump_dem(const char *dir, int ts, int nlocal, int nghost, int cap, const double *position,
+                     const double *linear_velocity, const double *angular_velocity, const double *force,
+                     const double *torque, const double *radius, const double *mass, const double *rotation_quat,
+                     const double *rotation_matrix, const int *uid, const int *flags, const int *type,
+                     const int *num_contacts, const int *contact_lists, const int *is_sticking,
+                     const double *tangential_spring_displacement, const double *impact_velocity_magnitude) {
+    char path[4096];
+    snprintf(path, sizeof(path), "%s/dem_%05d.bin", dir, ts + 1);
+    FILE *f = fopen(path, "wb");
+    if (!f) return;
+    int hdr[8] = {0x4D454400, ts, nlocal, nghost, cap, 0, 0, 0};
+    fwrite(hdr, sizeof(int), 8, f);
+    const long nall = (long) nlocal + nghost, nl = nlocal;
+    put_d(f, position, nall * 3);
+    put_d(f, linear_velocity, nall * 3);
+    put_d(f, angular_velocity, nall * 3);
+    put_d(f, force, nl * 3);
+    put_d(f, torque, nl * 3);
+    put_d(f, radius, nall);
+    put_d(f, mass, nall);
+    put_d(f, rotation_quat, nl * 4);
+    put_d(f, rotation_matrix, nl * 9);
+    put_i(f, uid, nall);
+    put_i(f, flags, nall);
+    put_i(f, type, nall);
+    put_i(f, num_contacts, nl);
+    put_i(f, contact_lists, nl * cap);
+    put_i(f, is_sticking, nl * cap);
+    put_d(f, tangential_spring_displacement, nl * cap * 3);
+    put_d(f, impact_velocity_magnitude, nl * cap);
+    fclose(f);
+}
+
 }  // namespace golden

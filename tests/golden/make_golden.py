@@ -227,20 +227,8 @@ def make_md(name, nx, ntypes, steps, half, keep=None):
     print(f"wrote {out} ({os.path.getsize(out)} bytes)")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--keep", default=None, help="keep the scratch build directory here")
-    args = ap.parse_args()
-    make_md("md_fcc4_full", 4, 1, 100, False, args.keep)
-    make_md("md_fcc4_half", 4, 1, 100, True, args.keep)
-
-
-if __name__ == "__main__":
-    main()
-
-
 # ---------------------------------------------------------------- DEM
-DEM_STEPS = 200
+DEM_STEPS = 500
 
 
 def specialise_dem(src, steps):
@@ -268,3 +256,160 @@ def generate_dem(workdir, steps):
     env["PYTHONPATH"] = os.path.join(REF, "src")
     subprocess.run([sys.executable, path, "cpu"], cwd=workdir, env=env, check=True, stdout=subprocess.DEVNULL)
     return os.path.join(workdir, "dem.cpp")
+
+
+DEM_ARGS = ("nlocal, nghost, neighbor_capacity, position, linear_velocity, angular_velocity, force, torque, "
+            "radius, mass, rotation_quat, rotation_matrix, uid, flags, type, num_contacts, contact_lists, "
+            "is_sticking, tangential_spring_displacement, impact_velocity_magnitude")
+
+
+def instrument_dem(cpp_path, dump_dir, steps):
+    src = open(cpp_path).read()
+    src = src.replace('#include "runtime/vtk.hpp"',
+                      '#include "runtime/vtk.hpp"\n#include "golden_dump.hpp"', 1)
+    main_at = src.index("int main(int argc, char **argv) {")
+    m = re.compile(r"for\(int (i\d+) = 0; \1 < %d; \1\+\+\) \{" % (steps + 1)).search(src, main_at)
+    assert m, "timestep loop not found"
+    ts_var = m.group(1)
+    init_call = f'golden::dump_dem("{dump_dir}", -1, {DEM_ARGS});\n    '
+    src = src[:m.start()] + init_call + src[m.start():]
+    end = src.index("    LIKWID_MARKER_CLOSE;")
+    close = src.rindex("    }\n", 0, end)
+    src = src[:close] + f'        golden::dump_dem("{dump_dir}", {ts_var}, {DEM_ARGS});\n' + src[close:]
+    with open(cpp_path, "w") as f:
+        f.write(src)
+
+
+def read_dem_dump(path):
+    raw = open(path, "rb").read()
+    hdr = np.frombuffer(raw, dtype=np.int32, count=8)
+    assert hdr[0] == 0x4D454400
+    ts, nlocal, nghost, cap = [int(x) for x in hdr[1:5]]
+    off = 32
+    nall = nlocal + nghost
+
+    def take(dtype, n, shape=None):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=n, offset=off).copy()
+        off += a.nbytes
+        return a.reshape(shape) if shape else a
+
+    d = dict(ts=ts, nlocal=nlocal, nghost=nghost, cap=cap)
+    d["position"] = take(np.float64, nall * 3, (nall, 3))
+    d["linear_velocity"] = take(np.float64, nall * 3, (nall, 3))
+    d["angular_velocity"] = take(np.float64, nall * 3, (nall, 3))
+    d["force"] = take(np.float64, nlocal * 3, (nlocal, 3))
+    d["torque"] = take(np.float64, nlocal * 3, (nlocal, 3))
+    d["radius"] = take(np.float64, nall)
+    d["mass"] = take(np.float64, nall)
+    d["rotation_quat"] = take(np.float64, nlocal * 4, (nlocal, 4))
+    d["rotation_matrix"] = take(np.float64, nlocal * 9, (nlocal, 9))
+    d["uid"] = take(np.int32, nall)
+    d["flags"] = take(np.int32, nall)
+    d["type"] = take(np.int32, nall)
+    d["num_contacts"] = take(np.int32, nlocal)
+    d["contact_lists"] = take(np.int32, nlocal * cap, (nlocal, cap))
+    d["is_sticking"] = take(np.int32, nlocal * cap, (nlocal, cap))
+    d["tangential_spring_displacement"] = take(np.float64, nlocal * cap * 3, (nlocal, cap, 3))
+    d["impact_velocity_magnitude"] = take(np.float64, nlocal * cap, (nlocal, cap))
+    return d
+
+
+# DEM steps whose full state is kept (every step's contact count is kept)
+DEM_STATE_STEPS = [0, 1, 2, 5, 10, 50, 100, 150, 200, 250, 300, 350, 400, 450, 499, 500]
+
+
+def pack_dem(dump_dir, out_path, meta):
+    dumps = {}
+    for p in sorted(glob.glob(os.path.join(dump_dir, "dem_*.bin"))):
+        d = read_dem_dump(p)
+        dumps[d["ts"]] = d
+    arrays = {}
+    init = dumps[-1]
+    n0 = init["nlocal"]
+    for k in ("position", "linear_velocity", "angular_velocity", "radius", "mass", "uid", "flags", "type",
+              "rotation_quat", "rotation_matrix"):
+        arrays["init_" + k] = init[k][:n0]
+    nsteps = max(dumps)
+    arrays["nlocal_all"] = np.array([dumps[t]["nlocal"] for t in range(nsteps + 1)], np.int32)
+    arrays["nghost_all"] = np.array([dumps[t]["nghost"] for t in range(nsteps + 1)], np.int32)
+    arrays["ncontacts_all"] = np.array([int(dumps[t]["num_contacts"].sum()) for t in range(nsteps + 1)], np.int32)
+    for t in DEM_STATE_STEPS:
+        if t not in dumps:
+            continue
+        d = dumps[t]
+        n = d["nlocal"]
+        for k in ("position", "linear_velocity", "angular_velocity", "uid", "rotation_quat"):
+            arrays[f"s{t}_{k}"] = d[k][:n]
+        arrays[f"s{t}_force"] = d["force"]
+        arrays[f"s{t}_torque"] = d["torque"]
+        arrays[f"s{t}_ghost_uid"] = d["uid"][n:]
+        arrays[f"s{t}_ghost_position"] = d["position"][n:]
+        nc = d["num_contacts"]
+        arrays[f"s{t}_num_contacts"] = nc
+        rows = [np.arange(nc[i]) for i in range(n)]
+        arrays[f"s{t}_contact_uid"] = np.concatenate(
+            [d["contact_lists"][i, r] for i, r in enumerate(rows)] or [np.zeros(0, np.int32)]).astype(np.int32)
+        arrays[f"s{t}_contact_sticking"] = np.concatenate(
+            [d["is_sticking"][i, r] for i, r in enumerate(rows)] or [np.zeros(0, np.int32)]).astype(np.int32)
+        arrays[f"s{t}_contact_tsd"] = np.concatenate(
+            [d["tangential_spring_displacement"][i, r] for i, r in enumerate(rows)] or [np.zeros((0, 3))])
+        arrays[f"s{t}_contact_impact"] = np.concatenate(
+            [d["impact_velocity_magnitude"][i, r] for i, r in enumerate(rows)] or [np.zeros(0)])
+    for k, v in meta.items():
+        arrays["meta_" + k] = np.array(v)
+    np.savez_compressed(out_path, **arrays)
+    return arrays
+
+
+def build_and_run_dem(workdir, cpp):
+    exe = os.path.join(workdir, "dem_ref")
+    libdir = os.path.join(workdir, "mpilib")
+    os.makedirs(libdir, exist_ok=True)
+    for lib in ("libmpi.so.12", "libgfortran.so.4", "libquadmath.so.0"):
+        dst = os.path.join(libdir, lib)
+        if not os.path.exists(dst):
+            os.symlink(os.path.join(MPI_LIB, lib), dst)
+    cmd = ["g++"] + CXXFLAGS + ["-I" + MPI_INC, "-I" + REF, "-I" + HERE, "-o", exe, cpp,
+                                os.path.join(REF, "runtime/pairs.cpp"),
+                                os.path.join(REF, "runtime/domain/regular_6d_stencil.cpp"),
+                                os.path.join(REF, "runtime/devices/dummy.cpp"),
+                                os.path.join(libdir, "libmpi.so.12"),
+                                "-Wl,-rpath," + libdir, "-Wl,-rpath-link," + libdir]
+    subprocess.run(cmd, check=True, cwd=workdir)
+    subprocess.run([exe], cwd=workdir, check=True, capture_output=True, text=True)
+
+
+def make_dem(name, steps, keep=None):
+    workdir = keep or tempfile.mkdtemp(prefix="pairs_golden_dem_")
+    os.makedirs(workdir, exist_ok=True)
+    dump_dir = os.path.join(workdir, "dumps_" + name)
+    shutil.rmtree(dump_dir, ignore_errors=True)
+    os.makedirs(dump_dir)
+    cpp = generate_dem(workdir, steps)
+    instrument_dem(cpp, dump_dir, steps)
+    build_and_run_dem(workdir, cpp)
+    meta = dict(domain=[0.04, 0.015, 0.04], steps=steps, spacing=0.005, diameter=0.0029, min_diameter=0.00261,
+                max_diameter=0.00319, initial_velocity=1.0, density=2550.0, dt=5e-5,
+                compiler="g++ 11.4 + MPICH 3.3.2 " + " ".join(CXXFLAGS))
+    out = os.path.join(HERE, f"{name}.npz")
+    pack_dem(dump_dir, out, meta)
+    if keep is None:
+        shutil.rmtree(workdir, ignore_errors=True)
+    print(f"wrote {out} ({os.path.getsize(out)} bytes)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--keep", default=None, help="keep the scratch build directory here")
+    ap.add_argument("--only", default=None, choices=["md", "dem"])
+    args = ap.parse_args()
+    if args.only in (None, "md"):
+        make_md("md_fcc4_full", 4, 1, 100, False, args.keep)
+        make_md("md_fcc4_half", 4, 1, 100, True, args.keep)
+    if args.only in (None, "dem"):
+        make_dem("dem_sc168", DEM_STEPS, args.keep)
+
+
+if __name__ == "__main__":
+    main()
