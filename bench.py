@@ -83,12 +83,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--time-every", type=int, default=10)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device once: a kernel of known HBM traffic that "
+                         "calibrates FETCH_SIZE / WRITE_SIZE counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the single-domain hot path; sharding is a design note)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe > 0:
+        import torch
+        n = args.copy_probe * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     full = run_gpu(args.steps, args.warmup, False, args.time_every)
     value = NATOMS * args.steps / full["seconds"]
     fused_bytes = force_kernel_bytes(full["nlocal"], full["nghost"], full["nneigh"], True)

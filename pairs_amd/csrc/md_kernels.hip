@@ -310,6 +310,18 @@ __global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ count
     }
 }
 
+constexpr int FB = 128;  // threads per force block (two waves)
+
+// A segment's tile as physical slots: entry p of the tile is particle
+// tslot[p] (the force kernels gather positions from HBM through it, so the
+// tile costs 4 B of LDS per entry and many workgroups share a CU).
+__device__ __forceinline__ void stage_slots(const int *r_start, const int *r_off, int T, int *tslot) {
+    for (int p = threadIdx.x; p < T; p += blockDim.x) {
+        const int r = run_of(r_off, p);
+        tslot[p] = r_start[r] + (p - r_off[r]);
+    }
+}
+
 // ---------------------------------------------------------------- neighbour lists
 
 // BuildNeighborLists (src/pairs/sim/neighbor_lists.py:21-48) over the cell
@@ -318,35 +330,49 @@ __global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ count
 // reference-index order; accepted when j != i (full) or i < j (half, point
 // masses) and rsq < (rc + skin)^2.
 //
-// One workgroup per segment.  Its tile (positions + reference indices) is
-// staged in LDS; per home cell the candidate order is materialised once as a
-// list of tile offsets; each wave takes one home particle at a time, its 64
-// lanes test 64 candidates, and a ballot with a lane prefix count compacts
-// the accepted ones in candidate order.  Lists are 16-bit tile offsets,
-// transposed through LDS so each row k is written as one contiguous run.
-constexpr int NB_CAND = 1024;
+// One workgroup of BB threads (BW waves) per segment; each wave takes home
+// cells of the segment in turn.  The tile (positions, + reference indices
+// for half lists) is staged in LDS and a table gives, per column of the tile,
+// each flat cell's local and ghost sub-range (tile offset, length).  For a
+// home cell the wave lays out its candidates in the reference's order (cell
+// 0; then per stencil column and dz: that cell's locals, then its ghosts) as
+// tile offsets in LDS; then each chunk of 64 candidates is held one per lane
+// while the home particles of the cell are taken one at a time (broadcast
+// from the lane that holds them): a ballot over the chunk and a lane prefix
+// count append the accepted candidates in candidate order.  Entries go to an
+// LDS row buffer (one column per home particle) and leave as packed groups of
+// four rows; rows past the buffer, or home particles past its columns, are
+// written to the list directly.
+constexpr int BB = 128;        // threads per build workgroup
+constexpr int BW = BB / 64;    // waves per build workgroup
+constexpr int LR_MAX = 96;     // list rows buffered in LDS (a multiple of 4)
+constexpr int HCAP = 192;      // home particles with an LDS buffer column
+constexpr int NB_CAND = 2048;  // candidates of one home cell
 
-__global__ __launch_bounds__(256) void k_build_neighbors(const int *__restrict__ counters, Geom g,
-                                                         const int *__restrict__ lstart,
-                                                         const int *__restrict__ gstart,
-                                                         const int *__restrict__ lcount,
-                                                         const int *__restrict__ gcount,
-                                                         const double *__restrict__ X, const double *__restrict__ Y,
-                                                         const double *__restrict__ Z, const int *__restrict__ ref,
-                                                         const int *__restrict__ flags, int half, double rc2,
-                                                         uint16_t *__restrict__ nl, int nl_stride, int ncap, int hb,
-                                                         int tcap, int *__restrict__ numneighs,
-                                                         int *__restrict__ ctr) {
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) b, l);
+    const int hi = __builtin_amdgcn_readlane((int) (b >> 32), l);
+    return __longlong_as_double(((long long) hi << 32) | lo);
+}
+
+template <bool HALF>
+__global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ counters, Geom g,
+                                                        const int *__restrict__ lstart,
+                                                        const int *__restrict__ gstart, const double *__restrict__ X,
+                                                        const double *__restrict__ Y, const double *__restrict__ Z,
+                                                        const int *__restrict__ ref, const int *__restrict__ flags,
+                                                        double rc2, uint16_t *__restrict__ nl, int nl_stride,
+                                                        int ncap, int lrows, int tcap, int zt,
+                                                        int *__restrict__ numneighs, int *__restrict__ ctr) {
     extern __shared__ char smem[];
-    double *tx = (double *) smem;
-    double *ty = tx + tcap;
-    double *tz = ty + tcap;
-    int *tref = (int *) (tz + tcap);
-    uint16_t *cand = (uint16_t *) (tref + tcap);
-    uint16_t *lbuf = cand + NB_CAND;  // [ncap][hb]
+    int *tslot = (int *) smem;                                  // [tcap] physical slot of each tile entry
+    uint16_t *lbuf = (uint16_t *) (tslot + tcap);               // [lrows][HCAP]
+    uint16_t *cand_all = lbuf + (size_t) lrows * HCAP;          // [BW][NB_CAND]
+    int *ctab = (int *) (cand_all + BW * NB_CAND);              // [9][zt][4]: local st, len, ghost st, len
+    int *tref = ctab + 9 * zt * 4;                              // half lists only
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
-    __shared__ int ncand_s, home_cnt[256];
-    __shared__ int sub_start[56], sub_len[56], sub_off[57];
+    __shared__ int s_cnt[HCAP];
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
     const int seg = xcd_block(blockIdx.x, gridDim.x);
@@ -356,117 +382,133 @@ __global__ __launch_bounds__(256) void k_build_neighbors(const int *__restrict__
     if (h1 <= h0) return;
     seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
     const int T = r_off[NRUNS];
-    if (T > tcap) {
+    const int nz = si.zb - si.za + 2;  // flat cells B+za-1 .. B+zb of each column
+    if (T > tcap || nz > zt) {
         if (threadIdx.x == 0) { atomicMax(&ctr[C_TILE_MAX], T); atomicMax(&ctr[C_OVF], OVF_TILE); }
         return;
     }
     if (threadIdx.x == 0) atomicMax(&ctr[C_TILE_MAX], T);
-    for (int p = threadIdx.x; p < T; p += 256) {
-        const int r = run_of(r_off, p);
-        const int ph = r_start[r] + (p - r_off[r]);
-        tx[p] = X[ph]; ty[p] = Y[ph]; tz[p] = Z[ph];
-        tref[p] = ref[ph];
+    stage_slots(r_start, r_off, T, tslot);
+    __syncthreads();
+    if (HALF)
+        for (int p = threadIdx.x; p < T; p += BB) tref[p] = ref[tslot[p]];
+    // sub-range table: a flat cell cc of column col is a stencil cell of the
+    // reference iff 0 < cc < ncells (interaction.py:92-98)
+    for (int e = threadIdx.x; e < 9 * nz; e += BB) {
+        const int col = e / nz, zz = e - col * nz;
+        const int dx = col / 3 - 1, dy = col % 3 - 1;
+        const int cc = ((si.cx + dx) * g.dim_cells[1] + (si.cy + dy)) * g.dim_cells[2] + 1 + si.za - 1 + zz;
+        int v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+        if (cc > 0 && cc < g.ncells) {
+            const int lo = col_lo[col];
+            v0 = r_off[2 + 2 * col] + (lstart[cc] - lstart[lo]);
+            v1 = lstart[cc + 1] - lstart[cc];
+            v2 = r_off[3 + 2 * col] + (gstart[cc] - gstart[lo]);
+            v3 = gstart[cc + 1] - gstart[cc];
+        }
+        int *t = ctab + (col * zt + zz) * 4;
+        t[0] = v0; t[1] = v1; t[2] = v2; t[3] = v3;
     }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int cz = si.za; cz < si.zb; cz++) {
-        const int c = si.ca + (cz - si.za);
-        const int nh = lcount[c];
-        __syncthreads();
-        if (nh == 0) continue;
-        // candidate tile offsets of this cell, in reference order: 56 sub-ranges
-        // (cell 0 locals, ghosts; then per stencil column and dz: locals,
-        // ghosts of that cell), one thread each, then a parallel fill
-        if (threadIdx.x < 56) {
-            const int t = threadIdx.x;
-            int st = 0, ln = 0;
-            if (t < 2) {
-                st = r_off[t];
-                ln = r_len[t];
-            } else {
-                const int u = t - 2, col = u / 6, dz = (u % 6) / 2 - 1, loc = (u & 1) == 0;
-                const int dx = col / 3 - 1, dy = col % 3 - 1;
-                const int B = ((si.cx + dx) * g.dim_cells[1] + (si.cy + dy)) * g.dim_cells[2] + 1;
-                const int cc = B + cz + dz;
-                if (cc > 0 && cc < g.ncells) {
-                    const int lo = col_lo[col];
-                    if (loc) { st = r_off[2 + 2 * col] + (lstart[cc] - lstart[lo]); ln = lcount[cc]; }
-                    else { st = r_off[3 + 2 * col] + (gstart[cc] - gstart[lo]); ln = gcount[cc]; }
-                }
-            }
-            sub_start[t] = st;
-            sub_len[t] = ln;
+    uint16_t *cand = cand_all + wave * NB_CAND;
+    // home particles are the locals of the centre column (col 4), table rows
+    // 1 .. nz-2; a home particle's index in the segment is its tile offset
+    // minus own_base (= its physical slot minus h0)
+    const int own_base = r_off[2 + 2 * 4] + (h0 - lstart[col_lo[4]]);
+    for (int zz = 1 + wave; zz <= nz - 2; zz += BW) {
+        const int *tc = ctab + (4 * zt + zz) * 4;
+        const int hst = tc[0], hlen = tc[1];
+        if (hlen == 0) continue;
+        // candidate sub-ranges, one per lane: cell 0 locals, ghosts; then per
+        // column (dx outer, dy inner) and dz = -1, 0, 1: locals, ghosts
+        int st = 0, ln = 0;
+        if (lane < 2) {
+            st = r_off[lane];
+            ln = r_len[lane];
+        } else if (lane < 56) {
+            const int u = lane - 2, col = u / 6, dz = (u % 6) / 2 - 1, gh = u & 1;
+            const int *t = ctab + (col * zt + zz + dz) * 4;
+            st = t[2 * gh];
+            ln = t[2 * gh + 1];
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int off = 0;
-            for (int t = 0; t < 56; t++) { sub_off[t] = off; off += sub_len[t]; }
-            sub_off[56] = off;
-            ncand_s = min(off, NB_CAND);
-            if (off > NB_CAND) atomicMax(&ctr[C_OVF], OVF_TILE);
+        int incl = ln;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
         }
-        __syncthreads();
-        const int ncand = ncand_s;
-        for (int q = threadIdx.x; q < ncand; q += 256) {
-            int lo_s = 0, hi_s = 55;  // last sub-range with sub_off <= q
-            while (lo_s < hi_s) {
-                const int mid = (lo_s + hi_s + 1) >> 1;
-                if (sub_off[mid] <= q) lo_s = mid; else hi_s = mid - 1;
-            }
-            cand[q] = (uint16_t) (sub_start[lo_s] + (q - sub_off[lo_s]));
+        const int ncand = __shfl(incl, 63, 64);
+        if (ncand > NB_CAND) {
+            if (lane == 0) { atomicMax(&ctr[C_TILE_MAX], tcap + 1); atomicMax(&ctr[C_OVF], OVF_TILE); }
+            continue;
         }
-        __syncthreads();
-        const int cstart = lstart[c];
-        // the home particle's own tile slot: column (0, 0) holds cell c's locals
-        const int own0 = r_off[2 + 2 * 4] + (cstart - lstart[col_lo[4]]);
-        for (int a = wave; a < nh; a += 4) {
-            const int i = cstart + a;           // physical home slot
-            const int hi = i - h0;              // index in the segment's home range
-            const double xi = tx[own0 + a], yi = ty[own0 + a], zi = tz[own0 + a];
-            const int refi = tref[own0 + a];
+        for (int k = 0, e0 = incl - ln; k < ln; k++) cand[e0 + k] = (uint16_t) (st + k);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int hb = 0; hb < hlen; hb += 64) {
+            const int nh = min(64, hlen - hb);
+            // lane h holds home particle h of this pass
+            const int oh = hst + hb + min(lane, nh - 1);
+            const int ph = tslot[oh];
+            const double hx = X[ph], hy = Y[ph], hz = Z[ph];
+            const int href = HALF ? tref[oh] : 0;
+            const int hidx = oh - own_base;
+            const int hfix = flags[h0 + hidx] & FLAG_FIXED;
             int cnt = 0;
-            if (!(flags[i] & FLAG_FIXED)) {
-                for (int q0 = 0; q0 < ncand; q0 += 64) {
-                    const int q = q0 + lane;
-                    bool acc = false;
-                    int o = 0;
-                    if (q < ncand) {
-                        o = cand[q];
-                        const int refj = tref[o];
-                        const bool keep = half ? (refi < refj) : (refj != refi);
-                        const double dx = xi - tx[o], dy = yi - ty[o], dz = zi - tz[o];
-                        double rsq = dx * dx + dy * dy;
-                        rsq = rsq + dz * dz;
-                        acc = keep && rsq < rc2;
-                    }
+            for (int c0 = 0; c0 < ncand; c0 += 64) {
+                const int q = c0 + lane;
+                const bool qv = q < ncand;
+                const int o = qv ? cand[q] : 0;
+                const int pc = tslot[o];
+                const double cx = X[pc], cy = Y[pc], cz = Z[pc];
+                const int cref = HALF ? tref[o] : 0;
+                for (int h = 0; h < nh; h++) {
+                    if (__builtin_amdgcn_readlane(hfix, h)) continue;
+                    const double xi = readlane_f64(hx, h), yi = readlane_f64(hy, h), zi = readlane_f64(hz, h);
+                    const double dx = xi - cx, dy = yi - cy, dz = zi - cz;
+                    double rsq = dx * dx + dy * dy;
+                    rsq = rsq + dz * dz;
+                    const int ownh = hst + hb + h;
+                    const bool keep = HALF ? (__builtin_amdgcn_readlane(href, h) < cref) : (o != ownh);
+                    const bool acc = qv && keep && rsq < rc2;
                     const unsigned long long m = __ballot(acc);
-                    const int k = cnt + lane_rank(m);
+                    const int base = __builtin_amdgcn_readlane(cnt, h);
+                    const int k = base + lane_rank(m);
                     if (acc && k < ncap) {
-                        if (hi < hb) lbuf[(size_t) k * hb + hi] = (uint16_t) o;
-                        else nl[(size_t) k * nl_stride + i] = (uint16_t) o;
+                        const int idx = ownh - own_base;
+                        if (k < lrows && idx < HCAP) lbuf[k * HCAP + idx] = (uint16_t) o;
+                        else nl[nl_at(k, h0 + idx, nl_stride)] = (uint16_t) o;
                     }
-                    cnt += __popcll(m);
+                    cnt = lane == h ? base + __popcll(m) : cnt;
                 }
             }
-            if (lane == 0) {
-                int K = cnt;
+            if (lane < nh) {
                 // reference overflow rule (generated resize check): count+1 >= capacity
+                int K = cnt;
                 if (K + 1 >= ncap) {
                     atomicMax(&ctr[C_REQ_NEIGH], K + 2);
                     K = ncap - 2;
                 }
-                numneighs[i] = K;
-                if (hi < hb) home_cnt[hi] = K;
+                numneighs[h0 + hidx] = K;
+                if (hidx < HCAP) s_cnt[hidx] = K;
             }
         }
     }
     __syncthreads();
-    // write the transposed rows: row k holds entry k of every home particle
-    const int nhome = min(h1 - h0, hb);
-    int kmax = 0;
-    for (int a = 0; a < nhome; a++) kmax = max(kmax, home_cnt[a]);
-    for (int k = 0; k < kmax; k++)
-        for (int a = threadIdx.x; a < nhome; a += 256)
-            if (k < home_cnt[a]) nl[(size_t) k * nl_stride + h0 + a] = lbuf[(size_t) k * hb + a];
+    // buffered rows leave as packed groups of four (lane order = physical order)
+    const int nhome = min(h1 - h0, HCAP);
+    uint64_t *nl4 = (uint64_t *) nl;
+    for (int a = threadIdx.x; a < nhome; a += BB) {
+        const int kb = (min(s_cnt[a], lrows) + 3) >> 2;
+        for (int gq = 0; gq < kb; gq++) {
+            const uint64_t v = (uint64_t) lbuf[(4 * gq) * HCAP + a] | ((uint64_t) lbuf[(4 * gq + 1) * HCAP + a] << 16) |
+                               ((uint64_t) lbuf[(4 * gq + 2) * HCAP + a] << 32) |
+                               ((uint64_t) lbuf[(4 * gq + 3) * HCAP + a] << 48);
+            nl4[(size_t) gq * nl_stride + h0 + a] = v;
+        }
+    }
 }
 
 // Expand the 16-bit tile-offset lists into reference indices (for download):
@@ -489,7 +531,7 @@ __global__ __launch_bounds__(256) void k_expand_neighbors(const int *__restrict_
     for (int i = h0 + threadIdx.x; i < h1; i += 256) {
         const int n = numneighs[i];
         for (int k = 0; k < n && k < ncap_out; k++) {
-            const int o = nl[(size_t) k * nl_stride + i];
+            const int o = nl[nl_at(k, i, nl_stride)];
             const int r = run_of(r_off, o);
             out[(size_t) i * ncap_out + k] = ref[r_start[r] + (o - r_off[r])];
         }
@@ -506,14 +548,20 @@ struct LJTab {
 // One pair of examples/md.py:5-8 as the reference emits it: delta,
 // rsq = dx*dx + dy*dy + dz*dz, sr2 = 1/rsq, sr6 = sr2*sr2*sr2*sigma6,
 // s = 48*sr6*(sr6-0.5)*sr2*eps, force contribution delta*s.
+// Branch-free: every lane evaluates the pair and a pair that is not counted
+// (past the lane's list, or rsq >= rc2) gets s = 0, so its contribution is a
+// signed zero.  A running sum that starts at +0.0 is never -0.0, and t + (+-0)
+// == t exactly for every other t, so the masked pairs leave the sum
+// bit-identical to the reference's conditional add; without branches the U
+// divisions of a group overlap.
 template <bool ENERGY>
-__device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double xj, double yj, double zj, double rc2,
-                                        double sg6, double ep, double &g0, double &g1, double &g2, double &u,
-                                        double &w) {
+__device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double xj, double yj, double zj, bool valid,
+                                        double rc2, double sg6, double ep, double &g0, double &g1, double &g2,
+                                        double &u, double &w) {
     const double dx = xi - xj, dy = yi - yj, dz = zi - zj;
     double rsq = dx * dx + dy * dy;
     rsq = rsq + dz * dz;
-    if (!(rsq < rc2)) return false;
+    const bool in = valid && rsq < rc2;
     const double sr2 = 1.0 / rsq;
     double sr6 = sr2 * sr2;
     sr6 = sr6 * sr2;
@@ -522,49 +570,52 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
     s = s * (sr6 - 0.5);
     s = s * sr2;
     s = s * ep;
+    s = in ? s : 0.0;
     g0 = dx * s; g1 = dy * s; g2 = dz * s;
     if (ENERGY) {
-        u = 4.0 * ep * sr6 * (sr6 - 1.0);
-        w = 48.0 * sr6 * (sr6 - 0.5) * ep;
+        u = in ? 4.0 * ep * sr6 * (sr6 - 1.0) : 0.0;
+        w = in ? 48.0 * sr6 * (sr6 - 0.5) * ep : 0.0;
     }
-    return true;
+    return in;
 }
 
-constexpr int FB = 128;  // threads per force block (two waves)
-
-// Stage a segment's tile in LDS: positions (+ type, + physical slot).
-template <bool WITH_TYPE, bool WITH_PHYS>
-__device__ __forceinline__ void stage_tile(const int *r_start, const int *r_off, int T, const double *__restrict__ X,
-                                           const double *__restrict__ Y, const double *__restrict__ Z,
-                                           const int *__restrict__ type, double *tx, double *ty, double *tz,
-                                           int *tt, int *tp) {
-    // SPT entries per thread in flight at once: all loads of a batch issue
-    // before the first LDS store waits on them
-    constexpr int SPT = 8;
-    for (int p0 = 0; p0 < T; p0 += SPT * (int) blockDim.x) {
-        double xv[SPT], yv[SPT], zv[SPT];
-        int tv[SPT], pv[SPT];
+// Walk a lane's neighbour list eight tile offsets per step, to the
+// wave-uniform depth nmax: body(k, on) gets entries k .. k+7.  A step
+// consumes two packed groups of four entries; the loads of the groups two
+// steps ahead are issued first, in a ring of three register pairs written
+// out (no register moves), so a list load never stalls the step that issues
+// it.  Groups past the list re-read its last group (in bounds, no branch);
+// offsets are clamped into the tile.
+template <typename Body>
+__device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stride, int n, int nmax, int T,
+                                          Body &&body) {
+    const uint64_t *lp = (const uint64_t *) nl + i;
+    const int gl = max((n + 3) / 4 - 1, 0), tlast = T - 1;
+    // a relaxed atomic load is an ordered memory access to the compiler: it
+    // stays where it is issued instead of being sunk next to its use
+    auto ld = [&](int gq) {
+        return __hip_atomic_load(lp + (size_t) min(gq, gl) * nl_stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    };
+    auto step = [&](int k, uint64_t a, uint64_t b) {
+        int on[8];
 #pragma unroll
-        for (int k = 0; k < SPT; k++) {
-            const int p = p0 + k * blockDim.x + threadIdx.x;
-            pv[k] = 0;
-            if (p < T) {
-                const int r = run_of(r_off, p);
-                const int ph = r_start[r] + (p - r_off[r]);
-                xv[k] = X[ph]; yv[k] = Y[ph]; zv[k] = Z[ph];
-                if (WITH_TYPE) tv[k] = type[ph];
-                pv[k] = ph;
-            }
+        for (int q = 0; q < 4; q++) {
+            on[q] = min((int) ((a >> (16 * q)) & 0xffff), tlast);
+            on[4 + q] = min((int) ((b >> (16 * q)) & 0xffff), tlast);
         }
-#pragma unroll
-        for (int k = 0; k < SPT; k++) {
-            const int p = p0 + k * blockDim.x + threadIdx.x;
-            if (p < T) {
-                tx[p] = xv[k]; ty[p] = yv[k]; tz[p] = zv[k];
-                if (WITH_TYPE) tt[p] = tv[k];
-                if (WITH_PHYS) tp[p] = pv[k];
-            }
-        }
+        body(k, on);
+    };
+    uint64_t r0a = ld(0), r0b = ld(1), r1a = ld(2), r1b = ld(3), r2a, r2b;
+    for (int k = 0; k < nmax; k += 24) {
+        const int g = k >> 2;
+        r2a = ld(g + 4); r2b = ld(g + 5);
+        step(k, r0a, r0b);
+        if (k + 8 >= nmax) break;
+        r0a = ld(g + 6); r0b = ld(g + 7);
+        step(k + 8, r1a, r1b);
+        if (k + 16 >= nmax) break;
+        r1a = ld(g + 8); r1b = ld(g + 9);
+        step(k + 16, r2a, r2b);
     }
 }
 
@@ -592,6 +643,7 @@ __global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters
                                                 double rc2, LJTab tab, double dt, double hdt, int do_vv2, int do_vv1,
                                                 double *__restrict__ partials, int pstride,
                                                 double *__restrict__ eatom, int tcap) {
+    static_assert(U == 8, "the list stream yields eight entries per step");
     extern __shared__ char smem[];
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     __shared__ double s_eps[16], s_sig6[16];
@@ -604,12 +656,9 @@ __global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters
         seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
         const int T = r_off[NRUNS];
         const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
-        double *tx = (double *) smem;
-        double *ty = tx + T;
-        double *tz = ty + T;
-        int *tt = (int *) (tz + T);
+        int *tphys = (int *) smem;
         if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
-        if (fits) stage_tile<NT != 1, false>(r_start, r_off, T, X, Y, Z, type, tx, ty, tz, tt, nullptr);
+        if (fits) stage_slots(r_start, r_off, T, tphys);
         __syncthreads();
         const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
         const int nt = tab.nt;
@@ -619,38 +668,31 @@ __global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters
             const int ii = live ? i : h0;
             const int fl = live ? flags[ii] : FLAG_FIXED;
             const int n = (live && !(fl & FLAG_FIXED)) ? numneighs[ii] : 0;
-            const int nmax = wave_max(n);
+            const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
             const double xi = X[ii], yi = Y[ii], zi = Z[ii];
             const int ti = NT != 1 ? type[ii] : 0;
-            const uint16_t *lp = nl + ii;
             double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
-            int on[U];
+            list_walk(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *on) {
+                int jp[U];
 #pragma unroll
-            for (int q = 0; q < U; q++) on[q] = q < n ? (int) __builtin_nontemporal_load(lp + (size_t) q * nl_stride) : 0;
-            for (int k = 0; k < nmax; k += U) {
+                for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
                 double xj[U], yj[U], zj[U];
                 int tj[U];
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    xj[q] = tx[on[q]]; yj[q] = ty[on[q]]; zj[q] = tz[on[q]];
-                    if (NT != 1) tj[q] = tt[on[q]];
-                }
-#pragma unroll
-                for (int q = 0; q < U; q++) {
-                    const int kk = k + U + q;
-                    on[q] = kk < n ? (int) __builtin_nontemporal_load(lp + (size_t) kk * nl_stride) : 0;
+                    xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]];
+                    if (NT != 1) tj[q] = type[jp[q]];
                 }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
                     double sg6 = sig0, ep = eps0;
                     if (NT != 1) { const int kt = ti * nt + tj[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
                     double g0, g1, g2, uu, ww;
-                    if (k + q < n && lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], rc2, sg6, ep, g0, g1, g2, uu, ww)) {
-                        t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
-                        if (RED) { u += uu; w += ww; }
-                    }
+                    lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < n, rc2, sg6, ep, g0, g1, g2, uu, ww);
+                    t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
+                    if (RED) { u += uu; w += ww; }
                 }
-            }
+            });
             if (live) {
                 if (RED) { pe += 0.5 * u; vir += 0.5 * w; }
                 if (MODE == 2) {
@@ -712,6 +754,7 @@ __global__ __launch_bounds__(FB) void k_lj_half(const int *__restrict__ counters
                                                 double *__restrict__ fx, double *__restrict__ fy,
                                                 double *__restrict__ fz, double rc2, LJTab tab,
                                                 double *__restrict__ partials, int pstride, int tcap) {
+    static_assert(U == 8, "the list stream yields eight entries per step");
     extern __shared__ char smem[];
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     __shared__ double s_eps[16], s_sig6[16];
@@ -724,13 +767,9 @@ __global__ __launch_bounds__(FB) void k_lj_half(const int *__restrict__ counters
         seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
         const int T = r_off[NRUNS];
         const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
-        double *tx = (double *) smem;
-        double *ty = tx + T;
-        double *tz = ty + T;
-        int *tp = (int *) (tz + T);
-        int *tt = tp + T;
+        int *tp = (int *) smem;
         if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
-        if (fits) stage_tile<NT != 1, true>(r_start, r_off, T, X, Y, Z, type, tx, ty, tz, tt, tp);
+        if (fits) stage_slots(r_start, r_off, T, tp);
         __syncthreads();
         const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
         const int nt = tab.nt;
@@ -739,40 +778,37 @@ __global__ __launch_bounds__(FB) void k_lj_half(const int *__restrict__ counters
             const bool live = i < h1;
             const int ii = live ? i : h0;
             const int n = (live && !(flags[ii] & FLAG_FIXED)) ? numneighs[ii] : 0;
-            const int nmax = wave_max(n);
+            const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
             const double xi = X[ii], yi = Y[ii], zi = Z[ii];
             const int ti = NT != 1 ? type[ii] : 0;
-            const uint16_t *lp = nl + ii;
             double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
-            int on[U];
+            list_walk(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *oc) {
+                double g0[U], g1[U], g2[U];
+                bool in[U];
+                int jp[U];
 #pragma unroll
-            for (int q = 0; q < U; q++) on[q] = q < n ? (int) __builtin_nontemporal_load(lp + (size_t) q * nl_stride) : 0;
-            for (int k = 0; k < nmax; k += U) {
-                int oc[U];
+                for (int q = 0; q < U; q++) jp[q] = tp[oc[q]];
+                double xj[U], yj[U], zj[U];
 #pragma unroll
-                for (int q = 0; q < U; q++) oc[q] = on[q];
+                for (int q = 0; q < U; q++) { xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]]; }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    const int kk = k + U + q;
-                    on[q] = kk < n ? (int) __builtin_nontemporal_load(lp + (size_t) kk * nl_stride) : 0;
+                    double sg6 = sig0, ep = eps0;
+                    if (NT != 1) { const int kt = ti * nt + type[jp[q]]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
+                    double uu, ww;
+                    in[q] = lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < n, rc2, sg6, ep, g0[q], g1[q], g2[q],
+                                         uu, ww);
+                    t0 = t0 + g0[q]; t1 = t1 + g1[q]; t2 = t2 + g2[q];
+                    if (RED) { const double wt = jp[q] < nlocal ? 1.0 : 0.5; u += wt * uu; w += wt * ww; }
                 }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    const int o = oc[q];
-                    double sg6 = sig0, ep = eps0;
-                    if (NT != 1) { const int kt = ti * nt + tt[o]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
-                    double g0, g1, g2, uu, ww;
-                    if (k + q < n && lj_pair<RED>(xi, yi, zi, tx[o], ty[o], tz[o], rc2, sg6, ep, g0, g1, g2, uu, ww)) {
-                        t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
-                        const int j = tp[o];
-                        const bool loc = j < nlocal;
-                        if (RED) { const double wt = loc ? 1.0 : 0.5; u += wt * uu; w += wt * ww; }
-                        if (loc && !(flags[j] & FLAG_FIXED)) {
-                            atomic_add_f64(fx + j, -g0); atomic_add_f64(fy + j, -g1); atomic_add_f64(fz + j, -g2);
-                        }
+                    const int j = jp[q];
+                    if (in[q] && j < nlocal && !(flags[j] & FLAG_FIXED)) {
+                        atomic_add_f64(fx + j, -g0[q]); atomic_add_f64(fy + j, -g1[q]); atomic_add_f64(fz + j, -g2[q]);
                     }
                 }
-            }
+            });
             if (n > 0) {
                 atomic_add_f64(fx + i, t0); atomic_add_f64(fy + i, t1); atomic_add_f64(fz + i, t2);
                 pe += u; vir += w;
@@ -1228,18 +1264,36 @@ void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g) {
                        d.sync_src, d.sync_mult);
 }
 
+int build_zt(const Geom &g) { return g.nchunks == 1 ? g.dim_cells[2] + 2 : g.zc + 3; }
+
+static int build_lrows(const DevState &d) { return min((d.ncap + 3) / 4 * 4, LR_MAX); }
+
+static size_t build_lds(const DevState &d, const Geom &g, int half, int tcap) {
+    return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) +
+           (size_t) build_lrows(d) * HCAP * sizeof(uint16_t) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
+           (size_t) 9 * build_zt(g) * 4 * sizeof(int);
+}
+
+int build_tile_limit(const DevState &d, const Geom &g, int half) {
+    const size_t fixed = build_lds(d, g, half, 0) + 2048;  // + static LDS
+    const size_t per = sizeof(int) + (half ? sizeof(int) : 0);
+    const int lim = d.max_lds > (int) fixed ? (int) ((d.max_lds - fixed) / per) : 0;
+    return min(lim, 65536);  // tile offsets are 16-bit
+}
+
 void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2) {
-    // LDS budget: tile (28 B/entry) + candidate list + list transposition rows
-    // of hb home particles (home particles past hb write their rows directly)
-    const int budget = d.max_lds - 4096 - NB_CAND * (int) sizeof(uint16_t);
-    int hb = min(256, max(32, (budget / 2) / (2 * d.ncap)));
-    int tcap = min(2048, (budget - 2 * d.ncap * hb) / 28);
-    tcap = max(tcap, 256);
-    const size_t lds = (size_t) tcap * 28 + NB_CAND * sizeof(uint16_t) + (size_t) d.ncap * hb * sizeof(uint16_t);
+    const int lrows = build_lrows(d);
+    const int tcap = min(d.tile_cap, build_tile_limit(d, g, half));
+    const size_t lds = build_lds(d, g, half, tcap);
     const int c = d.cur;
-    hipLaunchKernelGGL(k_build_neighbors, dim3(g.nsegs), dim3(256), lds, s, d.counters, g, d.lstart, d.gstart,
-                       d.lcount, d.gcount, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, half, rc2, d.nl, d.nl_stride,
-                       d.ncap, hb, tcap, d.numneighs, d.counters);
+    if (half)
+        hipLaunchKernelGGL(k_build_neighbors<true>, dim3(g.nsegs), dim3(BB), lds, s, d.counters, g, d.lstart,
+                           d.gstart, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, rc2, d.nl, d.nl_stride, d.ncap, lrows,
+                           tcap, build_zt(g), d.numneighs, d.counters);
+    else
+        hipLaunchKernelGGL(k_build_neighbors<false>, dim3(g.nsegs), dim3(BB), lds, s, d.counters, g, d.lstart,
+                           d.gstart, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, rc2, d.nl, d.nl_stride, d.ncap, lrows,
+                           tcap, build_zt(g), d.numneighs, d.counters);
 }
 
 void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out) {
@@ -1314,7 +1368,7 @@ int force_blocks(const DevState &d, const Geom &g) { return g.nsegs; }
 template <int NT, int MODE, bool RED>
 static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
                            int do_vv2, int do_vv1, double *eatom) {
-    const size_t lds = (size_t) d.tile_cap * (3 * sizeof(double) + (NT != 1 ? sizeof(int) : 0));
+    const size_t lds = (size_t) d.tile_cap * sizeof(int);
     const LJTab tab = make_tab(prm);
     hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, d.counters, g,
                        d.lstart, d.gstart, d.X[cur], d.Y[cur], d.Z[cur], d.X[next], d.Y[next], d.Z[next], d.type,
@@ -1343,7 +1397,7 @@ void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJPar
 
 template <int NT>
 static void lj_half_nt(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce) {
-    const size_t lds = (size_t) d.tile_cap * (3 * sizeof(double) + sizeof(int) + (NT != 1 ? sizeof(int) : 0));
+    const size_t lds = (size_t) d.tile_cap * sizeof(int);
     const LJTab tab = make_tab(prm);
     const int c = cur;
     if (reduce)

@@ -61,6 +61,13 @@ struct Geom {
     int nsegs;     // dim_cells[0] * dim_cells[1] * nchunks
 };
 
+// Neighbour lists: entry k of physical local i.  Rows come in groups of four
+// so that one 8-byte load gives a lane four consecutive entries; a group
+// holds, for every particle, its four entries side by side.
+__host__ __device__ __forceinline__ size_t nl_at(int k, int i, int nl_stride) {
+    return ((size_t) (k >> 2) * nl_stride + i) * 4 + (k & 3);
+}
+
 struct LJParams {
     double rc2, dt, hdt;
     int ntypes;
@@ -82,7 +89,7 @@ struct DevState {
     double *alt_vx, *alt_vy, *alt_vz, *alt_mass;
     int *alt_type, *alt_flags, *alt_uid, *alt_shape, *alt_ref, *alt_cell;
     int *numneighs;
-    uint16_t *nl;  // nl[k * nl_stride + physical local] = tile offset
+    uint16_t *nl;  // entry k of physical local i at nl_at(k, i, nl_stride): tile offset
     int *particle_cell, *slot, *perm;
     int *lcount, *gcount, *lstart, *gstart;  // per-cell counts and CSR starts (locals / ghosts)
     int *send_map, *send_mult, *exchg_flag;
@@ -106,6 +113,8 @@ void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *
 // cell order (swapping d's spare arrays in), then builds the ghost sync maps
 void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g);
 void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2);
+// largest tile (entries) the build and force kernels can stage in LDS
+int build_tile_limit(const DevState &d, const Geom &g, int half);
 void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out);
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid);
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel);

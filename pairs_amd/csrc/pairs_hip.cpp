@@ -128,7 +128,7 @@ static int alloc_neighbor_arrays(pairs_hip_ctx *ctx, DevState &d, int ncap) {
     dfree(d.nl);
     d.nl = nullptr;
     d.nl_stride = d.cap;
-    if (dalloc(ctx, &d.nl, (size_t) ncap * d.nl_stride)) return PAIRS_HIP_ERR_HIP;
+    if (dalloc(ctx, &d.nl, (size_t) ((ncap + 3) / 4) * 4 * d.nl_stride)) return PAIRS_HIP_ERR_HIP;
     d.ncap = ncap;
     return 0;
 }
@@ -710,12 +710,15 @@ int pairs_hip_bin_cells(pairs_hip_ctx *ctx) {
     return finish_module(ctx);
 }
 
-// the build kernel's LDS: tile (28 B/entry) + candidates + list transposition
+// LDS entries the kernels stage per segment tile: the largest tile of the
+// last build plus 1/8 and 64 entries of headroom (tiles change only at a
+// rebuild), never above what a workgroup's LDS holds; grows only, so captured
+// graphs stay valid while tiles fluctuate
 static void set_tile_cap(pairs_hip_ctx *ctx) {
     const int tmax = ctx->h_counters[C_TILE_MAX];
     int cap = tmax + tmax / 8 + 64;
     cap = (cap + 63) / 64 * 64;
-    const int lim = ctx->max_lds / 32;  // force kernels stage up to 32 B per tile entry
+    const int lim = build_tile_limit(ctx->d, ctx->geom, ctx->cfg.half);
     ctx->d.tile_cap = std::min(std::max(cap, ctx->d.tile_cap), lim);
 }
 
@@ -727,10 +730,15 @@ static int build_neighbors(pairs_hip_ctx *ctx) {
         CHECK_LAUNCH();
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
         if (ctx->h_counters[C_OVF] == OVF_TILE) {
-            // tile too large for one workgroup: fewer cells per segment
             launch_set_counters(ctx->stream, ctx->d.counters, C_OVF, 1, 0);
-            if (ctx->geom.zc <= 1) { ctx->err = "cell tile exceeds the workgroup's LDS"; return PAIRS_HIP_ERR_STATE; }
             drop_graphs(ctx);
+            const int tmax = ctx->h_counters[C_TILE_MAX];
+            if (tmax <= build_tile_limit(ctx->d, ctx->geom, ctx->cfg.half) && tmax > ctx->d.tile_cap) {
+                set_tile_cap(ctx);  // the tile fits a workgroup: stage more entries
+                continue;
+            }
+            // tile too large for one workgroup: fewer cells per segment
+            if (ctx->geom.zc <= 1) { ctx->err = "cell tile exceeds the workgroup's LDS"; return PAIRS_HIP_ERR_STATE; }
             setup_segments(ctx, ctx->geom.zc - 1);
             continue;
         }
