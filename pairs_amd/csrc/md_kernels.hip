@@ -322,6 +322,43 @@ __device__ __forceinline__ void stage_slots(const int *r_start, const int *r_off
     }
 }
 
+// Images of each local: every ghost's chain of sources ends at its root
+// local; a root gets a slot per image (order irrelevant: positions are
+// written to the next buffer from values of the current one).
+__global__ __launch_bounds__(256) void k_img_build(int *__restrict__ counters, const int *__restrict__ sync_src,
+                                                   int *__restrict__ img_cnt, int *__restrict__ img_slot, int cap) {
+    if (counters[C_OVF]) return;
+    const int nlocal = counters[C_NLOCAL], ng = counters[C_NGHOST];
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < ng; q += gridDim.x * 256) {
+        int root = sync_src[q];
+        for (int hop = 0; root >= nlocal && hop < 64; hop++) root = sync_src[root - nlocal];
+        if (root >= nlocal) { atomicMax(&counters[C_IMG_OVF], 1); continue; }
+        const int k = atomicAdd(&img_cnt[root], 1);
+        if (k < IMG_MAX) img_slot[(size_t) k * cap + root] = q;
+        else atomicMax(&counters[C_IMG_OVF], 1);
+    }
+}
+
+// Next positions of local i's images (the stale multi-hop refresh of
+// src/pairs/sim/comm.py:45-54,369-442 one step early): an image of i itself
+// is i's new position + shift, an image of a ghost is that ghost's current
+// position + shift -- exactly what the next step's synchronize packs.
+__device__ __forceinline__ void write_images(const Images &im, int i, int nlocal, double px, double py, double pz,
+                                             const double *__restrict__ X, const double *__restrict__ Y,
+                                             const double *__restrict__ Z, double *__restrict__ Xn,
+                                             double *__restrict__ Yn, double *__restrict__ Zn, const Geom &g) {
+    const int n = min(im.cnt[i], IMG_MAX);
+    for (int k = 0; k < n; k++) {
+        const int q = im.slot[(size_t) k * im.cap + i];
+        const int src = im.src[q];
+        const bool own = src == i;
+        const double bx = own ? px : X[src], by = own ? py : Y[src], bz = own ? pz : Z[src];
+        Xn[nlocal + q] = bx + (double) im.mult[3 * q + 0] * g.len[0];
+        Yn[nlocal + q] = by + (double) im.mult[3 * q + 1] * g.len[1];
+        Zn[nlocal + q] = bz + (double) im.mult[3 * q + 2] * g.len[2];
+    }
+}
+
 // ---------------------------------------------------------------- neighbour lists
 
 // BuildNeighborLists (src/pairs/sim/neighbor_lists.py:21-48) over the cell
@@ -540,11 +577,6 @@ __global__ __launch_bounds__(256) void k_expand_neighbors(const int *__restrict_
 
 // ---------------------------------------------------------------- Lennard-Jones
 
-struct LJTab {
-    int nt;
-    double eps[16], sig6[16];  // ntypes <= 4
-};
-
 // One pair of examples/md.py:5-8 as the reference emits it: delta,
 // rsq = dx*dx + dy*dy + dz*dz, sr2 = 1/rsq, sr6 = sr2*sr2*sr2*sigma6,
 // s = 48*sr6*(sr6-0.5)*sr2*eps, force contribution delta*s.
@@ -586,9 +618,10 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
 // out (no register moves), so a list load never stalls the step that issues
 // it.  Groups past the list re-read its last group (in bounds, no branch);
 // offsets are clamped into the tile.
-template <typename Body>
+template <int U, typename Body>
 __device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stride, int n, int nmax, int T,
                                           Body &&body) {
+    static_assert(U == 4 || U == 8, "a step takes one or two packed groups");
     const uint64_t *lp = (const uint64_t *) nl + i;
     const int gl = max((n + 3) / 4 - 1, 0), tlast = T - 1;
     // a relaxed atomic load is an ordered memory access to the compiler: it
@@ -605,6 +638,28 @@ __device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stri
         }
         body(k, on);
     };
+    if (U == 4) {
+        // one group per step, same ring of three
+        auto step4 = [&](int k, uint64_t a) {
+            int on[8];
+#pragma unroll
+            for (int q = 0; q < 4; q++) on[q] = min((int) ((a >> (16 * q)) & 0xffff), tlast);
+            body(k, on);
+        };
+        uint64_t r0 = ld(0), r1 = ld(1), r2;
+        for (int k = 0; k < nmax; k += 12) {
+            const int g = k >> 2;
+            r2 = ld(g + 2);
+            step4(k, r0);
+            if (k + 4 >= nmax) break;
+            r0 = ld(g + 3);
+            step4(k + 4, r1);
+            if (k + 8 >= nmax) break;
+            r1 = ld(g + 4);
+            step4(k + 8, r2);
+        }
+        return;
+    }
     uint64_t r0a = ld(0), r0b = ld(1), r1a = ld(2), r1b = ld(3), r2a, r2b;
     for (int k = 0; k < nmax; k += 24) {
         const int g = k >> 2;
@@ -629,26 +684,30 @@ __device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stri
 // group's offsets loading while the current one computes; each lane adds its
 // pair forces in list order, so the sum is bit-identical to the reference's.
 template <int U, int NT, int MODE, bool RED>
-__global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters, Geom g,
-                                                const int *__restrict__ lstart, const int *__restrict__ gstart,
-                                                const double *__restrict__ X, const double *__restrict__ Y,
-                                                const double *__restrict__ Z, double *__restrict__ Xn,
-                                                double *__restrict__ Yn, double *__restrict__ Zn,
-                                                const int *__restrict__ type, const uint16_t *__restrict__ nl,
-                                                int nl_stride, const int *__restrict__ numneighs,
-                                                const int *__restrict__ flags, double *__restrict__ fx,
-                                                double *__restrict__ fy, double *__restrict__ fz,
-                                                double *__restrict__ vx, double *__restrict__ vy,
-                                                double *__restrict__ vz, const double *__restrict__ mass,
-                                                double rc2, LJTab tab, double dt, double hdt, int do_vv2, int do_vv1,
-                                                double *__restrict__ partials, int pstride,
-                                                double *__restrict__ eatom, int tcap) {
-    static_assert(U == 8, "the list stream yields eight entries per step");
+__device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
+    static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
+    const int *__restrict__ counters = A.counters;
+    const Geom &g = A.g;
+    const int *__restrict__ lstart = A.lstart, *__restrict__ gstart = A.gstart;
+    const double *__restrict__ X = A.X, *__restrict__ Y = A.Y, *__restrict__ Z = A.Z;
+    double *__restrict__ Xn = A.Xn, *__restrict__ Yn = A.Yn, *__restrict__ Zn = A.Zn;
+    const int *__restrict__ type = A.type;
+    const uint16_t *__restrict__ nl = A.nl;
+    const int nl_stride = A.nl_stride;
+    const int *__restrict__ numneighs = A.numneighs, *__restrict__ flags = A.flags;
+    double *__restrict__ fx = A.fx, *__restrict__ fy = A.fy, *__restrict__ fz = A.fz;
+    double *__restrict__ vx = A.vx, *__restrict__ vy = A.vy, *__restrict__ vz = A.vz;
+    const double *__restrict__ mass = A.mass;
+    const double rc2 = A.rc2, dt = A.dt, hdt = A.hdt;
+    const LJTab &tab = A.tab;
+    const int do_vv2 = A.do_vv2, do_vv1 = A.do_vv1, pstride = A.pstride, tcap = A.tcap;
+    double *__restrict__ partials = A.partials;
+    double *__restrict__ eatom = A.eatom;
+    const Images &im = A.im;
     extern __shared__ char smem[];
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     __shared__ double s_eps[16], s_sig6[16];
     const int nlocal = counters[C_NLOCAL];
-    const int seg = xcd_block(blockIdx.x, gridDim.x);
     const SegInfo si = seg_info(seg, g);
     const int h0 = lstart[si.ca], h1 = lstart[si.cb];
     double ke = 0.0, pe = 0.0, vir = 0.0;
@@ -672,7 +731,7 @@ __global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters
             const double xi = X[ii], yi = Y[ii], zi = Z[ii];
             const int ti = NT != 1 ? type[ii] : 0;
             double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
-            list_walk(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *on) {
+            list_walk<U>(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *on) {
                 int jp[U];
 #pragma unroll
                 for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
@@ -721,7 +780,10 @@ __global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters
                         }
                         vx[i] = v0; vy[i] = v1; vz[i] = v2;
                     }
-                    if (do_vv1) { Xn[i] = px; Yn[i] = py; Zn[i] = pz; }
+                    if (do_vv1) {
+                        Xn[i] = px; Yn[i] = py; Zn[i] = pz;
+                        if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, g);
+                    }
                 }
             }
         }
@@ -741,6 +803,23 @@ __global__ __launch_bounds__(FB) void k_lj_full(const int *__restrict__ counters
     }
 }
 
+
+// one segment per workgroup of one system
+template <int U, int NT, int MODE, bool RED>
+__global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
+    lj_full_segment<U, NT, MODE, RED>(A, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// the segments of many independent systems in one launch: block b belongs to
+// system rep[b] and is its segment b - seg_base[rep[b]]
+template <int U, int NT, int MODE, bool RED>
+__global__ __launch_bounds__(FB) void k_lj_full_batch(const LJArgs *__restrict__ desc, const int *__restrict__ rep,
+                                                      const int *__restrict__ seg_base) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r = rep[b];
+    lj_full_segment<U, NT, MODE, RED>(desc[r], b - seg_base[r]);
+}
+
 // Half-list LJ (compute_half): i accumulates its own sum; the mirror force on a
 // local, non-fixed j is an FP64 atomic subtract (src/pairs/ir/apply.py:111-125).
 // Forces must be zeroed before the launch (reset_volatile_properties).
@@ -754,7 +833,7 @@ __global__ __launch_bounds__(FB) void k_lj_half(const int *__restrict__ counters
                                                 double *__restrict__ fx, double *__restrict__ fy,
                                                 double *__restrict__ fz, double rc2, LJTab tab,
                                                 double *__restrict__ partials, int pstride, int tcap) {
-    static_assert(U == 8, "the list stream yields eight entries per step");
+    static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
     extern __shared__ char smem[];
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     __shared__ double s_eps[16], s_sig6[16];
@@ -782,7 +861,7 @@ __global__ __launch_bounds__(FB) void k_lj_half(const int *__restrict__ counters
             const double xi = X[ii], yi = Y[ii], zi = Z[ii];
             const int ti = NT != 1 ? type[ii] : 0;
             double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
-            list_walk(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *oc) {
+            list_walk<U>(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *oc) {
                 double g0[U], g1[U], g2[U];
                 bool in[U];
                 int jp[U];
@@ -860,7 +939,7 @@ __global__ __launch_bounds__(256) void k_vv_post(const double *__restrict__ X, c
                                                  const double *__restrict__ fy, const double *__restrict__ fz,
                                                  const double *__restrict__ mass, const int *__restrict__ flags,
                                                  int nlocal, double dt, double hdt, int do_vv2, int do_vv1,
-                                                 double *__restrict__ partials) {
+                                                 double *__restrict__ partials, Images im, Geom g) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     double ke = 0.0;
     if (i < nlocal) {
@@ -886,7 +965,10 @@ __global__ __launch_bounds__(256) void k_vv_post(const double *__restrict__ X, c
         } else if (RED) {
             ke = m * (v0 * v0 + v1 * v1 + v2 * v2);
         }
-        if (do_vv1) { Xn[i] = px; Yn[i] = py; Zn[i] = pz; }
+        if (do_vv1) {
+            Xn[i] = px; Yn[i] = py; Zn[i] = pz;
+            if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, g);
+        }
     }
     if (RED) {
         __shared__ double red[4];
@@ -1262,6 +1344,8 @@ void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g) {
                        d.vy, d.vz, d.mass, d.type, d.flags, d.uid, d.shape, d.ref, d.particle_cell);
     hipLaunchKernelGGL(k_sync_maps, dim3(grid), dim3(256), 0, s, d.counters, d.ref, d.phys, d.send_map, d.send_mult,
                        d.sync_src, d.sync_mult);
+    launch_fill_i32(s, d.img_cnt, 0, d.cap);
+    hipLaunchKernelGGL(k_img_build, dim3(grid), dim3(256), 0, s, d.counters, d.sync_src, d.img_cnt, d.img_slot, d.cap);
 }
 
 int build_zt(const Geom &g) { return g.nchunks == 1 ? g.dim_cells[2] + 2 : g.zc + 3; }
@@ -1365,15 +1449,51 @@ constexpr int LJ_UNROLL = 8;
 
 int force_blocks(const DevState &d, const Geom &g) { return g.nsegs; }
 
+// image refresh only when the kernel writes next positions into the other
+// buffer and every local's images fit the table
+static Images images_of(const DevState &d, bool writes_next) {
+    Images im;
+    im.cnt = d.img_cnt; im.slot = d.img_slot; im.src = d.sync_src; im.mult = d.sync_mult;
+    im.cap = d.cap;
+    im.on = (writes_next && d.img_on) ? 1 : 0;
+    return im;
+}
+
+LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
+               int fused, double *eatom) {
+    LJArgs A;
+    A.counters = d.counters;
+    A.g = g;
+    A.lstart = d.lstart; A.gstart = d.gstart;
+    A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];
+    A.Xn = d.X[next]; A.Yn = d.Y[next]; A.Zn = d.Z[next];
+    A.type = d.type; A.nl = d.nl; A.nl_stride = d.nl_stride; A.numneighs = d.numneighs; A.flags = d.flags;
+    A.fx = d.fx; A.fy = d.fy; A.fz = d.fz; A.vx = d.vx; A.vy = d.vy; A.vz = d.vz; A.mass = d.mass;
+    A.rc2 = prm.rc2; A.tab = make_tab(prm); A.dt = prm.dt; A.hdt = prm.hdt;
+    A.do_vv2 = do_vv2; A.do_vv1 = do_vv1;
+    A.partials = d.partials; A.pstride = d.pstride; A.eatom = eatom; A.tcap = d.tile_cap;
+    A.im = images_of(d, fused && do_vv1 && next != cur);
+    return A;
+}
+
 template <int NT, int MODE, bool RED>
 static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
                            int do_vv2, int do_vv1, double *eatom) {
     const size_t lds = (size_t) d.tile_cap * sizeof(int);
-    const LJTab tab = make_tab(prm);
-    hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, d.counters, g,
-                       d.lstart, d.gstart, d.X[cur], d.Y[cur], d.Z[cur], d.X[next], d.Y[next], d.Z[next], d.type,
-                       d.nl, d.nl_stride, d.numneighs, d.flags, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.mass, prm.rc2,
-                       tab, prm.dt, prm.hdt, do_vv2, do_vv1, d.partials, d.pstride, eatom, d.tile_cap);
+    const LJArgs A = lj_args(d, g, prm, cur, next, do_vv2, do_vv1, MODE == 1, eatom);
+    hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
+}
+
+void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,
+                          int max_tile_cap, int ntypes1, int reduce) {
+    const size_t lds = (size_t) max_tile_cap * sizeof(int);
+    if (ntypes1) {
+        if (reduce) hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 1, 1, true>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
+        else hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 1, 1, false>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
+    } else {
+        if (reduce) hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 0, 1, true>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
+        else hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 0, 1, false>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
+    }
 }
 
 template <int NT>
@@ -1437,18 +1557,18 @@ void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm) {
                        d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt, prm.hdt);
 }
 
-void launch_vv_post(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
-                    int reduce) {
+void launch_vv_post(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
+                    int do_vv2, int do_vv1, int reduce) {
     if (d.nlocal <= 0) return;
     const int grid = (d.nlocal + 255) / 256;
     if (reduce)
         hipLaunchKernelGGL(k_vv_post<true>, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[next],
                            d.Y[next], d.Z[next], d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt,
-                           prm.hdt, do_vv2, do_vv1, d.partials);
+                           prm.hdt, do_vv2, do_vv1, d.partials, images_of(d, do_vv1 && next != cur), g);
     else
         hipLaunchKernelGGL(k_vv_post<false>, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[next],
                            d.Y[next], d.Z[next], d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt,
-                           prm.hdt, do_vv2, do_vv1, d.partials);
+                           prm.hdt, do_vv2, do_vv1, d.partials, images_of(d, do_vv1 && next != cur), g);
 }
 
 void launch_ke_partial(hipStream_t s, const DevState &d) {

@@ -44,7 +44,8 @@ enum {
     C_REQ_PART = 15,
     C_TILE_MAX = 16,  // largest segment tile of the current lists
     C_UNSUPPORTED = 17,
-    C_COUNT = 18
+    C_IMG_OVF = 18,   // a local has more periodic images than IMG_MAX
+    C_COUNT = 19
 };
 enum { OVF_NONE = 0, OVF_PART = 1, OVF_NEIGH = 2, OVF_SEND = 3, OVF_TILE = 4 };
 
@@ -67,6 +68,19 @@ struct Geom {
 __host__ __device__ __forceinline__ size_t nl_at(int k, int i, int nl_stride) {
     return ((size_t) (k >> 2) * nl_stride + i) * 4 + (k & 3);
 }
+
+constexpr int IMG_MAX = 8;
+
+// what a position-advancing kernel needs to refresh the ghost images
+struct Images {
+    const int *cnt, *slot, *src, *mult;
+    int cap, on;
+};
+
+struct LJTab {
+    int nt;
+    double eps[16], sig6[16];  // ntypes <= 4
+};
 
 struct LJParams {
     double rc2, dt, hdt;
@@ -95,6 +109,12 @@ struct DevState {
     int *send_map, *send_mult, *exchg_flag;
     int *sync_src;       // per ghost (physical order): physical source slot
     int *sync_mult;      // per ghost (physical order): 3 image multipliers
+    // periodic images of each local (its ghosts, and ghosts of those): the
+    // kernel that advances a local's position also writes its images' next
+    // positions, so no separate ghost refresh runs between steps
+    int *img_cnt;        // per physical local
+    int *img_slot;       // [k][cap]: ghost index (physical - nlocal) of image k
+    int img_on;          // host: every local's images fit IMG_MAX
     int *tile_cnt;
     int *counters;
     int *scratch_i;      // 4 * send_cap
@@ -124,10 +144,36 @@ void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJPar
 void launch_lj_half(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce);
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom);
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm);
-void launch_vv_post(hipStream_t s, const DevState &d, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
-                    int reduce);
+void launch_vv_post(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
+                    int do_vv2, int do_vv1, int reduce);
 void launch_ke_partial(hipStream_t s, const DevState &d);
 void launch_thermo_reduce(hipStream_t s, const DevState &d, int nblocks, int which_mask, const Geom &g, double *out);
 int force_blocks(const DevState &d, const Geom &g);
+
+// every argument of the full-list force kernel (one system)
+struct LJArgs {
+    const int *counters;
+    Geom g;
+    const int *lstart, *gstart;
+    const double *X, *Y, *Z;
+    double *Xn, *Yn, *Zn;
+    const int *type;
+    const uint16_t *nl;
+    int nl_stride;
+    const int *numneighs, *flags;
+    double *fx, *fy, *fz, *vx, *vy, *vz;
+    const double *mass;
+    double rc2, dt, hdt;
+    LJTab tab;
+    int do_vv2, do_vv1, pstride, tcap;
+    double *partials, *eatom;
+    Images im;
+};
+// the fused force + integrate arguments of one system (cur -> next buffer)
+LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
+               int fused, double *eatom);
+// fused force + integrate for many systems: block b is segment b - seg_base[rep[b]] of system rep[b]
+void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,
+                          int max_tile_cap, int ntypes1, int reduce);
 
 }  // namespace pk

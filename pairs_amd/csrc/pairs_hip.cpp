@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/pairs_hip.h"
+#include "../../include/pairs_hip_batch.h"
 #include "md_kernels.hpp"
 
 using namespace pk;
@@ -36,11 +37,13 @@ struct pairs_hip_ctx {
     int ghost_grid = 1;
     int max_lds = 65536;       // shared memory a workgroup may allocate
     int zc_override = 0;
+    int home_target = 112;     // home particles a segment aims at (sets zc)
     // graphs of one non-rebuild step, by parity of the current buffer
     bool use_graphs = true;
     hipGraphExec_t step_graph[2] = {nullptr, nullptr};
     int graph_ghost_grid[2] = {0, 0};
     int graph_tile_cap[2] = {0, 0};
+    int graph_img[2] = {0, 0};
     // force-kernel timing
     bool time_force = false;
     int time_every = 10;
@@ -102,7 +105,7 @@ static void free_particle_arrays(DevState &d) {
     for (auto p : dd) { dfree(*p); *p = nullptr; }
     int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
                   &d.alt_shape, &d.alt_ref, &d.alt_cell, &d.numneighs, &d.particle_cell, &d.slot, &d.perm,
-                  &d.exchg_flag, &d.tile_cnt, &d.sync_src, &d.sync_mult};
+                  &d.exchg_flag, &d.tile_cnt, &d.sync_src, &d.sync_mult, &d.img_cnt, &d.img_slot};
     for (auto p : ii) { dfree(*p); *p = nullptr; }
 }
 
@@ -116,6 +119,8 @@ static int alloc_particle_arrays(pairs_hip_ctx *ctx, DevState &d, int cap) {
                   &d.exchg_flag, &d.sync_src};
     for (auto p : ii) r |= dalloc(ctx, p, cap);
     r |= dalloc(ctx, &d.sync_mult, 3 * (size_t) cap);
+    r |= dalloc(ctx, &d.img_cnt, cap);
+    r |= dalloc(ctx, &d.img_slot, (size_t) IMG_MAX * cap);
     r |= dalloc(ctx, &d.tile_cnt, 2 * ((size_t) cap / 256 + 2));
     d.cap = cap;
     d.pstride = (int) pstride_of(d, ctx->geom);
@@ -164,15 +169,15 @@ static int zero_particle_arrays(pairs_hip_ctx *ctx) {
     return 0;
 }
 
-// segments: zc cells of one z-column, sized for about 112 home particles per
-// 128-thread block (override: PAIRS_HIP_ZC)
+// segments: zc cells of one z-column, sized for about home_target (112 by
+// default) home particles per 128-thread block (override: PAIRS_HIP_ZC)
 static void setup_segments(pairs_hip_ctx *ctx, int zc) {
     Geom &g = ctx->geom;
     if (zc <= 0) {
         const double vol = g.len[0] * g.len[1] * g.len[2];
         const double s3 = g.spacing * g.spacing * g.spacing;
         const double per_cell = ctx->d.nlocal > 0 && vol > 0 ? ctx->d.nlocal * s3 / vol : 16.0;
-        zc = (int) std::lround(112.0 / std::max(per_cell, 1e-3));
+        zc = (int) std::lround((double) ctx->home_target / std::max(per_cell, 1e-3));
     }
     zc = std::max(1, std::min(zc, std::max(1, g.dim_cells[2] - 1)));
     g.zc = zc;
@@ -774,7 +779,7 @@ int pairs_hip_vv_initial(pairs_hip_ctx *ctx) {
 int pairs_hip_vv_final(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     DevState &d = ctx->d;
-    launch_vv_post(ctx->stream, d, ctx->prm, d.cur, d.cur, 1, 0, 0);
+    launch_vv_post(ctx->stream, d, ctx->geom, ctx->prm, d.cur, d.cur, 1, 0, 0);
     CHECK_LAUNCH();
     return 0;
 }
@@ -825,11 +830,15 @@ static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 |
 static int rebuild(pairs_hip_ctx *ctx) {
     for (int attempt = 0; attempt < 8; attempt++) {
         hipStream_t s = ctx->stream;
+        launch_set_counters(s, ctx->d.counters, C_IMG_OVF, 1, 0);
         launch_exchange(s, ctx->d, ctx->geom, ctx->cfg.periodic);
         launch_borders(s, ctx->d, ctx->geom, ctx->cfg.periodic);
         launch_bin_sort(s, ctx->d, ctx->geom);
         CHECK_LAUNCH();
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+        // images refresh the ghosts from the position kernels unless some
+        // local has too many of them (then a separate refresh runs)
+        ctx->d.img_on = ctx->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
         int code = take_overflow(ctx);
         if (code == 0) {
             code = build_neighbors(ctx);
@@ -860,7 +869,7 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
     }
     if (ctx->cfg.half) launch_lj_half(s, d, ctx->geom, ctx->prm, d.cur, 0);
     else launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
-    if (ts > 0) launch_vv_post(s, d, ctx->prm, d.cur, d.cur, 1, 0, 0);
+    if (ts > 0) launch_vv_post(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 1, 0, 0);
     CHECK_LAUNCH();
     return 0;
 }
@@ -872,12 +881,12 @@ static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int
     DevState &d = ctx->d;
     hipStream_t s = ctx->stream;
     const int other = 1 - cur;
-    launch_sync_pp(s, d, ctx->geom, cur, other, ctx->ghost_grid);
+    if (!d.img_on) launch_sync_pp(s, d, ctx->geom, cur, other, ctx->ghost_grid);
     if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
     if (ctx->cfg.half) {
         launch_lj_half(s, d, ctx->geom, ctx->prm, cur, reduce);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-        launch_vv_post(s, d, ctx->prm, cur, other, vv2, vv1, reduce);
+        launch_vv_post(s, d, ctx->geom, ctx->prm, cur, other, vv2, vv1, reduce);
     } else {
         launch_lj_full(s, d, ctx->geom, ctx->prm, cur, other, 1, vv2, vv1, reduce);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
@@ -918,7 +927,7 @@ int pairs_hip_force_kernel_time(pairs_hip_ctx *ctx, double *avg_ms, int *launche
 
 static int graph_step(pairs_hip_ctx *ctx, int cur) {
     if (!ctx->step_graph[cur] || ctx->graph_ghost_grid[cur] < ctx->ghost_grid ||
-        ctx->graph_tile_cap[cur] != ctx->d.tile_cap) {
+        ctx->graph_tile_cap[cur] != ctx->d.tile_cap || ctx->graph_img[cur] != ctx->d.img_on) {
         if (ctx->step_graph[cur]) hipGraphExecDestroy(ctx->step_graph[cur]);
         ctx->step_graph[cur] = nullptr;
         // capture with headroom so that later rebuilds with a few more ghosts reuse it
@@ -932,6 +941,7 @@ static int graph_step(pairs_hip_ctx *ctx, int cur) {
         hipGraphDestroy(g);
         ctx->graph_ghost_grid[cur] = ctx->ghost_grid;
         ctx->graph_tile_cap[cur] = ctx->d.tile_cap;
+        ctx->graph_img[cur] = ctx->d.img_on;
         ctx->ghost_grid = saved;
         if (e != hipSuccess) {
             ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
@@ -978,7 +988,7 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             if (ctx->cfg.half) {
                 launch_lj_half(s, d, ctx->geom, ctx->prm, cur, T ? 1 : 0);
                 if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-                launch_vv_post(s, d, ctx->prm, cur, other, vv2, vv1, T ? 1 : 0);
+                launch_vv_post(s, d, ctx->geom, ctx->prm, cur, other, vv2, vv1, T ? 1 : 0);
             } else {
                 launch_lj_full(s, d, ctx->geom, ctx->prm, cur, other, 1, vv2, vv1, T ? 1 : 0);
                 if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
@@ -1009,6 +1019,368 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         ctx->force_launches = ctx->ev_used;
     }
     return nthermo;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batched ensemble (include/pairs_hip_batch.h)
+
+struct pairs_hip_batch {
+    std::vector<pairs_hip_ctx *> rep;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // descriptors of the fused force kernel, one per replica, for the two
+    // buffer phases after a rebuild (phase p: replica r reads buffer base_cur[r] ^ p)
+    std::vector<int> base_cur;
+    LJArgs *h_desc = nullptr;          // pinned staging, 3 * R
+    LJArgs *d_desc[3] = {nullptr, nullptr, nullptr};  // phase 0, phase 1, one-off
+    int *d_rep = nullptr, *d_base = nullptr;
+    int nblocks = 0, blocks_cap = 0, max_tile = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_blocks[2] = {0, 0}, graph_tile[2] = {0, 0};
+    std::vector<double> thermo;        // last thermo row per replica
+    bool ntypes1 = true;
+    int pending_home = 0;              // segment size to apply at the next rebuild
+};
+
+static void batch_drop_graphs(pairs_hip_batch *b) {
+    for (int p = 0; p < 2; p++) {
+        if (b->graph[p]) hipGraphExecDestroy(b->graph[p]);
+        b->graph[p] = nullptr;
+    }
+}
+
+#define BATCH_OK(expr)                                                                                 \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            b->err = std::string(#expr) + ": " + hipGetErrorString(e_);                                \
+            return PAIRS_HIP_ERR_HIP;                                                                  \
+        }                                                                                              \
+    } while (0)
+
+// block -> replica tables after (re)segmentation
+static int batch_tables(pairs_hip_batch *b) {
+    const int R = (int) b->rep.size();
+    std::vector<int> base(R + 1);
+    int tot = 0, mt = 0;
+    for (int r = 0; r < R; r++) {
+        base[r] = tot;
+        tot += b->rep[r]->geom.nsegs;
+        mt = std::max(mt, b->rep[r]->d.tile_cap);
+    }
+    base[R] = tot;
+    if (tot > b->blocks_cap) {
+        if (b->d_rep) hipFree(b->d_rep);
+        b->blocks_cap = tot + tot / 4 + 64;
+        BATCH_OK(hipMalloc((void **) &b->d_rep, b->blocks_cap * sizeof(int)));
+    }
+    std::vector<int> rep(std::max(tot, 1));
+    for (int r = 0; r < R; r++)
+        for (int k = base[r]; k < base[r + 1]; k++) rep[k] = r;
+    BATCH_OK(hipMemcpyAsync(b->d_rep, rep.data(), tot * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BATCH_OK(hipMemcpyAsync(b->d_base, base.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BATCH_OK(hipStreamSynchronize(b->stream));
+    b->nblocks = tot;
+    b->max_tile = mt;
+    return 0;
+}
+
+// descriptors of both phases (steady steps: final and initial integrate on)
+static int batch_descs(pairs_hip_batch *b) {
+    const int R = (int) b->rep.size();
+    b->base_cur.resize(R);
+    for (int p = 0; p < 2; p++)
+        for (int r = 0; r < R; r++) {
+            pairs_hip_ctx *c = b->rep[r];
+            const int cur = c->d.cur ^ p;
+            b->h_desc[p * R + r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, 1, 1, 1, nullptr);
+        }
+    for (int r = 0; r < R; r++) b->base_cur[r] = b->rep[r]->d.cur;
+    for (int p = 0; p < 2; p++)
+        BATCH_OK(hipMemcpyAsync(b->d_desc[p], b->h_desc + p * R, R * sizeof(LJArgs), hipMemcpyHostToDevice, b->stream));
+    BATCH_OK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+// every replica's rebuild kernels back to back, one synchronisation per
+// phase; replicas that overflow fall back to their own retry loop
+static int batch_rebuild(pairs_hip_batch *b) {
+    const int R = (int) b->rep.size();
+    hipStream_t s = b->stream;
+    if (b->pending_home > 0) {
+        // a new segment size: segments and lists change together, here
+        for (pairs_hip_ctx *c : b->rep) {
+            c->home_target = b->pending_home;
+            if (c->zc_override <= 0) setup_segments(c, 0);
+            drop_graphs(c);
+        }
+        batch_drop_graphs(b);
+        b->pending_home = 0;
+    }
+    for (pairs_hip_ctx *c : b->rep) {
+        launch_set_counters(s, c->d.counters, C_IMG_OVF, 1, 0);
+        launch_exchange(s, c->d, c->geom, c->cfg.periodic);
+        launch_borders(s, c->d, c->geom, c->cfg.periodic);
+        launch_bin_sort(s, c->d, c->geom);
+    }
+    BATCH_OK(hipStreamSynchronize(s));
+    std::vector<char> fallback(R, 0);
+    for (int r = 0; r < R; r++) {
+        pairs_hip_ctx *c = b->rep[r];
+        if (read_counters(c)) { b->err = c->err; return PAIRS_HIP_ERR_HIP; }
+        c->d.img_on = c->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
+        if (take_overflow(c)) fallback[r] = 1;
+        else {
+            launch_set_counters(s, c->d.counters, C_TILE_MAX, 1, 0);
+            launch_build_neighbors(s, c->d, c->geom, c->cfg.half, c->cfg.cell_spacing * c->cfg.cell_spacing);
+        }
+    }
+    BATCH_OK(hipStreamSynchronize(s));
+    for (int r = 0; r < R; r++) {
+        pairs_hip_ctx *c = b->rep[r];
+        if (!fallback[r]) {
+            if (read_counters(c)) { b->err = c->err; return PAIRS_HIP_ERR_HIP; }
+            if (c->h_counters[C_OVF] == OVF_TILE || c->h_counters[C_REQ_NEIGH] > 0) {
+                launch_set_counters(s, c->d.counters, C_OVF, 1, 0);
+                launch_set_counters(s, c->d.counters, C_REQ_NEIGH, 3, 0);
+                fallback[r] = 1;
+            } else {
+                set_tile_cap(c);
+            }
+        }
+        if (fallback[r]) {
+            drop_graphs(c);
+            const int code = rebuild(c);
+            if (code) { b->err = "replica " + std::to_string(r) + ": " + c->err; return code; }
+        }
+    }
+    int e = batch_tables(b);
+    if (!e) e = batch_descs(b);
+    return e;
+}
+
+static int batch_graph_launch(pairs_hip_batch *b, int phase) {
+    if (!b->graph[phase] || b->graph_blocks[phase] != b->nblocks || b->graph_tile[phase] != b->max_tile) {
+        if (b->graph[phase]) hipGraphExecDestroy(b->graph[phase]);
+        b->graph[phase] = nullptr;
+        hipGraph_t g;
+        BATCH_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (size_t r = 0; r < b->rep.size(); r++) {
+            pairs_hip_ctx *c = b->rep[r];
+            if (!c->d.img_on) {
+                const int cur = b->base_cur[r] ^ phase;
+                launch_sync_pp(b->stream, c->d, c->geom, cur, 1 - cur, c->ghost_grid);
+            }
+        }
+        launch_lj_full_batch(b->stream, b->d_desc[phase], b->d_rep, b->d_base, b->nblocks, b->max_tile, b->ntypes1, 0);
+        BATCH_OK(hipStreamEndCapture(b->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&b->graph[phase], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) { b->graph[phase] = nullptr; b->err = "batch graph instantiation failed"; return PAIRS_HIP_ERR_HIP; }
+        b->graph_blocks[phase] = b->nblocks;
+        b->graph_tile[phase] = b->max_tile;
+    }
+    BATCH_OK(hipGraphLaunch(b->graph[phase], b->stream));
+    return 0;
+}
+
+extern "C" {
+
+int pairs_hip_batch_create(pairs_hip_batch **out, int nreplicas, const pairs_hip_md_config *cfgs) {
+    if (!out || nreplicas <= 0 || !cfgs) return PAIRS_HIP_ERR_ARG;
+    *out = nullptr;
+    for (int r = 1; r < nreplicas; r++)
+        if (cfgs[r].reneighbor_every != cfgs[0].reneighbor_every || cfgs[r].thermo_every != cfgs[0].thermo_every ||
+            cfgs[r].half || cfgs[0].half || cfgs[r].ntypes != cfgs[0].ntypes)
+            return PAIRS_HIP_ERR_ARG;
+    pairs_hip_batch *b = new pairs_hip_batch();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return PAIRS_HIP_ERR_HIP; }
+    b->ntypes1 = cfgs[0].ntypes == 1;
+    for (int r = 0; r < nreplicas; r++) {
+        pairs_hip_md_config c = cfgs[r];
+        c.stream = (void *) b->stream;
+        pairs_hip_ctx *ctx = nullptr;
+        const int e = pairs_hip_create(&ctx, &c);
+        if (e) { pairs_hip_batch_destroy(b); return e; }
+        b->rep.push_back(ctx);
+    }
+    const size_t R = nreplicas;
+    if (hipHostMalloc((void **) &b->h_desc, 2 * R * sizeof(LJArgs)) != hipSuccess ||
+        hipMalloc((void **) &b->d_desc[0], R * sizeof(LJArgs)) != hipSuccess ||
+        hipMalloc((void **) &b->d_desc[1], R * sizeof(LJArgs)) != hipSuccess ||
+        hipMalloc((void **) &b->d_desc[2], R * sizeof(LJArgs)) != hipSuccess ||
+        hipMalloc((void **) &b->d_base, (R + 1) * sizeof(int)) != hipSuccess) {
+        pairs_hip_batch_destroy(b);
+        return PAIRS_HIP_ERR_HIP;
+    }
+    b->thermo.assign(5 * R, 0.0);
+    *out = b;
+    return 0;
+}
+
+int pairs_hip_batch_destroy(pairs_hip_batch *b) {
+    if (!b) return 0;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_drop_graphs(b);
+    for (pairs_hip_ctx *c : b->rep) pairs_hip_destroy(c);
+    if (b->h_desc) hipHostFree(b->h_desc);
+    for (int p = 0; p < 3; p++) if (b->d_desc[p]) hipFree(b->d_desc[p]);
+    if (b->d_rep) hipFree(b->d_rep);
+    if (b->d_base) hipFree(b->d_base);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+const char *pairs_hip_batch_last_error(const pairs_hip_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+pairs_hip_ctx *pairs_hip_batch_replica(pairs_hip_batch *b, int r) {
+    if (!b || r < 0 || r >= (int) b->rep.size()) return nullptr;
+    return b->rep[r];
+}
+
+int pairs_hip_batch_run(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps) {
+    if (!b || ts_begin < 0 || ts_end > nsteps + 1 || ts_begin > ts_end) return PAIRS_HIP_ERR_ARG;
+    const int R = (int) b->rep.size();
+    hipStream_t s = b->stream;
+    pairs_hip_ctx *c0 = b->rep[0];
+    bool prev_in_other = false;
+    int phase = 0;
+    if (b->nblocks == 0) {  // first use: tables and descriptors from the current segmentation
+        int e = batch_tables(b);
+        if (!e) e = batch_descs(b);
+        if (e) return e;
+    }
+    for (int ts = ts_begin; ts < ts_end; ts++) {
+        const bool Rb = every(ts, c0->cfg.reneighbor_every);
+        const bool T = every(ts, c0->cfg.thermo_every);
+        if (ts == ts_begin && ts > 0)
+            for (pairs_hip_ctx *c : b->rep) launch_vv_initial(s, c->d, c->prm);
+        const int vv2 = ts > 0, vv1 = (ts + 1 < ts_end) && (ts + 1 <= nsteps);
+        if (Rb) {
+            const int e = batch_rebuild(b);
+            if (e) return e;
+            prev_in_other = false;
+            phase = 0;
+        }
+        if (!Rb && !prev_in_other) {
+            // the first step of a run: ghosts refreshed in place, and the
+            // descriptors follow each replica's current buffer
+            for (pairs_hip_ctx *c : b->rep) launch_sync_inplace(s, c->d, c->geom, c->ghost_grid, 0);
+            const int e = batch_descs(b);
+            if (e) return e;
+            phase = 0;
+        }
+        if (prev_in_other && !Rb) {
+            for (int r = 0; r < R; r++) {
+                pairs_hip_ctx *c = b->rep[r];
+                if (!c->d.img_on && (!vv1 || T || !vv2)) {
+                    const int cur = b->base_cur[r] ^ phase;
+                    launch_sync_pp(s, c->d, c->geom, cur, 1 - cur, c->ghost_grid);
+                }
+            }
+        }
+        if (vv1 && vv2 && !T && prev_in_other && !Rb) {
+            const int e = batch_graph_launch(b, phase);
+            if (e) return e;
+        } else {
+            for (int r = 0; r < R; r++) {
+                pairs_hip_ctx *c = b->rep[r];
+                const int cur = b->base_cur[r] ^ phase;
+                b->h_desc[r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, vv2, vv1, 1, nullptr);
+            }
+            BATCH_OK(hipMemcpyAsync(b->d_desc[2], b->h_desc, R * sizeof(LJArgs), hipMemcpyHostToDevice, s));
+            BATCH_OK(hipStreamSynchronize(s));  // h_desc is reused by the next descriptor build
+            launch_lj_full_batch(s, b->d_desc[2], b->d_rep, b->d_base, b->nblocks, b->max_tile, b->ntypes1, T ? 1 : 0);
+            if (T) {
+                for (int r = 0; r < R; r++) {
+                    pairs_hip_ctx *c = b->rep[r];
+                    launch_thermo_reduce(s, c->d, c->geom.nsegs, 3, c->geom, c->d.thermo);
+                }
+            }
+        }
+        {
+            hipError_t e_ = hipGetLastError();
+            if (e_ != hipSuccess) { b->err = std::string("kernel launch: ") + hipGetErrorString(e_); return PAIRS_HIP_ERR_HIP; }
+        }
+        if (T) {
+            BATCH_OK(hipStreamSynchronize(s));
+            for (int r = 0; r < R; r++)
+                BATCH_OK(hipMemcpy(&b->thermo[5 * r], b->rep[r]->d.thermo, 5 * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        if (vv1) {
+            for (pairs_hip_ctx *c : b->rep) c->d.cur = 1 - c->d.cur;
+            phase ^= 1;
+            prev_in_other = true;
+        }
+    }
+    BATCH_OK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int pairs_hip_batch_run_each(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps) {
+    if (!b) return PAIRS_HIP_ERR_ARG;
+    for (size_t r = 0; r < b->rep.size(); r++) {
+        double th[5 * 16];
+        const int n = pairs_hip_run(b->rep[r], ts_begin, ts_end, nsteps, th, 16);
+        if (n < 0) { b->err = "replica " + std::to_string(r) + ": " + b->rep[r]->err; return n; }
+        if (n > 0) for (int k = 0; k < 5; k++) b->thermo[5 * r + k] = th[5 * (std::min(n, 16) - 1) + k];
+    }
+    b->nblocks = 0;  // per-replica runs may have re-segmented: rebuild tables on the next batched run
+    return 0;
+}
+
+int pairs_hip_batch_set_home(pairs_hip_batch *b, int home) {
+    if (!b || home < 8) return PAIRS_HIP_ERR_ARG;
+    b->pending_home = home;
+    return 0;
+}
+
+int pairs_hip_batch_autotune(pairs_hip_batch *b, int ts_begin, int steps, int nsteps, int ncand, const int *cand,
+                             double *ms_per_step, int *chosen, int *ts_next) {
+    if (!b || ncand <= 0 || !cand || steps <= 0) return PAIRS_HIP_ERR_ARG;
+    const int every_r = b->rep[0]->cfg.reneighbor_every;
+    int best = -1;
+    double best_ms = 1e300;
+    int ts = ts_begin;
+    hipEvent_t e0, e1;
+    BATCH_OK(hipEventCreate(&e0));
+    BATCH_OK(hipEventCreate(&e1));
+    for (int k = 0; k < ncand; k++) {
+        // advance to the next rebuild step (where a new segment size takes
+        // effect), rebuild with the candidate, then time `steps` steps
+        int tr = ts;
+        while (tr <= nsteps && !every(tr, every_r)) tr++;
+        if (tr + 1 + steps > nsteps + 1) break;
+        int e = tr > ts ? pairs_hip_batch_run(b, ts, tr, nsteps) : 0;
+        if (!e) e = pairs_hip_batch_set_home(b, cand[k]);
+        if (!e) e = pairs_hip_batch_run(b, tr, tr + 1, nsteps);
+        if (e) return e;
+        BATCH_OK(hipEventRecord(e0, b->stream));
+        e = pairs_hip_batch_run(b, tr + 1, tr + 1 + steps, nsteps);
+        if (e) return e;
+        BATCH_OK(hipEventRecord(e1, b->stream));
+        BATCH_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        BATCH_OK(hipEventElapsedTime(&ms, e0, e1));
+        ts = tr + 1 + steps;
+        const double per = ms / steps;
+        if (ms_per_step) ms_per_step[k] = per;
+        if (per < best_ms) { best_ms = per; best = k; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (ts_next) *ts_next = ts;
+    if (best < 0) { b->err = "autotune: not enough timesteps for the candidates"; return PAIRS_HIP_ERR_ARG; }
+    if (chosen) *chosen = cand[best];
+    return pairs_hip_batch_set_home(b, cand[best]);
+}
+
+int pairs_hip_batch_thermo(pairs_hip_batch *b, double *out) {
+    if (!b || !out) return PAIRS_HIP_ERR_ARG;
+    std::memcpy(out, b->thermo.data(), b->thermo.size() * sizeof(double));
+    return 0;
 }
 
 }  // extern "C"

@@ -80,7 +80,23 @@ class LJSystem:
         self.L = L
 
     # ------------------------------------------------------------ lifetime
+    @classmethod
+    def attach(cls, handle, cfg, box, owner):
+        """A view of a context owned elsewhere (a batch replica); never destroys it."""
+        sim = cls.__new__(cls)
+        sim.L = nat.lib()
+        sim.h = handle
+        sim.cfg = cfg
+        sim.ntypes = cfg.ntypes
+        sim.half = bool(cfg.half)
+        sim.box = [tuple(b) for b in box]
+        sim._owner = owner
+        return sim
+
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self.h = None
+            return
         if getattr(self, "h", None):
             self.L.pairs_hip_destroy(self.h)
             self.h = None

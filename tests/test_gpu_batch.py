@@ -1,0 +1,82 @@
+"""The batched ensemble (include/pairs_hip_batch.h): every replica's
+trajectory is the one its own single-system run (and the CPU oracle) gives --
+bit-identical positions, velocities and forces -- whether the replicas advance
+in one fused launch per step, in a per-replica launch loop, or after the
+autotuner changed their segment size."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [500, 1372, 2100, 3001, 864, 4000]
+RHO = [0.70, 0.90, 0.8442, 0.75, 0.82, 0.88]
+TEMP = [0.8, 2.0, 1.44, 1.1, 1.7, 1.0]
+
+
+def _batch():
+    from pairs_amd import batch
+    return batch
+
+
+def _single(b, r):
+    """replica r as its own LJSystem with the same initial state"""
+    from pairs_amd import md
+    n = b.sizes[r]
+    L = b.boxes[r]
+    s = md.LJSystem([(0.0, L)] * 3, particle_capacity=8 * n + 4096, send_capacity=8 * n + 4096,
+                    neighbor_capacity=128)
+    x, v = b.initial[r]
+    s.upload(x, v)
+    s._call("pairs_hip_setup_cells")
+    return s
+
+
+def _same(a, b, tag):
+    assert np.array_equal(a.positions(), b.positions()), tag
+    assert np.array_equal(a.velocities(), b.velocities()), tag
+    assert np.array_equal(a.forces(), b.forces()), tag
+
+
+def test_batched_equals_single_systems():
+    B = _batch().BatchEnsemble(SIZES, RHO, TEMP)
+    singles = [_single(B, r) for r in range(len(SIZES))]
+    for a, e in ((0, 1), (1, 45), (45, 122)):
+        B.run(a, e, 300)
+        for s in singles:
+            s.run(a, e, 300)
+        for r, s in enumerate(singles):
+            _same(B.replicas[r], s, (r, e))
+    th = B.thermo()
+    assert np.all(th[:, 0] > 0)
+
+
+def test_batched_equals_oracle():
+    B = _batch().BatchEnsemble(SIZES[:3], RHO[:3], TEMP[:3])
+    B.run(0, 81, 300)
+    for r in range(3):
+        L = B.boxes[r]
+        o = po.OracleMD([(0.0, L)] * 3, 8 * B.sizes[r] + 4096, ncap=128, send_cap=8 * B.sizes[r] + 4096)
+        x, v = B.initial[r]
+        o.set_particles(x, v)
+        o.setup_cells()
+        o.run(0, 81)
+        g = B.replicas[r]
+        assert np.array_equal(g.positions(True), o.positions(True)), r
+        assert np.array_equal(g.velocities(), o.velocities()), r
+        assert np.array_equal(g.forces(), o.forces()), r
+
+
+def test_run_each_and_autotune_keep_trajectories():
+    bm = _batch()
+    A = bm.BatchEnsemble(SIZES, RHO, TEMP)
+    C = bm.BatchEnsemble(SIZES, RHO, TEMP)
+    A.run(0, 1, 400)
+    chosen, times, ts = A.autotune(1, 10, 400, candidates=(48, 112, 200))
+    assert chosen in (48, 112, 200) and all(t > 0 for t in times.values())
+    assert ts == 70  # rebuilds at 19, 39, 59, ten timed steps after each
+    A.run(ts, 121, 400)
+    C.run_each(0, 121, 400)
+    for r in range(len(SIZES)):
+        _same(A.replicas[r], C.replicas[r], r)
