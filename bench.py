@@ -41,6 +41,23 @@ def force_kernel_bytes(n, g, s, fused):
     return b
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "lj_fcc32_force_pmc.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the fused force kernel, as measured by the
+    FETCH_SIZE / WRITE_SIZE counter runs (scaled by a copy of known size)
+    recorded in the profile summary; None when the summary is absent."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
+                    return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_gpu(steps, warmup, half, time_every):
     from pairs_amd import md
 
@@ -128,7 +145,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": profiled_traffic(),
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a "
+                              "512 MiB copy (profiles/lj_fcc32_force_pmc.md)",
             "kernel": "k_lj_full<8,true,false> (LJ force + final/initial integrate, fused)",
             "bytes_per_launch": fused_bytes,
             "lj_only_bytes_per_launch": lj_bytes,
