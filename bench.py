@@ -11,7 +11,12 @@ value = atoms x steps / wall time of the timed steps (inputs resident in HBM,
 set-up excluded), full neighbour list; the half-list form, the force kernel's
 HBM roofline and the CPU baseline are reported beside it.
 
+Beside it: the batched ensemble (512 uneven replicas), the MLP potential at
+131072 atoms (training, force check, MD throughput) and the DEM path on a
+larger domain, each as its own object of the same JSON line.
+
   python bench.py [--gpus 1] [--steps 1000] [--warmup 100] [--no-half] [--no-cpu]
+                  [--no-batched] [--no-mlp] [--no-dem]
 """
 import argparse
 import json
@@ -79,6 +84,130 @@ def run_gpu(steps, warmup, half, time_every):
                 thermo=thermo.tolist())
 
 
+def run_batched(steps, warmup, nreplicas):
+    """The uneven replica ensemble: batched (one launch per step, graphs,
+    autotuned segment size) against a per-replica launch loop over the same
+    replicas and steps."""
+    from pairs_amd import batch
+
+    t0 = time.perf_counter()
+    B = batch.uneven_ensemble(nreplicas)
+    setup_s = time.perf_counter() - t0
+    natoms = B.natoms
+    nsteps = 100000  # schedule length: longer than every range run here
+    B.run(0, warmup, nsteps)
+    chosen, cand_ms, ts = B.autotune(warmup, 10, nsteps)
+    B.run(ts, ts + 20, nsteps)  # re-enter the graph path after tuning
+    ts += 20
+    t0 = time.perf_counter()
+    B.run(ts, ts + steps, nsteps)
+    t_batched = time.perf_counter() - t0
+    ts += steps
+    t0 = time.perf_counter()
+    B.run_each(ts, ts + steps, nsteps)
+    t_each = time.perf_counter() - t0
+    B.close()
+    return {
+        "value": natoms * steps / t_batched,
+        "unit": "atom-steps/s",
+        "per_replica_loop_value": natoms * steps / t_each,
+        "speedup_vs_per_replica_loop": t_each / t_batched,
+        "ms_per_step": t_batched * 1e3 / steps,
+        "replicas": nreplicas,
+        "atoms": natoms,
+        "steps": steps,
+        "autotune": {"chosen_home_per_segment": chosen, "ms_per_step": cand_ms},
+        "setup_s": setup_s,
+        "workload": f"{nreplicas} periodic LJ replicas, N ~ U[500, 20000], rho ~ U[0.70, 0.90], "
+                    f"T ~ U[0.8, 2.0], full lists, rebuild every 20, FP64",
+    }
+
+
+def run_mlp(steps, warmup, train_steps):
+    """The MLP potential at 131072 atoms: GPU training on LJ snapshots, the
+    force check that decides MLP or LJ fallback, and the MLP MD throughput."""
+    from pairs_amd import md, mlp
+
+    m = mlp.MLPModel(seed=20241202, dataset_capacity=12 * NATOMS)
+    t0 = time.perf_counter()
+    rows = mlp.collect_lj_snapshots(m, lambda T: md.fcc_system(NX, temp=T), (0.8, 1.1, 1.4, 1.7, 2.0),
+                                    equilibrate=200, every=50, per_temp=2)
+    t_collect = time.perf_counter() - t0
+    m.fit_scaling()
+    rmse0 = m.rmse()
+    batch = 4096
+    t0 = time.perf_counter()
+    loss = m.train(train_steps, batch=batch, lr=1e-2, seed=11)
+    t_train = time.perf_counter() - t0
+    rmse1 = m.rmse()
+    # held-out state: plain LJ at T = 1.44 after 300 steps (the last a rebuild)
+    sim = md.fcc_system(NX)
+    sim.run(0, 300, 100000)
+    rel = mlp.force_error(sim, m)
+    attached, _ = mlp.attach(sim, m, 0.02)
+    # MLP MD throughput (attached regardless of the check, to time the path)
+    if not attached:
+        mlp.attach(sim, m, -1.0)
+    sim.run(300, 300 + warmup, 100000)
+    sim.sync()
+    sim.set_force_timing(10)
+    t0 = time.perf_counter()
+    sim.run(300 + warmup, 300 + warmup + steps, 100000)
+    sim.sync()
+    t_md = time.perf_counter() - t0
+    ms, launches = sim.force_kernel_time()
+    sim.close()
+    return {
+        "value": NATOMS * steps / t_md,
+        "unit": "atom-steps/s",
+        "ms_per_step": t_md * 1e3 / steps,
+        "force_eval_ms": ms,
+        "force_rel_rms_vs_lj": rel,
+        "fallback_to_lj": not attached,
+        "fallback_rule": "MLP used only if relative RMS force error vs LJ <= 0.02",
+        "energy_rmse_train_before": rmse0,
+        "energy_rmse_train_after": rmse1,
+        "training": {"rows": rows, "steps": train_steps, "batch": batch,
+                     "samples_per_s": train_steps * batch / t_train, "seconds": t_train,
+                     "final_loss": float(loss[-50:].mean()) if len(loss) else None,
+                     "snapshot_collection_s": t_collect},
+        "workload": "FCC 32^3 = 131072 atoms, rho 0.8442, T 1.44, descriptor 16 Gaussians rc 2.5, "
+                    "MLP 16-64-64-1 SiLU BF16 MFMA / FP32, FP64 integration; training on LJ snapshots "
+                    "(T 0.8..2.0, every 50 steps)",
+    }
+
+
+def run_dem(steps):
+    """The DEM path (examples/dem.py physics) on a larger domain: about 191k
+    spheres in 0.35 x 0.35 x 0.2 m, periodic x/y, planes at the bottom and top,
+    timed over the whole run from the initial grid (free fall, impacts and
+    settling on the bottom plane)."""
+    from pairs_amd import dem
+
+    s = dem.reference_system((0.35, 0.35, 0.2), particle_capacity=400000, send_capacity=80000)
+    n = s.nlocal
+    s.set_timing(50)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(0, steps)
+    s.sync()
+    t = time.perf_counter() - t0
+    ms, launches = s.contact_kernel_time()
+    nc = s.get("num_contacts")
+    s.close()
+    return {
+        "value": n * steps / t,
+        "unit": "particle-steps/s",
+        "ms_per_step": t * 1e3 / steps,
+        "contact_kernel_ms": ms,
+        "spheres": n,
+        "steps": steps,
+        "contacts_at_end": int(nc.sum()),
+        "workload": "examples/dem.py parameters (d 2.9 mm, rho 2550, dt 5e-5, e 0.1, mu 0.5, collision time "
+                    "5e-4) on a 0.35 x 0.35 x 0.2 m domain from the SC grid, rebuild every step, FP64",
+    }
+
+
 def run_cpu_baseline(sample_steps):
     """The oracle (single-threaded C restatement of the reference's generated
     CPU program) on a bounded sample of the same workload."""
@@ -100,6 +229,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--time-every", type=int, default=10)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--no-dem", action="store_true")
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--side-steps", type=int, default=200,
+                    help="timed steps of the batched, MLP and DEM lines")
+    ap.add_argument("--mlp-train-steps", type=int, default=12000)
+    ap.add_argument("--dem-steps", type=int, default=10000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once: a kernel of known HBM traffic that "
                          "calibrates FETCH_SIZE / WRITE_SIZE counter runs")
@@ -165,6 +302,13 @@ def main():
             "force_kernel_GBps": hb / (half["force_ms"] * 1e-3) / 1e9 if half["force_ms"] > 0 else None,
             "neighbour_entries": half["nneigh"],
         }
+    if not args.no_batched:
+        rec["batched"] = run_batched(args.side_steps, 40, args.replicas)
+    if not args.no_mlp:
+        rec["mlp"] = run_mlp(args.side_steps, 20, args.mlp_train_steps)
+        rec["mlp"]["vs_plain_lj"] = rec["mlp"]["value"] / value
+    if not args.no_dem:
+        rec["dem"] = run_dem(args.dem_steps)
     if not args.no_cpu:
         v, secs = run_cpu_baseline(args.cpu_steps)
         rec["cpu_baseline"] = {"value": v, "unit": "atom-steps/s", "cores": 1, "kind": "port",

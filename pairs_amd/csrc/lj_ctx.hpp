@@ -1,0 +1,74 @@
+// The LJ context behind include/pairs_hip.h (host side), shared by the C API
+// translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pairs_hip.h"
+#include "md_kernels.hpp"
+#include "mlp_kernels.hpp"
+
+using namespace pk;
+
+struct pairs_hip_ctx {
+    pairs_hip_md_config cfg;
+    LJParams prm;
+    Geom geom;
+    DevState d;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    int req[5] = {0, 0, 0, 0, 0};
+    int h_counters[C_COUNT];
+    int nghost = 0, nsend_all = 0;
+    int ghost_grid = 1;
+    int max_lds = 65536;       // shared memory a workgroup may allocate
+    int zc_override = 0;
+    int home_target = 112;     // home particles a segment aims at (sets zc)
+    // graphs of one non-rebuild step, by parity of the current buffer
+    bool use_graphs = true;
+    hipGraphExec_t step_graph[2] = {nullptr, nullptr};
+    int graph_ghost_grid[2] = {0, 0};
+    int graph_tile_cap[2] = {0, 0};
+    int graph_img[2] = {0, 0};
+    // force-kernel timing
+    bool time_force = false;
+    int time_every = 10;
+    std::vector<hipEvent_t> ev_begin, ev_end;
+    int ev_used = 0;
+    double force_ms_avg = 0.0;
+    int force_launches = 0;
+    double *d_eatom = nullptr;
+    int eatom_cap = 0;
+    int *d_expand = nullptr;
+    size_t expand_cap = 0;
+    // MLP force field (include/pairs_hip_mlp.h): when on, every force
+    // evaluation of the run uses the attached model instead of LJ
+    bool mlp_on = false;
+    MLPBind mlp{};
+    float *mlp_G = nullptr, *mlp_D = nullptr, *mlp_E = nullptr;  // per-slot work arrays
+    double *mlp_f = nullptr;                                      // [3][cap] saved forces
+    double *mlp_red = nullptr;                                    // reduction partials
+    int mlp_cap = 0;
+};
+
+#define HIP_OK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                              \
+            return PAIRS_HIP_ERR_HIP;                                                                  \
+        }                                                                                              \
+    } while (0)
+
+#define CHECK_LAUNCH()                                                                                 \
+    do {                                                                                               \
+        hipError_t e_ = hipGetLastError();                                                             \
+        if (e_ != hipSuccess) {                                                                        \
+            ctx->err = std::string("kernel launch: ") + hipGetErrorString(e_);                         \
+            return PAIRS_HIP_ERR_HIP;                                                                  \
+        }                                                                                              \
+    } while (0)
+

@@ -10,6 +10,9 @@
 // The storage layout (cell-ordered physical slots, reference-index
 // relabeling, segments and their LDS tiles) is described in md_kernels.hpp.
 #include "md_kernels.hpp"
+#include "md_segment.hpp"
+
+#include <stdlib.h>
 
 namespace pk {
 
@@ -59,12 +62,6 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds_waves, int *
     return r;
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 // ---------------------------------------------------------------- geometry
 
 // cell of a position: src/pairs/sim/cell_lists.py:114-126 (generated as
@@ -80,74 +77,6 @@ __device__ __forceinline__ int cell_of(double x, double y, double z, const Geom 
         idx = d == 0 ? c : idx * g.dim_cells[d] + c;
     }
     return idx + 1;
-}
-
-struct SegInfo {
-    int cx, cy, za, zb;  // column and z range [za, zb) of the home cells
-    int ca, cb;          // flat cell ids of the first home cell and one past the last
-};
-
-__device__ __forceinline__ SegInfo seg_info(int seg, const Geom &g) {
-    SegInfo s;
-    const int chunk = seg % g.nchunks, col = seg / g.nchunks;
-    s.cy = col % g.dim_cells[1];
-    s.cx = col / g.dim_cells[1];
-    s.za = chunk == 0 ? 0 : 1 + chunk * g.zc;
-    s.zb = chunk == g.nchunks - 1 ? g.dim_cells[2] : 1 + (chunk + 1) * g.zc;
-    const int base = (s.cx * g.dim_cells[1] + s.cy) * g.dim_cells[2] + 1;
-    s.ca = base + s.za;
-    s.cb = base + s.zb;
-    return s;
-}
-
-constexpr int NRUNS = 20;
-
-// The 20 physical runs of a segment's tile, in the reference's candidate
-// order of cells: cell 0 (locals, ghosts), then the columns (dx, dy) of the
-// stencil (dx outer, dy inner; src/pairs/sim/cell_lists.py:70-87), each over
-// flat cells [B + za - 1, B + zb] clipped to the grid (a stencil cell of the
-// reference is valid iff 0 < nc < ncells, interaction.py:92-98), locals then
-// ghosts.  Written by threads 0..19; r_off is the exclusive prefix (r_off[20]
-// = tile size).  lo of each column is kept for candidate enumeration.
-__device__ __forceinline__ void seg_runs(const SegInfo &si, const Geom &g, const int *__restrict__ lstart,
-                                         const int *__restrict__ gstart, int nlocal, int *r_start, int *r_len,
-                                         int *r_off, int *col_lo) {
-    const int t = threadIdx.x;
-    if (t < NRUNS) {
-        int start = 0, len = 0;
-        if (t < 2) {
-            if (t == 0) { start = lstart[0]; len = lstart[1] - lstart[0]; }
-            else { start = nlocal + gstart[0]; len = gstart[1] - gstart[0]; }
-        } else {
-            const int col = (t - 2) >> 1, loc = ((t - 2) & 1) == 0;
-            const int dx = col / 3 - 1, dy = col % 3 - 1;
-            const int B = ((si.cx + dx) * g.dim_cells[1] + (si.cy + dy)) * g.dim_cells[2] + 1;
-            int lo = B + si.za - 1, hi = B + si.zb;
-            lo = max(lo, 1);
-            hi = min(hi, g.ncells - 1);
-            if (lo <= hi) {
-                if (loc) { start = lstart[lo]; len = lstart[hi + 1] - lstart[lo]; }
-                else { start = nlocal + gstart[lo]; len = gstart[hi + 1] - gstart[lo]; }
-            }
-            if (loc) col_lo[col] = lo;
-        }
-        r_start[t] = start;
-        r_len[t] = len;
-    }
-    __syncthreads();
-    if (t == 0) {
-        int off = 0;
-        for (int r = 0; r < NRUNS; r++) { r_off[r] = off; off += r_len[r]; }
-        r_off[NRUNS] = off;
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ int run_of(const int *r_off, int p) {
-    int r = 0;
-#pragma unroll
-    for (int k = 1; k < NRUNS; k++) r += (p >= r_off[k]) ? 1 : 0;
-    return r;
 }
 
 // ---------------------------------------------------------------- cells (binning by counting sort)
@@ -310,18 +239,10 @@ __global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ count
     }
 }
 
-constexpr int FB = 128;  // threads per force block (two waves)
 
 // A segment's tile as physical slots: entry p of the tile is particle
 // tslot[p] (the force kernels gather positions from HBM through it, so the
 // tile costs 4 B of LDS per entry and many workgroups share a CU).
-__device__ __forceinline__ void stage_slots(const int *r_start, const int *r_off, int T, int *tslot) {
-    for (int p = threadIdx.x; p < T; p += blockDim.x) {
-        const int r = run_of(r_off, p);
-        tslot[p] = r_start[r] + (p - r_off[r]);
-    }
-}
-
 // Images of each local: every ghost's chain of sources ends at its root
 // local; a root gets a slot per image (order irrelevant: positions are
 // written to the next buffer from values of the current one).
@@ -611,69 +532,6 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
     return in;
 }
 
-// Walk a lane's neighbour list eight tile offsets per step, to the
-// wave-uniform depth nmax: body(k, on) gets entries k .. k+7.  A step
-// consumes two packed groups of four entries; the loads of the groups two
-// steps ahead are issued first, in a ring of three register pairs written
-// out (no register moves), so a list load never stalls the step that issues
-// it.  Groups past the list re-read its last group (in bounds, no branch);
-// offsets are clamped into the tile.
-template <int U, typename Body>
-__device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stride, int n, int nmax, int T,
-                                          Body &&body) {
-    static_assert(U == 4 || U == 8, "a step takes one or two packed groups");
-    const uint64_t *lp = (const uint64_t *) nl + i;
-    const int gl = max((n + 3) / 4 - 1, 0), tlast = T - 1;
-    // a relaxed atomic load is an ordered memory access to the compiler: it
-    // stays where it is issued instead of being sunk next to its use
-    auto ld = [&](int gq) {
-        return __hip_atomic_load(lp + (size_t) min(gq, gl) * nl_stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    };
-    auto step = [&](int k, uint64_t a, uint64_t b) {
-        int on[8];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            on[q] = min((int) ((a >> (16 * q)) & 0xffff), tlast);
-            on[4 + q] = min((int) ((b >> (16 * q)) & 0xffff), tlast);
-        }
-        body(k, on);
-    };
-    if (U == 4) {
-        // one group per step, same ring of three
-        auto step4 = [&](int k, uint64_t a) {
-            int on[8];
-#pragma unroll
-            for (int q = 0; q < 4; q++) on[q] = min((int) ((a >> (16 * q)) & 0xffff), tlast);
-            body(k, on);
-        };
-        uint64_t r0 = ld(0), r1 = ld(1), r2;
-        for (int k = 0; k < nmax; k += 12) {
-            const int g = k >> 2;
-            r2 = ld(g + 2);
-            step4(k, r0);
-            if (k + 4 >= nmax) break;
-            r0 = ld(g + 3);
-            step4(k + 4, r1);
-            if (k + 8 >= nmax) break;
-            r1 = ld(g + 4);
-            step4(k + 8, r2);
-        }
-        return;
-    }
-    uint64_t r0a = ld(0), r0b = ld(1), r1a = ld(2), r1b = ld(3), r2a, r2b;
-    for (int k = 0; k < nmax; k += 24) {
-        const int g = k >> 2;
-        r2a = ld(g + 4); r2b = ld(g + 5);
-        step(k, r0a, r0b);
-        if (k + 8 >= nmax) break;
-        r0a = ld(g + 6); r0b = ld(g + 7);
-        step(k + 8, r1a, r1b);
-        if (k + 16 >= nmax) break;
-        r1a = ld(g + 8); r1b = ld(g + 9);
-        step(k + 16, r2a, r2b);
-    }
-}
-
 // LJ force over the 16-bit tile-offset lists of one segment per block.
 // MODE 0: forces only (lennard_jones with reset_volatile_properties);
 // MODE 1: fused F(ts) + final_integrate(ts) + [thermo] + initial_integrate(ts+1)
@@ -686,6 +544,9 @@ __device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stri
 template <int U, int NT, int MODE, bool RED>
 __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
     static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
+    // one type and U = 4: the pipelined walk (positions of the next step in
+    // flight while the current one computes)
+    constexpr bool PIPE = NT == 1 && U == 4;
     const int *__restrict__ counters = A.counters;
     const Geom &g = A.g;
     const int *__restrict__ lstart = A.lstart, *__restrict__ gstart = A.gstart;
@@ -731,6 +592,18 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
             const double xi = X[ii], yi = Y[ii], zi = Z[ii];
             const int ti = NT != 1 ? type[ii] : 0;
             double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
+            if constexpr (PIPE) {
+                list_walk_pipe(nl, ii, nl_stride, n, nmax, T, tphys, X, Y, Z,
+                               [&](int k, const double *xj, const double *yj, const double *zj) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        double g0, g1, g2, uu, ww;
+                        lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < n, rc2, sig0, eps0, g0, g1, g2, uu, ww);
+                        t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
+                        if (RED) { u += uu; w += ww; }
+                    }
+                });
+            } else
             list_walk<U>(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *on) {
                 int jp[U];
 #pragma unroll
@@ -1481,7 +1354,9 @@ static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, cons
                            int do_vv2, int do_vv1, double *eatom) {
     const size_t lds = (size_t) d.tile_cap * sizeof(int);
     const LJArgs A = lj_args(d, g, prm, cur, next, do_vv2, do_vv1, MODE == 1, eatom);
-    hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    static const bool pipe = getenv("PAIRS_HIP_LJ_PIPE") && atoi(getenv("PAIRS_HIP_LJ_PIPE")) == 1;
+    if (pipe) hipLaunchKernelGGL((k_lj_full<4, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    else hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
 }
 
 void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,

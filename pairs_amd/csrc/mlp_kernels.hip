@@ -1,0 +1,677 @@
+// gfx950 kernels of the radial-descriptor MLP potential (mlp_kernels.hpp).
+//
+// The network runs on MFMA v_mfma_f32_16x16x16_bf16 with the activations kept
+// transposed (features x particles): a layer's 16x16 output block in the
+// accumulator layout (lane l holds rows 4(l/16)..4(l/16)+3 of column l%16) is
+// exactly the B-operand layout of the next layer's K block, so no data moves
+// between layers.  Weights are the A operands, converted to BF16 once per wave
+// and held in registers while the wave walks its 16-particle tiles.  The
+// first layer (and its transpose in the input gradient) carries each operand
+// as a pair of BF16 values hi + lo, three MFMAs per block: the energy is a
+// nearly linear function of correlated descriptors, and single-BF16 inputs
+// cost more accuracy than the network gains over a linear fit.
+// The library is compiled with -ffp-contract=off; the FP32 math here uses
+// explicit fmaf where a fused multiply-add is wanted.
+#include <algorithm>
+
+#include "md_segment.hpp"
+#include "mlp_kernels.hpp"
+
+namespace pk {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (inputs are finite)
+__device__ __forceinline__ short f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short) (u >> 16);
+}
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+    bf16x4 r;
+    r[0] = f2bf(a); r[1] = f2bf(b); r[2] = f2bf(c); r[3] = f2bf(d);
+    return r;
+}
+__device__ __forceinline__ bf16x4 pack4(const f32x4 &x) { return pack4(x[0], x[1], x[2], x[3]); }
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((uint32_t) (uint16_t) b) << 16); }
+// x = hi + lo with hi = bf16(x), lo = bf16(x - hi): a pair carries 16 mantissa bits
+__device__ __forceinline__ void split4(float a, float b, float c, float d, bf16x4 &hi, bf16x4 &lo) {
+    hi = pack4(a, b, c, d);
+    lo = pack4(a - bf2f(hi[0]), b - bf2f(hi[1]), c - bf2f(hi[2]), d - bf2f(hi[3]));
+}
+__device__ __forceinline__ void split4(const f32x4 &x, bf16x4 &hi, bf16x4 &lo) { split4(x[0], x[1], x[2], x[3], hi, lo); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// SiLU and its derivative
+__device__ __forceinline__ void silu(float z, float &h, float &dh) {
+    const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+    h = z * sg;
+    dh = sg * fmaf(z, 1.0f - sg, 1.0f);
+}
+
+// ---------------------------------------------------------------- network tile
+
+// A operands and per-lane constants of one wave (lane l: q = l / 16, col = l % 16)
+struct MLPRegs {
+    bf16x4 a1[4];      // W1 rows 16c + col, columns 4q..4q+3 (high BF16 part)
+    bf16x4 a1l[4];     // W1 - a1 (low part: the first layer runs at ~16-bit precision)
+    bf16x4 a2[4][4];   // W2 [c2][kc]: rows 16c2 + col, columns 16kc + 4q..
+    bf16x4 a2t[4][4];  // W2^T [c][kc]: W2[16kc + 4q + v][16c + col]
+    bf16x4 a1t[4];     // W1^T [kc]: W1[16kc + 4q + v][col] (high part)
+    bf16x4 a1tl[4];    // its low part
+    float b1[4][4], b2[4][4], w3[4][4];  // rows 16c + 4q + v
+    float b3, gm[4], gi[4], em, es;
+};
+
+template <bool BACK>
+__device__ __forceinline__ void mlp_load(MLPRegs &R, const float *__restrict__ p, const float *__restrict__ sc,
+                                         int lane) {
+    const int q = lane >> 4, col = lane & 15;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float *w = p + MLP_W1 + (16 * c + col) * MLP_ND + 4 * q;
+        split4(w[0], w[1], w[2], w[3], R.a1[c], R.a1l[c]);
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            const float *u = p + MLP_W2 + (16 * c + col) * MLP_H + 16 * kc + 4 * q;
+            R.a2[c][kc] = pack4(u[0], u[1], u[2], u[3]);
+            if (BACK) {
+                const float *t = p + MLP_W2 + (16 * kc + 4 * q) * MLP_H + 16 * c + col;
+                R.a2t[c][kc] = pack4(t[0], t[MLP_H], t[2 * MLP_H], t[3 * MLP_H]);
+            }
+        }
+        if (BACK) {
+            const float *t = p + MLP_W1 + (16 * c + 4 * q) * MLP_ND + col;
+            split4(t[0], t[MLP_ND], t[2 * MLP_ND], t[3 * MLP_ND], R.a1t[c], R.a1tl[c]);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int h = 16 * c + 4 * q + v;
+            R.b1[c][v] = p[MLP_B1 + h];
+            R.b2[c][v] = p[MLP_B2 + h];
+            R.w3[c][v] = p[MLP_W3 + h];
+        }
+    }
+    R.b3 = p[MLP_B3];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        R.gm[v] = sc[MLP_GMEAN + 4 * q + v];
+        R.gi[v] = sc[MLP_GISTD + 4 * q + v];
+    }
+    R.em = sc[MLP_EMEAN];
+    R.es = sc[MLP_ESTD];
+}
+
+// standardised descriptors of the lane's particle, features 4q..4q+3
+__device__ __forceinline__ f32x4 mlp_input(const MLPRegs &R, float4 g) {
+    f32x4 s;
+    s[0] = (g.x - R.gm[0]) * R.gi[0];
+    s[1] = (g.y - R.gm[1]) * R.gi[1];
+    s[2] = (g.z - R.gm[2]) * R.gi[2];
+    s[3] = (g.w - R.gm[3]) * R.gi[3];
+    return s;
+}
+
+// forward pass of one 16-particle tile: returns y of particle col (all four
+// lane groups hold it); keeps activations and SiLU derivatives
+__device__ __forceinline__ float mlp_forward(const MLPRegs &R, bf16x4 b0, bf16x4 b0l, f32x4 (&h1)[4],
+                                             f32x4 (&dh1)[4], bf16x4 (&bh1)[4], f32x4 (&h2)[4], f32x4 (&dh2)[4]) {
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        // W1 s with both operands split: hi*hi + lo*hi + hi*lo
+        f32x4 z = mfma16(R.a1l[c], b0, zero);
+        z = mfma16(R.a1[c], b0l, z);
+        z = mfma16(R.a1[c], b0, z);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float h, dh;
+            silu(z[v] + R.b1[c][v], h, dh);
+            h1[c][v] = h;
+            dh1[c][v] = dh;
+        }
+        bh1[c] = pack4(h1[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        f32x4 z = zero;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) z = mfma16(R.a2[c][kc], bh1[kc], z);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float h, dh;
+            silu(z[v] + R.b2[c][v], h, dh);
+            h2[c][v] = h;
+            dh2[c][v] = dh;
+        }
+    }
+    float y = 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) y = fmaf(R.w3[c][v], h2[c][v], y);
+    y += __shfl_xor(y, 16, 64);
+    y += __shfl_xor(y, 32, 64);
+    return y + R.b3;
+}
+
+// dy/d(standardised descriptor) of particle col, features 4q..4q+3
+__device__ __forceinline__ f32x4 mlp_input_grad(const MLPRegs &R, const f32x4 (&dh1)[4], const f32x4 (&dh2)[4]) {
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    bf16x4 bg2[4], bg1[4], bg1l[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        bg2[c] = pack4(R.w3[c][0] * dh2[c][0], R.w3[c][1] * dh2[c][1], R.w3[c][2] * dh2[c][2],
+                       R.w3[c][3] * dh2[c][3]);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        f32x4 a = zero;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) a = mfma16(R.a2t[c][kc], bg2[kc], a);
+        split4(a[0] * dh1[c][0], a[1] * dh1[c][1], a[2] * dh1[c][2], a[3] * dh1[c][3], bg1[c], bg1l[c]);
+    }
+    // W1^T g1, both split like the forward first layer
+    f32x4 d = zero;
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+        d = mfma16(R.a1tl[kc], bg1[kc], d);
+        d = mfma16(R.a1t[kc], bg1l[kc], d);
+        d = mfma16(R.a1t[kc], bg1[kc], d);
+    }
+    return d;
+}
+
+// E and dE/dG (raw descriptor units) of n rows; one wave per 16-row tile
+template <bool WANT_D>
+__global__ __launch_bounds__(256) void k_mlp_eval(const float *__restrict__ p, const float *__restrict__ sc,
+                                                  const float *__restrict__ G, int n, float *__restrict__ E,
+                                                  float *__restrict__ D) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nw = gridDim.x * 4;
+    const int ntiles = (n + 15) >> 4;
+    if (wave >= ntiles) return;  // wave-uniform
+    MLPRegs R;
+    mlp_load<WANT_D>(R, p, sc, lane);
+    for (int t = wave; t < ntiles; t += nw) {
+        const int row = t * 16 + col;
+        const bool ok = row < n;
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok) g = *(const float4 *) (G + (size_t) row * MLP_ND + 4 * q);
+        bf16x4 b0, b0l;
+        split4(mlp_input(R, g), b0, b0l);
+        f32x4 h1[4], dh1[4], h2[4], dh2[4];
+        bf16x4 bh1[4];
+        const float y = mlp_forward(R, b0, b0l, h1, dh1, bh1, h2, dh2);
+        if (ok && q == 0 && E) E[row] = fmaf(R.es, y, R.em);
+        if (WANT_D) {
+            const f32x4 d = mlp_input_grad(R, dh1, dh2);
+            if (ok) {
+                float4 o;
+                o.x = R.es * d[0] * R.gi[0];
+                o.y = R.es * d[1] * R.gi[1];
+                o.z = R.es * d[2] * R.gi[2];
+                o.w = R.es * d[3] * R.gi[3];
+                *(float4 *) (D + (size_t) row * MLP_ND + 4 * q) = o;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- descriptors and forces
+
+// log2(e) / (2 w^2): exp(-t^2 / (2 w^2)) = exp2(-t^2 * MLP_C)
+constexpr float MLP_C = 1.4426950408889634f / (2.0f * MLP_W * MLP_W);
+constexpr float MLP_IW2 = 1.0f / (MLP_W * MLP_W);
+constexpr float MLP_PIRC = 3.14159265358979323846f / (float) MLP_RC;
+
+struct MLPSegArgs {
+    const int *counters;
+    Geom g;
+    const int *lstart, *gstart;
+    const double *X, *Y, *Z;
+    const uint16_t *nl;
+    int nl_stride;
+    const int *numneighs;
+    int tcap, pstride;
+    const float *D, *E;
+    float *G;
+    double *fx, *fy, *fz;
+    double *partials;
+};
+
+// G of every home particle of one segment (MODE 0), or its force, energy
+// and virial (MODE 1).  One lane per home particle, the full list walked four
+// entries per step as in the LJ kernel; pairs at r >= 2.5 and entries past a
+// lane's own count are masked.
+template <int MODE, bool RED>
+__global__ __launch_bounds__(FB) void k_mlp_segment(MLPSegArgs A) {
+    const int seg = xcd_block(blockIdx.x, gridDim.x);
+    const Geom &g = A.g;
+    extern __shared__ char smem[];
+    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
+    const int nlocal = A.counters[C_NLOCAL];
+    const SegInfo si = seg_info(seg, g);
+    const int h0 = A.lstart[si.ca], h1 = A.lstart[si.cb];
+    const double rc2 = MLP_RC * MLP_RC;
+    double pe = 0.0, vir = 0.0;
+    if (h1 > h0) {
+        seg_runs(si, g, A.lstart, A.gstart, nlocal, r_start, r_len, r_off, col_lo);
+        const int T = r_off[NRUNS];
+        const bool fits = T <= A.tcap;
+        int *tphys = (int *) smem;
+        if (fits) stage_slots(r_start, r_off, T, tphys);
+        __syncthreads();
+        for (int base = h0; fits && base < h1; base += FB) {
+            const int i = base + threadIdx.x;
+            const bool live = i < h1;
+            const int ii = live ? i : h0;
+            const int n = live ? A.numneighs[ii] : 0;
+            const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
+            const double xi = A.X[ii], yi = A.Y[ii], zi = A.Z[ii];
+            float acc[MLP_ND], di[MLP_ND];
+#pragma unroll
+            for (int k = 0; k < MLP_ND; k++) acc[k] = 0.f;
+            if (MODE == 1) {
+                const float4 *dp = (const float4 *) (A.D + (size_t) ii * MLP_ND);
+#pragma unroll
+                for (int k4 = 0; k4 < 4; k4++) {
+                    const float4 d = dp[k4];
+                    di[4 * k4] = d.x; di[4 * k4 + 1] = d.y; di[4 * k4 + 2] = d.z; di[4 * k4 + 3] = d.w;
+                }
+            }
+            double f0 = 0.0, f1 = 0.0, f2 = 0.0, w = 0.0;
+            list_walk<4>(A.nl, ii, A.nl_stride, n, nmax, T, [&](int k, const int *on) {
+                int jp[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) jp[q] = tphys[on[q]];
+                double xj[4], yj[4], zj[4];
+                float4 dj[4][4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    xj[q] = A.X[jp[q]]; yj[q] = A.Y[jp[q]]; zj[q] = A.Z[jp[q]];
+                    if (MODE == 1) {
+                        const float4 *dp = (const float4 *) (A.D + (size_t) jp[q] * MLP_ND);
+#pragma unroll
+                        for (int k4 = 0; k4 < 4; k4++) dj[q][k4] = dp[k4];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const double dx = xi - xj[q], dy = yi - yj[q], dz = zi - zj[q];
+                    double rsq = dx * dx + dy * dy;
+                    rsq = rsq + dz * dz;
+                    const bool in = (k + q < n) && rsq < rc2;
+                    const float r = sqrtf((float) rsq);
+                    if (MODE == 0) {
+                        const float fc = in ? 0.5f * (cosf(MLP_PIRC * r) + 1.0f) : 0.0f;
+#pragma unroll
+                        for (int kk = 0; kk < MLP_ND; kk++) {
+                            const float t = r - (MLP_MU0 + (float) kk * MLP_DMU);
+                            acc[kk] = fmaf(exp2f(-(t * t) * MLP_C), fc, acc[kk]);
+                        }
+                    } else {
+                        float sn, cs;
+                        sincosf(MLP_PIRC * r, &sn, &cs);
+                        const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
+                        float ci = 0.f, cj = 0.f;
+                        const float *djq = (const float *) dj[q];
+#pragma unroll
+                        for (int kk = 0; kk < MLP_ND; kk++) {
+                            const float t = r - (MLP_MU0 + (float) kk * MLP_DMU);
+                            const float e = exp2f(-(t * t) * MLP_C);
+                            const float dphi = e * fmaf(-t * MLP_IW2, fc, dfc);
+                            ci = fmaf(di[kk], dphi, ci);
+                            cj = fmaf(djq[kk], dphi, cj);
+                        }
+                        const float cc = ci + cj;
+                        const double coef = in ? (double) (-cc / r) : 0.0;
+                        f0 += coef * dx; f1 += coef * dy; f2 += coef * dz;
+                        if (RED) w += in ? (double) (-cc * r) : 0.0;
+                    }
+                }
+            });
+            if (live) {
+                if (MODE == 0) {
+                    float4 *gp = (float4 *) (A.G + (size_t) i * MLP_ND);
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; k4++)
+                        gp[k4] = make_float4(acc[4 * k4], acc[4 * k4 + 1], acc[4 * k4 + 2], acc[4 * k4 + 3]);
+                } else {
+                    A.fx[i] = f0; A.fy[i] = f1; A.fz[i] = f2;
+                    if (RED) { pe += (double) A.E[i]; vir += 0.5 * w; }
+                }
+            }
+        }
+    }
+    if (MODE == 1 && RED) {
+        __shared__ double red[2][FB / 64];
+        const double a = wave_sum(pe), b = wave_sum(vir);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) { red[0][wave] = a; red[1][wave] = b; }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            double s = 0.0;
+            for (int k = 0; k < FB / 64; k++) s += red[threadIdx.x][k];
+            A.partials[(size_t) (1 + threadIdx.x) * A.pstride + seg] = s;
+        }
+    }
+}
+
+// dE/dG of every ghost = that of the local at the end of its source chain
+__global__ __launch_bounds__(256) void k_mlp_ghost_d(const int *__restrict__ counters, const int *__restrict__ sync_src,
+                                                     float *__restrict__ D) {
+    const int nlocal = counters[C_NLOCAL], ng = counters[C_NGHOST];
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < ng; q += gridDim.x * 256) {
+        int root = sync_src[q];
+        for (int hop = 0; root >= nlocal && hop < 64; hop++) root = sync_src[root - nlocal];
+        if (root >= nlocal) root = nlocal + q;  // unresolved chain: leave the slot as it is
+        const float4 *s = (const float4 *) (D + (size_t) root * MLP_ND);
+        float4 *d = (float4 *) (D + (size_t) (nlocal + q) * MLP_ND);
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) d[k4] = s[k4];
+    }
+}
+
+// ---------------------------------------------------------------- training
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// Gradient of L = (1/B) sum_b (y_b - (label_b - e_mean) / e_std)^2 over one
+// minibatch.  Every wave takes the same number of 16-row tiles (batch is a
+// multiple of 16 * waves); weight gradients are K = 16-particle MFMAs on
+// transposed tiles staged in LDS; bias gradients are lane sums.
+__global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ p, const float *__restrict__ sc,
+                                                  MLPTrain t, uint64_t seed, int step) {
+    __shared__ short lds[4][3 * MLP_H * 16 + MLP_ND * 16];
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15, wl = threadIdx.x >> 6;
+    const int wave = blockIdx.x * 4 + wl, nw = gridDim.x * 4;
+    short *sD2 = lds[wl], *sH1 = sD2 + MLP_H * 16, *sD1 = sH1 + MLP_H * 16, *sG = sD1 + MLP_H * 16;
+    MLPRegs R;
+    mlp_load<true>(R, p, sc, lane);
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4 gW2[4][4], gW1[4];
+    float gb1[4][4], gb2[4][4], gw3[4][4], gb3 = 0.f, lsum = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        gW1[a] = zero;
+#pragma unroll
+        for (int b = 0; b < 4; b++) { gW2[a][b] = zero; gb1[a][b] = 0.f; gb2[a][b] = 0.f; gw3[a][b] = 0.f; }
+    }
+    const float inv_es = 1.0f / R.es, scale = 2.0f / (float) t.batch;
+    const int ntiles = t.batch >> 4;
+    for (int tile = wave; tile < ntiles; tile += nw) {
+        const uint64_t j = (uint64_t) (tile * 16 + col);
+        const uint64_t h = splitmix64(seed ^ ((uint64_t) step << 32) ^ j);
+        const int idx = (int) (h % (uint64_t) t.rows);
+        const float4 g = *(const float4 *) (t.G + (size_t) idx * MLP_ND + 4 * q);
+        const float lab = t.label[idx];
+        bf16x4 b0, b0l;
+        split4(mlp_input(R, g), b0, b0l);
+#pragma unroll
+        for (int v = 0; v < 4; v++) sG[(4 * q + v) * 16 + col] = b0[v];
+        f32x4 h1[4], dh1[4], h2[4], dh2[4];
+        bf16x4 bh1[4];
+        const float y = mlp_forward(R, b0, b0l, h1, dh1, bh1, h2, dh2);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) sH1[(16 * c + 4 * q + v) * 16 + col] = bh1[c][v];
+        const float err = y - (lab - R.em) * inv_es;
+        const float e = scale * err;
+        if (q == 0) { lsum = fmaf(err, err, lsum); gb3 += e; }
+        bf16x4 bd2[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f32x4 d2;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                gw3[c][v] = fmaf(e, h2[c][v], gw3[c][v]);
+                d2[v] = e * R.w3[c][v] * dh2[c][v];
+                gb2[c][v] += d2[v];
+            }
+            bd2[c] = pack4(d2);
+#pragma unroll
+            for (int v = 0; v < 4; v++) sD2[(16 * c + 4 * q + v) * 16 + col] = bd2[c][v];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f32x4 a = zero;
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) a = mfma16(R.a2t[c][kc], bd2[kc], a);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float d1 = a[v] * dh1[c][v];
+                gb1[c][v] += d1;
+                sD1[(16 * c + 4 * q + v) * 16 + col] = f2bf(d1);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c2 = 0; c2 < 4; c2++) {
+            const bf16x4 a = *(const bf16x4 *) (sD2 + (16 * c2 + col) * 16 + 4 * q);
+#pragma unroll
+            for (int c1 = 0; c1 < 4; c1++) {
+                const bf16x4 b = *(const bf16x4 *) (sH1 + (16 * c1 + col) * 16 + 4 * q);
+                gW2[c2][c1] = mfma16(a, b, gW2[c2][c1]);
+            }
+        }
+        const bf16x4 bg = *(const bf16x4 *) (sG + col * 16 + 4 * q);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const bf16x4 a = *(const bf16x4 *) (sD1 + (16 * c + col) * 16 + 4 * q);
+            gW1[c] = mfma16(a, bg, gW1[c]);
+        }
+        __syncthreads();
+    }
+    float *out = t.partial + (size_t) wave * MLP_NP;
+#pragma unroll
+    for (int c2 = 0; c2 < 4; c2++)
+#pragma unroll
+        for (int c1 = 0; c1 < 4; c1++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[MLP_W2 + (16 * c2 + 4 * q + v) * MLP_H + 16 * c1 + col] = gW2[c2][c1][v];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[MLP_W1 + (16 * c + 4 * q + v) * MLP_ND + col] = gW1[c][v];
+    // bias sums over the 16 particles of a lane group
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float a = gb1[c][v], b = gb2[c][v], w = gw3[c][v];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                a += __shfl_xor(a, o, 64); b += __shfl_xor(b, o, 64); w += __shfl_xor(w, o, 64);
+            }
+            if (col == 0) {
+                const int hh = 16 * c + 4 * q + v;
+                out[MLP_B1 + hh] = a; out[MLP_B2 + hh] = b; out[MLP_W3 + hh] = w;
+            }
+        }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) { gb3 += __shfl_xor(gb3, o, 64); lsum += __shfl_xor(lsum, o, 64); }
+    if (lane == 0) { out[MLP_B3] = gb3; t.loss[wave] = lsum; }
+}
+
+// Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the summed wave partials
+__global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
+                                                  const float *__restrict__ partial, const float *__restrict__ loss,
+                                                  int nwaves, int batch, float lr, float bc1, float bc2,
+                                                  float *__restrict__ loss_out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < MLP_NP) {
+        float gsum = 0.f;
+        for (int w = 0; w < nwaves; w++) gsum += partial[(size_t) w * MLP_NP + k];
+        const float mm = 0.9f * m[k] + 0.1f * gsum;
+        const float vv = 0.999f * v[k] + 0.001f * gsum * gsum;
+        m[k] = mm;
+        v[k] = vv;
+        p[k] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + 1e-8f);
+    }
+    if (k == 0 && loss_out) {
+        float s = 0.f;
+        for (int w = 0; w < nwaves; w++) s += loss[w];
+        *loss_out = s / (float) batch;
+    }
+}
+
+// column moments: per block, 16 descriptor sums, 16 sums of squares, label sum and sum of squares
+__global__ __launch_bounds__(256) void k_mlp_moments(const float *__restrict__ G, const float *__restrict__ label,
+                                                     int n, double *__restrict__ partials) {
+    __shared__ double red[16][34];
+    const int col = threadIdx.x & 15, sub = threadIdx.x >> 4;
+    double s = 0.0, s2 = 0.0, l = 0.0, l2 = 0.0;
+    for (int r = blockIdx.x * 16 + sub; r < n; r += gridDim.x * 16) {
+        const double x = G[(size_t) r * MLP_ND + col];
+        s += x; s2 += x * x;
+        if (col == 0) { const double y = label[r]; l += y; l2 += y * y; }
+    }
+    red[sub][col] = s;
+    red[sub][16 + col] = s2;
+    if (col == 0) { red[sub][32] = l; red[sub][33] = l2; }
+    __syncthreads();
+    if (threadIdx.x < 34) {
+        double a = 0.0;
+        for (int k = 0; k < 16; k++) a += red[k][threadIdx.x];
+        partials[(size_t) blockIdx.x * 34 + threadIdx.x] = a;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_sqerr(const float *__restrict__ E, const float *__restrict__ label, int n,
+                                                   double *__restrict__ partials) {
+    double s = 0.0;
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) {
+        const double d = (double) E[r] - (double) label[r];
+        s += d * d;
+    }
+    __shared__ double red[4];
+    const double a = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_mlp_snapshot(const float *__restrict__ G, const double *__restrict__ eatom,
+                                                      int nlocal, float *__restrict__ dG, float *__restrict__ dl) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nlocal; i += gridDim.x * 256) {
+        const float4 *s = (const float4 *) (G + (size_t) i * MLP_ND);
+        float4 *d = (float4 *) (dG + (size_t) i * MLP_ND);
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) d[k4] = s[k4];
+        dl[i] = (float) eatom[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_force_diff(const double *__restrict__ ax, const double *__restrict__ ay,
+                                                    const double *__restrict__ az, const double *__restrict__ bx,
+                                                    const double *__restrict__ by, const double *__restrict__ bz,
+                                                    int n, double *__restrict__ partials) {
+    double s = 0.0, r = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double d0 = ax[i] - bx[i], d1 = ay[i] - by[i], d2 = az[i] - bz[i];
+        s += d0 * d0 + d1 * d1 + d2 * d2;
+        r += bx[i] * bx[i] + by[i] * by[i] + bz[i] * bz[i];
+    }
+    __shared__ double red[2][4];
+    const double a = wave_sum(s), b = wave_sum(r);
+    if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = a; red[1][threadIdx.x >> 6] = b; }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int k = threadIdx.x;
+        partials[2 * blockIdx.x + k] = ((red[k][0] + red[k][1]) + red[k][2]) + red[k][3];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+static MLPSegArgs seg_args(const DevState &d, const Geom &g, int cur) {
+    MLPSegArgs A{};
+    A.counters = d.counters;
+    A.g = g;
+    A.lstart = d.lstart;
+    A.gstart = d.gstart;
+    A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];
+    A.nl = d.nl;
+    A.nl_stride = d.nl_stride;
+    A.numneighs = d.numneighs;
+    A.tcap = d.tile_cap;
+    A.pstride = d.pstride;
+    return A;
+}
+
+void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int cur, float *G) {
+    if (d.nlocal <= 0) return;
+    MLPSegArgs A = seg_args(d, g, cur);
+    A.G = G;
+    hipLaunchKernelGGL((k_mlp_segment<0, false>), dim3(g.nsegs), dim3(FB), (size_t) d.tile_cap * sizeof(int), s, A);
+}
+
+static int eval_grid(int n) {
+    const int ntiles = (n + 15) / 16;
+    return std::max(1, std::min((ntiles + 3) / 4, 2048));
+}
+
+void launch_mlp_eval(hipStream_t s, const float *p, const float *sc, const float *G, int n, float *E, float *D) {
+    if (n <= 0) return;
+    if (D) hipLaunchKernelGGL(k_mlp_eval<true>, dim3(eval_grid(n)), dim3(256), 0, s, p, sc, G, n, E, D);
+    else hipLaunchKernelGGL(k_mlp_eval<false>, dim3(eval_grid(n)), dim3(256), 0, s, p, sc, G, n, E, D);
+}
+
+void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const MLPBind &b, int cur, int reduce,
+                       int ghost_grid) {
+    if (d.nlocal <= 0) return;
+    launch_mlp_descriptors(s, d, g, cur, b.G);
+    launch_mlp_eval(s, b.p, b.sc, b.G, d.nlocal, b.E, b.D);
+    hipLaunchKernelGGL(k_mlp_ghost_d, dim3(std::max(ghost_grid, 1)), dim3(256), 0, s, d.counters, d.sync_src, b.D);
+    MLPSegArgs A = seg_args(d, g, cur);
+    A.D = b.D;
+    A.E = b.E;
+    A.fx = d.fx; A.fy = d.fy; A.fz = d.fz;
+    A.partials = d.partials;
+    const size_t lds = (size_t) d.tile_cap * sizeof(int);
+    if (reduce) hipLaunchKernelGGL((k_mlp_segment<1, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    else hipLaunchKernelGGL((k_mlp_segment<1, false>), dim3(g.nsegs), dim3(FB), lds, s, A);
+}
+
+void launch_mlp_grad(hipStream_t s, const float *p, const float *sc, const MLPTrain &t, uint64_t seed, int step) {
+    hipLaunchKernelGGL(k_mlp_grad, dim3(t.nwaves / 4), dim3(256), 0, s, p, sc, t, seed, step);
+}
+
+void launch_mlp_adam(hipStream_t s, float *p, float *m, float *v, const MLPTrain &t, float lr, int step,
+                     float *loss_out) {
+    const float bc1 = 1.0f - powf(0.9f, (float) step), bc2 = 1.0f - powf(0.999f, (float) step);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NP + 255) / 256), dim3(256), 0, s, p, m, v, t.partial, t.loss,
+                       t.nwaves, t.batch, lr, bc1, bc2, loss_out);
+}
+
+void launch_mlp_moments(hipStream_t s, const float *G, const float *label, int n, double *partials, int nblocks) {
+    hipLaunchKernelGGL(k_mlp_moments, dim3(nblocks), dim3(256), 0, s, G, label, n, partials);
+}
+
+void launch_mlp_sqerr(hipStream_t s, const float *E, const float *label, int n, double *partials, int nblocks) {
+    hipLaunchKernelGGL(k_mlp_sqerr, dim3(nblocks), dim3(256), 0, s, E, label, n, partials);
+}
+
+void launch_mlp_snapshot(hipStream_t s, const float *G, const double *eatom, int nlocal, float *dst_G,
+                         float *dst_label) {
+    if (nlocal <= 0) return;
+    hipLaunchKernelGGL(k_mlp_snapshot, dim3(std::min((nlocal + 255) / 256, 2048)), dim3(256), 0, s, G, eatom, nlocal,
+                       dst_G, dst_label);
+}
+
+void launch_force_diff(hipStream_t s, const double *ax, const double *ay, const double *az, const double *bx,
+                       const double *by, const double *bz, int n, double *partials, int nblocks) {
+    hipLaunchKernelGGL(k_force_diff, dim3(nblocks), dim3(256), 0, s, ax, ay, az, bx, by, bz, n, partials);
+}
+
+}  // namespace pk

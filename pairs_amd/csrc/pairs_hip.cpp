@@ -19,61 +19,9 @@
 
 #include "../../include/pairs_hip.h"
 #include "../../include/pairs_hip_batch.h"
-#include "md_kernels.hpp"
-
-using namespace pk;
-
-struct pairs_hip_ctx {
-    pairs_hip_md_config cfg;
-    LJParams prm;
-    Geom geom;
-    DevState d;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    std::string err;
-    int req[5] = {0, 0, 0, 0, 0};
-    int h_counters[C_COUNT];
-    int nghost = 0, nsend_all = 0;
-    int ghost_grid = 1;
-    int max_lds = 65536;       // shared memory a workgroup may allocate
-    int zc_override = 0;
-    int home_target = 112;     // home particles a segment aims at (sets zc)
-    // graphs of one non-rebuild step, by parity of the current buffer
-    bool use_graphs = true;
-    hipGraphExec_t step_graph[2] = {nullptr, nullptr};
-    int graph_ghost_grid[2] = {0, 0};
-    int graph_tile_cap[2] = {0, 0};
-    int graph_img[2] = {0, 0};
-    // force-kernel timing
-    bool time_force = false;
-    int time_every = 10;
-    std::vector<hipEvent_t> ev_begin, ev_end;
-    int ev_used = 0;
-    double force_ms_avg = 0.0;
-    int force_launches = 0;
-    double *d_eatom = nullptr;
-    int eatom_cap = 0;
-    int *d_expand = nullptr;
-    size_t expand_cap = 0;
-};
-
-#define HIP_OK(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess) {                                                                        \
-            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                              \
-            return PAIRS_HIP_ERR_HIP;                                                                  \
-        }                                                                                              \
-    } while (0)
-
-#define CHECK_LAUNCH()                                                                                 \
-    do {                                                                                               \
-        hipError_t e_ = hipGetLastError();                                                             \
-        if (e_ != hipSuccess) {                                                                        \
-            ctx->err = std::string("kernel launch: ") + hipGetErrorString(e_);                         \
-            return PAIRS_HIP_ERR_HIP;                                                                  \
-        }                                                                                              \
-    } while (0)
+#include "../../include/pairs_hip_mlp.h"
+#include "lj_ctx.hpp"
+#include "mlp_model.hpp"
 
 static void drop_graphs(pairs_hip_ctx *ctx) {
     for (int p = 0; p < 2; p++) {
@@ -333,6 +281,7 @@ int pairs_hip_destroy(pairs_hip_ctx *ctx) {
     dfree(d.nl); dfree(d.send_map); dfree(d.send_mult); dfree(d.scratch_i); dfree(d.stage);
     dfree(d.lcount); dfree(d.gcount); dfree(d.lstart); dfree(d.gstart); dfree(d.counters); dfree(d.thermo);
     dfree(ctx->d_eatom); dfree(ctx->d_expand);
+    dfree(ctx->mlp_G); dfree(ctx->mlp_D); dfree(ctx->mlp_E); dfree(ctx->mlp_f); dfree(ctx->mlp_red);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -823,6 +772,29 @@ int pairs_hip_sync(pairs_hip_ctx *ctx) {
     return 0;
 }
 
+// per-slot MLP work arrays sized to the particle capacity (re-bound and the
+// step graphs dropped when they grow)
+static int ensure_mlp_work(pairs_hip_ctx *ctx) {
+    const int cap = ctx->d.cap;
+    if (ctx->mlp_cap < cap) {
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        drop_graphs(ctx);
+        dfree(ctx->mlp_G); dfree(ctx->mlp_D); dfree(ctx->mlp_E); dfree(ctx->mlp_f);
+        ctx->mlp_G = ctx->mlp_D = ctx->mlp_E = nullptr;
+        ctx->mlp_f = nullptr;
+        if (dalloc(ctx, &ctx->mlp_G, (size_t) cap * MLP_ND) || dalloc(ctx, &ctx->mlp_D, (size_t) cap * MLP_ND) ||
+            dalloc(ctx, &ctx->mlp_E, cap) || dalloc(ctx, &ctx->mlp_f, (size_t) 3 * cap))
+            return PAIRS_HIP_ERR_HIP;
+        ctx->mlp_cap = cap;
+    }
+    if (!ctx->mlp_red && dalloc(ctx, &ctx->mlp_red, 2048)) return PAIRS_HIP_ERR_HIP;
+    ctx->mlp.G = ctx->mlp_G;
+    ctx->mlp.D = ctx->mlp_D;
+    ctx->mlp.E = ctx->mlp_E;
+    ctx->mlp.cap = ctx->mlp_cap;
+    return 0;
+}
+
 static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 || ts == 0); }
 
 // exchange + borders + cells + neighbour lists, retried with grown capacities
@@ -842,6 +814,7 @@ static int rebuild(pairs_hip_ctx *ctx) {
         int code = take_overflow(ctx);
         if (code == 0) {
             code = build_neighbors(ctx);
+            if (code == 0 && ctx->mlp_on) code = ensure_mlp_work(ctx);
             if (code <= 0) return code;
         }
         int pc = 0, nc = 0, sc = 0;
@@ -867,7 +840,8 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
     } else {
         launch_sync_inplace(s, d, ctx->geom, ctx->ghost_grid, 0);
     }
-    if (ctx->cfg.half) launch_lj_half(s, d, ctx->geom, ctx->prm, d.cur, 0);
+    if (ctx->mlp_on) launch_mlp_forces(s, d, ctx->geom, ctx->mlp, d.cur, 0, ctx->ghost_grid);
+    else if (ctx->cfg.half) launch_lj_half(s, d, ctx->geom, ctx->prm, d.cur, 0);
     else launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
     if (ts > 0) launch_vv_post(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 1, 0, 0);
     CHECK_LAUNCH();
@@ -877,20 +851,31 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
 // one non-rebuild step of the fused double-buffered form:
 //   sync ghosts of ts into buffer cur (stale multi-hop values from the other)
 //   force(ts) + final_integrate(ts) + initial_integrate(ts+1) -> other buffer
-static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int reduce, bool timed) {
+// force(ts) on buffer cur, then final_integrate(ts) + initial_integrate(ts+1)
+// into buffer next (fused into the LJ kernel for full lists)
+static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, int vv1, int reduce, bool timed) {
     DevState &d = ctx->d;
     hipStream_t s = ctx->stream;
-    const int other = 1 - cur;
-    if (!d.img_on) launch_sync_pp(s, d, ctx->geom, cur, other, ctx->ghost_grid);
     if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
-    if (ctx->cfg.half) {
+    if (ctx->mlp_on) {
+        launch_mlp_forces(s, d, ctx->geom, ctx->mlp, cur, reduce, ctx->ghost_grid);
+        if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
+        launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
+    } else if (ctx->cfg.half) {
         launch_lj_half(s, d, ctx->geom, ctx->prm, cur, reduce);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-        launch_vv_post(s, d, ctx->geom, ctx->prm, cur, other, vv2, vv1, reduce);
+        launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
     } else {
-        launch_lj_full(s, d, ctx->geom, ctx->prm, cur, other, 1, vv2, vv1, reduce);
+        launch_lj_full(s, d, ctx->geom, ctx->prm, cur, next, 1, vv2, vv1, reduce);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
     }
+}
+
+static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int reduce, bool timed) {
+    DevState &d = ctx->d;
+    const int other = 1 - cur;
+    if (!d.img_on) launch_sync_pp(ctx->stream, d, ctx->geom, cur, other, ctx->ghost_grid);
+    launch_force_post(ctx, cur, other, vv2, vv1, reduce, timed);
 }
 
 static int ensure_events(pairs_hip_ctx *ctx, int n) {
@@ -984,19 +969,11 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             launch_fused_step(ctx, cur, vv2, vv1, T ? 1 : 0, timed);
         } else {
             // ghosts already current in buffer cur: force + integrators only
-            if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
-            if (ctx->cfg.half) {
-                launch_lj_half(s, d, ctx->geom, ctx->prm, cur, T ? 1 : 0);
-                if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-                launch_vv_post(s, d, ctx->geom, ctx->prm, cur, other, vv2, vv1, T ? 1 : 0);
-            } else {
-                launch_lj_full(s, d, ctx->geom, ctx->prm, cur, other, 1, vv2, vv1, T ? 1 : 0);
-                if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-            }
+            launch_force_post(ctx, cur, other, vv2, vv1, T ? 1 : 0, timed);
         }
         if (timed) ctx->ev_used++;
         if (T && nthermo < 4096) {
-            const int nb_ke = ctx->cfg.half ? (d.nlocal + 255) / 256 : ctx->geom.nsegs;
+            const int nb_ke = (ctx->cfg.half || ctx->mlp_on) ? (d.nlocal + 255) / 256 : ctx->geom.nsegs;
             launch_thermo_reduce(s, d, nb_ke, 3, ctx->geom, d.thermo + 5 * nthermo);
             nthermo++;
         }
@@ -1020,6 +997,132 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
     }
     return nthermo;
 }
+
+// ---------------------------------------------------------------- MLP force field (include/pairs_hip_mlp.h)
+
+static int mlp_prepare(pairs_hip_ctx *ctx, pairs_hip_mlp *m) {
+    if (ctx->cfg.half) {
+        ctx->err = "the MLP potential runs on full neighbour lists (cfg.half = 0)";
+        return PAIRS_HIP_ERR_STATE;
+    }
+    if (int r = ensure_mlp_work(ctx)) return r;
+    if (m) HIP_OK(hipStreamSynchronize(m->stream));  // the model's own stream (training) is done
+    return 0;
+}
+
+static MLPBind mlp_bind(pairs_hip_ctx *ctx, pairs_hip_mlp *m) {
+    MLPBind b = ctx->mlp;
+    b.p = m->p;
+    b.sc = m->sc;
+    return b;
+}
+
+int pairs_hip_mlp_descriptors(pairs_hip_ctx *ctx, float *G, int count) {
+    if (!ctx || !G || count < ctx->d.nlocal) return PAIRS_HIP_ERR_ARG;
+    if (int r = mlp_prepare(ctx, nullptr)) return r;
+    DevState &d = ctx->d;
+    const int n = d.nlocal;
+    launch_mlp_descriptors(ctx->stream, d, ctx->geom, d.cur, ctx->mlp_G);
+    CHECK_LAUNCH();
+    std::vector<int> phys;
+    if (host_phys(ctx, n, phys)) return PAIRS_HIP_ERR_HIP;
+    std::vector<float> tmp((size_t) n * MLP_ND);
+    if (download_raw(ctx, tmp.data(), ctx->mlp_G, tmp.size() * sizeof(float))) return PAIRS_HIP_ERR_HIP;
+    for (int r = 0; r < n; r++)
+        std::memcpy(G + (size_t) r * MLP_ND, tmp.data() + (size_t) phys[r] * MLP_ND, MLP_ND * sizeof(float));
+    return 0;
+}
+
+int pairs_hip_mlp_add_snapshot(pairs_hip_mlp *m, pairs_hip_ctx *ctx) {
+    if (!ctx || !m) return PAIRS_HIP_ERR_ARG;
+    if (int r = mlp_prepare(ctx, m)) return r;
+    if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
+    DevState &d = ctx->d;
+    const int n = d.nlocal;
+    if (m->ds_rows + n > m->ds_cap) {
+        ctx->err = "the model's training set is full";
+        return PAIRS_HIP_ERR_STATE;
+    }
+    hipStream_t s = ctx->stream;
+    launch_mlp_descriptors(s, d, ctx->geom, d.cur, ctx->mlp_G);
+    launch_energy(s, d, ctx->geom, ctx->prm, 0, ctx->d_eatom);
+    launch_mlp_snapshot(s, ctx->mlp_G, ctx->d_eatom, n, m->ds_G + (size_t) m->ds_rows * MLP_ND,
+                        m->ds_label + m->ds_rows);
+    CHECK_LAUNCH();
+    HIP_OK(hipStreamSynchronize(s));
+    m->ds_rows += n;
+    return n;
+}
+
+int pairs_hip_mlp_forces(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double *energy, double *virial) {
+    if (!ctx || !m) return PAIRS_HIP_ERR_ARG;
+    if (int r = mlp_prepare(ctx, m)) return r;
+    DevState &d = ctx->d;
+    launch_mlp_forces(ctx->stream, d, ctx->geom, mlp_bind(ctx, m), d.cur, 1, ctx->ghost_grid);
+    launch_thermo_reduce(ctx->stream, d, 0, 2, ctx->geom, d.thermo);
+    CHECK_LAUNCH();
+    double out[5];
+    if (download_raw(ctx, out, d.thermo, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;
+    if (energy) *energy = out[2];
+    if (virial) *virial = out[3];
+    return 0;
+}
+
+int pairs_hip_mlp_force_error(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double *rel_rms) {
+    if (!ctx || !m || !rel_rms) return PAIRS_HIP_ERR_ARG;
+    if (int r = mlp_prepare(ctx, m)) return r;
+    DevState &d = ctx->d;
+    hipStream_t s = ctx->stream;
+    const int n = d.nlocal;
+    if (n <= 0) { *rel_rms = 0.0; return 0; }
+    const size_t cap = (size_t) ctx->mlp_cap;
+    launch_mlp_forces(s, d, ctx->geom, mlp_bind(ctx, m), d.cur, 0, ctx->ghost_grid);
+    HIP_OK(hipMemcpyAsync(ctx->mlp_f, d.fx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(ctx->mlp_f + cap, d.fy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(ctx->mlp_f + 2 * cap, d.fz, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    const int nb = std::min((n + 255) / 256, 1024);
+    launch_force_diff(s, ctx->mlp_f, ctx->mlp_f + cap, ctx->mlp_f + 2 * cap, d.fx, d.fy, d.fz, n, ctx->mlp_red, nb);
+    CHECK_LAUNCH();
+    std::vector<double> part(2 * nb);
+    if (download_raw(ctx, part.data(), ctx->mlp_red, part.size() * sizeof(double))) return PAIRS_HIP_ERR_HIP;
+    double num = 0.0, den = 0.0;
+    for (int b = 0; b < nb; b++) { num += part[2 * b]; den += part[2 * b + 1]; }
+    *rel_rms = den > 0.0 ? std::sqrt(num / den) : std::sqrt(num);
+    return 0;
+}
+
+int pairs_hip_mlp_attach(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double max_rel_rms, double *rel_rms) {
+    if (!ctx || !m) return PAIRS_HIP_ERR_ARG;
+    if (int r = mlp_prepare(ctx, m)) return r;
+    if (rel_rms) *rel_rms = -1.0;
+    drop_graphs(ctx);
+    if (max_rel_rms >= 0.0) {
+        double rel = 0.0;
+        if (int r = pairs_hip_mlp_force_error(ctx, m, &rel)) return r;
+        if (rel_rms) *rel_rms = rel;
+        if (!(rel <= max_rel_rms)) {  // fallback: keep LJ (its forces are in the force array)
+            ctx->mlp_on = false;
+            return 1;
+        }
+    }
+    ctx->mlp = mlp_bind(ctx, m);
+    ctx->mlp_on = true;
+    DevState &d = ctx->d;
+    launch_mlp_forces(ctx->stream, d, ctx->geom, ctx->mlp, d.cur, 0, ctx->ghost_grid);
+    CHECK_LAUNCH();
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int pairs_hip_mlp_detach(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    drop_graphs(ctx);
+    ctx->mlp_on = false;
+    return 0;
+}
+
+int pairs_hip_mlp_active(const pairs_hip_ctx *ctx) { return ctx && ctx->mlp_on ? 1 : 0; }
 
 }  // extern "C"
 

@@ -1,0 +1,106 @@
+"""Pins the FP64 MLP restatement (oracle/mlp_oracle.py) by mathematics: the
+model has no reference counterpart, so its checks are closed forms and finite
+differences, each chosen so that a dropped term, a wrong sign or a transposed
+weight fails."""
+import numpy as np
+
+from oracle import mlp_oracle as mo
+
+
+def _scale(seed=3):
+    rng = np.random.default_rng(seed)
+    sc = np.zeros(mo.NSCALE)
+    sc[0:16] = rng.uniform(0.0, 3.0, 16)
+    sc[16:32] = rng.uniform(0.3, 2.0, 16)
+    sc[32], sc[33] = -5.5, 0.7
+    return sc
+
+
+def _cluster(n=40, L=7.0, seed=5):
+    """particles on a jittered grid (no pair closer than about 0.9)"""
+    rng = np.random.default_rng(seed)
+    k = int(np.ceil(n ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(k)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n]
+    return (g + 0.5) * (L / k) + rng.uniform(-0.12, 0.12, (n, 3)), L
+
+
+def test_pair_descriptor_closed_form():
+    r = 1.37
+    x = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
+    G = mo.descriptors(x, None)
+    t = r - (0.8 + np.arange(16) * 1.7 / 15)
+    expect = np.exp(-t * t / (2 * 0.15 ** 2)) * 0.5 * (np.cos(np.pi * r / 2.5) + 1)
+    assert np.allclose(G[0], expect, rtol=1e-14, atol=0) and np.allclose(G[1], expect, rtol=1e-14, atol=0)
+    assert mo.fc(2.5) < 1e-15 and abs(mo.fc(0.0) - 1.0) < 1e-15
+    phi, _ = mo.basis(mo.MU[7])
+    assert abs(phi[7] - mo.fc(mo.MU[7])) < 1e-15
+    # beyond the cutoff nothing counts
+    x[1, 0] = 1.0 + 2.5001
+    assert np.all(mo.descriptors(x, None) == 0.0)
+
+
+def test_basis_derivative_matches_finite_difference():
+    r = np.linspace(0.85, 2.45, 33)
+    h = 1e-6
+    _, d = mo.basis(r)
+    fp, _ = mo.basis(r + h)
+    fm, _ = mo.basis(r - h)
+    assert np.allclose(d, (fp - fm) / (2 * h), rtol=1e-7, atol=1e-9)
+
+
+def test_backward_matches_finite_difference_on_G():
+    p, sc = mo.xavier(11), _scale()
+    rng = np.random.default_rng(2)
+    G = rng.uniform(0.0, 4.0, (5, 16))
+    E, D = mo.forward(p, sc, G)
+    h = 1e-6
+    for k in range(16):
+        Gp, Gm = G.copy(), G.copy()
+        Gp[:, k] += h
+        Gm[:, k] -= h
+        fd = (mo.forward(p, sc, Gp)[0] - mo.forward(p, sc, Gm)[0]) / (2 * h)
+        assert np.allclose(D[:, k], fd, rtol=1e-6, atol=1e-9), k
+
+
+def test_zero_output_layer():
+    p, sc = mo.xavier(1), _scale()
+    p[5248:5312] = 0.0
+    E, D = mo.forward(p, sc, np.ones((3, 16)))
+    assert np.allclose(E, sc[32] + sc[33] * p[5312]) and np.all(D == 0.0)
+
+
+def test_forces_are_minus_energy_gradient():
+    x, L = _cluster()
+    p, sc = mo.xavier(7), _scale()
+    F, _ = mo.forces(p, sc, x, L)
+    h = 1e-5
+    for i in (0, 3, 17, 39):
+        for d in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, d] += h
+            xm[i, d] -= h
+            fd = -(mo.energy(p, sc, xp, L) - mo.energy(p, sc, xm, L)) / (2 * h)
+            assert abs(F[i, d] - fd) <= 1e-6 * max(1.0, abs(fd)), (i, d, F[i, d], fd)
+    # translation invariance: the forces sum to zero
+    assert np.allclose(F.sum(0), 0.0, atol=1e-9)
+
+
+def test_lj_atom_energy_pair():
+    r = 1.2
+    x = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+    E = mo.lj_atom_energy(x, None)
+    u = 4 * (r ** -12 - r ** -6)
+    assert np.allclose(E, [0.5 * u, 0.5 * u], rtol=1e-15)
+
+
+def test_bf16_rounding_and_budget():
+    # nearest-even at the tie, exact values kept
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 3.14159])
+    assert np.array_equal(mo.bf16(x), [1.0, 1.0, 1.0 + 2 ** -6, 1.0, -2.5, 3.140625])
+    # the device's rounding points cost well under 2 % of the signal
+    p, sc = mo.xavier(7), _scale()
+    G = np.random.default_rng(1).uniform(0, 4, (500, 16))
+    E, D = mo.forward(p, sc, G)
+    Eb, Db = mo.forward_bf16(p, sc, G)
+    assert np.sqrt(np.mean((E - Eb) ** 2)) < 0.01 * E.std()
+    assert np.sqrt(np.mean((D - Db) ** 2) / np.mean(D ** 2)) < 0.01
