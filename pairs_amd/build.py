@@ -12,8 +12,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpairs_hip.so")
 ARCH = os.environ.get("PAIRS_HIP_ARCH", "gfx950")
 
-SOURCES = ["md_kernels.hip", "dem_kernels.hip", "batch_kernels.hip", "mlp_kernels.hip", "pairs_hip.cpp",
-           "dem_capi.cpp", "batch_capi.cpp", "mlp_capi.cpp"]
+SOURCES = ["md_kernels.hip", "dem_kernels.hip", "mlp_kernels.hip", "pairs_hip.cpp", "dem_capi.cpp", "mlp_capi.cpp",
+           "io.cpp"]
 
 # -ffp-contract=off: every multiply and add rounds separately, as in the
 #   reference's generated CPU code (bit-exact parity of the FP64 paths).
@@ -32,7 +32,7 @@ def hipcc():
 
 
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + FLAGS + ["-I" + os.path.join(HERE, "..", "include"), "-o", tmp]
     for s in srcs:

@@ -305,6 +305,7 @@ constexpr int BB = 128;        // threads per build workgroup
 constexpr int BW = BB / 64;    // waves per build workgroup
 constexpr int LR_MAX = 96;     // list rows buffered in LDS (a multiple of 4)
 constexpr int HCAP = 192;      // home particles with an LDS buffer column
+constexpr int LBS = HCAP + 2;  // row stride of the LDS list buffer (odd word count: rows fall on different banks)
 constexpr int NB_CAND = 2048;  // candidates of one home cell
 
 __device__ __forceinline__ double readlane_f64(double v, int l) {
@@ -325,8 +326,8 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
                                                         int *__restrict__ numneighs, int *__restrict__ ctr) {
     extern __shared__ char smem[];
     int *tslot = (int *) smem;                                  // [tcap] physical slot of each tile entry
-    uint16_t *lbuf = (uint16_t *) (tslot + tcap);               // [lrows][HCAP]
-    uint16_t *cand_all = lbuf + (size_t) lrows * HCAP;          // [BW][NB_CAND]
+    uint16_t *lbuf = (uint16_t *) (tslot + tcap);               // [lrows][LBS]
+    uint16_t *cand_all = lbuf + (size_t) lrows * LBS;           // [BW][NB_CAND]
     int *ctab = (int *) (cand_all + BW * NB_CAND);              // [9][zt][4]: local st, len, ghost st, len
     int *tref = ctab + 9 * zt * 4;                              // half lists only
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
@@ -415,13 +416,21 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
             const int hidx = oh - own_base;
             const int hfix = flags[h0 + hidx] & FLAG_FIXED;
             int cnt = 0;
+            // candidate chunk c0 + 64 is fetched (LDS offsets, then positions
+            // from memory) while the home particles test chunk c0
+            int on = lane < ncand ? cand[lane] : 0;
+            double px = X[tslot[on]], py = Y[tslot[on]], pz = Z[tslot[on]];
             for (int c0 = 0; c0 < ncand; c0 += 64) {
                 const int q = c0 + lane;
                 const bool qv = q < ncand;
-                const int o = qv ? cand[q] : 0;
-                const int pc = tslot[o];
-                const double cx = X[pc], cy = Y[pc], cz = Z[pc];
+                const int o = on;
+                const double cx = px, cy = py, cz = pz;
                 const int cref = HALF ? tref[o] : 0;
+                if (c0 + 64 < ncand) {
+                    on = q + 64 < ncand ? cand[q + 64] : 0;
+                    const int pn = tslot[on];
+                    px = X[pn]; py = Y[pn]; pz = Z[pn];
+                }
                 for (int h = 0; h < nh; h++) {
                     if (__builtin_amdgcn_readlane(hfix, h)) continue;
                     const double xi = readlane_f64(hx, h), yi = readlane_f64(hy, h), zi = readlane_f64(hz, h);
@@ -436,7 +445,7 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
                     const int k = base + lane_rank(m);
                     if (acc && k < ncap) {
                         const int idx = ownh - own_base;
-                        if (k < lrows && idx < HCAP) lbuf[k * HCAP + idx] = (uint16_t) o;
+                        if (k < lrows && idx < HCAP) lbuf[k * LBS + idx] = (uint16_t) o;
                         else nl[nl_at(k, h0 + idx, nl_stride)] = (uint16_t) o;
                     }
                     cnt = lane == h ? base + __popcll(m) : cnt;
@@ -461,9 +470,9 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
     for (int a = threadIdx.x; a < nhome; a += BB) {
         const int kb = (min(s_cnt[a], lrows) + 3) >> 2;
         for (int gq = 0; gq < kb; gq++) {
-            const uint64_t v = (uint64_t) lbuf[(4 * gq) * HCAP + a] | ((uint64_t) lbuf[(4 * gq + 1) * HCAP + a] << 16) |
-                               ((uint64_t) lbuf[(4 * gq + 2) * HCAP + a] << 32) |
-                               ((uint64_t) lbuf[(4 * gq + 3) * HCAP + a] << 48);
+            const uint64_t v = (uint64_t) lbuf[(4 * gq) * LBS + a] | ((uint64_t) lbuf[(4 * gq + 1) * LBS + a] << 16) |
+                               ((uint64_t) lbuf[(4 * gq + 2) * LBS + a] << 32) |
+                               ((uint64_t) lbuf[(4 * gq + 3) * LBS + a] << 48);
             nl4[(size_t) gq * nl_stride + h0 + a] = v;
         }
     }
@@ -541,7 +550,7 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
 // wave-uniform depth (entries past a lane's own count are masked), the next
 // group's offsets loading while the current one computes; each lane adds its
 // pair forces in list order, so the sum is bit-identical to the reference's.
-template <int U, int NT, int MODE, bool RED>
+template <int U, int NT, int MODE, bool RED, bool LDSPOS = false>
 __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
     static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
     // one type and U = 4: the pipelined walk (positions of the next step in
@@ -577,8 +586,20 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
         const int T = r_off[NRUNS];
         const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
         int *tphys = (int *) smem;
+        // LDSPOS: the tile's positions themselves are staged (x, y, z arrays)
+        double *tpx = (double *) smem, *tpy = tpx + tcap, *tpz = tpy + tcap;
         if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
-        if (fits) stage_slots(r_start, r_off, T, tphys);
+        if (fits) {
+            if (LDSPOS) {
+                for (int p = threadIdx.x; p < T; p += blockDim.x) {
+                    const int r = run_of(r_off, p);
+                    const int q = r_start[r] + (p - r_off[r]);
+                    tpx[p] = X[q]; tpy[p] = Y[q]; tpz[p] = Z[q];
+                }
+            } else {
+                stage_slots(r_start, r_off, T, tphys);
+            }
+        }
         __syncthreads();
         const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
         const int nt = tab.nt;
@@ -605,15 +626,23 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
                 });
             } else
             list_walk<U>(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *on) {
-                int jp[U];
-#pragma unroll
-                for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
                 double xj[U], yj[U], zj[U];
                 int tj[U];
+                if (LDSPOS) {
 #pragma unroll
-                for (int q = 0; q < U; q++) {
-                    xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]];
-                    if (NT != 1) tj[q] = type[jp[q]];
+                    for (int q = 0; q < U; q++) {
+                        xj[q] = tpx[on[q]]; yj[q] = tpy[on[q]]; zj[q] = tpz[on[q]];
+                        if (NT != 1) tj[q] = type[tphys[on[q]]];
+                    }
+                } else {
+                    int jp[U];
+#pragma unroll
+                    for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
+#pragma unroll
+                    for (int q = 0; q < U; q++) {
+                        xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]];
+                        if (NT != 1) tj[q] = type[jp[q]];
+                    }
                 }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
@@ -678,9 +707,9 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
 
 
 // one segment per workgroup of one system
-template <int U, int NT, int MODE, bool RED>
+template <int U, int NT, int MODE, bool RED, bool LDSPOS = false>
 __global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
-    lj_full_segment<U, NT, MODE, RED>(A, xcd_block(blockIdx.x, gridDim.x));
+    lj_full_segment<U, NT, MODE, RED, LDSPOS>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // the segments of many independent systems in one launch: block b belongs to
@@ -1227,7 +1256,7 @@ static int build_lrows(const DevState &d) { return min((d.ncap + 3) / 4 * 4, LR_
 
 static size_t build_lds(const DevState &d, const Geom &g, int half, int tcap) {
     return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) +
-           (size_t) build_lrows(d) * HCAP * sizeof(uint16_t) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
+           (size_t) build_lrows(d) * LBS * sizeof(uint16_t) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
            (size_t) 9 * build_zt(g) * 4 * sizeof(int);
 }
 
@@ -1355,6 +1384,12 @@ static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, cons
     const size_t lds = (size_t) d.tile_cap * sizeof(int);
     const LJArgs A = lj_args(d, g, prm, cur, next, do_vv2, do_vv1, MODE == 1, eatom);
     static const bool pipe = getenv("PAIRS_HIP_LJ_PIPE") && atoi(getenv("PAIRS_HIP_LJ_PIPE")) == 1;
+    static const bool ldspos = getenv("PAIRS_HIP_LJ_LDS") && atoi(getenv("PAIRS_HIP_LJ_LDS")) == 1;
+    if (ldspos && NT == 1 && (size_t) d.tile_cap * 3 * sizeof(double) <= (size_t) d.max_lds - 2048) {
+        hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED, true>), dim3(g.nsegs), dim3(FB),
+                           (size_t) d.tile_cap * 3 * sizeof(double), s, A);
+        return;
+    }
     if (pipe) hipLaunchKernelGGL((k_lj_full<4, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
     else hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
 }

@@ -157,12 +157,17 @@ static int setup_geometry(pairs_hip_ctx *ctx) {
     return 0;
 }
 
-static int read_counters(pairs_hip_ctx *ctx) {
-    HIP_OK(hipMemcpyAsync(ctx->h_counters, ctx->d.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(hipStreamSynchronize(ctx->stream));
+// host-side bookkeeping of a counters snapshot already in ctx->h_counters
+static void take_counters(pairs_hip_ctx *ctx) {
     ctx->nghost = ctx->h_counters[C_NGHOST];
     ctx->nsend_all = ctx->h_counters[C_NSEND_ALL];
     ctx->ghost_grid = std::max(1, std::min((ctx->nsend_all + 255) / 256 + 1, 4096));
+}
+
+static int read_counters(pairs_hip_ctx *ctx) {
+    HIP_OK(hipMemcpyAsync(ctx->h_counters, ctx->d.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    take_counters(ctx);
     return 0;
 }
 
@@ -249,6 +254,7 @@ int pairs_hip_create(pairs_hip_ctx **out, const pairs_hip_md_config *cfg) {
         ctx->max_lds = lds;
     DevState &d = ctx->d;
     d.max_lds = ctx->max_lds;
+    if (const char *e = getenv("PAIRS_HIP_HOME")) ctx->home_target = std::max(8, atoi(e));
     setup_geometry(ctx);
     // the cell grid is fixed by the box; its arrays are sized to fit it
     // (the generated code grows ncells_capacity to twice the need on overflow)
@@ -1144,6 +1150,12 @@ struct pairs_hip_batch {
     std::vector<double> thermo;        // last thermo row per replica
     bool ntypes1 = true;
     int pending_home = 0;              // segment size to apply at the next rebuild
+    // rebuild streams: replica r's rebuild kernels run on rs[r % NRS], so the
+    // small per-replica kernels of different replicas overlap on the device
+    static constexpr int NRS = 4;
+    hipStream_t rs[NRS] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_main = nullptr;      // end of the main stream's work before a rebuild
+    int *h_cnt = nullptr;              // pinned, [R][C_COUNT] counters of every replica
 };
 
 static void batch_drop_graphs(pairs_hip_batch *b) {
@@ -1221,29 +1233,45 @@ static int batch_rebuild(pairs_hip_batch *b) {
         batch_drop_graphs(b);
         b->pending_home = 0;
     }
-    for (pairs_hip_ctx *c : b->rep) {
-        launch_set_counters(s, c->d.counters, C_IMG_OVF, 1, 0);
-        launch_exchange(s, c->d, c->geom, c->cfg.periodic);
-        launch_borders(s, c->d, c->geom, c->cfg.periodic);
-        launch_bin_sort(s, c->d, c->geom);
+    // the replicas' rebuilds start after the main stream's last step
+    BATCH_OK(hipEventRecord(b->ev_main, s));
+    for (int k = 0; k < pairs_hip_batch::NRS; k++) BATCH_OK(hipStreamWaitEvent(b->rs[k], b->ev_main, 0));
+    auto rstream = [&](int r) { return b->rs[r % pairs_hip_batch::NRS]; };
+    // every replica's counters into one pinned array, then one wait for all
+    auto fetch_counters = [&]() -> int {
+        for (int r = 0; r < R; r++)
+            BATCH_OK(hipMemcpyAsync(b->h_cnt + (size_t) r * C_COUNT, b->rep[r]->d.counters, sizeof(int) * C_COUNT,
+                                    hipMemcpyDeviceToHost, rstream(r)));
+        for (int k = 0; k < pairs_hip_batch::NRS; k++) BATCH_OK(hipStreamSynchronize(b->rs[k]));
+        for (int r = 0; r < R; r++) {
+            std::memcpy(b->rep[r]->h_counters, b->h_cnt + (size_t) r * C_COUNT, sizeof(int) * C_COUNT);
+            take_counters(b->rep[r]);
+        }
+        return 0;
+    };
+    for (int r = 0; r < R; r++) {
+        pairs_hip_ctx *c = b->rep[r];
+        hipStream_t rs = rstream(r);
+        launch_set_counters(rs, c->d.counters, C_IMG_OVF, 1, 0);
+        launch_exchange(rs, c->d, c->geom, c->cfg.periodic);
+        launch_borders(rs, c->d, c->geom, c->cfg.periodic);
+        launch_bin_sort(rs, c->d, c->geom);
     }
-    BATCH_OK(hipStreamSynchronize(s));
+    if (int e = fetch_counters()) return e;
     std::vector<char> fallback(R, 0);
     for (int r = 0; r < R; r++) {
         pairs_hip_ctx *c = b->rep[r];
-        if (read_counters(c)) { b->err = c->err; return PAIRS_HIP_ERR_HIP; }
         c->d.img_on = c->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
         if (take_overflow(c)) fallback[r] = 1;
         else {
-            launch_set_counters(s, c->d.counters, C_TILE_MAX, 1, 0);
-            launch_build_neighbors(s, c->d, c->geom, c->cfg.half, c->cfg.cell_spacing * c->cfg.cell_spacing);
+            launch_set_counters(rstream(r), c->d.counters, C_TILE_MAX, 1, 0);
+            launch_build_neighbors(rstream(r), c->d, c->geom, c->cfg.half, c->cfg.cell_spacing * c->cfg.cell_spacing);
         }
     }
-    BATCH_OK(hipStreamSynchronize(s));
+    if (int e = fetch_counters()) return e;
     for (int r = 0; r < R; r++) {
         pairs_hip_ctx *c = b->rep[r];
         if (!fallback[r]) {
-            if (read_counters(c)) { b->err = c->err; return PAIRS_HIP_ERR_HIP; }
             if (c->h_counters[C_OVF] == OVF_TILE || c->h_counters[C_REQ_NEIGH] > 0) {
                 launch_set_counters(s, c->d.counters, C_OVF, 1, 0);
                 launch_set_counters(s, c->d.counters, C_REQ_NEIGH, 3, 0);
@@ -1317,6 +1345,16 @@ int pairs_hip_batch_create(pairs_hip_batch **out, int nreplicas, const pairs_hip
         pairs_hip_batch_destroy(b);
         return PAIRS_HIP_ERR_HIP;
     }
+    for (int k = 0; k < pairs_hip_batch::NRS; k++)
+        if (hipStreamCreateWithFlags(&b->rs[k], hipStreamNonBlocking) != hipSuccess) {
+            pairs_hip_batch_destroy(b);
+            return PAIRS_HIP_ERR_HIP;
+        }
+    if (hipEventCreateWithFlags(&b->ev_main, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void **) &b->h_cnt, R * C_COUNT * sizeof(int)) != hipSuccess) {
+        pairs_hip_batch_destroy(b);
+        return PAIRS_HIP_ERR_HIP;
+    }
     b->thermo.assign(5 * R, 0.0);
     *out = b;
     return 0;
@@ -1331,6 +1369,10 @@ int pairs_hip_batch_destroy(pairs_hip_batch *b) {
     for (int p = 0; p < 3; p++) if (b->d_desc[p]) hipFree(b->d_desc[p]);
     if (b->d_rep) hipFree(b->d_rep);
     if (b->d_base) hipFree(b->d_base);
+    for (int k = 0; k < pairs_hip_batch::NRS; k++)
+        if (b->rs[k]) { hipStreamSynchronize(b->rs[k]); hipStreamDestroy(b->rs[k]); }
+    if (b->ev_main) hipEventDestroy(b->ev_main);
+    if (b->h_cnt) hipHostFree(b->h_cnt);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return 0;

@@ -154,6 +154,28 @@ class DEMSystem:
         return self._check(self.L.pairs_hip_dem_add_spheres(self.ctx, n, ip(u), ip(t), dp(r), dp(m), dp(x), dp(v),
                                                             ip(f)), "add_spheres")
 
+    def read_particle_data(self, filename, properties, shape):
+        """psim.read_particle_data(filename, properties, shape) of examples/dem.py:
+        rows of the CSV file (pairs_amd.io.read_particle_data) added as
+        half-spaces or spheres; returns the number added."""
+        from . import io
+        d = io.read_particle_data(filename, properties, [(0.0, self.domain[k]) for k in range(3)])
+        n = len(next(iter(d.values()))) if d else 0
+        if shape == "halfspace":
+            for k in range(n):
+                self.add_halfspace(int(d["uid"][k]), d["position"][k], d["normal"][k],
+                                   flags=int(d["flags"][k]) if "flags" in d else 13,
+                                   type_=int(d["type"][k]) if "type" in d else 0,
+                                   mass=float(d["mass"][k]) if "mass" in d else 1.0)
+            return n
+        if shape != "sphere":
+            raise ValueError(f"unsupported shape {shape}")
+        if n:
+            # masses come from the density, as for the grid (the file's mass column is ignored)
+            self.add_spheres(d["uid"], d["radius"], d["position"], d["linear_velocity"],
+                             type_=d.get("type"), flags=d.get("flags"))
+        return n
+
     def setup(self):
         self._check(self.L.pairs_hip_dem_setup(self.ctx), "setup")
 

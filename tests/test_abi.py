@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 def test_python_binding_covers_header():
     import importlib
     from pairs_amd import native
-    for mod in ("dem", "batch", "mlp"):  # each registers its own entry points
+    for mod in ("dem", "batch", "mlp", "io"):  # each registers its own entry points
         if os.path.exists(os.path.join(ROOT, "pairs_amd", mod + ".py")):
             importlib.import_module("pairs_amd." + mod)
     assert set(_declared()) <= set(native.exported_symbols())

@@ -93,3 +93,26 @@ def test_larger_domain_vs_oracle():
         assert np.array_equal(gu, ou) and np.array_equal(gs, os_), b
         assert _rel(gt, ot) <= REL and _rel(gi, oi) <= REL, b
     assert int(nc.sum()) > 500
+
+
+def test_planes_read_from_csv_equal_builtin_planes(tmp_path):
+    """examples/dem.py reads its planes with read_particle_data: the CSV path
+    gives the same system, step for step; the VTK writer sees every local"""
+    dm = _dem()
+    from pairs_amd import io
+    a = dm.reference_system()
+    b = dm.DEMSystem((0.04, 0.015, 0.04))
+    b.sc_grid((0.04, 0.015, 0.04))
+    here = os.path.dirname(os.path.abspath(__file__))
+    n = b.read_particle_data(os.path.join(here, "golden", "planes.input"),
+                             ["uid", "type", "mass", "position", "normal", "flags"], "halfspace")
+    assert n == 2
+    b.setup()
+    a.run(0, 60)
+    b.run(0, 60)
+    for name in ("position", "linear_velocity", "angular_velocity", "uid"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    assert io.vtk_write(b, str(tmp_path / "dem"), 60)
+    text = open(str(tmp_path / "dem_60.vtk")).read()
+    nsph = int(np.sum(b.get("shape") == 0))
+    assert f"POINTS {nsph} double" in text  # the two planes are Infinite

@@ -183,3 +183,14 @@ def test_c2_size_properties_and_oracle_100_steps():
     assert np.array_equal(g.velocities(), o.velocities())
     assert np.array_equal(g.forces(), o.forces())
     assert g.nghost == o.nghost
+
+
+def test_vtk_output_of_lj_system(tmp_path):
+    from pairs_amd import md, io
+    s = md.fcc_system(4)
+    s.run(0, 10, 100)
+    assert io.vtk_write(s, str(tmp_path / "md"), 10, 5)
+    lines = open(str(tmp_path / "md_10.vtk")).read().split("\n")
+    assert lines[4] == "POINTS 256 double"
+    x = s.positions()
+    assert lines[5] == f"{x[0, 0]:.8f} {x[0, 1]:.8f} {x[0, 2]:.8f}"

@@ -2,8 +2,9 @@
 restatement (oracle/mlp_oracle.py, pinned by finite differences and closed
 forms in tests/test_oracle_mlp.py).
 
-Tolerances: descriptors are FP32 sums of at most ~80 terms of size <= 1
-(rtol 1e-5, atol 5e-6).  The network runs in BF16 on MFMA; against the
+Tolerances: descriptors are FP32 sums of at most ~80 terms of size <= 1,
+each Gaussian from a two-anchor recurrence with relative error below 1e-5
+(rtol 3e-5, atol 1e-5).  The network runs in BF16 on MFMA; against the
 oracle with the device's rounding points emulated (forward_bf16) the energy
 and gradients agree to float32-accumulation noise (rel 5e-3 RMS; a transposed
 operand or wrong lane layout gives errors of order 1), and the forces to
@@ -50,7 +51,7 @@ def test_descriptors_vs_oracle():
     G = mlp.descriptors(s)
     Gr = mo.descriptors(s.positions(), L)
     assert G.shape == (256, 16)
-    assert np.allclose(G, Gr, rtol=1e-5, atol=5e-6), np.abs(G - Gr).max()
+    assert np.allclose(G, Gr, rtol=3e-5, atol=1e-5), np.abs(G - Gr).max()
     assert Gr.max() > 1.0  # the shells are populated
 
 
