@@ -144,7 +144,7 @@ class BatchEnsemble:
     def set_home(self, home):
         self._check(self.L.pairs_hip_batch_set_home(self.h, home), "batch_set_home")
 
-    def autotune(self, ts_begin, steps, nsteps, candidates=(64, 96, 112, 160, 224)):
+    def autotune(self, ts_begin, steps, nsteps, candidates=(32, 48, 64, 96, 160)):
         """-> (chosen home size, {candidate: ms per step}, next timestep)"""
         cand = np.ascontiguousarray(candidates, np.int32)
         ms = np.zeros(len(cand))

@@ -26,7 +26,7 @@ struct pairs_hip_ctx {
     int ghost_grid = 1;
     int max_lds = 65536;       // shared memory a workgroup may allocate
     int zc_override = 0;
-    int home_target = 112;     // home particles a segment aims at (sets zc)
+    int home_target = 40;      // home particles a segment aims at (sets zc; about 3 cells at rho 0.84)
     // graphs of one non-rebuild step, by parity of the current buffer
     bool use_graphs = true;
     hipGraphExec_t step_graph[2] = {nullptr, nullptr};

@@ -12,8 +12,6 @@
 #include "md_kernels.hpp"
 #include "md_segment.hpp"
 
-#include <stdlib.h>
-
 namespace pk {
 
 // ---------------------------------------------------------------- utilities
@@ -406,6 +404,25 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // the cell's candidates, chunk c on lane c * 64 + lane; the first
+        // CREG chunks stay in registers while every home particle tests them
+        constexpr int CREG = 8;
+        const int nchunk = (ncand + 63) >> 6;
+        double rcx[CREG], rcy[CREG], rcz[CREG];
+        int rco[CREG], rcr[CREG];
+#pragma unroll
+        for (int c = 0; c < CREG; c++) {
+            rco[c] = 0; rcr[c] = 0;
+            rcx[c] = 0.0; rcy[c] = 0.0; rcz[c] = 0.0;
+            if (c < nchunk) {
+                const int q = c * 64 + lane;
+                const int o = q < ncand ? cand[q] : 0;
+                const int pc = tslot[o];
+                rco[c] = o;
+                rcx[c] = X[pc]; rcy[c] = Y[pc]; rcz[c] = Z[pc];
+                if (HALF) rcr[c] = tref[o];
+            }
+        }
         for (int hb = 0; hb < hlen; hb += 64) {
             const int nh = min(64, hlen - hb);
             // lane h holds home particle h of this pass
@@ -413,53 +430,50 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
             const int ph = tslot[oh];
             const double hx = X[ph], hy = Y[ph], hz = Z[ph];
             const int href = HALF ? tref[oh] : 0;
-            const int hidx = oh - own_base;
-            const int hfix = flags[h0 + hidx] & FLAG_FIXED;
-            int cnt = 0;
-            // candidate chunk c0 + 64 is fetched (LDS offsets, then positions
-            // from memory) while the home particles test chunk c0
-            int on = lane < ncand ? cand[lane] : 0;
-            double px = X[tslot[on]], py = Y[tslot[on]], pz = Z[tslot[on]];
-            for (int c0 = 0; c0 < ncand; c0 += 64) {
-                const int q = c0 + lane;
-                const bool qv = q < ncand;
-                const int o = on;
-                const double cx = px, cy = py, cz = pz;
-                const int cref = HALF ? tref[o] : 0;
-                if (c0 + 64 < ncand) {
-                    on = q + 64 < ncand ? cand[q + 64] : 0;
-                    const int pn = tslot[on];
-                    px = X[pn]; py = Y[pn]; pz = Z[pn];
-                }
-                for (int h = 0; h < nh; h++) {
-                    if (__builtin_amdgcn_readlane(hfix, h)) continue;
+            const int hfix = flags[h0 + (oh - own_base)] & FLAG_FIXED;
+            for (int h = 0; h < nh; h++) {
+                const int ownh = hst + hb + h;
+                const int idx = ownh - own_base;
+                int cnt = 0;  // wave-uniform: entries of home particle h so far
+                if (!__builtin_amdgcn_readlane(hfix, h)) {
                     const double xi = readlane_f64(hx, h), yi = readlane_f64(hy, h), zi = readlane_f64(hz, h);
-                    const double dx = xi - cx, dy = yi - cy, dz = zi - cz;
-                    double rsq = dx * dx + dy * dy;
-                    rsq = rsq + dz * dz;
-                    const int ownh = hst + hb + h;
-                    const bool keep = HALF ? (__builtin_amdgcn_readlane(href, h) < cref) : (o != ownh);
-                    const bool acc = qv && keep && rsq < rc2;
-                    const unsigned long long m = __ballot(acc);
-                    const int base = __builtin_amdgcn_readlane(cnt, h);
-                    const int k = base + lane_rank(m);
-                    if (acc && k < ncap) {
-                        const int idx = ownh - own_base;
-                        if (k < lrows && idx < HCAP) lbuf[k * LBS + idx] = (uint16_t) o;
-                        else nl[nl_at(k, h0 + idx, nl_stride)] = (uint16_t) o;
+                    const int hr = HALF ? __builtin_amdgcn_readlane(href, h) : 0;
+                    // one chunk: accepted candidates append in lane (= candidate) order
+                    auto test = [&](int c0, int o, double cx, double cy, double cz, int cr) {
+                        const bool qv = c0 + lane < ncand;
+                        const double dx = xi - cx, dy = yi - cy, dz = zi - cz;
+                        double rsq = dx * dx + dy * dy;
+                        rsq = rsq + dz * dz;
+                        const bool keep = HALF ? (hr < cr) : (o != ownh);
+                        const bool acc = qv && keep && rsq < rc2;
+                        const unsigned long long m = __ballot(acc);
+                        const int k = cnt + lane_rank(m);
+                        if (acc && k < ncap) {
+                            if (k < lrows && idx < HCAP) lbuf[k * LBS + idx] = (uint16_t) o;
+                            else nl[nl_at(k, h0 + idx, nl_stride)] = (uint16_t) o;
+                        }
+                        cnt += __popcll(m);
+                    };
+#pragma unroll
+                    for (int c = 0; c < CREG; c++)
+                        if (c < nchunk) test(c * 64, rco[c], rcx[c], rcy[c], rcz[c], rcr[c]);
+                    for (int c = CREG; c < nchunk; c++) {  // more candidates than the registers hold
+                        const int q = c * 64 + lane;
+                        const int o = q < ncand ? cand[q] : 0;
+                        const int pc = tslot[o];
+                        test(c * 64, o, X[pc], Y[pc], Z[pc], HALF ? tref[o] : 0);
                     }
-                    cnt = lane == h ? base + __popcll(m) : cnt;
                 }
-            }
-            if (lane < nh) {
-                // reference overflow rule (generated resize check): count+1 >= capacity
-                int K = cnt;
-                if (K + 1 >= ncap) {
-                    atomicMax(&ctr[C_REQ_NEIGH], K + 2);
-                    K = ncap - 2;
+                if (lane == 0) {
+                    // reference overflow rule (generated resize check): count+1 >= capacity
+                    int K = cnt;
+                    if (K + 1 >= ncap) {
+                        atomicMax(&ctr[C_REQ_NEIGH], K + 2);
+                        K = ncap - 2;
+                    }
+                    numneighs[h0 + idx] = K;
+                    if (idx < HCAP) s_cnt[idx] = K;
                 }
-                numneighs[h0 + hidx] = K;
-                if (hidx < HCAP) s_cnt[hidx] = K;
             }
         }
     }
@@ -550,12 +564,9 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
 // wave-uniform depth (entries past a lane's own count are masked), the next
 // group's offsets loading while the current one computes; each lane adds its
 // pair forces in list order, so the sum is bit-identical to the reference's.
-template <int U, int NT, int MODE, bool RED, bool LDSPOS = false>
+template <int U, int NT, int MODE, bool RED>
 __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
     static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
-    // one type and U = 4: the pipelined walk (positions of the next step in
-    // flight while the current one computes)
-    constexpr bool PIPE = NT == 1 && U == 4;
     const int *__restrict__ counters = A.counters;
     const Geom &g = A.g;
     const int *__restrict__ lstart = A.lstart, *__restrict__ gstart = A.gstart;
@@ -586,20 +597,8 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
         const int T = r_off[NRUNS];
         const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
         int *tphys = (int *) smem;
-        // LDSPOS: the tile's positions themselves are staged (x, y, z arrays)
-        double *tpx = (double *) smem, *tpy = tpx + tcap, *tpz = tpy + tcap;
         if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
-        if (fits) {
-            if (LDSPOS) {
-                for (int p = threadIdx.x; p < T; p += blockDim.x) {
-                    const int r = run_of(r_off, p);
-                    const int q = r_start[r] + (p - r_off[r]);
-                    tpx[p] = X[q]; tpy[p] = Y[q]; tpz[p] = Z[q];
-                }
-            } else {
-                stage_slots(r_start, r_off, T, tphys);
-            }
-        }
+        if (fits) stage_slots(r_start, r_off, T, tphys);
         __syncthreads();
         const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
         const int nt = tab.nt;
@@ -613,36 +612,16 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
             const double xi = X[ii], yi = Y[ii], zi = Z[ii];
             const int ti = NT != 1 ? type[ii] : 0;
             double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
-            if constexpr (PIPE) {
-                list_walk_pipe(nl, ii, nl_stride, n, nmax, T, tphys, X, Y, Z,
-                               [&](int k, const double *xj, const double *yj, const double *zj) {
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        double g0, g1, g2, uu, ww;
-                        lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < n, rc2, sig0, eps0, g0, g1, g2, uu, ww);
-                        t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
-                        if (RED) { u += uu; w += ww; }
-                    }
-                });
-            } else
             list_walk<U>(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *on) {
+                int jp[U];
+#pragma unroll
+                for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
                 double xj[U], yj[U], zj[U];
                 int tj[U];
-                if (LDSPOS) {
 #pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        xj[q] = tpx[on[q]]; yj[q] = tpy[on[q]]; zj[q] = tpz[on[q]];
-                        if (NT != 1) tj[q] = type[tphys[on[q]]];
-                    }
-                } else {
-                    int jp[U];
-#pragma unroll
-                    for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
-#pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]];
-                        if (NT != 1) tj[q] = type[jp[q]];
-                    }
+                for (int q = 0; q < U; q++) {
+                    xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]];
+                    if (NT != 1) tj[q] = type[jp[q]];
                 }
 #pragma unroll
                 for (int q = 0; q < U; q++) {
@@ -707,9 +686,9 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
 
 
 // one segment per workgroup of one system
-template <int U, int NT, int MODE, bool RED, bool LDSPOS = false>
+template <int U, int NT, int MODE, bool RED>
 __global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
-    lj_full_segment<U, NT, MODE, RED, LDSPOS>(A, xcd_block(blockIdx.x, gridDim.x));
+    lj_full_segment<U, NT, MODE, RED>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // the segments of many independent systems in one launch: block b belongs to
@@ -1383,15 +1362,7 @@ static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, cons
                            int do_vv2, int do_vv1, double *eatom) {
     const size_t lds = (size_t) d.tile_cap * sizeof(int);
     const LJArgs A = lj_args(d, g, prm, cur, next, do_vv2, do_vv1, MODE == 1, eatom);
-    static const bool pipe = getenv("PAIRS_HIP_LJ_PIPE") && atoi(getenv("PAIRS_HIP_LJ_PIPE")) == 1;
-    static const bool ldspos = getenv("PAIRS_HIP_LJ_LDS") && atoi(getenv("PAIRS_HIP_LJ_LDS")) == 1;
-    if (ldspos && NT == 1 && (size_t) d.tile_cap * 3 * sizeof(double) <= (size_t) d.max_lds - 2048) {
-        hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED, true>), dim3(g.nsegs), dim3(FB),
-                           (size_t) d.tile_cap * 3 * sizeof(double), s, A);
-        return;
-    }
-    if (pipe) hipLaunchKernelGGL((k_lj_full<4, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
-    else hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
 }
 
 void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,

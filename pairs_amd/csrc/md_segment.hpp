@@ -154,57 +154,4 @@ __device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stri
 }
 
 
-// Three-stage form of the walk, four entries per step: while step k
-// computes, the positions of step k + 1 are in flight from memory and the
-// physical slots of step k + 2 are read from the LDS tile (list groups load
-// three steps ahead).  body(k, xj, yj, zj) gets the positions of entries
-// k .. k+3; entries past the list repeat its last group (in bounds, masked by
-// the body).  Loads are relaxed atomics so that the compiler keeps them at the
-// point of issue instead of next to their use.
-template <typename Body>
-__device__ __forceinline__ void list_walk_pipe(const uint16_t *nl, int i, int nl_stride, int n, int nmax, int T,
-                                               const int *tslot, const double *X, const double *Y, const double *Z,
-                                               Body &&body) {
-    const uint64_t *lp = (const uint64_t *) nl + i;
-    const int gl = max((n + 3) / 4 - 1, 0), tlast = T - 1;
-    auto ld = [&](int gq) {
-        return __hip_atomic_load(lp + (size_t) min(gq, gl) * nl_stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-    };
-    auto slots = [&](uint64_t a, int (&jp)[4]) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) jp[q] = tslot[min((int) ((a >> (16 * q)) & 0xffff), tlast)];
-    };
-    auto gather = [&](const int (&jp)[4], double (&x)[4], double (&y)[4], double (&z)[4]) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            x[q] = __hip_atomic_load(X + jp[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            y[q] = __hip_atomic_load(Y + jp[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-            z[q] = __hip_atomic_load(Z + jp[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-        }
-    };
-    // two position sets A and B alternate roles (no register copies, so no
-    // wait on a load before its step)
-    int ja[4], jb[4];
-    double xa[4], ya[4], za[4], xb[4], yb[4], zb[4];
-    slots(ld(0), ja);
-    gather(ja, xa, ya, za);
-    slots(ld(1), jb);
-    uint64_t la = ld(2), lb = ld(3);
-    for (int k = 0; k < nmax; k += 8) {
-        const int g = k >> 2;
-        gather(jb, xb, yb, zb);  // step g + 1
-        slots(la, ja);           // step g + 2
-        la = ld(g + 4);
-        // keep the scheduler from moving step g's arithmetic above the loads
-        __builtin_amdgcn_sched_barrier(0);
-        body(k, xa, ya, za);
-        if (k + 4 >= nmax) break;
-        gather(ja, xa, ya, za);  // step g + 2
-        slots(lb, jb);           // step g + 3
-        lb = ld(g + 5);
-        __builtin_amdgcn_sched_barrier(0);
-        body(k + 4, xb, yb, zb);
-    }
-}
-
 }  // namespace pk

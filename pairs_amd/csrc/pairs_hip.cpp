@@ -117,8 +117,8 @@ static int zero_particle_arrays(pairs_hip_ctx *ctx) {
     return 0;
 }
 
-// segments: zc cells of one z-column, sized for about home_target (112 by
-// default) home particles per 128-thread block (override: PAIRS_HIP_ZC)
+// segments: zc cells of one z-column, sized for about home_target (40 by
+// default) home particles per force workgroup (overrides: PAIRS_HIP_ZC, PAIRS_HIP_HOME)
 static void setup_segments(pairs_hip_ctx *ctx, int zc) {
     Geom &g = ctx->geom;
     if (zc <= 0) {

@@ -67,12 +67,14 @@ class DEMSystem:
     """One DEM domain on the GPU (the reference's PairsSimulation for dem.py)."""
 
     def __init__(self, domain, periodic=(True, True, False), particle_capacity=8192, contact_capacity=20,
-                 send_capacity=8192, ncells_capacity=None, params=None, ntypes=1):
+                 send_capacity=8192, ncells_capacity=None, params=None, ntypes=1, kappa=None, ln_restitution=None,
+                 cell_width=None):
         p = dict(REFERENCE)
         p.update(params or {})
         self.params = p
-        kappa = 2.0 * (1.0 - p["poisson"]) / (2.0 - p["poisson"])  # Thornton et al. (dem.py:132)
-        width = 1.01 * (p["diameter"] * 1.1)                          # linkedCellWidth (dem.py:137)
+        if kappa is None:
+            kappa = 2.0 * (1.0 - p["poisson"]) / (2.0 - p["poisson"])  # Thornton et al. (dem.py:132)
+        width = cell_width if cell_width is not None else 1.01 * (p["diameter"] * 1.1)  # linkedCellWidth (dem.py:137)
         self.cell_width = width
         dims = [math.ceil((domain[d] + 2 * width) / width) + 1 for d in range(3)]
         ncells = dims[0] * dims[1] * dims[2] + 1
@@ -86,7 +88,8 @@ class DEMSystem:
         cfg.cell_width = width
         cfg.dt, cfg.gravity = p["dt"], p["gravity"]
         cfg.density_particle, cfg.density_fluid = p["density_particle"], p["density_fluid"]
-        cfg.kappa, cfg.ln_restitution, cfg.collision_time = kappa, math.log(p["restitution"]), p["collision_time"]
+        lnr = math.log(p["restitution"]) if ln_restitution is None else ln_restitution
+        cfg.kappa, cfg.ln_restitution, cfg.collision_time = kappa, lnr, p["collision_time"]
         cfg.ntypes = ntypes
         for k in range(ntypes * ntypes):
             cfg.friction_static[k] = p["friction_static"]
@@ -120,12 +123,15 @@ class DEMSystem:
         return code
 
     # ---- set-up (examples/dem.py:158-201)
-    def sc_grid(self, domain=None, spacing=None, diameter=None, initial_velocity=None, density=None):
+    def sc_grid(self, domain=None, spacing=None, diameter=None, initial_velocity=None, density=None,
+                min_diameter=None, max_diameter=None):
         p = self.params
         d = diameter if diameter is not None else p["diameter"]
         dom = domain or self.domain
+        dmin = d * 0.9 if min_diameter is None else min_diameter
+        dmax = d * 1.1 if max_diameter is None else max_diameter
         return self._check(self.L.pairs_hip_dem_sc_grid(
-            self.ctx, dom[0], dom[1], dom[2], spacing or p["spacing"], d, d * 0.9, d * 1.1,
+            self.ctx, dom[0], dom[1], dom[2], spacing or p["spacing"], d, dmin, dmax,
             p["initial_velocity"] if initial_velocity is None else initial_velocity,
             density or p["density_particle"], self.ntypes), "sc_grid")
 
