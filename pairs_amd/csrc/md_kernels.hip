@@ -302,8 +302,9 @@ __device__ __forceinline__ void write_images(const Images &im, int i, int nlocal
 constexpr int BB = 128;        // threads per build workgroup
 constexpr int BW = BB / 64;    // waves per build workgroup
 constexpr int LR_MAX = 96;     // list rows buffered in LDS (a multiple of 4)
-constexpr int HCAP = 192;      // home particles with an LDS buffer column
-constexpr int LBS = HCAP + 2;  // row stride of the LDS list buffer (odd word count: rows fall on different banks)
+// home particles with an LDS buffer column: g.hcap (a multiple of 4, set
+// from the segment size); rows are hcap + 2 entries long (an odd number of
+// 4-byte words, so consecutive rows start on different banks)
 constexpr int NB_CAND = 2048;  // candidates of one home cell
 
 __device__ __forceinline__ double readlane_f64(double v, int l) {
@@ -323,13 +324,14 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
                                                         int ncap, int lrows, int tcap, int zt,
                                                         int *__restrict__ numneighs, int *__restrict__ ctr) {
     extern __shared__ char smem[];
+    const int HCAP = g.hcap, LBS = g.hcap + 2;
     int *tslot = (int *) smem;                                  // [tcap] physical slot of each tile entry
-    uint16_t *lbuf = (uint16_t *) (tslot + tcap);               // [lrows][LBS]
+    int *s_cnt = tslot + tcap;                                  // [hcap] list length per buffered column
+    int *ctab = s_cnt + HCAP;                                   // [9][zt][4]: local st, len, ghost st, len
+    int *tref = ctab + 9 * zt * 4;                              // [tcap] half lists only
+    uint16_t *lbuf = (uint16_t *) (tref + (HALF ? tcap : 0));   // [lrows][LBS]
     uint16_t *cand_all = lbuf + (size_t) lrows * LBS;           // [BW][NB_CAND]
-    int *ctab = (int *) (cand_all + BW * NB_CAND);              // [9][zt][4]: local st, len, ghost st, len
-    int *tref = ctab + 9 * zt * 4;                              // half lists only
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
-    __shared__ int s_cnt[HCAP];
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
     const int seg = xcd_block(blockIdx.x, gridDim.x);
@@ -564,7 +566,8 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
 // wave-uniform depth (entries past a lane's own count are masked), the next
 // group's offsets loading while the current one computes; each lane adds its
 // pair forces in list order, so the sum is bit-identical to the reference's.
-template <int U, int NT, int MODE, bool RED>
+// PXY: gather (x, y) as one 16-byte load from the packed copy A.XY
+template <int U, int NT, int MODE, bool RED, bool PXY = false>
 __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
     static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
     const int *__restrict__ counters = A.counters;
@@ -620,7 +623,13 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
                 int tj[U];
 #pragma unroll
                 for (int q = 0; q < U; q++) {
-                    xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]];
+                    if (PXY) {
+                        const double2 p = A.XY[jp[q]];
+                        xj[q] = p.x; yj[q] = p.y;
+                    } else {
+                        xj[q] = X[jp[q]]; yj[q] = Y[jp[q]];
+                    }
+                    zj[q] = Z[jp[q]];
                     if (NT != 1) tj[q] = type[jp[q]];
                 }
 #pragma unroll
@@ -686,9 +695,16 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
 
 
 // one segment per workgroup of one system
-template <int U, int NT, int MODE, bool RED>
+template <int U, int NT, int MODE, bool RED, bool PXY = true>
 __global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
-    lj_full_segment<U, NT, MODE, RED>(A, xcd_block(blockIdx.x, gridDim.x));
+    lj_full_segment<U, NT, MODE, RED, PXY>(A, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// the packed (x, y) copy the single-system force kernel gathers from
+__global__ __launch_bounds__(256) void k_pack_xy(const int *__restrict__ counters, const double *__restrict__ X,
+                                                 const double *__restrict__ Y, double2 *__restrict__ XY) {
+    const int n = counters[C_NLOCAL] + counters[C_NGHOST];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) XY[i] = make_double2(X[i], Y[i]);
 }
 
 // the segments of many independent systems in one launch: block b belongs to
@@ -1234,8 +1250,8 @@ int build_zt(const Geom &g) { return g.nchunks == 1 ? g.dim_cells[2] + 2 : g.zc 
 static int build_lrows(const DevState &d) { return min((d.ncap + 3) / 4 * 4, LR_MAX); }
 
 static size_t build_lds(const DevState &d, const Geom &g, int half, int tcap) {
-    return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) +
-           (size_t) build_lrows(d) * LBS * sizeof(uint16_t) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
+    return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) + (size_t) g.hcap * sizeof(int) +
+           (size_t) build_lrows(d) * (g.hcap + 2) * sizeof(uint16_t) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
            (size_t) 9 * build_zt(g) * 4 * sizeof(int);
 }
 
@@ -1347,6 +1363,7 @@ LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, i
     A.g = g;
     A.lstart = d.lstart; A.gstart = d.gstart;
     A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];
+    A.XY = d.XY;
     A.Xn = d.X[next]; A.Yn = d.Y[next]; A.Zn = d.Z[next];
     A.type = d.type; A.nl = d.nl; A.nl_stride = d.nl_stride; A.numneighs = d.numneighs; A.flags = d.flags;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz; A.vx = d.vx; A.vy = d.vy; A.vz = d.vz; A.mass = d.mass;
@@ -1389,9 +1406,16 @@ static void lj_full_nt(hipStream_t s, const DevState &d, const Geom &g, const LJ
     }
 }
 
+static void launch_pack_xy(hipStream_t s, const DevState &d, int cur) {
+    hipLaunchKernelGGL(k_pack_xy, dim3(std::max(1, std::min((d.cap + 255) / 256, 1024))), dim3(256), 0, s,
+                       d.counters, d.X[cur], d.Y[cur], d.XY);
+}
+
 void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
-                    int do_vv2, int do_vv1, int reduce) {
+                    int do_vv2, int do_vv1, int reduce, hipEvent_t after_pack) {
     if (d.nlocal <= 0) return;
+    launch_pack_xy(s, d, cur);
+    if (after_pack) hipEventRecord(after_pack, s);
     if (prm.ntypes == 1) lj_full_nt<1>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
     else lj_full_nt<0>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
 }
@@ -1427,6 +1451,7 @@ void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJPara
         launch_lj_half(s, d, g, prm, d.cur, 1);
         return;
     }
+    launch_pack_xy(s, d, d.cur);
     if (prm.ntypes == 1) lj_full_launch<1, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
     else lj_full_launch<0, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
 }

@@ -60,6 +60,7 @@ struct Geom {
     int zc;        // cells per segment along z
     int nchunks;   // segments per z-column
     int nsegs;     // dim_cells[0] * dim_cells[1] * nchunks
+    int hcap;      // home particles of a segment whose lists the build buffers in LDS
 };
 
 // Neighbour lists: entry k of physical local i.  Rows come in groups of four
@@ -96,6 +97,7 @@ struct DevState {
     int pstride;  // row stride of the partials array
     int max_lds;  // shared memory a workgroup may allocate
     double *X[2], *Y[2], *Z[2];
+    double2 *XY;         // (x, y) of the force kernel's current buffer, packed for one 16-byte gather
     double *vx, *vy, *vz, *fx, *fy, *fz, *mass;
     int *type, *flags, *uid, *shape, *ref;
     int *phys;    // physical slot of each reference index
@@ -139,8 +141,10 @@ void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, in
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid);
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel);
 // mode: 0 force only, 1 force + integrators (fused), 2 energy only (no writes)
+// (the (x, y) pack of buffer cur runs first; after_pack, if given, is
+// recorded between the pack and the force kernel)
 void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
-                    int do_vv2, int do_vv1, int reduce);
+                    int do_vv2, int do_vv1, int reduce, hipEvent_t after_pack = nullptr);
 void launch_lj_half(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce);
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom);
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm);
@@ -156,6 +160,7 @@ struct LJArgs {
     Geom g;
     const int *lstart, *gstart;
     const double *X, *Y, *Z;
+    const double2 *XY;   // packed (x, y) of X, Y (single-system launches)
     double *Xn, *Yn, *Zn;
     const int *type;
     const uint16_t *nl;

@@ -81,7 +81,7 @@ __device__ __forceinline__ int run_of(const int *r_off, int p) {
     return r;
 }
 
-constexpr int FB = 128;  // threads per force block (two waves)
+constexpr int FB = 64;   // threads per force workgroup (one wave)
 
 __device__ __forceinline__ void stage_slots(const int *r_start, const int *r_off, int T, int *tslot) {
     for (int p = threadIdx.x; p < T; p += blockDim.x) {

@@ -48,6 +48,8 @@ static size_t pstride_of(const DevState &d, const Geom &g) {
 
 static void free_particle_arrays(DevState &d) {
     for (int b = 0; b < 2; b++) { dfree(d.X[b]); dfree(d.Y[b]); dfree(d.Z[b]); d.X[b] = d.Y[b] = d.Z[b] = nullptr; }
+    dfree(d.XY);
+    d.XY = nullptr;
     double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz,
                      &d.alt_mass, &d.partials};
     for (auto p : dd) { dfree(*p); *p = nullptr; }
@@ -60,6 +62,7 @@ static void free_particle_arrays(DevState &d) {
 static int alloc_particle_arrays(pairs_hip_ctx *ctx, DevState &d, int cap) {
     int r = 0;
     for (int b = 0; b < 2; b++) { r |= dalloc(ctx, &d.X[b], cap); r |= dalloc(ctx, &d.Y[b], cap); r |= dalloc(ctx, &d.Z[b], cap); }
+    r |= dalloc(ctx, &d.XY, cap);
     double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz, &d.alt_mass};
     for (auto p : dd) r |= dalloc(ctx, p, cap);
     int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
@@ -131,6 +134,15 @@ static void setup_segments(pairs_hip_ctx *ctx, int zc) {
     g.zc = zc;
     g.nchunks = std::max(1, (g.dim_cells[2] - 1 + zc - 1) / zc);
     g.nsegs = g.dim_cells[0] * g.dim_cells[1] * g.nchunks;
+    // LDS list columns for twice the mean home count (denser segments write
+    // their extra particles' lists to memory directly), a multiple of 4
+    {
+        const double vol = g.len[0] * g.len[1] * g.len[2];
+        const double s3 = g.spacing * g.spacing * g.spacing;
+        const double per_cell = ctx->d.nlocal > 0 && vol > 0 ? ctx->d.nlocal * s3 / vol : 16.0;
+        const int want = (int) std::ceil(2.0 * per_cell * zc) + 8;
+        g.hcap = std::max(32, std::min(192, (want + 3) / 4 * 4));
+    }
 }
 
 // build_cell_lists_stencil (src/pairs/sim/cell_lists.py:46-87)
@@ -862,7 +874,7 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
 static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, int vv1, int reduce, bool timed) {
     DevState &d = ctx->d;
     hipStream_t s = ctx->stream;
-    if (timed) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
+    if (timed && (ctx->mlp_on || ctx->cfg.half)) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
     if (ctx->mlp_on) {
         launch_mlp_forces(s, d, ctx->geom, ctx->mlp, cur, reduce, ctx->ghost_grid);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
@@ -872,7 +884,9 @@ static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, in
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
         launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
     } else {
-        launch_lj_full(s, d, ctx->geom, ctx->prm, cur, next, 1, vv2, vv1, reduce);
+        // the timed region is the force kernel itself (after its (x, y) pack)
+        launch_lj_full(s, d, ctx->geom, ctx->prm, cur, next, 1, vv2, vv1, reduce,
+                       timed ? ctx->ev_begin[ctx->ev_used] : nullptr);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
     }
 }
