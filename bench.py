@@ -37,12 +37,13 @@ def force_kernel_bytes(n, g, s, fused):
 
     LJ only (the neighbour-list force): 24(N+G) positions + 4N counts + 4S list
     indices + 24N force write.  The fused kernel of the timed loop also does
-    final_integrate + initial_integrate of its own atoms: + 8N mass + 48N
-    velocity read/write + 24N position write (its forces stay in registers).
+    final_integrate + initial_integrate of its own atoms and writes the next
+    positions of its atoms and of their ghost images: + 8N mass + 48N velocity
+    read/write + 24(N+G) position write + 16(N+G) packed (x, y) write.
     """
     b = 24 * (n + g) + 4 * n + 4 * s + 24 * n
     if fused:
-        b += 8 * n + 48 * n + 24 * n
+        b += 8 * n + 48 * n + 24 * (n + g) + 16 * (n + g)
     return b
 
 

@@ -46,6 +46,8 @@ struct pairs_hip_ctx {
     size_t expand_cap = 0;
     // MLP force field (include/pairs_hip_mlp.h): when on, every force
     // evaluation of the run uses the attached model instead of LJ
+    // XY[b] (packed x, y) matches X[b], Y[b] for every particle
+    bool xy_valid[2] = {false, false};
     bool mlp_on = false;
     MLPBind mlp{};
     float *mlp_G = nullptr, *mlp_D = nullptr, *mlp_E = nullptr;  // per-slot work arrays

@@ -102,21 +102,39 @@ __global__ __launch_bounds__(1024) void k_scan_cells(const int *__restrict__ cou
                                                      int *__restrict__ gstart, int ncells) {
     __shared__ int lds[20];
     if (counters[C_OVF]) return;
-    for (int which = 0; which < 2; which++) {
-        const int *cnt = which ? gcount : lcount;
-        int *st = which ? gstart : lstart;
-        int carry = 0;
-        for (int base = 0; base < ncells; base += 1024) {
-            const int c = base + threadIdx.x;
-            const int v = c < ncells ? cnt[c] : 0;
-            int total;
-            const int ex = block_exclusive_scan<1024>(v, lds, &total);
-            if (c < ncells) st[c] = carry + ex;
-            carry += total;
+    // each thread owns a contiguous run of cells (up to PT per round, all
+    // loads issued together); one block scan of the run totals per round
+    constexpr int PT = 16;
+    int carry_l = 0, carry_g = 0;
+    for (int r0 = 0; r0 < ncells; r0 += 1024 * PT) {
+        const int rn = min(ncells - r0, 1024 * PT);
+        const int per = (rn + 1023) / 1024;
+        const int c0 = r0 + min((int) threadIdx.x * per, rn);
+        const int c1 = r0 + min((int) threadIdx.x * per + per, rn);
+        int vl[PT], vg[PT];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int c = c0 + k;
+            vl[k] = c < c1 ? lcount[c] : 0;
+            vg[k] = c < c1 ? gcount[c] : 0;
         }
-        if (threadIdx.x == 0) st[ncells] = carry;
-        __syncthreads();
+        int sl = 0, sg = 0;
+#pragma unroll
+        for (int k = 0; k < PT; k++) { sl += vl[k]; sg += vg[k]; }
+        int tl, tg;
+        int al = carry_l + block_exclusive_scan<1024>(sl, lds, &tl);
+        int ag = carry_g + block_exclusive_scan<1024>(sg, lds, &tg);
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int c = c0 + k;
+            if (c < c1) { lstart[c] = al; gstart[c] = ag; }
+            al += vl[k];
+            ag += vg[k];
+        }
+        carry_l += tl;
+        carry_g += tg;
     }
+    if (threadIdx.x == 0) { lstart[ncells] = carry_l; gstart[ncells] = carry_g; }
 }
 
 __global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ counters, const int *__restrict__ cell,
@@ -265,16 +283,20 @@ __global__ __launch_bounds__(256) void k_img_build(int *__restrict__ counters, c
 __device__ __forceinline__ void write_images(const Images &im, int i, int nlocal, double px, double py, double pz,
                                              const double *__restrict__ X, const double *__restrict__ Y,
                                              const double *__restrict__ Z, double *__restrict__ Xn,
-                                             double *__restrict__ Yn, double *__restrict__ Zn, const Geom &g) {
+                                             double *__restrict__ Yn, double *__restrict__ Zn, const Geom &g,
+                                             double2 *__restrict__ XYn = nullptr) {
     const int n = min(im.cnt[i], IMG_MAX);
     for (int k = 0; k < n; k++) {
         const int q = im.slot[(size_t) k * im.cap + i];
         const int src = im.src[q];
         const bool own = src == i;
         const double bx = own ? px : X[src], by = own ? py : Y[src], bz = own ? pz : Z[src];
-        Xn[nlocal + q] = bx + (double) im.mult[3 * q + 0] * g.len[0];
-        Yn[nlocal + q] = by + (double) im.mult[3 * q + 1] * g.len[1];
+        const double x = bx + (double) im.mult[3 * q + 0] * g.len[0];
+        const double y = by + (double) im.mult[3 * q + 1] * g.len[1];
+        Xn[nlocal + q] = x;
+        Yn[nlocal + q] = y;
         Zn[nlocal + q] = bz + (double) im.mult[3 * q + 2] * g.len[2];
+        if (XYn) XYn[nlocal + q] = make_double2(x, y);
     }
 }
 
@@ -672,7 +694,8 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
                     }
                     if (do_vv1) {
                         Xn[i] = px; Yn[i] = py; Zn[i] = pz;
-                        if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, g);
+                        if (A.XYn) A.XYn[i] = make_double2(px, py);
+                        if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, g, A.XYn);
                     }
                 }
             }
@@ -1146,7 +1169,8 @@ __global__ __launch_bounds__(256) void k_sync_pp(double *__restrict__ Xc, double
                                                  double *__restrict__ Zc, const double *__restrict__ Xp,
                                                  const double *__restrict__ Yp, const double *__restrict__ Zp,
                                                  const int *__restrict__ counters, const int *__restrict__ sync_src,
-                                                 const int *__restrict__ sync_mult, Geom g) {
+                                                 const int *__restrict__ sync_mult, Geom g,
+                                                 double2 *__restrict__ XYc) {
     const int n = counters[C_NSEND_ALL];
     const int nlocal = counters[C_NLOCAL];
     for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
@@ -1155,9 +1179,12 @@ __global__ __launch_bounds__(256) void k_sync_pp(double *__restrict__ Xc, double
         const double x = loc ? Xc[src] : Xp[src];
         const double y = loc ? Yc[src] : Yp[src];
         const double z = loc ? Zc[src] : Zp[src];
-        Xc[nlocal + q] = x + (double) sync_mult[3 * q + 0] * g.len[0];
-        Yc[nlocal + q] = y + (double) sync_mult[3 * q + 1] * g.len[1];
+        const double xg = x + (double) sync_mult[3 * q + 0] * g.len[0];
+        const double yg = y + (double) sync_mult[3 * q + 1] * g.len[1];
+        Xc[nlocal + q] = xg;
+        Yc[nlocal + q] = yg;
         Zc[nlocal + q] = z + (double) sync_mult[3 * q + 2] * g.len[2];
+        XYc[nlocal + q] = make_double2(xg, yg);
     }
 }
 
@@ -1298,9 +1325,9 @@ void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int 
     const int c = d.cur;
     for (int dim = 0; dim < 3; dim++) {
         if (!periodic[dim]) continue;
-        // nsend_all = 0; nghost = 0 (comm.py:105-118)
-        launch_set_counters(s, d.counters, C_NSEND_ALL, 1, 0);
-        launch_set_counters(s, d.counters, C_NGHOST, 1, 0);
+        // nsend_all = 0; nghost = 0 (comm.py:105-118; adjacent counters)
+        static_assert(C_NSEND_ALL == C_NGHOST + 1, "counter layout");
+        launch_set_counters(s, d.counters, C_NGHOST, 2, 0);
         launch_select_dim(s, d, g, dim, 1);
         int *sc = d.scratch_i;
         hipLaunchKernelGGL(k_exchange_relabel, dim3(1), dim3(1024), 0, s, d.X[c], d.Y[c], d.Z[c], d.ref, d.phys,
@@ -1310,9 +1337,9 @@ void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int 
 }
 
 void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *periodic) {
-    launch_set_counters(s, d.counters, C_NSEND_ALL, 1, 0);
-    launch_set_counters(s, d.counters, C_NGHOST, 1, 0);
-    launch_set_counters(s, d.counters, C_NSEND, 6, 0);
+    // nghost, nsend_all and the six per-side send counts (adjacent counters)
+    static_assert(C_NSEND_ALL == C_NGHOST + 1 && C_NSEND == C_NSEND_ALL + 1, "counter layout");
+    launch_set_counters(s, d.counters, C_NGHOST, 8, 0);
     for (int dim = 0; dim < 3; dim++) {
         if (!periodic[dim]) continue;
         launch_select_dim(s, d, g, dim, 0);
@@ -1321,7 +1348,7 @@ void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *
 
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid) {
     hipLaunchKernelGGL(k_sync_pp, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[prev], d.Y[prev],
-                       d.Z[prev], d.counters, d.sync_src, d.sync_mult, g);
+                       d.Z[prev], d.counters, d.sync_src, d.sync_mult, g, d.XY[cur]);
 }
 
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel) {
@@ -1357,13 +1384,14 @@ static Images images_of(const DevState &d, bool writes_next) {
 }
 
 LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
-               int fused, double *eatom) {
+               int fused, double *eatom, bool packed) {
     LJArgs A;
     A.counters = d.counters;
     A.g = g;
     A.lstart = d.lstart; A.gstart = d.gstart;
     A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];
-    A.XY = d.XY;
+    A.XY = d.XY[cur];
+    A.XYn = (packed && fused && do_vv1 && next != cur) ? d.XY[next] : nullptr;
     A.Xn = d.X[next]; A.Yn = d.Y[next]; A.Zn = d.Z[next];
     A.type = d.type; A.nl = d.nl; A.nl_stride = d.nl_stride; A.numneighs = d.numneighs; A.flags = d.flags;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz; A.vx = d.vx; A.vy = d.vy; A.vz = d.vz; A.mass = d.mass;
@@ -1408,13 +1436,13 @@ static void lj_full_nt(hipStream_t s, const DevState &d, const Geom &g, const LJ
 
 static void launch_pack_xy(hipStream_t s, const DevState &d, int cur) {
     hipLaunchKernelGGL(k_pack_xy, dim3(std::max(1, std::min((d.cap + 255) / 256, 1024))), dim3(256), 0, s,
-                       d.counters, d.X[cur], d.Y[cur], d.XY);
+                       d.counters, d.X[cur], d.Y[cur], d.XY[cur]);
 }
 
 void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
-                    int do_vv2, int do_vv1, int reduce, hipEvent_t after_pack) {
+                    int do_vv2, int do_vv1, int reduce, int pack, hipEvent_t after_pack) {
     if (d.nlocal <= 0) return;
-    launch_pack_xy(s, d, cur);
+    if (pack) launch_pack_xy(s, d, cur);
     if (after_pack) hipEventRecord(after_pack, s);
     if (prm.ntypes == 1) lj_full_nt<1>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
     else lj_full_nt<0>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);

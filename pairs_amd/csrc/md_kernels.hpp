@@ -97,7 +97,7 @@ struct DevState {
     int pstride;  // row stride of the partials array
     int max_lds;  // shared memory a workgroup may allocate
     double *X[2], *Y[2], *Z[2];
-    double2 *XY;         // (x, y) of the force kernel's current buffer, packed for one 16-byte gather
+    double2 *XY[2];      // (x, y) of each position buffer, packed for one 16-byte gather
     double *vx, *vy, *vz, *fx, *fy, *fz, *mass;
     int *type, *flags, *uid, *shape, *ref;
     int *phys;    // physical slot of each reference index
@@ -141,10 +141,12 @@ void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, in
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid);
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel);
 // mode: 0 force only, 1 force + integrators (fused), 2 energy only (no writes)
-// (the (x, y) pack of buffer cur runs first; after_pack, if given, is
-// recorded between the pack and the force kernel)
+// The full-list force kernel gathers (x, y) from XY[cur]: with pack, a pass
+// first packs it from X, Y; after_pack, if given, is recorded between that
+// pass and the force kernel.  With fuse and do_vv1 the kernel also writes
+// XY[next] for every local and (images on) every ghost.
 void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
-                    int do_vv2, int do_vv1, int reduce, hipEvent_t after_pack = nullptr);
+                    int do_vv2, int do_vv1, int reduce, int pack = 1, hipEvent_t after_pack = nullptr);
 void launch_lj_half(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce);
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom);
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm);
@@ -161,6 +163,7 @@ struct LJArgs {
     const int *lstart, *gstart;
     const double *X, *Y, *Z;
     const double2 *XY;   // packed (x, y) of X, Y (single-system launches)
+    double2 *XYn;        // packed (x, y) of the next positions, or null
     double *Xn, *Yn, *Zn;
     const int *type;
     const uint16_t *nl;
@@ -176,7 +179,7 @@ struct LJArgs {
 };
 // the fused force + integrate arguments of one system (cur -> next buffer)
 LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
-               int fused, double *eatom);
+               int fused, double *eatom, bool packed = true);
 // fused force + integrate for many systems: block b is segment b - seg_base[rep[b]] of system rep[b]
 void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,
                           int max_tile_cap, int ntypes1, int reduce);

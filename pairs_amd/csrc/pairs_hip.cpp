@@ -48,8 +48,7 @@ static size_t pstride_of(const DevState &d, const Geom &g) {
 
 static void free_particle_arrays(DevState &d) {
     for (int b = 0; b < 2; b++) { dfree(d.X[b]); dfree(d.Y[b]); dfree(d.Z[b]); d.X[b] = d.Y[b] = d.Z[b] = nullptr; }
-    dfree(d.XY);
-    d.XY = nullptr;
+    for (int b = 0; b < 2; b++) { dfree(d.XY[b]); d.XY[b] = nullptr; }
     double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz,
                      &d.alt_mass, &d.partials};
     for (auto p : dd) { dfree(*p); *p = nullptr; }
@@ -62,7 +61,8 @@ static void free_particle_arrays(DevState &d) {
 static int alloc_particle_arrays(pairs_hip_ctx *ctx, DevState &d, int cap) {
     int r = 0;
     for (int b = 0; b < 2; b++) { r |= dalloc(ctx, &d.X[b], cap); r |= dalloc(ctx, &d.Y[b], cap); r |= dalloc(ctx, &d.Z[b], cap); }
-    r |= dalloc(ctx, &d.XY, cap);
+    r |= dalloc(ctx, &d.XY[0], cap);
+    r |= dalloc(ctx, &d.XY[1], cap);
     double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz, &d.alt_mass};
     for (auto p : dd) r |= dalloc(ctx, p, cap);
     int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
@@ -367,6 +367,7 @@ int pairs_hip_reserve(pairs_hip_ctx *ctx, int particle_capacity, int neighbor_ca
 
 int pairs_hip_upload(pairs_hip_ctx *ctx, int n, const double *x, const double *v, const double *m, const int *type,
                      const int *flags, const int *uid) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (!ctx || n < 0 || !x || !v) return PAIRS_HIP_ERR_ARG;
     DevState &d = ctx->d;
     if (n > d.cap) {
@@ -430,6 +431,7 @@ static double park_miller(int *seed) {
 
 // pairs::copper_fcc_lattice (runtime/copper_fcc_lattice.hpp:64-145), on the host
 int pairs_hip_copper_fcc_lattice(pairs_hip_ctx *ctx, int nx, int ny, int nz, double rho, int ntypes) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (!ctx || nx <= 0 || ny <= 0 || nz <= 0 || rho <= 0.0 || ntypes < 1) return PAIRS_HIP_ERR_ARG;
     const double *lo = ctx->cfg.box_lo, *hi = ctx->cfg.box_hi;
     const double xprd = hi[0] - lo[0], yprd = hi[1] - lo[1], zprd = hi[2] - lo[2];
@@ -655,6 +657,7 @@ static int finish_module(pairs_hip_ctx *ctx) {
 }
 
 int pairs_hip_exchange(pairs_hip_ctx *ctx) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     drop_graphs(ctx);
     launch_exchange(ctx->stream, ctx->d, ctx->geom, ctx->cfg.periodic);
@@ -662,6 +665,7 @@ int pairs_hip_exchange(pairs_hip_ctx *ctx) {
 }
 
 int pairs_hip_borders(pairs_hip_ctx *ctx) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     drop_graphs(ctx);
     launch_borders(ctx->stream, ctx->d, ctx->geom, ctx->cfg.periodic);
@@ -669,6 +673,7 @@ int pairs_hip_borders(pairs_hip_ctx *ctx) {
 }
 
 int pairs_hip_sync_ghosts(pairs_hip_ctx *ctx) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     launch_sync_inplace(ctx->stream, ctx->d, ctx->geom, ctx->ghost_grid, 0);
     CHECK_LAUNCH();
@@ -676,6 +681,7 @@ int pairs_hip_sync_ghosts(pairs_hip_ctx *ctx) {
 }
 
 int pairs_hip_bin_cells(pairs_hip_ctx *ctx) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     drop_graphs(ctx);
     launch_bin_sort(ctx->stream, ctx->d, ctx->geom);
@@ -731,12 +737,16 @@ int pairs_hip_lj(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     DevState &d = ctx->d;
     if (ctx->cfg.half) launch_lj_half(ctx->stream, d, ctx->geom, ctx->prm, d.cur, 0);
-    else launch_lj_full(ctx->stream, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    else {
+        launch_lj_full(ctx->stream, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0, !ctx->xy_valid[d.cur]);
+        ctx->xy_valid[d.cur] = true;
+    }
     CHECK_LAUNCH();
     return 0;
 }
 
 int pairs_hip_vv_initial(pairs_hip_ctx *ctx) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     launch_vv_initial(ctx->stream, ctx->d, ctx->prm);
     CHECK_LAUNCH();
@@ -818,6 +828,7 @@ static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 |
 // exchange + borders + cells + neighbour lists, retried with grown capacities
 // on overflow like the generated resize loops (transformations/modules.py:159-205)
 static int rebuild(pairs_hip_ctx *ctx) {
+    if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     for (int attempt = 0; attempt < 8; attempt++) {
         hipStream_t s = ctx->stream;
         launch_set_counters(s, ctx->d.counters, C_IMG_OVF, 1, 0);
@@ -851,6 +862,7 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
     DevState &d = ctx->d;
     hipStream_t s = ctx->stream;
     if (ts > 0) launch_vv_initial(s, d, ctx->prm);
+    ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     if (every(ts, ctx->cfg.reneighbor_every)) {
         drop_graphs(ctx);
         int r = rebuild(ctx);
@@ -860,7 +872,10 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
     }
     if (ctx->mlp_on) launch_mlp_forces(s, d, ctx->geom, ctx->mlp, d.cur, 0, ctx->ghost_grid);
     else if (ctx->cfg.half) launch_lj_half(s, d, ctx->geom, ctx->prm, d.cur, 0);
-    else launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    else {
+        launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0, 1);
+        ctx->xy_valid[d.cur] = true;
+    }
     if (ts > 0) launch_vv_post(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 1, 0, 0);
     CHECK_LAUNCH();
     return 0;
@@ -884,11 +899,16 @@ static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, in
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
         launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
     } else {
-        // the timed region is the force kernel itself (after its (x, y) pack)
-        launch_lj_full(s, d, ctx->geom, ctx->prm, cur, next, 1, vv2, vv1, reduce,
+        // the timed region is the force kernel itself (after a pack of
+        // (x, y) when the previous kernel did not leave one)
+        launch_lj_full(s, d, ctx->geom, ctx->prm, cur, next, 1, vv2, vv1, reduce, !ctx->xy_valid[cur],
                        timed ? ctx->ev_begin[ctx->ev_used] : nullptr);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
+        ctx->xy_valid[cur] = true;
+        if (vv1 && next != cur) ctx->xy_valid[next] = true;  // written by the fused kernel
+        return;
     }
+    if (vv1 && next != cur) ctx->xy_valid[next] = false;  // moved without the packed copy
 }
 
 static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int reduce, bool timed) {
@@ -969,13 +989,17 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
     for (int ts = ts_begin; ts < ts_end; ts++) {
         const bool R = every(ts, ctx->cfg.reneighbor_every);
         const bool T = every(ts, ctx->cfg.thermo_every);
-        if (ts == ts_begin && ts > 0) launch_vv_initial(s, d, ctx->prm);
+        if (ts == ts_begin && ts > 0) {
+            launch_vv_initial(s, d, ctx->prm);
+            ctx->xy_valid[d.cur] = false;
+        }
         const int vv2 = ts > 0, vv1 = (ts + 1 < ts_end) && (ts + 1 <= nsteps);
         const bool timed = ctx->time_force && !R && ((ts % ctx->time_every) == 0) && ctx->ev_used < 4096;
         if (!R && !T && vv2 && vv1 && prev_in_other && ctx->use_graphs && !timed) {
             int r = graph_step(ctx, d.cur);
             if (r) return r;
             d.cur = 1 - d.cur;
+            ctx->xy_valid[d.cur] = !ctx->mlp_on && !ctx->cfg.half;  // the replayed LJ kernel wrote it
             continue;
         }
         if (R) {
@@ -984,7 +1008,10 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             prev_in_other = false;
         }
         const int cur = d.cur, other = 1 - cur;
-        if (!R && !prev_in_other) launch_sync_inplace(s, d, ctx->geom, ctx->ghost_grid, 0);
+        if (!R && !prev_in_other) {
+            launch_sync_inplace(s, d, ctx->geom, ctx->ghost_grid, 0);
+            ctx->xy_valid[d.cur] = false;
+        }
         if (!R && prev_in_other) {
             launch_fused_step(ctx, cur, vv2, vv1, T ? 1 : 0, timed);
         } else {
@@ -1100,7 +1127,8 @@ int pairs_hip_mlp_force_error(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double *rel_
     HIP_OK(hipMemcpyAsync(ctx->mlp_f, d.fx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIP_OK(hipMemcpyAsync(ctx->mlp_f + cap, d.fy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIP_OK(hipMemcpyAsync(ctx->mlp_f + 2 * cap, d.fz, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0);
+    launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0, !ctx->xy_valid[d.cur]);
+    ctx->xy_valid[d.cur] = true;
     const int nb = std::min((n + 255) / 256, 1024);
     launch_force_diff(s, ctx->mlp_f, ctx->mlp_f + cap, ctx->mlp_f + 2 * cap, d.fx, d.fy, d.fz, n, ctx->mlp_red, nb);
     CHECK_LAUNCH();
@@ -1223,7 +1251,7 @@ static int batch_descs(pairs_hip_batch *b) {
         for (int r = 0; r < R; r++) {
             pairs_hip_ctx *c = b->rep[r];
             const int cur = c->d.cur ^ p;
-            b->h_desc[p * R + r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, 1, 1, 1, nullptr);
+            b->h_desc[p * R + r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, 1, 1, 1, nullptr, false);
         }
     for (int r = 0; r < R; r++) b->base_cur[r] = b->rep[r]->d.cur;
     for (int p = 0; p < 2; p++)
@@ -1447,7 +1475,7 @@ int pairs_hip_batch_run(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps
             for (int r = 0; r < R; r++) {
                 pairs_hip_ctx *c = b->rep[r];
                 const int cur = b->base_cur[r] ^ phase;
-                b->h_desc[r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, vv2, vv1, 1, nullptr);
+                b->h_desc[r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, vv2, vv1, 1, nullptr, false);
             }
             BATCH_OK(hipMemcpyAsync(b->d_desc[2], b->h_desc, R * sizeof(LJArgs), hipMemcpyHostToDevice, s));
             BATCH_OK(hipStreamSynchronize(s));  // h_desc is reused by the next descriptor build
