@@ -78,7 +78,7 @@ static void free_all(pairs_hip_dem_ctx *ctx) {
                     p.nx, p.ny, p.nz, p.inv_i, p.rot_m, p.rot_q, p.uid, p.flags, p.type, p.shape, p.num_contacts,
                     p.c_uid, p.c_used, p.c_stick, p.c_tsd, p.c_ivm, p.cell_count, p.cell_start, p.cell_fill,
                     p.cell_particles, p.particle_cell, p.nshapes, p.send_map, p.send_mult, p.exchg_flag, p.copy_to,
-                    p.blk, p.scan_tmp, p.buf, p.hbuf, ctx->counters};
+                    p.blk, p.scan_tmp, p.buf, p.hbuf, p.cand_n, p.cand, ctx->counters};
     for (void *q : ptrs)
         if (q) hipFree(q);
     if (ctx->h_counters) hipHostFree(ctx->h_counters);
@@ -171,6 +171,8 @@ int pairs_hip_dem_create(pairs_hip_dem_ctx **out, const pairs_hip_dem_config *cf
     r |= dalloc(ctx, &p.blk, 2 * (C / 256 + 2));
     r |= dalloc(ctx, &p.buf, 40 * S);
     r |= dalloc(ctx, &p.hbuf, (2 + 5 * K) * S);
+    r |= dalloc(ctx, &p.cand_n, C);
+    r |= dalloc(ctx, &p.cand, DCAND * C);
     r |= dalloc(ctx, &ctx->counters, D_COUNT);
     if (r) return fail(-1);
     if (hipHostMalloc((void **) &ctx->h_counters, D_COUNT * sizeof(int)) != hipSuccess) return fail(-1);

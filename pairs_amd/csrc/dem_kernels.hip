@@ -523,11 +523,94 @@ __device__ __forceinline__ int contact_slot(const DemPtrs &p, int *c, int i, int
     return slot;
 }
 
-// One thread per local particle: reset_contact_history_usage_status,
-// reset_volatile_properties, gravity, linear_spring_dashpot (spheres of cell 0
-// and the 27-cell stencil, then the half-spaces; force += sphere + plane sums,
+// contact test of i (position xi, radius ri) against candidate j of the given
+// pass (0: sphere, 1: half-space), as linear_spring_dashpot's guard
+// (examples/dem.py:18-30 with the sphere/half-space geometry of
+// src/pairs/sim/interaction.py); on contact fills the normal, the contact
+// point and the penetration depth
+__device__ __forceinline__ bool contact_geom(const DemPtrs &p, const double xi[3], double ri, int j, int pass,
+                                             double nv[3], double cpt[3], double &delta) {
+    const double xj[3] = {p.x[j], p.y[j], p.z[j]};
+    if (pass == 0) {
+        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+        const double rsq = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+        const double rj = p.r[j];
+        const double rs = ri + rj;
+        if (!(rsq < rs * (rs + 0.0))) return false;
+        const double dist = sqrt(rsq);
+        const double pen = (dist - ri) - rj;
+        delta = -(pen);
+        if (delta < 0.0) return false;
+        const double inv = 1.0 / dist;
+        for (int e = 0; e < 3; e++) nv[e] = d[e] * inv;
+        const double reach = rj + 0.5 * pen;
+        for (int e = 0; e < 3; e++) cpt[e] = xj[e] + nv[e] * reach;
+    } else {
+        const double nj[3] = {p.nx[j], p.ny[j], p.nz[j]};
+        const double pen = (((nj[0] * xi[0] + nj[1] * xi[1]) + nj[2] * xi[2]) - ri) -
+                           ((nj[0] * xj[0] + nj[1] * xj[1]) + nj[2] * xj[2]);
+        if (!(pen < 0.0)) return false;
+        delta = -(pen);
+        if (delta < 0.0) return false;
+        const double reach = ri + pen;
+        for (int e = 0; e < 3; e++) { nv[e] = nj[e]; cpt[e] = xi[e] - reach * nj[e]; }
+    }
+    return true;
+}
+
+// Candidates of i in the reference's visiting order: the spheres of cell 0
+// and of the 27 stencil cells, then the half-spaces of the same cells
+// (cell_lists.py partitions each cell spheres-first).  fn(j, pass) is called
+// for each candidate j != i and returns false to stop the walk.
+template <typename F>
+__device__ __forceinline__ void walk_candidates(const DemPtrs &p, const DemGeom &g, int i, int ci, F &&fn) {
+    for (int pass = 0; pass < 2; pass++) {
+        for (int s = -1; s < 27; s++) {
+            const int cell = s < 0 ? 0 : ci + g.stencil[s];
+            if (!(s < 0 || (cell > 0 && cell < g.ncells))) continue;
+            const int st = p.cell_start[cell], nsph = p.nshapes[2 * cell];
+            const int q0 = pass == 0 ? 0 : nsph;
+            const int q1 = pass == 0 ? nsph : nsph + p.nshapes[2 * cell + 1];
+            for (int q = q0; q < q1; q++) {
+                const int j = p.cell_particles[st + q];
+                if (j == i) continue;
+                if (!fn(j, pass)) return;
+            }
+        }
+    }
+}
+
+// Pass 1 of the contact step, one thread per local particle: the partners
+// that touch i, in visiting order, as cand[t][i] (half-spaces stored as ~j);
+// cand_n[i] = DCAND + 1 when more than DCAND touch (the force pass then walks
+// the cells itself).  Only geometry is read: no history is touched.
+__global__ __launch_bounds__(256) void k_contact_find(DemPtrs p, const int *__restrict__ c, DemGeom g) {
+    const int nl = c[D_NLOCAL];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nl || c[D_OVF]) return;
+    int n = 0;
+    if (!(p.flags[i] & DFLAG_FIXED)) {
+        const double ri = p.r[i];
+        const double xi[3] = {p.x[i], p.y[i], p.z[i]};
+        walk_candidates(p, g, i, p.particle_cell[i], [&](int j, int pass) {
+            double nv[3], cpt[3], delta;
+            if (!contact_geom(p, xi, ri, j, pass, nv, cpt, delta)) return true;
+            if (n == DCAND) { n = DCAND + 1; return false; }
+            p.cand[(size_t) n * p.cap + i] = pass == 0 ? j : ~j;
+            n++;
+            return true;
+        });
+    }
+    p.cand_n[i] = n;
+}
+
+// Pass 2, one thread per local particle: reset_contact_history_usage_status,
+// reset_volatile_properties, gravity, linear_spring_dashpot over the touching
+// partners of pass 1 (spheres, then half-spaces; force += sphere + plane sums,
 // torque += plane + sphere sums) and clear_unused_contact_history -- every
-// one of them touches only the thread's own particle and history rows.
+// one of them touches only the thread's own particle and history rows.  A
+// wave runs the spring-dashpot as often as its busiest lane has contacts,
+// not once per candidate.
 __global__ __launch_bounds__(128) void k_contacts(DemPtrs p, int *__restrict__ c, DemGeom g, DemParams prm) {
     const int nl = c[D_NLOCAL];
     const int i = blockIdx.x * 128 + threadIdx.x;
@@ -541,49 +624,22 @@ __global__ __launch_bounds__(128) void k_contacts(DemPtrs p, int *__restrict__ c
         const double vol = (((4.0 / 3.0) * prm.pi) * ri * ri) * ri;
         f2 = f2 - (prm.drho * vol) * prm.grav;
         Acc3 fs = {0.0, 0.0, 0.0}, fp = {0.0, 0.0, 0.0}, ts = {0.0, 0.0, 0.0}, tp = {0.0, 0.0, 0.0};
-        const int ci = p.particle_cell[i];
         const double xi[3] = {p.x[i], p.y[i], p.z[i]};
-        for (int pass = 0; pass < 2; pass++) {
-            for (int s = -1; s < 27; s++) {
-                const int cell = s < 0 ? 0 : ci + g.stencil[s];
-                if (!(s < 0 || (cell > 0 && cell < g.ncells))) continue;
-                const int st = p.cell_start[cell], nsph = p.nshapes[2 * cell];
-                const int q0 = pass == 0 ? 0 : nsph;
-                const int q1 = pass == 0 ? nsph : nsph + p.nshapes[2 * cell + 1];
-                for (int q = q0; q < q1; q++) {
-                    const int j = p.cell_particles[st + q];
-                    if (j == i) continue;
-                    const double xj[3] = {p.x[j], p.y[j], p.z[j]};
-                    double nv[3], cpt[3], delta;
-                    if (pass == 0) {
-                        const double d[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
-                        const double rsq = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-                        const double rj = p.r[j];
-                        const double rs = ri + rj;
-                        if (!(rsq < rs * (rs + 0.0))) continue;
-                        const double dist = sqrt(rsq);
-                        const double pen = (dist - ri) - rj;
-                        delta = -(pen);
-                        if (delta < 0.0) continue;
-                        const double inv = 1.0 / dist;
-                        for (int e = 0; e < 3; e++) nv[e] = d[e] * inv;
-                        const double reach = rj + 0.5 * pen;
-                        for (int e = 0; e < 3; e++) cpt[e] = xj[e] + nv[e] * reach;
-                    } else {
-                        const double nj[3] = {p.nx[j], p.ny[j], p.nz[j]};
-                        const double pen = (((nj[0] * xi[0] + nj[1] * xi[1]) + nj[2] * xi[2]) - ri) -
-                                           ((nj[0] * xj[0] + nj[1] * xj[1]) + nj[2] * xj[2]);
-                        if (!(pen < 0.0)) continue;
-                        delta = -(pen);
-                        if (delta < 0.0) continue;
-                        const double reach = ri + pen;
-                        for (int e = 0; e < 3; e++) { nv[e] = nj[e]; cpt[e] = xi[e] - reach * nj[e]; }
-                    }
-                    const int slot = contact_slot(p, c, i, p.uid[j]);
-                    if (slot < 0) continue;
-                    spring_dashpot(p, prm, i, j, slot, nv, delta, cpt, pass == 0 ? fs : fp, pass == 0 ? ts : tp);
-                }
+        auto touch = [&](int j, int pass) {
+            double nv[3], cpt[3], delta;
+            if (!contact_geom(p, xi, ri, j, pass, nv, cpt, delta)) return true;
+            const int slot = contact_slot(p, c, i, p.uid[j]);
+            if (slot >= 0) spring_dashpot(p, prm, i, j, slot, nv, delta, cpt, pass == 0 ? fs : fp, pass == 0 ? ts : tp);
+            return true;
+        };
+        const int nt = p.cand_n[i];
+        if (nt <= DCAND) {
+            for (int t = 0; t < nt; t++) {
+                const int e = p.cand[(size_t) t * p.cap + i];
+                touch(e < 0 ? ~e : e, e < 0 ? 1 : 0);
             }
+        } else {
+            walk_candidates(p, g, i, p.particle_cell[i], touch);
         }
         f0 = f0 + (fs.x + fp.x); f1 = f1 + (fs.y + fp.y); f2 = f2 + (fs.z + fp.z);
         t0 = t0 + (tp.x + ts.x); t1 = t1 + (tp.y + ts.y); t2 = t2 + (tp.z + ts.z);
@@ -760,6 +816,7 @@ void launch_prepare(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g) {
 
 // forces + history bookkeeping, then integration
 void launch_contacts(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, const DemParams &prm) {
+    hipLaunchKernelGGL(k_contact_find, dim3(grid_of(p.cap, 256)), dim3(256), 0, s, p, c, g);
     hipLaunchKernelGGL(k_contacts, dim3(grid_of(p.cap, 128)), dim3(128), 0, s, p, c, g, prm);
 }
 

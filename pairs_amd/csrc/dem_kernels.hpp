@@ -25,6 +25,10 @@ enum {
     D_REQ_PART = 15,
     D_COUNT = 16
 };
+// touching partners recorded per particle by the contact pass; a particle
+// with more walks its cells again in the force pass
+constexpr int DCAND = 24;
+
 enum { DOVF_NONE = 0, DOVF_PART = 1, DOVF_CONTACT = 2, DOVF_SEND = 3, DOVF_CELLS = 4 };
 
 struct DemGeom {
@@ -54,6 +58,7 @@ struct DemPtrs {
     int *cell_count, *cell_start, *cell_fill, *cell_particles, *particle_cell, *nshapes;
     int *send_map, *send_mult, *exchg_flag, *copy_to, *blk, *scan_tmp;
     double *buf, *hbuf;
+    int *cand_n, *cand;  // touching partners found by the contact pass: [t][cap], t < DCAND
 };
 
 void launch_fill(hipStream_t s, int *p, int v, int n);
