@@ -194,6 +194,7 @@ def run_dem(steps):
     s.sync()
     t = time.perf_counter() - t0
     ms, launches = s.contact_kernel_time()
+    phases = s.phase_times()
     nc = s.get("num_contacts")
     s.close()
     return {
@@ -201,6 +202,8 @@ def run_dem(steps):
         "unit": "particle-steps/s",
         "ms_per_step": t * 1e3 / steps,
         "contact_kernel_ms": ms,
+        "phase_ms": dict(zip(("prepare", "contact_search", "history_slots", "spring_dashpot", "sums", "euler"),
+                             phases)),
         "spheres": n,
         "steps": steps,
         "contacts_at_end": int(nc.sum()),

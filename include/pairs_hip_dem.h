@@ -118,6 +118,11 @@ int pairs_hip_dem_contacts(pairs_hip_dem_ctx *ctx, int *num_contacts, int *partn
    last pairs_hip_dem_run (set every_n > 0 first; 0 disables timing). */
 int pairs_hip_dem_set_timing(pairs_hip_dem_ctx *ctx, int every_n);
 int pairs_hip_dem_contact_kernel_time(pairs_hip_dem_ctx *ctx, double *avg_ms, int *launches);
+/* The same timed steps split into phases, avg_ms[6]: prepare (exchange,
+   borders, cell lists), contact search, history slots, spring-dashpot per
+   contact, per-particle sums + history clean-up, euler.
+   No reference counterpart (the reference's timers are host-side). */
+int pairs_hip_dem_phase_times(pairs_hip_dem_ctx *ctx, double *avg_ms);
 
 #ifdef __cplusplus
 }

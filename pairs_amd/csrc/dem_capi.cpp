@@ -35,7 +35,10 @@ struct pairs_hip_dem_ctx {
     bool use_graphs = true;
     hipGraphExec_t graph = nullptr;
     int time_every = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // timed steps: before prepare, before the contact search, after it, after
+    // the slot and the pair kernels, after the sums, after euler
+    hipEvent_t ev[7] = {};
+    double t_phase[6] = {};
     double t_sum = 0.0;
     int t_count = 0;
     std::mt19937 generator;     // runtime/dem_sc_grid.hpp:13-20: default seeded, shared by all grids
@@ -78,7 +81,7 @@ static void free_all(pairs_hip_dem_ctx *ctx) {
                     p.nx, p.ny, p.nz, p.inv_i, p.rot_m, p.rot_q, p.uid, p.flags, p.type, p.shape, p.num_contacts,
                     p.c_uid, p.c_used, p.c_stick, p.c_tsd, p.c_ivm, p.cell_count, p.cell_start, p.cell_fill,
                     p.cell_particles, p.particle_cell, p.nshapes, p.send_map, p.send_mult, p.exchg_flag, p.copy_to,
-                    p.blk, p.scan_tmp, p.buf, p.hbuf, p.cand_n, p.cand, ctx->counters};
+                    p.blk, p.scan_tmp, p.buf, p.hbuf, p.cand_n, p.cand, p.cslot, p.cf, ctx->counters};
     for (void *q : ptrs)
         if (q) hipFree(q);
     if (ctx->h_counters) hipHostFree(ctx->h_counters);
@@ -173,6 +176,8 @@ int pairs_hip_dem_create(pairs_hip_dem_ctx **out, const pairs_hip_dem_config *cf
     r |= dalloc(ctx, &p.hbuf, (2 + 5 * K) * S);
     r |= dalloc(ctx, &p.cand_n, C);
     r |= dalloc(ctx, &p.cand, DCAND * C);
+    r |= dalloc(ctx, &p.cslot, DCAND * C);
+    r |= dalloc(ctx, &p.cf, 6 * DCAND * C);
     r |= dalloc(ctx, &ctx->counters, D_COUNT);
     if (r) return fail(-1);
     if (hipHostMalloc((void **) &ctx->h_counters, D_COUNT * sizeof(int)) != hipSuccess) return fail(-1);
@@ -186,8 +191,8 @@ int pairs_hip_dem_destroy(pairs_hip_dem_ctx *ctx) {
     if (!ctx) return 0;
     hipStreamSynchronize(ctx->stream);
     drop_graph(ctx);
-    if (ctx->ev0) hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->ev)
+        if (e) hipEventDestroy(e);
     free_all(ctx);
     if (ctx->own_stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -346,11 +351,13 @@ static int overflow_code(pairs_hip_dem_ctx *ctx) {
 }
 
 static void enqueue_step(pairs_hip_dem_ctx *ctx, bool timed) {
+    if (timed) hipEventRecord(ctx->ev[0], ctx->stream);
     launch_prepare(ctx->stream, ctx->p, ctx->counters, ctx->geom);
-    if (timed) hipEventRecord(ctx->ev0, ctx->stream);
-    launch_contacts(ctx->stream, ctx->p, ctx->counters, ctx->geom, ctx->prm);
-    if (timed) hipEventRecord(ctx->ev1, ctx->stream);
+    if (timed) hipEventRecord(ctx->ev[1], ctx->stream);
+    launch_contacts(ctx->stream, ctx->p, ctx->counters, ctx->geom, ctx->prm, timed ? ctx->ev + 2 : nullptr);
+    if (timed) hipEventRecord(ctx->ev[5], ctx->stream);
     launch_euler(ctx->stream, ctx->p, ctx->counters, ctx->prm);
+    if (timed) hipEventRecord(ctx->ev[6], ctx->stream);
 }
 
 int pairs_hip_dem_step(pairs_hip_dem_ctx *ctx, int ts) {
@@ -375,14 +382,20 @@ int pairs_hip_dem_run(pairs_hip_dem_ctx *ctx, int ts_begin, int ts_end) {
     }
     ctx->t_sum = 0.0;
     ctx->t_count = 0;
+    for (double &t : ctx->t_phase) t = 0.0;
     for (int ts = ts_begin; ts < ts_end; ts++) {
         const bool timed = ctx->time_every > 0 && (ts - ts_begin) % ctx->time_every == ctx->time_every - 1;
         if (timed) {
             enqueue_step(ctx, true);
             CHECK_LAUNCH();
-            HIP_OK(hipEventSynchronize(ctx->ev1));
+            HIP_OK(hipEventSynchronize(ctx->ev[6]));
+            for (int k = 0; k < 6; k++) {
+                float ms = 0.0f;
+                HIP_OK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+                ctx->t_phase[k] += ms;
+            }
             float ms = 0.0f;
-            HIP_OK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            HIP_OK(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[5]));
             ctx->t_sum += ms;
             ctx->t_count++;
         } else if (ctx->use_graphs) {
@@ -502,10 +515,8 @@ int pairs_hip_dem_contacts(pairs_hip_dem_ctx *ctx, int *num_contacts, int *partn
 int pairs_hip_dem_set_timing(pairs_hip_dem_ctx *ctx, int every_n) {
     if (!ctx || every_n < 0) return -2;
     ctx->time_every = every_n;
-    if (every_n > 0 && !ctx->ev0) {
-        HIP_OK(hipEventCreate(&ctx->ev0));
-        HIP_OK(hipEventCreate(&ctx->ev1));
-    }
+    if (every_n > 0 && !ctx->ev[0])
+        for (hipEvent_t &e : ctx->ev) HIP_OK(hipEventCreate(&e));
     return 0;
 }
 
@@ -513,6 +524,12 @@ int pairs_hip_dem_contact_kernel_time(pairs_hip_dem_ctx *ctx, double *avg_ms, in
     if (!ctx || !avg_ms || !launches) return -2;
     *avg_ms = ctx->t_count ? ctx->t_sum / ctx->t_count : 0.0;
     *launches = ctx->t_count;
+    return 0;
+}
+
+int pairs_hip_dem_phase_times(pairs_hip_dem_ctx *ctx, double *avg_ms) {
+    if (!ctx || !avg_ms) return -2;
+    for (int k = 0; k < 6; k++) avg_ms[k] = ctx->t_count ? ctx->t_phase[k] / ctx->t_count : 0.0;
     return 0;
 }
 

@@ -427,10 +427,11 @@ __global__ __launch_bounds__(256) void k_cell_order(DemPtrs p, int ncells) {
 struct Acc3 { double x, y, z; };
 
 // linear_spring_dashpot (examples/dem.py:18-74) for one contact: normal n,
-// penetration delta, contact point cp; history slot a (k-major index base)
+// penetration delta, contact point cp, history slot k; updates the slot and
+// returns the contact's force F and torque term T = (cp - x_i) x F
 __device__ __forceinline__ void spring_dashpot(const DemPtrs &p, const DemParams &prm, int i, int j, int k,
-                                               const double n[3], double delta, const double cp[3], Acc3 &fa,
-                                               Acc3 &ta) {
+                                               const double n[3], double delta, const double cp[3], double F[3],
+                                               double T[3]) {
     const size_t a = (size_t) k * p.cap + i;
     const bool sticking = p.c_stick[a] == 1;
     const double xi[3] = {p.x[i], p.y[i], p.z[i]}, xj[3] = {p.x[j], p.y[j], p.z[j]};
@@ -487,13 +488,17 @@ __device__ __forceinline__ void spring_dashpot(const DemPtrs &p, const DemParams
     p.c_ivm[a] = (iv > 0.0) ? iv : rel_len;
     p.c_stick[a] = ((cond1 || cond2) || ftls_len < f_dynamic) ? 1 : 0;
     const double ft_abs = (f_fric < ftls_len) ? f_fric : ftls_len;
-    double F[3];
     for (int d = 0; d < 3; d++) F[d] = fn[d] + ft_abs * tdir[d];
-    fa.x = fa.x + F[0]; fa.y = fa.y + F[1]; fa.z = fa.z + F[2];
     const double c0 = cp[0] - xi[0], c1 = cp[1] - xi[1], c2 = cp[2] - xi[2];
-    ta.x = ta.x + (c1 * F[2] - c2 * F[1]);
-    ta.y = ta.y + (c2 * F[0] - c0 * F[2]);
-    ta.z = ta.z + (c0 * F[1] - c1 * F[0]);
+    T[0] = c1 * F[2] - c2 * F[1];
+    T[1] = c2 * F[0] - c0 * F[2];
+    T[2] = c0 * F[1] - c1 * F[0];
+}
+
+// force += F, torque += T of one contact, in the caller's contact order
+__device__ __forceinline__ void accumulate(Acc3 &fa, Acc3 &ta, const double F[3], const double T[3]) {
+    fa.x = fa.x + F[0]; fa.y = fa.y + F[1]; fa.z = fa.z + F[2];
+    ta.x = ta.x + T[0]; ta.y = ta.y + T[1]; ta.z = ta.z + T[2];
 }
 
 // contact slot of uid_j in i's history: the last match, or a new slot with
@@ -580,43 +585,136 @@ __device__ __forceinline__ void walk_candidates(const DemPtrs &p, const DemGeom 
     }
 }
 
-// Pass 1 of the contact step, one thread per local particle: the partners
-// that touch i, in visiting order, as cand[t][i] (half-spaces stored as ~j);
-// cand_n[i] = DCAND + 1 when more than DCAND touch (the force pass then walks
-// the cells itself).  Only geometry is read: no history is touched.
+// inclusive sum over the 32 lanes of each half-wave
+__device__ __forceinline__ int half_scan(int x) {
+    const int l = threadIdx.x & 31;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+        const int y = __shfl_up(x, off, 32);
+        if (l >= off) x += y;
+    }
+    return x;
+}
+
+// Contact search, 32 lanes per local particle: lane 0 takes cell 0, lane
+// 1 + s stencil cell s.  Each lane counts the touching spheres and
+// half-spaces of its cell; prefix sums over the lanes give every hit its
+// place in the reference's visiting order (the spheres of cell 0 and of the
+// stencil cells, then the half-spaces of the same cells; cell_lists.py
+// partitions each cell spheres-first), and a second walk over the same
+// candidates stores them as cand[t][i] (half-spaces as ~j).  cand_n[i] =
+// DCAND + 1 when more than DCAND touch (the force pass then walks the cells
+// itself).  Only geometry is read: no history is touched.
 __global__ __launch_bounds__(256) void k_contact_find(DemPtrs p, const int *__restrict__ c, DemGeom g) {
+    if (c[D_OVF]) return;
+    const int nl = c[D_NLOCAL];
+    const int i = (blockIdx.x * 256 + threadIdx.x) >> 5;
+    const int l = threadIdx.x & 31;
+    const bool act = i < nl;
+    const int ii = act ? i : 0;
+    const bool fixed = !act || (p.flags[ii] & DFLAG_FIXED);
+    int st = 0, nsph = 0, nhs = 0;
+    if (!fixed && l < 28) {
+        const int cell = l == 0 ? 0 : p.particle_cell[ii] + g.stencil[l - 1];
+        if (l == 0 || (cell > 0 && cell < g.ncells)) {
+            st = p.cell_start[cell];
+            const int2 ns = reinterpret_cast<const int2 *>(p.nshapes)[cell];
+            nsph = ns.x;
+            nhs = ns.y;
+        }
+    }
+    const double ri = p.r[ii];
+    const double xi[3] = {p.x[ii], p.y[ii], p.z[ii]};
+    // hits of the first 32 candidates of the cell kept as a bit mask, so the
+    // store walk repeats the geometry test only beyond them
+    int hs = 0, hh = 0;
+    unsigned hit = 0u;
+    for (int q = 0; q < nsph + nhs; q++) {
+        const int j = p.cell_particles[st + q];
+        double nv[3], cpt[3], delta;
+        if (j != ii && contact_geom(p, xi, ri, j, q < nsph ? 0 : 1, nv, cpt, delta)) {
+            if (q < nsph) hs++; else hh++;
+            if (q < 32) hit |= 1u << q;
+        }
+    }
+    const int ps = half_scan(hs), ph = half_scan(hh);
+    const int tot_s = __shfl(ps, 31, 32), tot_h = __shfl(ph, 31, 32);
+    const int n = tot_s + tot_h;
+    if (n <= DCAND) {
+        int os = ps - hs, oh = tot_s + ph - hh;
+        while (hit) {
+            const int q = __builtin_ctz(hit);
+            hit &= hit - 1u;
+            const int j = p.cell_particles[st + q];
+            const int t = q < nsph ? os++ : oh++;
+            p.cand[(size_t) t * p.cap + ii] = q < nsph ? j : ~j;
+        }
+        for (int q = 32; q < nsph + nhs; q++) {
+            const int j = p.cell_particles[st + q];
+            double nv[3], cpt[3], delta;
+            if (j != ii && contact_geom(p, xi, ri, j, q < nsph ? 0 : 1, nv, cpt, delta)) {
+                const int t = q < nsph ? os++ : oh++;
+                p.cand[(size_t) t * p.cap + ii] = q < nsph ? j : ~j;
+            }
+        }
+    }
+    if (act && l == 0) p.cand_n[i] = n <= DCAND ? n : DCAND + 1;
+}
+
+// One thread per local particle: reset_contact_history_usage_status, then the
+// history slot of each touching partner in order (the last match, or a new
+// slot; src/pairs/mapping/funcs.py:230-270), marked used.
+__global__ __launch_bounds__(256) void k_contact_slots(DemPtrs p, int *__restrict__ c) {
     const int nl = c[D_NLOCAL];
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nl || c[D_OVF]) return;
-    int n = 0;
-    if (!(p.flags[i] & DFLAG_FIXED)) {
-        const double ri = p.r[i];
-        const double xi[3] = {p.x[i], p.y[i], p.z[i]};
-        walk_candidates(p, g, i, p.particle_cell[i], [&](int j, int pass) {
-            double nv[3], cpt[3], delta;
-            if (!contact_geom(p, xi, ri, j, pass, nv, cpt, delta)) return true;
-            if (n == DCAND) { n = DCAND + 1; return false; }
-            p.cand[(size_t) n * p.cap + i] = pass == 0 ? j : ~j;
-            n++;
-            return true;
-        });
+    const int nc0 = p.num_contacts[i];
+    for (int k = 0; k < nc0; k++) p.c_used[(size_t) k * p.cap + i] = 0;
+    const int nt = p.cand_n[i];
+    if (nt > DCAND) return;
+    for (int t = 0; t < nt; t++) {
+        const int e = p.cand[(size_t) t * p.cap + i];
+        p.cslot[(size_t) t * p.cap + i] = contact_slot(p, c, i, p.uid[e < 0 ? ~e : e]);
     }
-    p.cand_n[i] = n;
 }
 
-// Pass 2, one thread per local particle: reset_contact_history_usage_status,
-// reset_volatile_properties, gravity, linear_spring_dashpot over the touching
-// partners of pass 1 (spheres, then half-spaces; force += sphere + plane sums,
-// torque += plane + sphere sums) and clear_unused_contact_history -- every
-// one of them touches only the thread's own particle and history rows.  A
-// wave runs the spring-dashpot as often as its busiest lane has contacts,
-// not once per candidate.
+// One thread per (partner t, local particle i): linear_spring_dashpot of the
+// contact, its history slot updated, its force and torque term stored as
+// cf[6 t + 0..5][i] for the in-order sums of k_contacts.
+__global__ __launch_bounds__(256) void k_contact_pairs(DemPtrs p, const int *__restrict__ c, DemParams prm) {
+    const int nl = c[D_NLOCAL];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.y;
+    if (i >= nl || c[D_OVF]) return;
+    const int nt = p.cand_n[i];
+    if (t >= nt || nt > DCAND) return;
+    const size_t a = (size_t) t * p.cap + i;
+    const int slot = p.cslot[a];
+    if (slot < 0) return;
+    const int e = p.cand[a];
+    const int j = e < 0 ? ~e : e;
+    const double ri = p.r[i];
+    const double xi[3] = {p.x[i], p.y[i], p.z[i]};
+    double nv[3], cpt[3], delta, F[3], T[3];
+    if (!contact_geom(p, xi, ri, j, e < 0 ? 1 : 0, nv, cpt, delta)) return;
+    spring_dashpot(p, prm, i, j, slot, nv, delta, cpt, F, T);
+    for (int d = 0; d < 3; d++) {
+        p.cf[(6 * (size_t) t + d) * p.cap + i] = F[d];
+        p.cf[(6 * (size_t) t + 3 + d) * p.cap + i] = T[d];
+    }
+}
+
+// One thread per local particle: reset_volatile_properties, gravity, the
+// contacts' forces and torques summed in contact order (force += sphere +
+// plane sums, torque += plane + sphere sums) and
+// clear_unused_contact_history -- every one of them touches only the
+// thread's own particle and history rows.  A particle with more than DCAND
+// touching partners walks its cells here, slot search and spring-dashpot
+// inline.
 __global__ __launch_bounds__(128) void k_contacts(DemPtrs p, int *__restrict__ c, DemGeom g, DemParams prm) {
     const int nl = c[D_NLOCAL];
     const int i = blockIdx.x * 128 + threadIdx.x;
     if (i >= nl || c[D_OVF]) return;
-    const int nc0 = p.num_contacts[i];
-    for (int k = 0; k < nc0; k++) p.c_used[(size_t) k * p.cap + i] = 0;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
     if (!(p.flags[i] & DFLAG_FIXED)) {
         // gravity (examples/dem.py:88-90)
@@ -624,22 +722,31 @@ __global__ __launch_bounds__(128) void k_contacts(DemPtrs p, int *__restrict__ c
         const double vol = (((4.0 / 3.0) * prm.pi) * ri * ri) * ri;
         f2 = f2 - (prm.drho * vol) * prm.grav;
         Acc3 fs = {0.0, 0.0, 0.0}, fp = {0.0, 0.0, 0.0}, ts = {0.0, 0.0, 0.0}, tp = {0.0, 0.0, 0.0};
-        const double xi[3] = {p.x[i], p.y[i], p.z[i]};
-        auto touch = [&](int j, int pass) {
-            double nv[3], cpt[3], delta;
-            if (!contact_geom(p, xi, ri, j, pass, nv, cpt, delta)) return true;
-            const int slot = contact_slot(p, c, i, p.uid[j]);
-            if (slot >= 0) spring_dashpot(p, prm, i, j, slot, nv, delta, cpt, pass == 0 ? fs : fp, pass == 0 ? ts : tp);
-            return true;
-        };
         const int nt = p.cand_n[i];
         if (nt <= DCAND) {
             for (int t = 0; t < nt; t++) {
-                const int e = p.cand[(size_t) t * p.cap + i];
-                touch(e < 0 ? ~e : e, e < 0 ? 1 : 0);
+                const size_t a = (size_t) t * p.cap + i;
+                if (p.cslot[a] < 0) continue;
+                double F[3], T[3];
+                for (int d = 0; d < 3; d++) {
+                    F[d] = p.cf[(6 * (size_t) t + d) * p.cap + i];
+                    T[d] = p.cf[(6 * (size_t) t + 3 + d) * p.cap + i];
+                }
+                if (p.cand[a] >= 0) accumulate(fs, ts, F, T);
+                else accumulate(fp, tp, F, T);
             }
         } else {
-            walk_candidates(p, g, i, p.particle_cell[i], touch);
+            const double xi[3] = {p.x[i], p.y[i], p.z[i]};
+            walk_candidates(p, g, i, p.particle_cell[i], [&](int j, int pass) {
+                double nv[3], cpt[3], delta, F[3], T[3];
+                if (!contact_geom(p, xi, ri, j, pass, nv, cpt, delta)) return true;
+                const int slot = contact_slot(p, c, i, p.uid[j]);
+                if (slot < 0) return true;
+                spring_dashpot(p, prm, i, j, slot, nv, delta, cpt, F, T);
+                if (pass == 0) accumulate(fs, ts, F, T);
+                else accumulate(fp, tp, F, T);
+                return true;
+            });
         }
         f0 = f0 + (fs.x + fp.x); f1 = f1 + (fs.y + fp.y); f2 = f2 + (fs.z + fp.z);
         t0 = t0 + (tp.x + ts.x); t1 = t1 + (tp.y + ts.y); t2 = t2 + (tp.z + ts.z);
@@ -815,8 +922,14 @@ void launch_prepare(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g) {
 }
 
 // forces + history bookkeeping, then integration
-void launch_contacts(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, const DemParams &prm) {
-    hipLaunchKernelGGL(k_contact_find, dim3(grid_of(p.cap, 256)), dim3(256), 0, s, p, c, g);
+void launch_contacts(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, const DemParams &prm,
+                     const hipEvent_t *between) {
+    hipLaunchKernelGGL(k_contact_find, dim3(grid_of(p.cap, 8)), dim3(256), 0, s, p, c, g);
+    if (between) hipEventRecord(between[0], s);
+    hipLaunchKernelGGL(k_contact_slots, dim3(grid_of(p.cap, 256)), dim3(256), 0, s, p, c);
+    if (between) hipEventRecord(between[1], s);
+    hipLaunchKernelGGL(k_contact_pairs, dim3(grid_of(p.cap, 256), DCAND), dim3(256), 0, s, p, c, prm);
+    if (between) hipEventRecord(between[2], s);
     hipLaunchKernelGGL(k_contacts, dim3(grid_of(p.cap, 128)), dim3(128), 0, s, p, c, g, prm);
 }
 

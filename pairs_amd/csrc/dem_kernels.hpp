@@ -25,9 +25,9 @@ enum {
     D_REQ_PART = 15,
     D_COUNT = 16
 };
-// touching partners recorded per particle by the contact pass; a particle
+// touching partners recorded per particle by the contact search; a particle
 // with more walks its cells again in the force pass
-constexpr int DCAND = 24;
+constexpr int DCAND = 16;
 
 enum { DOVF_NONE = 0, DOVF_PART = 1, DOVF_CONTACT = 2, DOVF_SEND = 3, DOVF_CELLS = 4 };
 
@@ -58,7 +58,9 @@ struct DemPtrs {
     int *cell_count, *cell_start, *cell_fill, *cell_particles, *particle_cell, *nshapes;
     int *send_map, *send_mult, *exchg_flag, *copy_to, *blk, *scan_tmp;
     double *buf, *hbuf;
-    int *cand_n, *cand;  // touching partners found by the contact pass: [t][cap], t < DCAND
+    int *cand_n, *cand;  // touching partners found by the contact search: [t][cap], t < DCAND
+    int *cslot;          // their history slots [t][cap]
+    double *cf;          // their force and torque terms [6 t + d][cap]
 };
 
 void launch_fill(hipStream_t s, int *p, int v, int n);
@@ -66,7 +68,10 @@ void launch_mass_inertia(hipStream_t s, const DemPtrs &p, int nl);
 // one step = prepare (exchange, borders, cells) + contacts (+ gravity and
 // history bookkeeping) + euler
 void launch_prepare(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g);
-void launch_contacts(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, const DemParams &prm);
+// between: three events recorded after the search, the slot and the pair
+// kernels (timed steps only)
+void launch_contacts(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, const DemParams &prm,
+                     const hipEvent_t *between = nullptr);
 void launch_euler(hipStream_t s, const DemPtrs &p, int *c, const DemParams &prm);
 
 }  // namespace dk

@@ -27,6 +27,8 @@ static void drop_graphs(pairs_hip_ctx *ctx) {
     for (int p = 0; p < 2; p++) {
         if (ctx->step_graph[p]) hipGraphExecDestroy(ctx->step_graph[p]);
         ctx->step_graph[p] = nullptr;
+        if (ctx->reb_graph[p]) hipGraphExecDestroy(ctx->reb_graph[p]);
+        ctx->reb_graph[p] = nullptr;
     }
 }
 
@@ -827,15 +829,51 @@ static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 |
 
 // exchange + borders + cells + neighbour lists, retried with grown capacities
 // on overflow like the generated resize loops (transformations/modules.py:159-205)
+static void enqueue_rebuild_cells(pairs_hip_ctx *ctx) {
+    hipStream_t s = ctx->stream;
+    launch_set_counters(s, ctx->d.counters, C_IMG_OVF, 1, 0);
+    launch_exchange(s, ctx->d, ctx->geom, ctx->cfg.periodic);
+    launch_borders(s, ctx->d, ctx->geom, ctx->cfg.periodic);
+    launch_bin_sort(s, ctx->d, ctx->geom);  // flips d.cur
+}
+
+// exchange + borders + cell binning: about 35 small kernels whose grids
+// depend only on capacities and whose counts live on the device, replayed
+// from a graph (one per position buffer) so that they run back to back
+static int launch_rebuild_cells(pairs_hip_ctx *ctx) {
+    const int cur = ctx->d.cur;
+    if (!ctx->use_graphs) {
+        enqueue_rebuild_cells(ctx);
+        CHECK_LAUNCH();
+        return 0;
+    }
+    if (!ctx->reb_graph[cur]) {
+        hipGraph_t g;
+        HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_rebuild_cells(ctx);
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        ctx->d.cur = cur;
+        if (ec != hipSuccess) {
+            ctx->err = std::string("rebuild capture: ") + hipGetErrorString(ec);
+            return PAIRS_HIP_ERR_HIP;
+        }
+        const hipError_t e = hipGraphInstantiate(&ctx->reb_graph[cur], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+            ctx->reb_graph[cur] = nullptr;
+            return PAIRS_HIP_ERR_HIP;
+        }
+    }
+    HIP_OK(hipGraphLaunch(ctx->reb_graph[cur], ctx->stream));
+    ctx->d.cur = 1 - cur;
+    return 0;
+}
+
 static int rebuild(pairs_hip_ctx *ctx) {
     if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     for (int attempt = 0; attempt < 8; attempt++) {
-        hipStream_t s = ctx->stream;
-        launch_set_counters(s, ctx->d.counters, C_IMG_OVF, 1, 0);
-        launch_exchange(s, ctx->d, ctx->geom, ctx->cfg.periodic);
-        launch_borders(s, ctx->d, ctx->geom, ctx->cfg.periodic);
-        launch_bin_sort(s, ctx->d, ctx->geom);
-        CHECK_LAUNCH();
+        if (int r = launch_rebuild_cells(ctx)) return r;
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
         // images refresh the ghosts from the position kernels unless some
         // local has too many of them (then a separate refresh runs)

@@ -55,6 +55,7 @@ native.register({
     "pairs_hip_dem_contacts": (_I, [_P, _IP, _IP, _IP, _DP, _DP]),
     "pairs_hip_dem_set_timing": (_I, [_P, _I]),
     "pairs_hip_dem_contact_kernel_time": (_I, [_P, _DP, _IP]),
+    "pairs_hip_dem_phase_times": (_I, [_P, _DP]),
 })
 
 # examples/dem.py:113-140
@@ -203,6 +204,13 @@ class DEMSystem:
         self._check(self.L.pairs_hip_dem_contact_kernel_time(self.ctx, ctypes.byref(ms), ctypes.byref(n)),
                     "contact_kernel_time")
         return ms.value, n.value
+
+    def phase_times(self):
+        """Average ms of the timed steps' phases: prepare, contact search,
+        history slots, spring-dashpot, sums + history clean-up, euler."""
+        ms = (_D * 6)()
+        self._check(self.L.pairs_hip_dem_phase_times(self.ctx, ms), "phase_times")
+        return tuple(ms)
 
     def sync(self):
         self._check(self.L.pairs_hip_dem_sync(self.ctx), "sync")
