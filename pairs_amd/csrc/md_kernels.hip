@@ -328,6 +328,10 @@ constexpr int LR_MAX = 96;     // list rows buffered in LDS (a multiple of 4)
 // from the segment size); rows are hcap + 2 entries long (an odd number of
 // 4-byte words, so consecutive rows start on different banks)
 constexpr int NB_CAND = 2048;  // candidates of one home cell
+// half lists: mirror entries of one home particle collected in the top
+// RMAX slots of the wave's candidate buffer before they are sorted
+constexpr int RMAX = 256;
+constexpr int REF_MASK = (1 << 30) - 1;  // tref: reference index; bit 30: FIXED
 
 __device__ __forceinline__ double readlane_f64(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -344,15 +348,19 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
                                                         const int *__restrict__ ref, const int *__restrict__ flags,
                                                         double rc2, uint16_t *__restrict__ nl, int nl_stride,
                                                         int ncap, int lrows, int tcap, int zt,
-                                                        int *__restrict__ numneighs, int *__restrict__ ctr) {
+                                                        int *__restrict__ numneighs, int *__restrict__ ctr,
+                                                        int *__restrict__ nrev, int rev0) {
     extern __shared__ char smem[];
     const int HCAP = g.hcap, LBS = g.hcap + 2;
     int *tslot = (int *) smem;                                  // [tcap] physical slot of each tile entry
     int *s_cnt = tslot + tcap;                                  // [hcap] list length per buffered column
-    int *ctab = s_cnt + HCAP;                                   // [9][zt][4]: local st, len, ghost st, len
+                                                                // (+ [hcap] mirror lengths, half lists)
+    int *ctab = s_cnt + (HALF ? 2 : 1) * HCAP;                  // [9][zt][4]: local st, len, ghost st, len
     int *tref = ctab + 9 * zt * 4;                              // [tcap] half lists only
     uint16_t *lbuf = (uint16_t *) (tref + (HALF ? tcap : 0));   // [lrows][LBS]
     uint16_t *cand_all = lbuf + (size_t) lrows * LBS;           // [BW][NB_CAND]
+    // half lists: the row buffer's upper half holds the mirror rows
+    const int lr_own = HALF ? (lrows / 8) * 4 : lrows, lr_mir = lrows - lr_own;
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
@@ -372,7 +380,8 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
     stage_slots(r_start, r_off, T, tslot);
     __syncthreads();
     if (HALF)
-        for (int p = threadIdx.x; p < T; p += BB) tref[p] = ref[tslot[p]];
+        for (int p = threadIdx.x; p < T; p += BB)
+            tref[p] = ref[tslot[p]] | ((flags[tslot[p]] & FLAG_FIXED) ? (1 << 30) : 0);
     // sub-range table: a flat cell cc of column col is a stencil cell of the
     // reference iff 0 < cc < ncells (interaction.py:92-98)
     for (int e = threadIdx.x; e < 9 * nz; e += BB) {
@@ -420,7 +429,7 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
             if (lane >= off) incl += v;
         }
         const int ncand = __shfl(incl, 63, 64);
-        if (ncand > NB_CAND) {
+        if (ncand > (HALF ? NB_CAND - RMAX : NB_CAND)) {
             if (lane == 0) { atomicMax(&ctr[C_TILE_MAX], tcap + 1); atomicMax(&ctr[C_OVF], OVF_TILE); }
             continue;
         }
@@ -453,12 +462,14 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
             const int oh = hst + hb + min(lane, nh - 1);
             const int ph = tslot[oh];
             const double hx = X[ph], hy = Y[ph], hz = Z[ph];
-            const int href = HALF ? tref[oh] : 0;
+            const int href = HALF ? (tref[oh] & REF_MASK) : 0;
             const int hfix = flags[h0 + (oh - own_base)] & FLAG_FIXED;
             for (int h = 0; h < nh; h++) {
                 const int ownh = hst + hb + h;
                 const int idx = ownh - own_base;
-                int cnt = 0;  // wave-uniform: entries of home particle h so far
+                int cnt = 0;   // wave-uniform: entries of home particle h so far
+                int cntr = 0;  // half lists: its mirror entries so far
+                uint16_t *rbuf = cand + (NB_CAND - RMAX);
                 if (!__builtin_amdgcn_readlane(hfix, h)) {
                     const double xi = readlane_f64(hx, h), yi = readlane_f64(hy, h), zi = readlane_f64(hz, h);
                     const int hr = HALF ? __builtin_amdgcn_readlane(href, h) : 0;
@@ -468,15 +479,24 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
                         const double dx = xi - cx, dy = yi - cy, dz = zi - cz;
                         double rsq = dx * dx + dy * dy;
                         rsq = rsq + dz * dz;
-                        const bool keep = HALF ? (hr < cr) : (o != ownh);
+                        const int crf = cr & REF_MASK;
+                        const bool keep = HALF ? (hr < crf) : (o != ownh);
                         const bool acc = qv && keep && rsq < rc2;
                         const unsigned long long m = __ballot(acc);
                         const int k = cnt + lane_rank(m);
                         if (acc && k < ncap) {
-                            if (k < lrows && idx < HCAP) lbuf[k * LBS + idx] = (uint16_t) o;
+                            if (k < lr_own && idx < HCAP) lbuf[k * LBS + idx] = (uint16_t) o;
                             else nl[nl_at(k, h0 + idx, nl_stride)] = (uint16_t) o;
                         }
                         cnt += __popcll(m);
+                        if (HALF) {
+                            // a mirror entry: an earlier-index local, not fixed
+                            const bool mir = qv && crf < hr && !(cr >> 30) && rsq < rc2;
+                            const unsigned long long mr = __ballot(mir);
+                            const int kr = cntr + lane_rank(mr);
+                            if (mir && kr < RMAX) rbuf[kr] = (uint16_t) o;
+                            cntr += __popcll(mr);
+                        }
                     };
 #pragma unroll
                     for (int c = 0; c < CREG; c++)
@@ -487,6 +507,53 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
                         const int pc = tslot[o];
                         test(c * 64, o, X[pc], Y[pc], Z[pc], HALF ? tref[o] : 0);
                     }
+                }
+                if (HALF) {
+                    // the mirror entries go to rows rev0.. sorted by reference
+                    // index: an entry's rank is the number of entries whose
+                    // index is smaller (indices are distinct)
+                    if (cntr > RMAX) {
+                        if (lane == 0) { atomicMax(&ctr[C_TILE_MAX], tcap + 1); atomicMax(&ctr[C_OVF], OVF_TILE); }
+                        cntr = RMAX;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    int KR = cntr;
+                    if (KR + 1 >= rev0) {
+                        if (lane == 0) atomicMax(&ctr[C_REQ_NEIGH], KR + 2);
+                        KR = rev0 - 2;
+                    }
+                    if (cntr <= 64) {
+                        // one entry per lane, its index compared with the
+                        // others' through readlane (no memory round trips)
+                        const int oe = lane < cntr ? rbuf[lane] : 0;
+                        const int re = lane < cntr ? (tref[oe] & REF_MASK) : 0;
+                        int rank = 0;
+                        for (int q = 0; q < cntr; q++) rank += __builtin_amdgcn_readlane(re, q) < re ? 1 : 0;
+                        if (lane < cntr && rank < KR) {
+                            if (rank < lr_mir && idx < HCAP) lbuf[(lr_own + rank) * LBS + idx] = (uint16_t) oe;
+                            else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint16_t) oe;
+                        }
+                    } else {
+                        for (int e = lane; e < cntr; e += 64) {
+                            const int oe = rbuf[e];
+                            const int re = tref[oe] & REF_MASK;
+                            int rank = 0;
+                            for (int q = 0; q < cntr; q++) rank += (tref[rbuf[q]] & REF_MASK) < re ? 1 : 0;
+                            if (rank < KR) {
+                                if (rank < lr_mir && idx < HCAP) lbuf[(lr_own + rank) * LBS + idx] = (uint16_t) oe;
+                                else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint16_t) oe;
+                            }
+                        }
+                    }
+                    if (lane == 0) {
+                        nrev[h0 + idx] = KR;
+                        if (idx < HCAP) s_cnt[HCAP + idx] = KR;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 }
                 if (lane == 0) {
                     // reference overflow rule (generated resize check): count+1 >= capacity
@@ -505,14 +572,18 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
     // buffered rows leave as packed groups of four (lane order = physical order)
     const int nhome = min(h1 - h0, HCAP);
     uint64_t *nl4 = (uint64_t *) nl;
-    for (int a = threadIdx.x; a < nhome; a += BB) {
-        const int kb = (min(s_cnt[a], lrows) + 3) >> 2;
-        for (int gq = 0; gq < kb; gq++) {
-            const uint64_t v = (uint64_t) lbuf[(4 * gq) * LBS + a] | ((uint64_t) lbuf[(4 * gq + 1) * LBS + a] << 16) |
-                               ((uint64_t) lbuf[(4 * gq + 2) * LBS + a] << 32) |
-                               ((uint64_t) lbuf[(4 * gq + 3) * LBS + a] << 48);
-            nl4[(size_t) gq * nl_stride + h0 + a] = v;
+    auto flush = [&](int a, int row0, int nrows, int grp0) {
+        for (int gq = 0; gq < (nrows + 3) >> 2; gq++) {
+            const int r = row0 + 4 * gq;
+            const uint64_t v = (uint64_t) lbuf[r * LBS + a] | ((uint64_t) lbuf[(r + 1) * LBS + a] << 16) |
+                               ((uint64_t) lbuf[(r + 2) * LBS + a] << 32) |
+                               ((uint64_t) lbuf[(r + 3) * LBS + a] << 48);
+            nl4[(size_t) (grp0 + gq) * nl_stride + h0 + a] = v;
         }
+    };
+    for (int a = threadIdx.x; a < nhome; a += BB) {
+        flush(a, 0, min(s_cnt[a], lr_own), 0);
+        if (HALF) flush(a, lr_own, min(s_cnt[HCAP + a], lr_mir), rev0 >> 2);
     }
 }
 
@@ -589,7 +660,16 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
 // group's offsets loading while the current one computes; each lane adds its
 // pair forces in list order, so the sum is bit-identical to the reference's.
 // PXY: gather (x, y) as one 16-byte load from the packed copy A.XY
-template <int U, int NT, int MODE, bool RED, bool PXY = false>
+// HALF: the system uses half lists.  lennard_jones then adds, for local i,
+// each pair of its half list into its own sum T_i and the mirror force of
+// every earlier-index local j whose half list holds i into force[i] before
+// i's turn, so force[i] = R_i + T_i with R_i the mirror contributions summed
+// in reference-index order of j (src/pairs/ir/apply.py:111-125 with the
+// sequential loop of the generated CPU code).  Both sums are taken here from
+// i's side -- the mirror pair's force recomputed from i is exactly the
+// negated force j computes -- so the result is the reference's bit for bit,
+// without atomics.
+template <int U, int NT, int MODE, bool RED, bool PXY = false, bool HALF = false>
 __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
     static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
     const int *__restrict__ counters = A.counters;
@@ -636,42 +716,60 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
             const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
             const double xi = X[ii], yi = Y[ii], zi = Z[ii];
             const int ti = NT != 1 ? type[ii] : 0;
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
-            list_walk<U>(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *on) {
-                int jp[U];
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0, u = 0.0, w = 0.0, ue = 0.0;
+            // one list part walked in order into (a0, a1, a2).  Energy and
+            // virial: a full list weighs every pair 1/2 (u, w are halved
+            // below); a half list weighs a local pair 1 and a local-ghost
+            // pair 1/2 over its own part only, and its per-atom energy is
+            // half the own part's sum (ue)
+            auto walk = [&](const uint16_t *rows, int cnt, int cmax, bool mirror, double &a0, double &a1, double &a2) {
+                list_walk<U>(rows, ii, nl_stride, cnt, cmax, T, [&](int k, const int *on) {
+                    int jp[U];
 #pragma unroll
-                for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
-                double xj[U], yj[U], zj[U];
-                int tj[U];
+                    for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
+                    double xj[U], yj[U], zj[U];
+                    int tj[U];
 #pragma unroll
-                for (int q = 0; q < U; q++) {
-                    if (PXY) {
-                        const double2 p = A.XY[jp[q]];
-                        xj[q] = p.x; yj[q] = p.y;
-                    } else {
-                        xj[q] = X[jp[q]]; yj[q] = Y[jp[q]];
+                    for (int q = 0; q < U; q++) {
+                        if (PXY) {
+                            const double2 p = A.XY[jp[q]];
+                            xj[q] = p.x; yj[q] = p.y;
+                        } else {
+                            xj[q] = X[jp[q]]; yj[q] = Y[jp[q]];
+                        }
+                        zj[q] = Z[jp[q]];
+                        if (NT != 1) tj[q] = type[jp[q]];
                     }
-                    zj[q] = Z[jp[q]];
-                    if (NT != 1) tj[q] = type[jp[q]];
-                }
 #pragma unroll
-                for (int q = 0; q < U; q++) {
-                    double sg6 = sig0, ep = eps0;
-                    if (NT != 1) { const int kt = ti * nt + tj[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
-                    double g0, g1, g2, uu, ww;
-                    lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < n, rc2, sg6, ep, g0, g1, g2, uu, ww);
-                    t0 = t0 + g0; t1 = t1 + g1; t2 = t2 + g2;
-                    if (RED) { u += uu; w += ww; }
-                }
-            });
+                    for (int q = 0; q < U; q++) {
+                        double sg6 = sig0, ep = eps0;
+                        if (NT != 1) { const int kt = ti * nt + tj[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
+                        double g0, g1, g2, uu, ww;
+                        lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < cnt, rc2, sg6, ep, g0, g1, g2, uu, ww);
+                        a0 = a0 + g0; a1 = a1 + g1; a2 = a2 + g2;
+                        if (RED && !HALF) { u += uu; w += ww; }
+                        if (RED && HALF && !mirror) {
+                            const double wt = jp[q] < nlocal ? 2.0 : 1.0;
+                            u += wt * uu; w += wt * ww; ue += uu;
+                        }
+                    }
+                });
+            };
+            walk(nl, n, nmax, false, t0, t1, t2);
+            if (HALF) {
+                const int nr = (live && !(fl & FLAG_FIXED)) ? A.nrev[ii] : 0;
+                const int nrmax = __builtin_amdgcn_readfirstlane(wave_max(nr));
+                walk(nl + (size_t) A.rev0 * nl_stride, nr, nrmax, true, r0, r1, r2);
+            }
             if (live) {
                 if (RED) { pe += 0.5 * u; vir += 0.5 * w; }
                 if (MODE == 2) {
-                    if (eatom) eatom[i] = 0.5 * u;
+                    if (eatom) eatom[i] = 0.5 * (HALF ? ue : u);
                     continue;
                 }
-                // reset_volatile_properties then force[i] = force[i] + reduction
-                const double f0 = 0.0 + t0, f1 = 0.0 + t1, f2 = 0.0 + t2;
+                // reset_volatile_properties, the mirror contributions (half
+                // lists), then force[i] = force[i] + reduction
+                const double f0 = r0 + t0, f1 = r1 + t1, f2 = r2 + t2;
                 fx[i] = f0; fy[i] = f1; fz[i] = f2;
                 if (MODE == 1) {
                     const double m = mass[i];
@@ -718,9 +816,9 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
 
 
 // one segment per workgroup of one system
-template <int U, int NT, int MODE, bool RED, bool PXY = true>
+template <int U, int NT, int MODE, bool RED, bool PXY = true, bool HALF = false>
 __global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
-    lj_full_segment<U, NT, MODE, RED, PXY>(A, xcd_block(blockIdx.x, gridDim.x));
+    lj_full_segment<U, NT, MODE, RED, PXY, HALF>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // the packed (x, y) copy the single-system force kernel gathers from
@@ -740,94 +838,6 @@ __global__ __launch_bounds__(FB) void k_lj_full_batch(const LJArgs *__restrict__
     lj_full_segment<U, NT, MODE, RED>(desc[r], b - seg_base[r]);
 }
 
-// Half-list LJ (compute_half): i accumulates its own sum; the mirror force on a
-// local, non-fixed j is an FP64 atomic subtract (src/pairs/ir/apply.py:111-125).
-// Forces must be zeroed before the launch (reset_volatile_properties).
-template <int U, int NT, bool RED>
-__global__ __launch_bounds__(FB) void k_lj_half(const int *__restrict__ counters, Geom g,
-                                                const int *__restrict__ lstart, const int *__restrict__ gstart,
-                                                const double *__restrict__ X, const double *__restrict__ Y,
-                                                const double *__restrict__ Z, const int *__restrict__ type,
-                                                const uint16_t *__restrict__ nl, int nl_stride,
-                                                const int *__restrict__ numneighs, const int *__restrict__ flags,
-                                                double *__restrict__ fx, double *__restrict__ fy,
-                                                double *__restrict__ fz, double rc2, LJTab tab,
-                                                double *__restrict__ partials, int pstride, int tcap) {
-    static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
-    extern __shared__ char smem[];
-    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
-    __shared__ double s_eps[16], s_sig6[16];
-    const int nlocal = counters[C_NLOCAL];
-    const int seg = xcd_block(blockIdx.x, gridDim.x);
-    const SegInfo si = seg_info(seg, g);
-    const int h0 = lstart[si.ca], h1 = lstart[si.cb];
-    double pe = 0.0, vir = 0.0;
-    if (h1 > h0) {
-        seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
-        const int T = r_off[NRUNS];
-        const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
-        int *tp = (int *) smem;
-        if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
-        if (fits) stage_slots(r_start, r_off, T, tp);
-        __syncthreads();
-        const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
-        const int nt = tab.nt;
-        for (int base = h0; fits && base < h1; base += FB) {
-            const int i = base + threadIdx.x;
-            const bool live = i < h1;
-            const int ii = live ? i : h0;
-            const int n = (live && !(flags[ii] & FLAG_FIXED)) ? numneighs[ii] : 0;
-            const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
-            const double xi = X[ii], yi = Y[ii], zi = Z[ii];
-            const int ti = NT != 1 ? type[ii] : 0;
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0, u = 0.0, w = 0.0;
-            list_walk<U>(nl, ii, nl_stride, n, nmax, T, [&](int k, const int *oc) {
-                double g0[U], g1[U], g2[U];
-                bool in[U];
-                int jp[U];
-#pragma unroll
-                for (int q = 0; q < U; q++) jp[q] = tp[oc[q]];
-                double xj[U], yj[U], zj[U];
-#pragma unroll
-                for (int q = 0; q < U; q++) { xj[q] = X[jp[q]]; yj[q] = Y[jp[q]]; zj[q] = Z[jp[q]]; }
-#pragma unroll
-                for (int q = 0; q < U; q++) {
-                    double sg6 = sig0, ep = eps0;
-                    if (NT != 1) { const int kt = ti * nt + type[jp[q]]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
-                    double uu, ww;
-                    in[q] = lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < n, rc2, sg6, ep, g0[q], g1[q], g2[q],
-                                         uu, ww);
-                    t0 = t0 + g0[q]; t1 = t1 + g1[q]; t2 = t2 + g2[q];
-                    if (RED) { const double wt = jp[q] < nlocal ? 1.0 : 0.5; u += wt * uu; w += wt * ww; }
-                }
-#pragma unroll
-                for (int q = 0; q < U; q++) {
-                    const int j = jp[q];
-                    if (in[q] && j < nlocal && !(flags[j] & FLAG_FIXED)) {
-                        atomic_add_f64(fx + j, -g0[q]); atomic_add_f64(fy + j, -g1[q]); atomic_add_f64(fz + j, -g2[q]);
-                    }
-                }
-            });
-            if (n > 0) {
-                atomic_add_f64(fx + i, t0); atomic_add_f64(fy + i, t1); atomic_add_f64(fz + i, t2);
-                pe += u; vir += w;
-            }
-        }
-    }
-    if (RED) {
-        __shared__ double red[2][FB / 64];
-        const double a = wave_sum(pe), bb = wave_sum(vir);
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) { red[0][wave] = a; red[1][wave] = bb; }
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            const int r = threadIdx.x;
-            double s = 0.0;
-            for (int k = 0; k < FB / 64; k++) s += red[r][k];
-            partials[(size_t) (1 + r) * pstride + seg] = s;
-        }
-    }
-}
 
 // ---------------------------------------------------------------- integration
 
@@ -1277,7 +1287,7 @@ int build_zt(const Geom &g) { return g.nchunks == 1 ? g.dim_cells[2] + 2 : g.zc 
 static int build_lrows(const DevState &d) { return min((d.ncap + 3) / 4 * 4, LR_MAX); }
 
 static size_t build_lds(const DevState &d, const Geom &g, int half, int tcap) {
-    return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) + (size_t) g.hcap * sizeof(int) +
+    return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) + (size_t) g.hcap * sizeof(int) * (half ? 2 : 1) +
            (size_t) build_lrows(d) * (g.hcap + 2) * sizeof(uint16_t) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
            (size_t) 9 * build_zt(g) * 4 * sizeof(int);
 }
@@ -1297,11 +1307,11 @@ void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int
     if (half)
         hipLaunchKernelGGL(k_build_neighbors<true>, dim3(g.nsegs), dim3(BB), lds, s, d.counters, g, d.lstart,
                            d.gstart, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, rc2, d.nl, d.nl_stride, d.ncap, lrows,
-                           tcap, build_zt(g), d.numneighs, d.counters);
+                           tcap, build_zt(g), d.numneighs, d.counters, d.nrev, d.rev0);
     else
         hipLaunchKernelGGL(k_build_neighbors<false>, dim3(g.nsegs), dim3(BB), lds, s, d.counters, g, d.lstart,
                            d.gstart, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, rc2, d.nl, d.nl_stride, d.ncap, lrows,
-                           tcap, build_zt(g), d.numneighs, d.counters);
+                           tcap, build_zt(g), d.numneighs, d.counters, d.nrev, d.rev0);
 }
 
 void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out) {
@@ -1394,6 +1404,7 @@ LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, i
     A.XYn = (packed && fused && do_vv1 && next != cur) ? d.XY[next] : nullptr;
     A.Xn = d.X[next]; A.Yn = d.Y[next]; A.Zn = d.Z[next];
     A.type = d.type; A.nl = d.nl; A.nl_stride = d.nl_stride; A.numneighs = d.numneighs; A.flags = d.flags;
+    A.nrev = d.nrev; A.rev0 = d.rev0;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz; A.vx = d.vx; A.vy = d.vy; A.vz = d.vz; A.mass = d.mass;
     A.rc2 = prm.rc2; A.tab = make_tab(prm); A.dt = prm.dt; A.hdt = prm.hdt;
     A.do_vv2 = do_vv2; A.do_vv1 = do_vv1;
@@ -1407,7 +1418,10 @@ static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, cons
                            int do_vv2, int do_vv1, double *eatom) {
     const size_t lds = (size_t) d.tile_cap * sizeof(int);
     const LJArgs A = lj_args(d, g, prm, cur, next, do_vv2, do_vv1, MODE == 1, eatom);
-    hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    if (d.rev0 > 0)
+        hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED, true, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    else
+        hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED, true, false>), dim3(g.nsegs), dim3(FB), lds, s, A);
 }
 
 void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,
@@ -1448,37 +1462,9 @@ void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJPar
     else lj_full_nt<0>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
 }
 
-template <int NT>
-static void lj_half_nt(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce) {
-    const size_t lds = (size_t) d.tile_cap * sizeof(int);
-    const LJTab tab = make_tab(prm);
-    const int c = cur;
-    if (reduce)
-        hipLaunchKernelGGL((k_lj_half<LJ_UNROLL, NT, true>), dim3(g.nsegs), dim3(FB), lds, s, d.counters, g, d.lstart,
-                           d.gstart, d.X[c], d.Y[c], d.Z[c], d.type, d.nl, d.nl_stride, d.numneighs, d.flags, d.fx,
-                           d.fy, d.fz, prm.rc2, tab, d.partials, d.pstride, d.tile_cap);
-    else
-        hipLaunchKernelGGL((k_lj_half<LJ_UNROLL, NT, false>), dim3(g.nsegs), dim3(FB), lds, s, d.counters, g,
-                           d.lstart, d.gstart, d.X[c], d.Y[c], d.Z[c], d.type, d.nl, d.nl_stride, d.numneighs, d.flags,
-                           d.fx, d.fy, d.fz, prm.rc2, tab, d.partials, d.pstride, d.tile_cap);
-}
-
-void launch_lj_half(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce) {
-    if (d.nlocal <= 0) return;
-    launch_fill_f64(s, d.fx, 0.0, d.nlocal);
-    launch_fill_f64(s, d.fy, 0.0, d.nlocal);
-    launch_fill_f64(s, d.fz, 0.0, d.nlocal);
-    if (prm.ntypes == 1) lj_half_nt<1>(s, d, g, prm, cur, reduce);
-    else lj_half_nt<0>(s, d, g, prm, cur, reduce);
-}
-
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom) {
     if (d.nlocal <= 0) return;
-    if (half) {
-        // the half-list kernel recomputes (and re-adds) forces; energies come from its partials
-        launch_lj_half(s, d, g, prm, d.cur, 1);
-        return;
-    }
+    (void) half;  // half lists: the kernel adds the mirror part (d.rev0 > 0)
     launch_pack_xy(s, d, d.cur);
     if (prm.ntypes == 1) lj_full_launch<1, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
     else lj_full_launch<0, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);

@@ -106,6 +106,11 @@ struct DevState {
     int *alt_type, *alt_flags, *alt_uid, *alt_shape, *alt_ref, *alt_cell;
     int *numneighs;
     uint16_t *nl;  // entry k of physical local i at nl_at(k, i, nl_stride): tile offset
+    // half lists: rows [rev0, 2 rev0) hold each local's mirror part (the
+    // earlier-index locals whose half list holds it, by reference index);
+    // rev0 = 0 for full lists
+    int rev0;
+    int *nrev;
     int *particle_cell, *slot, *perm;
     int *lcount, *gcount, *lstart, *gstart;  // per-cell counts and CSR starts (locals / ghosts)
     int *send_map, *send_mult, *exchg_flag;
@@ -147,7 +152,6 @@ void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int gr
 // XY[next] for every local and (images on) every ghost.
 void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
                     int do_vv2, int do_vv1, int reduce, int pack = 1, hipEvent_t after_pack = nullptr);
-void launch_lj_half(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int reduce);
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom);
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm);
 void launch_vv_post(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
@@ -169,6 +173,8 @@ struct LJArgs {
     const uint16_t *nl;
     int nl_stride;
     const int *numneighs, *flags;
+    const int *nrev;     // half lists: length of each particle's mirror part
+    int rev0;            // half lists: first list row of the mirror part
     double *fx, *fy, *fz, *vx, *vy, *vz;
     const double *mass;
     double rc2, dt, hdt;

@@ -56,7 +56,7 @@ static void free_particle_arrays(DevState &d) {
     for (auto p : dd) { dfree(*p); *p = nullptr; }
     int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
                   &d.alt_shape, &d.alt_ref, &d.alt_cell, &d.numneighs, &d.particle_cell, &d.slot, &d.perm,
-                  &d.exchg_flag, &d.tile_cnt, &d.sync_src, &d.sync_mult, &d.img_cnt, &d.img_slot};
+                  &d.exchg_flag, &d.tile_cnt, &d.sync_src, &d.sync_mult, &d.img_cnt, &d.img_slot, &d.nrev};
     for (auto p : ii) { dfree(*p); *p = nullptr; }
 }
 
@@ -69,7 +69,7 @@ static int alloc_particle_arrays(pairs_hip_ctx *ctx, DevState &d, int cap) {
     for (auto p : dd) r |= dalloc(ctx, p, cap);
     int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
                   &d.alt_shape, &d.alt_ref, &d.alt_cell, &d.numneighs, &d.particle_cell, &d.slot, &d.perm,
-                  &d.exchg_flag, &d.sync_src};
+                  &d.exchg_flag, &d.sync_src, &d.nrev};
     for (auto p : ii) r |= dalloc(ctx, p, cap);
     r |= dalloc(ctx, &d.sync_mult, 3 * (size_t) cap);
     r |= dalloc(ctx, &d.img_cnt, cap);
@@ -86,7 +86,10 @@ static int alloc_neighbor_arrays(pairs_hip_ctx *ctx, DevState &d, int ncap) {
     dfree(d.nl);
     d.nl = nullptr;
     d.nl_stride = d.cap;
-    if (dalloc(ctx, &d.nl, (size_t) ((ncap + 3) / 4) * 4 * d.nl_stride)) return PAIRS_HIP_ERR_HIP;
+    // half lists: as many rows again for the mirror parts
+    const int rows = (ncap + 3) / 4 * 4;
+    d.rev0 = ctx->cfg.half ? rows : 0;
+    if (dalloc(ctx, &d.nl, (size_t) rows * (ctx->cfg.half ? 2 : 1) * d.nl_stride)) return PAIRS_HIP_ERR_HIP;
     d.ncap = ncap;
     return 0;
 }
@@ -738,11 +741,8 @@ int pairs_hip_build_neighbors(pairs_hip_ctx *ctx) {
 int pairs_hip_lj(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     DevState &d = ctx->d;
-    if (ctx->cfg.half) launch_lj_half(ctx->stream, d, ctx->geom, ctx->prm, d.cur, 0);
-    else {
-        launch_lj_full(ctx->stream, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0, !ctx->xy_valid[d.cur]);
-        ctx->xy_valid[d.cur] = true;
-    }
+    launch_lj_full(ctx->stream, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0, !ctx->xy_valid[d.cur]);
+    ctx->xy_valid[d.cur] = true;
     CHECK_LAUNCH();
     return 0;
 }
@@ -909,7 +909,6 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
         launch_sync_inplace(s, d, ctx->geom, ctx->ghost_grid, 0);
     }
     if (ctx->mlp_on) launch_mlp_forces(s, d, ctx->geom, ctx->mlp, d.cur, 0, ctx->ghost_grid);
-    else if (ctx->cfg.half) launch_lj_half(s, d, ctx->geom, ctx->prm, d.cur, 0);
     else {
         launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0, 1);
         ctx->xy_valid[d.cur] = true;
@@ -923,17 +922,14 @@ int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
 //   sync ghosts of ts into buffer cur (stale multi-hop values from the other)
 //   force(ts) + final_integrate(ts) + initial_integrate(ts+1) -> other buffer
 // force(ts) on buffer cur, then final_integrate(ts) + initial_integrate(ts+1)
-// into buffer next (fused into the LJ kernel for full lists)
+// into buffer next (fused into the LJ kernel; half lists walk their mirror
+// part in the same kernel)
 static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, int vv1, int reduce, bool timed) {
     DevState &d = ctx->d;
     hipStream_t s = ctx->stream;
-    if (timed && (ctx->mlp_on || ctx->cfg.half)) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
+    if (timed && ctx->mlp_on) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
     if (ctx->mlp_on) {
         launch_mlp_forces(s, d, ctx->geom, ctx->mlp, cur, reduce, ctx->ghost_grid);
-        if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-        launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
-    } else if (ctx->cfg.half) {
-        launch_lj_half(s, d, ctx->geom, ctx->prm, cur, reduce);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
         launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
     } else {
@@ -1037,7 +1033,7 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             int r = graph_step(ctx, d.cur);
             if (r) return r;
             d.cur = 1 - d.cur;
-            ctx->xy_valid[d.cur] = !ctx->mlp_on && !ctx->cfg.half;  // the replayed LJ kernel wrote it
+            ctx->xy_valid[d.cur] = !ctx->mlp_on;  // the replayed LJ kernel wrote it
             continue;
         }
         if (R) {
@@ -1058,7 +1054,7 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         }
         if (timed) ctx->ev_used++;
         if (T && nthermo < 4096) {
-            const int nb_ke = (ctx->cfg.half || ctx->mlp_on) ? (d.nlocal + 255) / 256 : ctx->geom.nsegs;
+            const int nb_ke = ctx->mlp_on ? (d.nlocal + 255) / 256 : ctx->geom.nsegs;
             launch_thermo_reduce(s, d, nb_ke, 3, ctx->geom, d.thermo + 5 * nthermo);
             nthermo++;
         }

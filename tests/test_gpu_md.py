@@ -17,7 +17,6 @@ from oracle import pyoracle as po
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-REL = 1e-10  # FP64 tolerance of the half-list (atomic) path
 
 
 def _md():
@@ -33,16 +32,13 @@ def _rel(a, b):
     return np.abs(a - b).max() / scale
 
 
-def _compare_state(g, o, exact, tag):
+def _compare_state(g, o, tag):
+    """bit-identical positions (ghosts included), velocities and forces; for
+    half lists too: the mirror forces are summed in the reference's order"""
     assert g.nlocal == o.nlocal and g.nghost == o.nghost, tag
-    if exact:
-        assert np.array_equal(g.positions(True), o.positions(True)), tag
-        assert np.array_equal(g.velocities(), o.velocities()), tag
-        assert np.array_equal(g.forces(), o.forces()), tag
-    else:
-        assert _rel(g.positions(True), o.positions(True)) <= REL, tag
-        assert _rel(g.velocities(), o.velocities()) <= REL, tag
-        assert _rel(g.forces(), o.forces()) <= REL, tag
+    for name, a, b in (("position", g.positions(True), o.positions(True)), ("velocity", g.velocities(), o.velocities()),
+                       ("force", g.forces(), o.forces())):
+        assert np.array_equal(a, b), f"{tag} {name}: max |diff| {np.abs(a - b).max()} (rel {_rel(a, b):.3g})"
 
 
 def _compare_lists(g, o, tag):
@@ -69,7 +65,7 @@ def test_modules_step_by_step_vs_oracle_fcc4(half):
     for ts in range(101):
         g.timestep(ts, 100)
         o.step(ts)
-        _compare_state(g, o, exact=not half, tag=f"ts={ts}")
+        _compare_state(g, o, tag=f"ts={ts}")
         if (ts + 1) % 20 == 0 or ts == 0:
             _compare_lists(g, o, tag=f"ts={ts}")
 
@@ -88,20 +84,12 @@ def test_fused_run_vs_golden_fixture(name, half):
         ts = stop + 1
         x, v, f = g.positions(), g.velocities(), g.forces()
         if f"s{stop}_position" in z.files:
-            if half:
-                assert _rel(x, z[f"s{stop}_position"]) <= REL, stop
-                assert _rel(v, z[f"s{stop}_velocity"]) <= REL, stop
-                assert _rel(f, z[f"s{stop}_force"]) <= REL, stop
-            else:
-                assert np.array_equal(x, z[f"s{stop}_position"]), stop
-                assert np.array_equal(v, z[f"s{stop}_velocity"]), stop
-                assert np.array_equal(f, z[f"s{stop}_force"]), stop
+            assert np.array_equal(x, z[f"s{stop}_position"]), stop
+            assert np.array_equal(v, z[f"s{stop}_velocity"]), stop
+            assert np.array_equal(f, z[f"s{stop}_force"]), stop
         if f"s{stop}_ghost_position" in z.files:
             gp = g.positions(True)[g.nlocal:]
-            if half:
-                assert _rel(gp, z[f"s{stop}_ghost_position"]) <= REL, stop
-            else:
-                assert np.array_equal(gp, z[f"s{stop}_ghost_position"]), stop
+            assert np.array_equal(gp, z[f"s{stop}_ghost_position"]), stop
         if stop in (0, 99):
             row = z["thermo_printed"][0 if stop == 0 else 1]
             assert len(thermo) >= 1
@@ -116,7 +104,7 @@ def test_fused_run_vs_oracle_fcc8_300_steps(graphs):
     o = po.fcc_system(8)
     th = g.run(0, 301, 300)
     o.run(0, 301)
-    _compare_state(g, o, exact=True, tag="fcc8")
+    _compare_state(g, o, tag="fcc8")
     _compare_lists(g, o, tag="fcc8")
     # thermo records at ts = 0, 99, 199, 299; the last one against the oracle's
     assert len(th) == 4
@@ -130,7 +118,7 @@ def test_half_list_fcc8_vs_oracle():
     o = po.fcc_system(8, half=True)
     g.run(0, 201, 200)
     o.run(0, 201)
-    _compare_state(g, o, exact=False, tag="fcc8-half")
+    _compare_state(g, o, tag="fcc8-half")
     _compare_lists(g, o, tag="fcc8-half")
 
 
@@ -142,15 +130,16 @@ def test_resize_protocol_small_capacities():
     o = po.fcc_system(4)
     g.run(0, 41, 40)
     o.run(0, 41)
-    _compare_state(g, o, exact=True, tag="resize")
+    _compare_state(g, o, tag="resize")
     _compare_lists(g, o, tag="resize")
     assert g.count(6) >= g.nsend_all and g.count(4) > 8
 
 
-def test_energy_and_virial_vs_oracle():
+@pytest.mark.parametrize("half", [False, True])
+def test_energy_and_virial_vs_oracle(half):
     md = _md()
-    g = md.fcc_system(6)
-    o = po.fcc_system(6)
+    g = md.fcc_system(6, half=half)
+    o = po.fcc_system(6, half=half)
     g.run(0, 51, 50)
     o.run(0, 51)
     pe_g, w_g = g.energy()
