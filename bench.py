@@ -212,16 +212,30 @@ def run_dem(steps):
     }
 
 
-def run_cpu_baseline(sample_steps):
-    """The oracle (single-threaded C restatement of the reference's generated
-    CPU program) on a bounded sample of the same workload."""
+def run_cpu_baseline(sample_steps, threads):
+    """The oracle (C restatement of the reference's generated CPU program) on a
+    bounded sample of the same workload, with the given number of OpenMP
+    threads (1 = the reference's serial target_cpu())."""
     from oracle import pyoracle as po
 
+    po.set_threads(threads)
     s = po.fcc_system(NX)
     t0 = time.perf_counter()
     s.run(0, sample_steps)
     t1 = time.perf_counter()
     return NATOMS * sample_steps / (t1 - t0), t1 - t0
+
+
+def host_threads():
+    """Cores for the parallel CPU baseline: OMP_NUM_THREADS when set, else the
+    cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def main():
@@ -314,10 +328,14 @@ def main():
     if not args.no_dem:
         rec["dem"] = run_dem(args.dem_steps)
     if not args.no_cpu:
-        v, secs = run_cpu_baseline(args.cpu_steps)
-        rec["cpu_baseline"] = {"value": v, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"same workload, iterations ts=0..{args.cpu_steps - 1} "
-                                         f"({secs:.1f} s, includes {args.cpu_steps // 20} rebuilds), one thread"}
+        v1, secs1 = run_cpu_baseline(args.cpu_steps, 1)
+        nt = host_threads()
+        vp, secsp = run_cpu_baseline(args.cpu_steps, nt) if nt > 1 else (v1, secs1)
+        sample = f"same workload, iterations ts=0..{args.cpu_steps - 1} (includes {args.cpu_steps // 20} rebuilds)"
+        rec["cpu_baseline"] = {"value": vp, "unit": "atom-steps/s", "cores": nt, "kind": "port",
+                               "sample": f"{sample}: {secsp:.1f} s on {nt} OpenMP threads "
+                                         "(neighbour build, force and integration loops)",
+                               "serial": {"value": v1, "cores": 1, "seconds": secs1}}
     print(json.dumps(rec), flush=True)
 
 

@@ -25,6 +25,7 @@
  *     force over explicit images.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -267,6 +268,8 @@ static void build_cells(omd_t *s) {
 static void build_neighbors(omd_t *s) {
     const double rc2 = s->spacing * s->spacing;
     for (int i = 0; i < s->nlocal; i++) s->numneighs[i] = 0;
+    /* each i fills only its own list: the threads' results equal the serial loop's */
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < s->nlocal; i++) {
         if (s->flags[i] & FLAG_FIXED) continue;
         int ci = s->particle_cell[i];
@@ -288,8 +291,12 @@ static void build_neighbors(omd_t *s) {
                 if (rsq < rc2) {
                     int n = s->numneighs[i];
                     s->neighborlists[(size_t) i * s->ncap + n] = j;
-                    if (n + 1 + 1 >= s->ncap) s->overflow |= 4;
-                    else s->numneighs[i] = n + 1;
+                    if (n + 1 + 1 >= s->ncap) {
+                        #pragma omp atomic
+                        s->overflow |= 4;
+                    } else {
+                        s->numneighs[i] = n + 1;
+                    }
                 }
             }
         }
@@ -434,6 +441,9 @@ static void lennard_jones(omd_t *s) {
     for (int i = 0; i < s->nlocal; i++) {
         s->f[3 * i] = 0.0; s->f[3 * i + 1] = 0.0; s->f[3 * i + 2] = 0.0;  /* reset_volatile_properties */
     }
+    /* full lists: each i writes only force[i], so threads may split the loop;
+       half lists add mirror forces to other particles in loop order (serial) */
+    #pragma omp parallel for schedule(static) if (!s->half)
     for (int i = 0; i < s->nlocal; i++) {
         if (s->flags[i] & FLAG_FIXED) continue;
         double t0 = 0.0, t1 = 0.0, t2 = 0.0;
@@ -471,6 +481,7 @@ static void lennard_jones(omd_t *s) {
 /* initial_integrate (examples/md.py:11-13) */
 static void initial_integrate(omd_t *s) {
     const double hdt = s->dt * 0.5;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < s->nlocal; i++) {
         if (s->flags[i] & FLAG_FIXED) continue;
         for (int d = 0; d < 3; d++) {
@@ -485,6 +496,7 @@ static void initial_integrate(omd_t *s) {
 /* final_integrate (examples/md.py:16-17) */
 static void final_integrate(omd_t *s) {
     const double hdt = s->dt * 0.5;
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < s->nlocal; i++) {
         if (s->flags[i] & FLAG_FIXED) continue;
         for (int d = 0; d < 3; d++) {
@@ -531,6 +543,10 @@ void omd_step(omd_t *s, int ts) {
         (void) V;
     }
 }
+
+/* threads of the OpenMP loops (1: the reference's serial target_cpu()) */
+void omd_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int omd_threads(void) { return omp_get_max_threads(); }
 
 void omd_run(omd_t *s, int ts_begin, int ts_end) {
     for (int ts = ts_begin; ts < ts_end; ts++) omd_step(s, ts);

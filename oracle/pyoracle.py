@@ -58,8 +58,19 @@ def lib():
         L.omd_last_temperature.argtypes = [ctypes.c_void_p]
         L.omd_last_pressure.restype = ctypes.c_double
         L.omd_last_pressure.argtypes = [ctypes.c_void_p]
+        L.omd_set_threads.argtypes = [ctypes.c_int]
+        L.omd_threads.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the LJ oracle's per-particle loops (1 = serial)."""
+    lib().omd_set_threads(int(n))
+
+
+def threads():
+    return lib().omd_threads()
 
 
 def _dp(a):

@@ -18,9 +18,8 @@ SOURCES = ["md_kernels.hip", "dem_kernels.hip", "mlp_kernels.hip", "pairs_hip.cp
 # -ffp-contract=off: every multiply and add rounds separately, as in the
 #   reference's generated CPU code (bit-exact parity of the FP64 paths).
 # -packed-fp32-ops off: no v_pk_{add,mul,fma}_f32 generated from float code.
-# -munsafe-fp-atomics: hardware FP64 atomic add for the half-list forces.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}", "-ffp-contract=off",
-         "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-command-line-argument"]
 
 

@@ -298,9 +298,9 @@ def _canonical(stmts):
 #
 # Each definition is what a hand-written kernel computes, written in the
 # DSL with its operation order (the traced graph fixes the order):
-#   lennard_jones       -> k_lj_full / k_lj_half       (md_kernels.hip)
+#   lennard_jones       -> k_lj_full (full and half lists) (md_kernels.hip)
 #   initial/final VV    -> fused into k_lj_full, k_vv_* (md_kernels.hip)
-#   linear_spring_dashpot, gravity -> k_contacts    (dem_kernels.hip)
+#   linear_spring_dashpot, gravity -> k_contact_* + k_contacts (dem_kernels.hip)
 #   euler               -> k_euler                    (dem_kernels.hip)
 #   update_mass_and_inertia -> k_mass_inertia         (dem_kernels.hip)
 
