@@ -705,31 +705,49 @@ static void set_tile_cap(pairs_hip_ctx *ctx) {
     ctx->d.tile_cap = std::min(std::max(cap, ctx->d.tile_cap), lim);
 }
 
-static int build_neighbors(pairs_hip_ctx *ctx) {
+static void enqueue_build(pairs_hip_ctx *ctx) {
     const double rl = ctx->cfg.cell_spacing;
+    launch_set_counters(ctx->stream, ctx->d.counters, C_TILE_MAX, 1, 0);
+    launch_build_neighbors(ctx->stream, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
+}
+
+// a segment's tile outgrew the LDS entries the build staged (counters just
+// read): stage more, or cut the segments when a workgroup cannot hold it
+static int grow_tile(pairs_hip_ctx *ctx) {
+    launch_set_counters(ctx->stream, ctx->d.counters, C_OVF, 1, 0);
+    drop_graphs(ctx);
+    const int tmax = ctx->h_counters[C_TILE_MAX];
+    if (tmax <= build_tile_limit(ctx->d, ctx->geom, ctx->cfg.half) && tmax > ctx->d.tile_cap) {
+        set_tile_cap(ctx);
+        return 0;
+    }
+    if (ctx->geom.zc <= 1) { ctx->err = "cell tile exceeds the workgroup's LDS"; return PAIRS_HIP_ERR_STATE; }
+    setup_segments(ctx, ctx->geom.zc - 1);
+    return 0;
+}
+
+// the build enqueued, counters read: repeat the build while its tile grows,
+// then report the other overflows
+static int finish_build(pairs_hip_ctx *ctx) {
     for (int attempt = 0; attempt < 64; attempt++) {
-        launch_set_counters(ctx->stream, ctx->d.counters, C_TILE_MAX, 1, 0);
-        launch_build_neighbors(ctx->stream, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
+        if (ctx->h_counters[C_OVF] != OVF_TILE) {
+            set_tile_cap(ctx);
+            return take_overflow(ctx);
+        }
+        if (int r = grow_tile(ctx)) return r;
+        enqueue_build(ctx);
         CHECK_LAUNCH();
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
-        if (ctx->h_counters[C_OVF] == OVF_TILE) {
-            launch_set_counters(ctx->stream, ctx->d.counters, C_OVF, 1, 0);
-            drop_graphs(ctx);
-            const int tmax = ctx->h_counters[C_TILE_MAX];
-            if (tmax <= build_tile_limit(ctx->d, ctx->geom, ctx->cfg.half) && tmax > ctx->d.tile_cap) {
-                set_tile_cap(ctx);  // the tile fits a workgroup: stage more entries
-                continue;
-            }
-            // tile too large for one workgroup: fewer cells per segment
-            if (ctx->geom.zc <= 1) { ctx->err = "cell tile exceeds the workgroup's LDS"; return PAIRS_HIP_ERR_STATE; }
-            setup_segments(ctx, ctx->geom.zc - 1);
-            continue;
-        }
-        set_tile_cap(ctx);
-        return take_overflow(ctx);
     }
     ctx->err = "neighbour build did not converge";
     return PAIRS_HIP_ERR_STATE;
+}
+
+static int build_neighbors(pairs_hip_ctx *ctx) {
+    enqueue_build(ctx);
+    CHECK_LAUNCH();
+    if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+    return finish_build(ctx);
 }
 
 int pairs_hip_build_neighbors(pairs_hip_ctx *ctx) {
@@ -874,14 +892,20 @@ static int rebuild(pairs_hip_ctx *ctx) {
     if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     for (int attempt = 0; attempt < 8; attempt++) {
         if (int r = launch_rebuild_cells(ctx)) return r;
+        // the neighbour build runs right behind the cell phase, with one
+        // counter read for both: every rebuild kernel returns at once when an
+        // earlier one has set an overflow, and the whole rebuild is repeated
+        // with grown capacities
+        enqueue_build(ctx);
+        CHECK_LAUNCH();
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
         // images refresh the ghosts from the position kernels unless some
         // local has too many of them (then a separate refresh runs)
         ctx->d.img_on = ctx->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
-        int code = take_overflow(ctx);
+        int code = ctx->h_counters[C_OVF] == OVF_TILE ? finish_build(ctx) : take_overflow(ctx);
         if (code == 0) {
-            code = build_neighbors(ctx);
-            if (code == 0 && ctx->mlp_on) code = ensure_mlp_work(ctx);
+            set_tile_cap(ctx);
+            if (ctx->mlp_on) code = ensure_mlp_work(ctx);
             if (code <= 0) return code;
         }
         int pc = 0, nc = 0, sc = 0;
