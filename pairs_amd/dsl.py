@@ -261,6 +261,15 @@ def trace(fn_or_src, properties, symbols, name=None):
         ns[p] = _Prop(p)
     for s in symbols:
         ns[s] = _Node("sym", s)
+    # older-API pair modules read the pair's squared distance and separation
+    # as bare names (examples/lj_onetype.py:5-8)
+    if len(fdef.args.args) == 2:
+        called = {n.func.id for n in ast.walk(fdef) if isinstance(n, ast.Call) and isinstance(n.func, ast.Name)}
+        used = {n.id for n in ast.walk(fdef) if isinstance(n, ast.Name) and isinstance(n.ctx, ast.Load)}
+        if "rsq" in used and "rsq" not in ns:
+            ns["rsq"] = _Node("squared_distance", _Node("i"), _Node("j"))
+        if "delta" in used and "delta" not in called:
+            ns["delta"] = _Node("delta", _Node("i"), _Node("j"))
     ns["__builtins__"] = {}
     exec(compile(mod, f"<module {fdef.name}>", "exec"), ns)
     f = ns[fdef.name]
@@ -449,7 +458,7 @@ class Simulation:
         self.position_name = None
         self.domain = None
         self.partitioner = None
-        self.periodic = [True, True, True]
+        self.pbc_flags = [True, True, True]
         self.setups = []            # ("fcc", ...), ("sc_grid", ...), ("read", ...)
         self.setup_modules = []     # (fn, symbols)
         self.modules = []           # dict(fn, cutoff, symbols, pre_step, skip_first)
@@ -474,7 +483,7 @@ class Simulation:
         self.half = True
 
     def pbc(self, config):
-        self.periodic = [bool(x) for x in config]
+        self.pbc_flags = [bool(x) for x in config]
 
     def add_position(self, name, value=(0.0, 0.0, 0.0), volatile=False, layout=None):
         self.position_name = name
@@ -529,6 +538,48 @@ class Simulation:
     def setup(self, func, symbols={}):
         self.setup_modules.append((func, dict(symbols)))
 
+    # ---- the older API (examples/lj_onetype.py:29-45, lj_embedded.py:13-30).
+    # The reference's current Simulation defines none of these calls; they
+    # are thin aliases onto the calls above, so such scripts configure.
+    # Their modules are traced and recognised like any other: modules
+    # outside this build's kernels fail at generate() with the reason.
+    def add_real_property(self, name, value=0.0, vol=False):
+        self.add_property(name, _REAL, value, vol)
+        return _Prop(name)
+
+    def add_int_property(self, name, value=0, vol=False):
+        self.add_property(name, _INT32, value, vol)
+        return _Prop(name)
+
+    def add_vector_property(self, name, value=(0.0, 0.0, 0.0), vol=False):
+        self.add_property(name, _VECTOR, value, vol)
+        return _Prop(name)
+
+    def add_matrix_property(self, name, value=(0.0,) * 9, vol=False):
+        self.add_property(name, _MATRIX, value, vol)
+        return _Prop(name)
+
+    def from_file(self, filename, prop_names, shape=None):
+        self.read_particle_data(filename, prop_names, shape or _POINT_MASS)
+
+    def create_cell_lists(self, spacing, cutoff_radius=None):
+        self.build_cell_lists(spacing)
+
+    def create_neighbor_lists(self):
+        if self.cell_spacing is None:
+            raise UnsupportedModule("create_neighbor_lists() needs create_cell_lists() first")
+        self.neighbor_radius = self.cell_spacing
+
+    def periodic(self, cutoff_radius=None, flags=(True, True, True)):
+        self.pbc(flags)
+
+    def particle_pairs(self, *args, **kw):
+        raise UnsupportedModule("loop-style kernels (particle_pairs / particles) are not traced: write the "
+                                "module as a function of (i, j) or (i) and pass it to compute()")
+
+    def particles(self, *args, **kw):
+        return self.particle_pairs()
+
     # ---- lowering onto the C ABI
     def _plan(self):
         if self._target is not None and self._target.kind != "gpu":
@@ -570,7 +621,7 @@ class Simulation:
                           cutoff=lj["cutoff"], skin=self.neighbor_radius - lj["cutoff"],
                           dt=self.modules[0]["symbols"]["dt"], half=self.half, ntypes=nt, epsilon=eps,
                           sigma6=sg6, reneighbor_every=self.reneighbor, thermo_every=self.thermo_every,
-                          periodic=self.periodic)
+                          periodic=self.pbc_flags)
         sim.copper_fcc_lattice(nx, ny, nz, rho, nt)
         sim.adjust_thermo(temp)
         sim._call("pairs_hip_setup_cells")
@@ -594,7 +645,7 @@ class Simulation:
                       density_particle=grav["densityParticle_SI"], spacing=spacing, initial_velocity=v0, dt=dt,
                       friction=fd, friction_static=fs, restitution=math.exp(con["lnDryResCoeff"]),
                       collision_time=con["collisionTime_SI"])
-        sysm = dem.DEMSystem(tuple(d[1] for d in self.domain), periodic=tuple(self.periodic),
+        sysm = dem.DEMSystem(tuple(d[1] for d in self.domain), periodic=tuple(self.pbc_flags),
                              particle_capacity=min(self.particle_capacity, 4 << 20),
                              contact_capacity=self.neighbor_capacity, send_capacity=200000, params=params,
                              ntypes=ntypes, kappa=con["kappa"], ln_restitution=con["lnDryResCoeff"],
@@ -671,7 +722,7 @@ int main() {{
     c.send_capacity = {max(200000, 8 * nx * ny * nz)};
     c.ncells_capacity = 100000;
     const double len[3] = {{{nx * a!r}, {ny * a!r}, {nz * a!r}}};
-    for (int d = 0; d < 3; d++) {{ c.box_lo[d] = 0.0; c.box_hi[d] = len[d]; c.periodic[d] = {int(self.periodic[0])}; }}
+    for (int d = 0; d < 3; d++) {{ c.box_lo[d] = 0.0; c.box_hi[d] = len[d]; c.periodic[d] = {int(self.pbc_flags[0])}; }}
     c.cell_spacing = {self.neighbor_radius!r};
     c.cutoff = {lj["cutoff"]!r};
     c.dt = {vv["symbols"]["dt"]!r};
@@ -717,7 +768,7 @@ int main() {{
     c.send_capacity = 200000;
     c.ncells_capacity = 1 << 22;
     c.box_hi[0] = {self.domain[0][1]!r}; c.box_hi[1] = {self.domain[1][1]!r}; c.box_hi[2] = {self.domain[2][1]!r};
-    c.periodic[0] = {int(self.periodic[0])}; c.periodic[1] = {int(self.periodic[1])}; c.periodic[2] = {int(self.periodic[2])};
+    c.periodic[0] = {int(self.pbc_flags[0])}; c.periodic[1] = {int(self.pbc_flags[1])}; c.periodic[2] = {int(self.pbc_flags[2])};
     c.cell_width = {self.cell_spacing!r};
     c.dt = {eul["symbols"]["dt"]!r};
     c.density_particle = {grav["symbols"]["densityParticle_SI"]!r};
@@ -754,9 +805,13 @@ _SCRIPT = {"run": True, "emit_cpp": None, "quiet": False}
 _CREATED = []
 
 
-def simulation(ref, shapes, dims=3, timesteps=100, double_prec=False, use_contact_history=False,
+def simulation(ref, shapes=None, dims=3, timesteps=100, double_prec=None, use_contact_history=False,
                particle_capacity=800000, neighbor_capacity=100, debug=False):
-    sim = Simulation(ref, shapes, dims, timesteps, double_prec, use_contact_history, particle_capacity,
+    """pairs.simulation (src/pairs/__init__.py:9-22).  The older API's call
+    (examples/lj_onetype.py:29) names neither shapes nor precision: shapes
+    default to point masses and an unstated precision runs in FP64."""
+    sim = Simulation(ref, shapes if shapes is not None else [_POINT_MASS], dims, timesteps,
+                     True if double_prec is None else double_prec, use_contact_history, particle_capacity,
                      neighbor_capacity, debug)
     _CREATED.append(sim)
     return sim

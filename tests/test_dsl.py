@@ -80,3 +80,29 @@ def test_emitted_host_program_compiles(tmp_path, script):
     r = subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), out],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_older_api_aliases():
+    """The older calls configure a simulation (properties, file input, cells,
+    neighbour lists, periodicity); its modules are traced with rsq / delta as
+    bare names, and generate() names the module this build has no kernel for
+    (Euler-integrated LJ with the older operation order)."""
+    script = os.path.join(HERE, "dsl", "older_api.py")
+    data = os.path.join(HERE, "golden", "minimd_setup_4x4x4_onetype.input")
+    with pytest.raises(dsl.UnsupportedModule, match="module lj is not one of the hot-path modules"):
+        dsl.run_script(script, [data], run=False)
+    sim = dsl._CREATED[-1]
+    assert set(sim.props) == {"mass", "position", "velocity", "force"}
+    assert sim.props["force"][2] is True and sim.position_name == "position"
+    assert sim.setups == [("read", data, ["mass", "position", "velocity"], dsl.point_mass())]
+    assert sim.cell_spacing == 2.8 and sim.neighbor_radius == 2.8 and sim.pbc_flags == [True, True, True]
+    # the pair module traced: 1/rsq, the pair's separation, an update of force[i]
+    form = dsl.trace(sim.modules[0]["fn"], ("force", "mass", "position", "velocity"), ("sigma6", "epsilon"))
+    ids = dict(line.split("=", 1) for line in form.splitlines() if "=" in line and not line.startswith("set"))
+    rsq = next(k for k, v in ids.items() if v == "squared_distance(i,j)")
+    dlt = next(k for k, v in ids.items() if v == "delta(i,j)")
+    assert f"div(const:1.0,{rsq})" in ids.values()
+    assert f"mul({dlt},const:48.0)" in ids.values()  # the older order: delta * 48.0 first
+    assert form.splitlines()[-1].startswith("set[](name:force,i,")
+    with pytest.raises(dsl.UnsupportedModule, match="loop-style kernels"):
+        sim.particle_pairs(2.5, None)
