@@ -1448,7 +1448,7 @@ static void lj_full_nt(hipStream_t s, const DevState &d, const Geom &g, const LJ
     }
 }
 
-static void launch_pack_xy(hipStream_t s, const DevState &d, int cur) {
+void launch_pack_xy(hipStream_t s, const DevState &d, int cur) {
     hipLaunchKernelGGL(k_pack_xy, dim3(std::max(1, std::min((d.cap + 255) / 256, 1024))), dim3(256), 0, s,
                        d.counters, d.X[cur], d.Y[cur], d.XY[cur]);
 }

@@ -131,6 +131,8 @@ struct DevState {
 };
 
 void launch_fill_i32(hipStream_t s, int *p, int v, int n);
+// XY[cur] = (X[cur], Y[cur]) for locals and ghosts
+void launch_pack_xy(hipStream_t s, const DevState &d, int cur);
 void launch_fill_f64(hipStream_t s, double *p, double v, int n);
 void launch_set_counters(hipStream_t s, int *c, int first, int count, int v);
 void launch_identity(hipStream_t s, int *p, int n);

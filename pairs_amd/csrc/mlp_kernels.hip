@@ -233,6 +233,7 @@ struct MLPSegArgs {
     Geom g;
     const int *lstart, *gstart;
     const double *X, *Y, *Z;
+    const double2 *XY;   // (x, y) of X, Y packed: one 16-byte gather
     const uint16_t *nl;
     int nl_stride;
     const int *numneighs;
@@ -293,7 +294,8 @@ __global__ __launch_bounds__(FB) void k_mlp_segment(MLPSegArgs A) {
                 float4 dj[4][4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    xj[q] = A.X[jp[q]]; yj[q] = A.Y[jp[q]]; zj[q] = A.Z[jp[q]];
+                    const double2 pxy = A.XY[jp[q]];
+                    xj[q] = pxy.x; yj[q] = pxy.y; zj[q] = A.Z[jp[q]];
                     if (MODE == 1) {
                         const float4 *dp = (const float4 *) (A.D + (size_t) jp[q] * MLP_ND);
 #pragma unroll
@@ -619,6 +621,7 @@ static MLPSegArgs seg_args(const DevState &d, const Geom &g, int cur) {
     A.lstart = d.lstart;
     A.gstart = d.gstart;
     A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];
+    A.XY = d.XY[cur];
     A.nl = d.nl;
     A.nl_stride = d.nl_stride;
     A.numneighs = d.numneighs;
@@ -629,6 +632,7 @@ static MLPSegArgs seg_args(const DevState &d, const Geom &g, int cur) {
 
 void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int cur, float *G) {
     if (d.nlocal <= 0) return;
+    launch_pack_xy(s, d, cur);  // both passes gather (x, y) packed
     MLPSegArgs A = seg_args(d, g, cur);
     A.G = G;
     hipLaunchKernelGGL((k_mlp_segment<0, false>), dim3(g.nsegs), dim3(FB), (size_t) d.tile_cap * sizeof(int), s, A);
