@@ -30,7 +30,8 @@ struct pairs_hip_ctx {
     // graphs of one non-rebuild step, by parity of the current buffer
     bool use_graphs = true;
     hipGraphExec_t step_graph[2] = {nullptr, nullptr};
-    // exchange + borders + cell binning of a rebuild, by position buffer
+    // a rebuild (exchange, borders, cell binning, neighbour build), by
+    // position buffer
     hipGraphExec_t reb_graph[2] = {nullptr, nullptr};
     int graph_ghost_grid[2] = {0, 0};
     int graph_tile_cap[2] = {0, 0};

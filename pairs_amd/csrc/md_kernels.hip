@@ -30,6 +30,13 @@ __global__ void k_set_counters(int *__restrict__ c, int first, int count, int v)
     if (t < count) c[first + t] = v;
 }
 
+// the counters of many systems into one array, [r][C_COUNT]
+__global__ __launch_bounds__(256) void k_gather_counters(const int *const *__restrict__ src, int n,
+                                                         int *__restrict__ dst) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n * C_COUNT) dst[t] = src[t / C_COUNT][t % C_COUNT];
+}
+
 // exclusive scan of one value per thread over a block of BS threads
 template <int BS>
 __device__ __forceinline__ int block_exclusive_scan(int v, int *lds_waves, int *total) {
@@ -81,7 +88,7 @@ __device__ __forceinline__ int cell_of(double x, double y, double z, const Geom 
 
 // BuildCellLists (src/pairs/sim/cell_lists.py:90-134), histogram half: cell
 // of every local and ghost, per-cell counts of locals and of ghosts.
-__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ X, const double *__restrict__ Y,
+__device__ __forceinline__ void k_bin_count_body(const double *__restrict__ X, const double *__restrict__ Y,
                                                    const double *__restrict__ Z, const int *__restrict__ flags,
                                                    const int *__restrict__ counters, Geom g,
                                                    int *__restrict__ cell, int *__restrict__ lcount,
@@ -96,8 +103,16 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ X,
     }
 }
 
+__global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ X, const double *__restrict__ Y,
+                                                   const double *__restrict__ Z, const int *__restrict__ flags,
+                                                   const int *__restrict__ counters, Geom g,
+                                                   int *__restrict__ cell, int *__restrict__ lcount,
+                                                   int *__restrict__ gcount, int *__restrict__ slot) {
+    k_bin_count_body(X, Y, Z, flags, counters, g, cell, lcount, gcount, slot);
+}
+
 // exclusive scans of the local and ghost cell counts -> lstart, gstart [0..ncells]
-__global__ __launch_bounds__(1024) void k_scan_cells(const int *__restrict__ counters, const int *__restrict__ lcount,
+__device__ __forceinline__ void k_scan_cells_body(const int *__restrict__ counters, const int *__restrict__ lcount,
                                                      const int *__restrict__ gcount, int *__restrict__ lstart,
                                                      int *__restrict__ gstart, int ncells) {
     __shared__ int lds[20];
@@ -137,7 +152,13 @@ __global__ __launch_bounds__(1024) void k_scan_cells(const int *__restrict__ cou
     if (threadIdx.x == 0) { lstart[ncells] = carry_l; gstart[ncells] = carry_g; }
 }
 
-__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ counters, const int *__restrict__ cell,
+__global__ __launch_bounds__(1024) void k_scan_cells(const int *__restrict__ counters, const int *__restrict__ lcount,
+                                                     const int *__restrict__ gcount, int *__restrict__ lstart,
+                                                     int *__restrict__ gstart, int ncells) {
+    k_scan_cells_body(counters, lcount, gcount, lstart, gstart, ncells);
+}
+
+__device__ __forceinline__ void k_bin_scatter_body(const int *__restrict__ counters, const int *__restrict__ cell,
                                                      const int *__restrict__ slot, const int *__restrict__ lstart,
                                                      const int *__restrict__ gstart, int *__restrict__ perm) {
     if (counters[C_OVF]) return;
@@ -150,17 +171,23 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ cou
     }
 }
 
+__global__ __launch_bounds__(256) void k_bin_scatter(const int *__restrict__ counters, const int *__restrict__ cell,
+                                                     const int *__restrict__ slot, const int *__restrict__ lstart,
+                                                     const int *__restrict__ gstart, int *__restrict__ perm) {
+    k_bin_scatter_body(counters, cell, slot, lstart, gstart, perm);
+}
+
 // Order the locals and the ghosts of every cell by reference index -- the
 // order in which the CPU program's sequential insertion fills cell_particles.
 // One wave per (cell, locals|ghosts) run: a rank sort of up to 64 keys in
 // registers, a serial insertion sort beyond that.
-__global__ __launch_bounds__(256) void k_sort_cells(const int *__restrict__ counters, const int *__restrict__ lstart,
+// one wave per item (cell c = item / 2; its locals, or its ghosts)
+__device__ __forceinline__ void k_sort_cells_body(const int *__restrict__ counters, const int *__restrict__ lstart,
                                                     const int *__restrict__ gstart, const int *__restrict__ ref,
-                                                    int *__restrict__ perm, int ncells) {
+                                                    int *__restrict__ perm, int ncells, int item) {
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
     const int lane = threadIdx.x & 63;
-    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (item >= 2 * ncells) return;
     const int c = item >> 1, ghosts = item & 1;
     const int b = ghosts ? nlocal + gstart[c] : lstart[c];
@@ -186,9 +213,15 @@ __global__ __launch_bounds__(256) void k_sort_cells(const int *__restrict__ coun
     }
 }
 
+__global__ __launch_bounds__(256) void k_sort_cells(const int *__restrict__ counters, const int *__restrict__ lstart,
+                                                    const int *__restrict__ gstart, const int *__restrict__ ref,
+                                                    int *__restrict__ perm, int ncells) {
+    k_sort_cells_body(counters, lstart, gstart, ref, perm, ncells, blockIdx.x * 4 + (threadIdx.x >> 6));
+}
+
 // Move every particle array into cell order (into the spare arrays) and
 // update phys[reference index].
-__global__ __launch_bounds__(256) void k_permute(const int *__restrict__ counters, const int *__restrict__ perm,
+__device__ __forceinline__ void k_permute_body(const int *__restrict__ counters, const int *__restrict__ perm,
                                                  const double *__restrict__ X, const double *__restrict__ Y,
                                                  const double *__restrict__ Z, double *__restrict__ Xn,
                                                  double *__restrict__ Yn, double *__restrict__ Zn,
@@ -217,7 +250,25 @@ __global__ __launch_bounds__(256) void k_permute(const int *__restrict__ counter
     }
 }
 
-__global__ __launch_bounds__(256) void k_copy_back(const int *__restrict__ counters, const double *__restrict__ avx,
+__global__ __launch_bounds__(256) void k_permute(const int *__restrict__ counters, const int *__restrict__ perm,
+                                                 const double *__restrict__ X, const double *__restrict__ Y,
+                                                 const double *__restrict__ Z, double *__restrict__ Xn,
+                                                 double *__restrict__ Yn, double *__restrict__ Zn,
+                                                 const double *__restrict__ vx, const double *__restrict__ vy,
+                                                 const double *__restrict__ vz, const double *__restrict__ mass,
+                                                 const int *__restrict__ type, const int *__restrict__ flags,
+                                                 const int *__restrict__ uid, const int *__restrict__ shape,
+                                                 const int *__restrict__ ref, const int *__restrict__ cell,
+                                                 double *__restrict__ avx, double *__restrict__ avy,
+                                                 double *__restrict__ avz, double *__restrict__ amass,
+                                                 int *__restrict__ atype, int *__restrict__ aflags,
+                                                 int *__restrict__ auid, int *__restrict__ ashape,
+                                                 int *__restrict__ aref, int *__restrict__ acell,
+                                                 int *__restrict__ phys) {
+    k_permute_body(counters, perm, X, Y, Z, Xn, Yn, Zn, vx, vy, vz, mass, type, flags, uid, shape, ref, cell, avx, avy, avz, amass, atype, aflags, auid, ashape, aref, acell, phys);
+}
+
+__device__ __forceinline__ void k_copy_back_body(const int *__restrict__ counters, const double *__restrict__ avx,
                                                    const double *__restrict__ avy, const double *__restrict__ avz,
                                                    const double *__restrict__ amass, const int *__restrict__ atype,
                                                    const int *__restrict__ aflags, const int *__restrict__ auid,
@@ -237,10 +288,24 @@ __global__ __launch_bounds__(256) void k_copy_back(const int *__restrict__ count
     }
 }
 
+__global__ __launch_bounds__(256) void k_copy_back(const int *__restrict__ counters, const double *__restrict__ avx,
+                                                   const double *__restrict__ avy, const double *__restrict__ avz,
+                                                   const double *__restrict__ amass, const int *__restrict__ atype,
+                                                   const int *__restrict__ aflags, const int *__restrict__ auid,
+                                                   const int *__restrict__ ashape, const int *__restrict__ aref,
+                                                   const int *__restrict__ acell, double *__restrict__ vx,
+                                                   double *__restrict__ vy, double *__restrict__ vz,
+                                                   double *__restrict__ mass, int *__restrict__ type,
+                                                   int *__restrict__ flags, int *__restrict__ uid,
+                                                   int *__restrict__ shape, int *__restrict__ ref,
+                                                   int *__restrict__ cell) {
+    k_copy_back_body(counters, avx, avy, avz, amass, atype, aflags, auid, ashape, aref, acell, vx, vy, vz, mass, type, flags, uid, shape, ref, cell);
+}
+
 // Ghost refresh maps in physical order: the physical slot of each ghost's
 // source and its image multipliers (send_map / send_mult are in reference
 // ghost order: ghost g = reference index nlocal + g).
-__global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ counters, const int *__restrict__ ref,
+__device__ __forceinline__ void k_sync_maps_body(const int *__restrict__ counters, const int *__restrict__ ref,
                                                    const int *__restrict__ phys, const int *__restrict__ send_map,
                                                    const int *__restrict__ send_mult, int *__restrict__ sync_src,
                                                    int *__restrict__ sync_mult) {
@@ -255,6 +320,13 @@ __global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ count
     }
 }
 
+__global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ counters, const int *__restrict__ ref,
+                                                   const int *__restrict__ phys, const int *__restrict__ send_map,
+                                                   const int *__restrict__ send_mult, int *__restrict__ sync_src,
+                                                   int *__restrict__ sync_mult) {
+    k_sync_maps_body(counters, ref, phys, send_map, send_mult, sync_src, sync_mult);
+}
+
 
 // A segment's tile as physical slots: entry p of the tile is particle
 // tslot[p] (the force kernels gather positions from HBM through it, so the
@@ -262,7 +334,7 @@ __global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ count
 // Images of each local: every ghost's chain of sources ends at its root
 // local; a root gets a slot per image (order irrelevant: positions are
 // written to the next buffer from values of the current one).
-__global__ __launch_bounds__(256) void k_img_build(int *__restrict__ counters, const int *__restrict__ sync_src,
+__device__ __forceinline__ void k_img_build_body(int *__restrict__ counters, const int *__restrict__ sync_src,
                                                    int *__restrict__ img_cnt, int *__restrict__ img_slot, int cap) {
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL], ng = counters[C_NGHOST];
@@ -274,6 +346,11 @@ __global__ __launch_bounds__(256) void k_img_build(int *__restrict__ counters, c
         if (k < IMG_MAX) img_slot[(size_t) k * cap + root] = q;
         else atomicMax(&counters[C_IMG_OVF], 1);
     }
+}
+
+__global__ __launch_bounds__(256) void k_img_build(int *__restrict__ counters, const int *__restrict__ sync_src,
+                                                   int *__restrict__ img_cnt, int *__restrict__ img_slot, int cap) {
+    k_img_build_body(counters, sync_src, img_cnt, img_slot, cap);
 }
 
 // Next positions of local i's images (the stale multi-hop refresh of
@@ -341,15 +418,16 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
 }
 
 template <bool HALF>
-__global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ counters, Geom g,
-                                                        const int *__restrict__ lstart,
-                                                        const int *__restrict__ gstart, const double *__restrict__ X,
-                                                        const double *__restrict__ Y, const double *__restrict__ Z,
-                                                        const int *__restrict__ ref, const int *__restrict__ flags,
-                                                        double rc2, uint16_t *__restrict__ nl, int nl_stride,
-                                                        int ncap, int lrows, int tcap, int zt,
-                                                        int *__restrict__ numneighs, int *__restrict__ ctr,
-                                                        int *__restrict__ nrev, int rev0) {
+__device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
+    const int *__restrict__ counters = A.counters;
+    const Geom &g = A.g;
+    const int *__restrict__ lstart = A.lstart, *__restrict__ gstart = A.gstart;
+    const double *__restrict__ X = A.X, *__restrict__ Y = A.Y, *__restrict__ Z = A.Z;
+    const int *__restrict__ ref = A.ref, *__restrict__ flags = A.flags;
+    const double rc2 = A.rc2;
+    uint16_t *__restrict__ nl = A.nl;
+    const int nl_stride = A.nl_stride, ncap = A.ncap, lrows = A.lrows, tcap = A.tcap, zt = A.zt, rev0 = A.rev0;
+    int *__restrict__ numneighs = A.numneighs, *__restrict__ ctr = A.counters_rw, *__restrict__ nrev = A.nrev;
     extern __shared__ char smem[];
     const int HCAP = g.hcap, LBS = g.hcap + 2;
     int *tslot = (int *) smem;                                  // [tcap] physical slot of each tile entry
@@ -364,7 +442,6 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
-    const int seg = xcd_block(blockIdx.x, gridDim.x);
     if (seg >= g.nsegs) return;
     const SegInfo si = seg_info(seg, g);
     const int h0 = lstart[si.ca], h1 = lstart[si.cb];
@@ -585,6 +662,22 @@ __global__ __launch_bounds__(BB) void k_build_neighbors(const int *__restrict__ 
         flush(a, 0, min(s_cnt[a], lr_own), 0);
         if (HALF) flush(a, lr_own, min(s_cnt[HCAP + a], lr_mir), rev0 >> 2);
     }
+}
+
+template <bool HALF>
+__global__ __launch_bounds__(BB) void k_build_neighbors(BuildArgs A) {
+    build_segment<HALF>(A, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// the segments of many independent systems in one launch: block b belongs to
+// system rep[b] and is its segment b - seg_base[rep[b]]
+template <bool HALF>
+__global__ __launch_bounds__(BB) void k_build_neighbors_batch(const BuildArgs *__restrict__ desc,
+                                                              const int *__restrict__ rep,
+                                                              const int *__restrict__ seg_base) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r = rep[b];
+    build_segment<HALF>(desc[r], b - seg_base[r]);
 }
 
 // Expand the 16-bit tile-offset lists into reference indices (for download):
@@ -928,9 +1021,9 @@ __global__ __launch_bounds__(256) void k_ke_partial(const double *__restrict__ v
 
 // Fixed-order sums of the per-block partials (row 0 kinetic over nb_ke blocks,
 // rows 1-2 energy/virial over nb_f blocks); thermo as runtime/thermo.hpp:11-51.
-__global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict__ partials, int pstride, int nb_ke,
-                                                       int nb_f, int which_mask, int natoms, double lx, double ly,
-                                                       double lz, double *__restrict__ out) {
+__device__ __forceinline__ void thermo_reduce_body(const double *__restrict__ partials, int pstride, int nb_ke,
+                                                   int nb_f, int which_mask, int natoms, double lx, double ly,
+                                                   double lz, double *__restrict__ out) {
     __shared__ double red[3][4];
     double s[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nb_ke; b += 256) s[0] += partials[b];
@@ -955,6 +1048,24 @@ __global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict__ partials, int pstride, int nb_ke,
+                                                       int nb_f, int which_mask, int natoms, double lx, double ly,
+                                                       double lz, double *__restrict__ out) {
+    thermo_reduce_body(partials, pstride, nb_ke, nb_f, which_mask, natoms, lx, ly, lz, out);
+}
+
+// the thermo rows of many systems after a fused step with reduction
+// (kinetic energy per segment): out[5 r + k], one workgroup per system
+__global__ __launch_bounds__(256) void k_thermo_reduce_batch(const RepDesc *__restrict__ D, double *__restrict__ out) {
+    const RepDesc &R = D[blockIdx.x];
+    thermo_reduce_body(R.d.partials, R.d.pstride, R.g.nsegs, R.g.nsegs, 3, R.d.nlocal, R.g.len[0], R.g.len[1],
+                       R.g.len[2], out + 5 * (size_t) blockIdx.x);
+}
+
+void launch_thermo_batch(hipStream_t s, const RepDesc *D, int n, double *out) {
+    if (n > 0) hipLaunchKernelGGL(k_thermo_reduce_batch, dim3(n), dim3(256), 0, s, D, out);
+}
+
 // ---------------------------------------------------------------- ghosts / exchange
 
 // Selection predicate of DimensionRanges.ghost_particles
@@ -968,7 +1079,7 @@ __device__ __forceinline__ void ghost_sel(double x, int fl, int dim, const Geom 
 }
 
 // pass 1 of the stable two-sided compaction over reference indices: per-tile counts
-__global__ __launch_bounds__(256) void k_sel_count(const double *__restrict__ Xd, const int *__restrict__ flags,
+__device__ __forceinline__ void k_sel_count_body(const double *__restrict__ Xd, const int *__restrict__ flags,
                                                    const int *__restrict__ phys, int *__restrict__ counters, Geom g,
                                                    int dim, int exch, int *__restrict__ tile_cnt,
                                                    int *__restrict__ exchg_flag) {
@@ -997,9 +1108,16 @@ __global__ __launch_bounds__(256) void k_sel_count(const double *__restrict__ Xd
     }
 }
 
+__global__ __launch_bounds__(256) void k_sel_count(const double *__restrict__ Xd, const int *__restrict__ flags,
+                                                   const int *__restrict__ phys, int *__restrict__ counters, Geom g,
+                                                   int dim, int exch, int *__restrict__ tile_cnt,
+                                                   int *__restrict__ exchg_flag) {
+    k_sel_count_body(Xd, flags, phys, counters, g, dim, exch, tile_cnt, exchg_flag);
+}
+
 // pass 2: exclusive scan of the tile counts (side 0 entries precede side 1),
 // counts per direction, capacity checks.
-__global__ __launch_bounds__(1024) void k_sel_scan(int *__restrict__ counters, int *__restrict__ tile_cnt, int dim,
+__device__ __forceinline__ void k_sel_scan_body(int *__restrict__ counters, int *__restrict__ tile_cnt, int dim,
                                                    int exch, int send_cap, int cap) {
     __shared__ int lds[20];
     if (counters[C_OVF]) return;
@@ -1042,12 +1160,17 @@ __global__ __launch_bounds__(1024) void k_sel_scan(int *__restrict__ counters, i
     }
 }
 
+__global__ __launch_bounds__(1024) void k_sel_scan(int *__restrict__ counters, int *__restrict__ tile_cnt, int dim,
+                                                   int exch, int send_cap, int cap) {
+    k_sel_scan_body(counters, tile_cnt, dim, exch, send_cap, cap);
+}
+
 // pass 3: stable scatter into the send map (reference indices); for borders
 // the ghost is created in the same pass (pack + communicateData self-copy +
 // unpack of comm.py:291-366: uid, type, mass, position + shift, velocity,
 // shape) at physical slot = reference index nlocal + g.  Ghost flags are not
 // sent by the reference; a ghost slot's flags read 0 (zeroed memory).
-__global__ __launch_bounds__(256) void k_sel_scatter(double *__restrict__ X, double *__restrict__ Y,
+__device__ __forceinline__ void k_sel_scatter_body(double *__restrict__ X, double *__restrict__ Y,
                                                      double *__restrict__ Z, double *__restrict__ vx,
                                                      double *__restrict__ vy, double *__restrict__ vz,
                                                      double *__restrict__ mass, int *__restrict__ type,
@@ -1104,6 +1227,18 @@ __global__ __launch_bounds__(256) void k_sel_scatter(double *__restrict__ X, dou
     }
 }
 
+__global__ __launch_bounds__(256) void k_sel_scatter(double *__restrict__ X, double *__restrict__ Y,
+                                                     double *__restrict__ Z, double *__restrict__ vx,
+                                                     double *__restrict__ vy, double *__restrict__ vz,
+                                                     double *__restrict__ mass, int *__restrict__ type,
+                                                     int *__restrict__ uid, int *__restrict__ shape,
+                                                     int *__restrict__ flags, int *__restrict__ ref,
+                                                     int *__restrict__ phys, const int *__restrict__ counters, Geom g,
+                                                     int dim, int exch, const int *__restrict__ tile_off,
+                                                     int *__restrict__ send_map, int *__restrict__ send_mult) {
+    k_sel_scatter_body(X, Y, Z, vx, vy, vz, mass, type, uid, shape, flags, ref, phys, counters, g, dim, exch, tile_off, send_map, send_mult);
+}
+
 // Exchange of one dimension after its send map is built, as a relabeling:
 // the reference packs the leavers (position + shift), fills their holes from
 // the tail (remove_exchanged_particles_pt1/pt2, copy guard "src > 0") and
@@ -1113,7 +1248,7 @@ __global__ __launch_bounds__(256) void k_sel_scatter(double *__restrict__ X, dou
 // into a hole get that hole's index.  pt1's sequential tail walk gives the
 // r-th exchanged particle below nlocal - n the r-th non-exchanged reference
 // index below nlocal (descending).  One workgroup: a rebuild moves few particles.
-__global__ __launch_bounds__(1024) void k_exchange_relabel(double *__restrict__ X, double *__restrict__ Y,
+__device__ __forceinline__ void k_exchange_relabel_body(double *__restrict__ X, double *__restrict__ Y,
                                                            double *__restrict__ Z, int *__restrict__ ref,
                                                            int *__restrict__ phys, const int *__restrict__ counters,
                                                            Geom g, const int *__restrict__ send_map,
@@ -1168,6 +1303,17 @@ __global__ __launch_bounds__(1024) void k_exchange_relabel(double *__restrict__ 
         ref[p] = nr;
         phys[nr] = p;
     }
+}
+
+__global__ __launch_bounds__(1024) void k_exchange_relabel(double *__restrict__ X, double *__restrict__ Y,
+                                                           double *__restrict__ Z, int *__restrict__ ref,
+                                                           int *__restrict__ phys, const int *__restrict__ counters,
+                                                           Geom g, const int *__restrict__ send_map,
+                                                           const int *__restrict__ send_mult,
+                                                           const int *__restrict__ exchg_flag, int *__restrict__ tail,
+                                                           int *__restrict__ copy_to, int *__restrict__ p_leaver,
+                                                           int *__restrict__ p_src) {
+    k_exchange_relabel_body(X, Y, Z, ref, phys, counters, g, send_map, send_mult, exchg_flag, tail, copy_to, p_leaver, p_src);
 }
 
 // Comm.synchronize with the stale multi-hop semantics of the reference (all
@@ -1251,6 +1397,11 @@ void launch_set_counters(hipStream_t s, int *c, int first, int count, int v) {
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, c, first, count, v);
 }
 
+void launch_gather_counters(hipStream_t s, const int *const *src, int n, int *dst) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_counters, dim3((n * C_COUNT + 255) / 256), dim3(256), 0, s, src, n, dst);
+}
+
 static int tile_grid(int cap) { return max(1, min((cap + 255) / 256, 2048)); }
 
 void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g) {
@@ -1282,6 +1433,152 @@ void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g) {
     hipLaunchKernelGGL(k_img_build, dim3(grid), dim3(256), 0, s, d.counters, d.sync_src, d.img_cnt, d.img_slot, d.cap);
 }
 
+// ---------------------------------------------------------------- batched cell phase
+//
+// The cell phase of a rebuild (exchange, borders, binning) of many systems,
+// each kernel one launch over all of them: blockIdx.y is the system, whose
+// arguments come from its descriptor; the x grid is the largest system's and
+// every kernel loops or bounds-checks by the system's own device counts.
+
+#define REP_ARGS                                     \
+    const RepDesc &R = D[blockIdx.y];                \
+    const DevState &d = R.d;                         \
+    const Geom &g = R.g;                             \
+    const int c = d.cur;                             \
+    (void) g;                                        \
+    (void) c
+
+// dim < 0: every system; else only systems periodic in dim
+__global__ void k_set_counters_batch(const RepDesc *__restrict__ D, int dim, int first, int count, int v) {
+    REP_ARGS;
+    if (dim >= 0 && !R.periodic[dim]) return;
+    if ((int) threadIdx.x < count) d.counters[first + threadIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_sel_count_batch(const RepDesc *__restrict__ D, int dim, int exch) {
+    REP_ARGS;
+    if (!R.periodic[dim]) return;
+    const double *Xd = dim == 0 ? d.X[c] : (dim == 1 ? d.Y[c] : d.Z[c]);
+    k_sel_count_body(Xd, d.flags, d.phys, d.counters, g, dim, exch, d.tile_cnt, d.exchg_flag);
+}
+
+__global__ __launch_bounds__(1024) void k_sel_scan_batch(const RepDesc *__restrict__ D, int dim, int exch) {
+    REP_ARGS;
+    if (!R.periodic[dim]) return;
+    k_sel_scan_body(d.counters, d.tile_cnt, dim, exch, d.send_cap, d.cap);
+}
+
+__global__ __launch_bounds__(256) void k_sel_scatter_batch(const RepDesc *__restrict__ D, int dim, int exch) {
+    REP_ARGS;
+    if (!R.periodic[dim]) return;
+    k_sel_scatter_body(d.X[c], d.Y[c], d.Z[c], d.vx, d.vy, d.vz, d.mass, d.type, d.uid, d.shape, d.flags, d.ref,
+                       d.phys, d.counters, g, dim, exch, d.tile_cnt, d.send_map, d.send_mult);
+}
+
+__global__ __launch_bounds__(1024) void k_exchange_relabel_batch(const RepDesc *__restrict__ D, int dim) {
+    REP_ARGS;
+    if (!R.periodic[dim]) return;
+    int *sc = d.scratch_i;
+    k_exchange_relabel_body(d.X[c], d.Y[c], d.Z[c], d.ref, d.phys, d.counters, g, d.send_map, d.send_mult,
+                            d.exchg_flag, sc, sc + d.send_cap, sc + 2 * d.send_cap, sc + 3 * d.send_cap);
+}
+
+// which: 0 lcount, 1 gcount (ncells + 1 entries), 2 img_cnt (cap entries)
+__global__ __launch_bounds__(256) void k_fill_batch(const RepDesc *__restrict__ D, int which, int v) {
+    REP_ARGS;
+    int *p = which == 0 ? d.lcount : (which == 1 ? d.gcount : d.img_cnt);
+    const int n = which == 2 ? d.cap : g.ncells + 1;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    k_bin_count_body(d.X[c], d.Y[c], d.Z[c], d.flags, d.counters, g, d.particle_cell, d.lcount, d.gcount, d.slot);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_cells_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    k_scan_cells_body(d.counters, d.lcount, d.gcount, d.lstart, d.gstart, g.ncells);
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    k_bin_scatter_body(d.counters, d.particle_cell, d.slot, d.lstart, d.gstart, d.perm);
+}
+
+__global__ __launch_bounds__(256) void k_sort_cells_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    for (int item = blockIdx.x * 4 + (threadIdx.x >> 6); item < 2 * g.ncells; item += gridDim.x * 4)
+        k_sort_cells_body(d.counters, d.lstart, d.gstart, d.ref, d.perm, g.ncells, item);
+}
+
+__global__ __launch_bounds__(256) void k_permute_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    const int o = 1 - c;
+    k_permute_body(d.counters, d.perm, d.X[c], d.Y[c], d.Z[c], d.X[o], d.Y[o], d.Z[o], d.vx, d.vy, d.vz, d.mass,
+                   d.type, d.flags, d.uid, d.shape, d.ref, d.particle_cell, d.alt_vx, d.alt_vy, d.alt_vz, d.alt_mass,
+                   d.alt_type, d.alt_flags, d.alt_uid, d.alt_shape, d.alt_ref, d.alt_cell, d.phys);
+}
+
+__global__ __launch_bounds__(256) void k_copy_back_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    k_copy_back_body(d.counters, d.alt_vx, d.alt_vy, d.alt_vz, d.alt_mass, d.alt_type, d.alt_flags, d.alt_uid,
+                     d.alt_shape, d.alt_ref, d.alt_cell, d.vx, d.vy, d.vz, d.mass, d.type, d.flags, d.uid, d.shape,
+                     d.ref, d.particle_cell);
+}
+
+__global__ __launch_bounds__(256) void k_sync_maps_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    k_sync_maps_body(d.counters, d.ref, d.phys, d.send_map, d.send_mult, d.sync_src, d.sync_mult);
+}
+
+__global__ __launch_bounds__(256) void k_img_build_batch(const RepDesc *__restrict__ D) {
+    REP_ARGS;
+    k_img_build_body(d.counters, d.sync_src, d.img_cnt, d.img_slot, d.cap);
+}
+
+#undef REP_ARGS
+
+void launch_cells_batch(hipStream_t s, const RepDesc *D, int n, int max_cap, int max_ncells) {
+    if (n <= 0) return;
+    // every kernel but the single-workgroup ones strides over its system's
+    // range, so a few workgroups per system fill the device
+    const int per = std::max(1, std::min(tile_grid(max_cap), 32));
+    const dim3 one(1, n), tiles(per, n), cells(per, n), fills(per, n);
+    (void) max_ncells;
+    hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, -1, C_IMG_OVF, 1, 0);
+    // exchange (comm.py:105-151), one dimension at a time
+    static_assert(C_NSEND_ALL == C_NGHOST + 1 && C_NSEND == C_NSEND_ALL + 1, "counter layout");
+    for (int dim = 0; dim < 3; dim++) {
+        hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, dim, C_NGHOST, 2, 0);
+        hipLaunchKernelGGL(k_sel_count_batch, tiles, dim3(256), 0, s, D, dim, 1);
+        hipLaunchKernelGGL(k_sel_scan_batch, one, dim3(1024), 0, s, D, dim, 1);
+        hipLaunchKernelGGL(k_sel_scatter_batch, tiles, dim3(256), 0, s, D, dim, 1);
+        hipLaunchKernelGGL(k_exchange_relabel_batch, one, dim3(1024), 0, s, D, dim);
+    }
+    // borders
+    hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, -1, C_NGHOST, 8, 0);
+    for (int dim = 0; dim < 3; dim++) {
+        hipLaunchKernelGGL(k_sel_count_batch, tiles, dim3(256), 0, s, D, dim, 0);
+        hipLaunchKernelGGL(k_sel_scan_batch, one, dim3(1024), 0, s, D, dim, 0);
+        hipLaunchKernelGGL(k_sel_scatter_batch, tiles, dim3(256), 0, s, D, dim, 0);
+    }
+    // cell binning and the permutation into cell order (flips each system's
+    // position buffer: the caller flips d.cur on the host)
+    hipLaunchKernelGGL(k_fill_batch, fills, dim3(256), 0, s, D, 0, 0);
+    hipLaunchKernelGGL(k_fill_batch, fills, dim3(256), 0, s, D, 1, 0);
+    hipLaunchKernelGGL(k_bin_count_batch, tiles, dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_scan_cells_batch, one, dim3(1024), 0, s, D);
+    hipLaunchKernelGGL(k_bin_scatter_batch, tiles, dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_sort_cells_batch, cells, dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_permute_batch, tiles, dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_copy_back_batch, tiles, dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_sync_maps_batch, tiles, dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_fill_batch, fills, dim3(256), 0, s, D, 2, 0);
+    hipLaunchKernelGGL(k_img_build_batch, tiles, dim3(256), 0, s, D);
+    hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, -1, C_TILE_MAX, 1, 0);
+}
+
 int build_zt(const Geom &g) { return g.nchunks == 1 ? g.dim_cells[2] + 2 : g.zc + 3; }
 
 static int build_lrows(const DevState &d) { return min((d.ncap + 3) / 4 * 4, LR_MAX); }
@@ -1299,19 +1596,39 @@ int build_tile_limit(const DevState &d, const Geom &g, int half) {
     return min(lim, 65536);  // tile offsets are 16-bit
 }
 
-void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2) {
-    const int lrows = build_lrows(d);
-    const int tcap = min(d.tile_cap, build_tile_limit(d, g, half));
-    const size_t lds = build_lds(d, g, half, tcap);
+BuildArgs build_args(const DevState &d, const Geom &g, int half, double rc2) {
+    BuildArgs A;
     const int c = d.cur;
-    if (half)
-        hipLaunchKernelGGL(k_build_neighbors<true>, dim3(g.nsegs), dim3(BB), lds, s, d.counters, g, d.lstart,
-                           d.gstart, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, rc2, d.nl, d.nl_stride, d.ncap, lrows,
-                           tcap, build_zt(g), d.numneighs, d.counters, d.nrev, d.rev0);
-    else
-        hipLaunchKernelGGL(k_build_neighbors<false>, dim3(g.nsegs), dim3(BB), lds, s, d.counters, g, d.lstart,
-                           d.gstart, d.X[c], d.Y[c], d.Z[c], d.ref, d.flags, rc2, d.nl, d.nl_stride, d.ncap, lrows,
-                           tcap, build_zt(g), d.numneighs, d.counters, d.nrev, d.rev0);
+    A.counters = d.counters; A.counters_rw = d.counters;
+    A.g = g;
+    A.lstart = d.lstart; A.gstart = d.gstart;
+    A.X = d.X[c]; A.Y = d.Y[c]; A.Z = d.Z[c];
+    A.ref = d.ref; A.flags = d.flags;
+    A.rc2 = rc2;
+    A.nl = d.nl; A.nl_stride = d.nl_stride; A.ncap = d.ncap;
+    A.lrows = build_lrows(d);
+    A.tcap = min(d.tile_cap, build_tile_limit(d, g, half));
+    A.zt = build_zt(g);
+    A.numneighs = d.numneighs; A.nrev = d.nrev; A.rev0 = d.rev0;
+    return A;
+}
+
+size_t build_lds_bytes(const DevState &d, const Geom &g, int half) {
+    return build_lds(d, g, half, min(d.tile_cap, build_tile_limit(d, g, half)));
+}
+
+void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2) {
+    const BuildArgs A = build_args(d, g, half, rc2);
+    const size_t lds = build_lds(d, g, half, A.tcap);
+    if (half) hipLaunchKernelGGL(k_build_neighbors<true>, dim3(g.nsegs), dim3(BB), lds, s, A);
+    else hipLaunchKernelGGL(k_build_neighbors<false>, dim3(g.nsegs), dim3(BB), lds, s, A);
+}
+
+void launch_build_batch(hipStream_t s, const BuildArgs *desc, const int *rep, const int *seg_base, int nblocks,
+                        size_t lds, int half) {
+    if (nblocks <= 0) return;
+    if (half) hipLaunchKernelGGL(k_build_neighbors_batch<true>, dim3(nblocks), dim3(BB), lds, s, desc, rep, seg_base);
+    else hipLaunchKernelGGL(k_build_neighbors_batch<false>, dim3(nblocks), dim3(BB), lds, s, desc, rep, seg_base);
 }
 
 void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out) {

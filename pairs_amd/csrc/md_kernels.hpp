@@ -135,6 +135,8 @@ void launch_fill_i32(hipStream_t s, int *p, int v, int n);
 void launch_pack_xy(hipStream_t s, const DevState &d, int cur);
 void launch_fill_f64(hipStream_t s, double *p, double v, int n);
 void launch_set_counters(hipStream_t s, int *c, int first, int count, int v);
+// dst[r * C_COUNT + k] = src[r][k] for r < n (device array of counter pointers)
+void launch_gather_counters(hipStream_t s, const int *const *src, int n, int *dst);
 void launch_identity(hipStream_t s, int *p, int n);
 void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int *periodic);
 void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *periodic);
@@ -144,6 +146,40 @@ void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g);
 void launch_build_neighbors(hipStream_t s, const DevState &d, const Geom &g, int half, double rc2);
 // largest tile (entries) the build and force kernels can stage in LDS
 int build_tile_limit(const DevState &d, const Geom &g, int half);
+
+// one system of a batched cell phase
+struct RepDesc {
+    DevState d;       // d.cur: the position buffer before the phase
+    Geom g;
+    int periodic[3];
+};
+// exchange, borders and cell binning of n systems, each kernel one launch
+// (blockIdx.y = system); max_cap / max_ncells: the largest system's
+void launch_cells_batch(hipStream_t s, const RepDesc *D, int n, int max_cap, int max_ncells);
+// thermo rows (T, P, PE, W, P total) of n systems after a fused step with
+// reduction, out[5 r + k]
+void launch_thermo_batch(hipStream_t s, const RepDesc *D, int n, double *out);
+
+// the neighbour build's arguments for one system (position buffer d.cur)
+struct BuildArgs {
+    const int *counters;
+    int *counters_rw;
+    Geom g;
+    const int *lstart, *gstart;
+    const double *X, *Y, *Z;
+    const int *ref, *flags;
+    double rc2;
+    uint16_t *nl;
+    int nl_stride, ncap, lrows, tcap, zt, rev0;
+    int *numneighs, *nrev;
+};
+BuildArgs build_args(const DevState &d, const Geom &g, int half, double rc2);
+// dynamic LDS of one system's build workgroup
+size_t build_lds_bytes(const DevState &d, const Geom &g, int half);
+// the builds of many systems in one launch (block b: system rep[b], its
+// segment b - seg_base[rep[b]]); lds = the largest system's need
+void launch_build_batch(hipStream_t s, const BuildArgs *desc, const int *rep, const int *seg_base, int nblocks,
+                        size_t lds, int half);
 void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out);
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid);
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel);

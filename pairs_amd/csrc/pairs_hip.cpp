@@ -23,13 +23,19 @@
 #include "lj_ctx.hpp"
 #include "mlp_model.hpp"
 
+static void drop_reb_graphs(pairs_hip_ctx *ctx) {
+    for (int p = 0; p < 2; p++) {
+        if (ctx->reb_graph[p]) hipGraphExecDestroy(ctx->reb_graph[p]);
+        ctx->reb_graph[p] = nullptr;
+    }
+}
+
 static void drop_graphs(pairs_hip_ctx *ctx) {
     for (int p = 0; p < 2; p++) {
         if (ctx->step_graph[p]) hipGraphExecDestroy(ctx->step_graph[p]);
         ctx->step_graph[p] = nullptr;
-        if (ctx->reb_graph[p]) hipGraphExecDestroy(ctx->reb_graph[p]);
-        ctx->reb_graph[p] = nullptr;
     }
+    drop_reb_graphs(ctx);
 }
 
 template <typename T>
@@ -702,14 +708,18 @@ static void set_tile_cap(pairs_hip_ctx *ctx) {
     int cap = tmax + tmax / 8 + 64;
     cap = (cap + 63) / 64 * 64;
     const int lim = build_tile_limit(ctx->d, ctx->geom, ctx->cfg.half);
-    ctx->d.tile_cap = std::min(std::max(cap, ctx->d.tile_cap), lim);
+    const int grown = std::min(std::max(cap, ctx->d.tile_cap), lim);
+    if (grown != ctx->d.tile_cap) drop_reb_graphs(ctx);  // they stage tile_cap entries
+    ctx->d.tile_cap = grown;
 }
 
-static void enqueue_build(pairs_hip_ctx *ctx) {
+static void enqueue_build(pairs_hip_ctx *ctx, hipStream_t s) {
     const double rl = ctx->cfg.cell_spacing;
-    launch_set_counters(ctx->stream, ctx->d.counters, C_TILE_MAX, 1, 0);
-    launch_build_neighbors(ctx->stream, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
+    launch_set_counters(s, ctx->d.counters, C_TILE_MAX, 1, 0);
+    launch_build_neighbors(s, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
 }
+
+static void enqueue_build(pairs_hip_ctx *ctx) { enqueue_build(ctx, ctx->stream); }
 
 // a segment's tile outgrew the LDS entries the build staged (counters just
 // read): stage more, or cut the segments when a workgroup cannot hold it
@@ -847,43 +857,48 @@ static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 |
 
 // exchange + borders + cells + neighbour lists, retried with grown capacities
 // on overflow like the generated resize loops (transformations/modules.py:159-205)
-static void enqueue_rebuild_cells(pairs_hip_ctx *ctx) {
-    hipStream_t s = ctx->stream;
+// exchange, borders, cell binning and the neighbour build of a rebuild
+static void enqueue_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
     launch_set_counters(s, ctx->d.counters, C_IMG_OVF, 1, 0);
     launch_exchange(s, ctx->d, ctx->geom, ctx->cfg.periodic);
     launch_borders(s, ctx->d, ctx->geom, ctx->cfg.periodic);
     launch_bin_sort(s, ctx->d, ctx->geom);  // flips d.cur
+    // the build runs right behind the cell phase; every rebuild kernel
+    // returns at once when an earlier one has set an overflow
+    enqueue_build(ctx, s);
 }
 
-// exchange + borders + cell binning: about 35 small kernels whose grids
-// depend only on capacities and whose counts live on the device, replayed
-// from a graph (one per position buffer) so that they run back to back
-static int launch_rebuild_cells(pairs_hip_ctx *ctx) {
+// The rebuild's ~37 small kernels have grids that depend only on
+// capacities, and their counts live on the device: they are replayed from a
+// graph (one per position buffer, captured on the context's stream), so they
+// run back to back and cost one launch on the host.
+static int launch_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
     const int cur = ctx->d.cur;
     if (!ctx->use_graphs) {
-        enqueue_rebuild_cells(ctx);
+        enqueue_rebuild(ctx, s);
         CHECK_LAUNCH();
         return 0;
     }
-    if (!ctx->reb_graph[cur]) {
+    hipGraphExec_t *slot = &ctx->reb_graph[cur];
+    if (!*slot) {
         hipGraph_t g;
         HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_rebuild_cells(ctx);
+        enqueue_rebuild(ctx, ctx->stream);
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         ctx->d.cur = cur;
         if (ec != hipSuccess) {
             ctx->err = std::string("rebuild capture: ") + hipGetErrorString(ec);
             return PAIRS_HIP_ERR_HIP;
         }
-        const hipError_t e = hipGraphInstantiate(&ctx->reb_graph[cur], g, nullptr, nullptr, 0);
+        const hipError_t e = hipGraphInstantiate(slot, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) {
             ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
-            ctx->reb_graph[cur] = nullptr;
+            *slot = nullptr;
             return PAIRS_HIP_ERR_HIP;
         }
     }
-    HIP_OK(hipGraphLaunch(ctx->reb_graph[cur], ctx->stream));
+    HIP_OK(hipGraphLaunch(*slot, s));
     ctx->d.cur = 1 - cur;
     return 0;
 }
@@ -891,13 +906,9 @@ static int launch_rebuild_cells(pairs_hip_ctx *ctx) {
 static int rebuild(pairs_hip_ctx *ctx) {
     if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     for (int attempt = 0; attempt < 8; attempt++) {
-        if (int r = launch_rebuild_cells(ctx)) return r;
-        // the neighbour build runs right behind the cell phase, with one
-        // counter read for both: every rebuild kernel returns at once when an
-        // earlier one has set an overflow, and the whole rebuild is repeated
-        // with grown capacities
-        enqueue_build(ctx);
-        CHECK_LAUNCH();
+        // one counter read for the whole rebuild: after an overflow it is
+        // repeated with grown capacities
+        if (int r = launch_rebuild(ctx, ctx->stream)) return r;
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
         // images refresh the ghosts from the position kernels unless some
         // local has too many of them (then a separate refresh runs)
@@ -1250,12 +1261,14 @@ struct pairs_hip_batch {
     std::vector<double> thermo;        // last thermo row per replica
     bool ntypes1 = true;
     int pending_home = 0;              // segment size to apply at the next rebuild
-    // rebuild streams: replica r's rebuild kernels run on rs[r % NRS], so the
-    // small per-replica kernels of different replicas overlap on the device
-    static constexpr int NRS = 4;
-    hipStream_t rs[NRS] = {nullptr, nullptr, nullptr, nullptr};
-    hipEvent_t ev_main = nullptr;      // end of the main stream's work before a rebuild
     int *h_cnt = nullptr;              // pinned, [R][C_COUNT] counters of every replica
+    RepDesc *h_rdesc = nullptr, *d_rdesc = nullptr;    // every replica's state for the batched kernels
+    bool rdesc_valid = false;          // d_rdesc holds every replica's current arrays
+    double *d_thermo = nullptr, *h_thermo = nullptr;   // [R][5] thermo rows
+    BuildArgs *h_bdesc = nullptr, *d_bdesc = nullptr;  // every replica's neighbour-build arguments
+    const int **h_cptr = nullptr, **d_cptr = nullptr;  // every replica's device counters
+    int *d_call = nullptr;             // [R][C_COUNT] gathered counters
+    bool tables_valid = false;         // d_rep / d_base match the replicas' segments
 };
 
 static void batch_drop_graphs(pairs_hip_batch *b) {
@@ -1298,6 +1311,7 @@ static int batch_tables(pairs_hip_batch *b) {
     BATCH_OK(hipStreamSynchronize(b->stream));
     b->nblocks = tot;
     b->max_tile = mt;
+    b->tables_valid = true;
     return 0;
 }
 
@@ -1332,61 +1346,68 @@ static int batch_rebuild(pairs_hip_batch *b) {
         }
         batch_drop_graphs(b);
         b->pending_home = 0;
+        b->tables_valid = false;
     }
-    // the replicas' rebuilds start after the main stream's last step
-    BATCH_OK(hipEventRecord(b->ev_main, s));
-    for (int k = 0; k < pairs_hip_batch::NRS; k++) BATCH_OK(hipStreamWaitEvent(b->rs[k], b->ev_main, 0));
-    auto rstream = [&](int r) { return b->rs[r % pairs_hip_batch::NRS]; };
-    // every replica's counters into one pinned array, then one wait for all
-    auto fetch_counters = [&]() -> int {
-        for (int r = 0; r < R; r++)
-            BATCH_OK(hipMemcpyAsync(b->h_cnt + (size_t) r * C_COUNT, b->rep[r]->d.counters, sizeof(int) * C_COUNT,
-                                    hipMemcpyDeviceToHost, rstream(r)));
-        for (int k = 0; k < pairs_hip_batch::NRS; k++) BATCH_OK(hipStreamSynchronize(b->rs[k]));
-        for (int r = 0; r < R; r++) {
-            std::memcpy(b->rep[r]->h_counters, b->h_cnt + (size_t) r * C_COUNT, sizeof(int) * C_COUNT);
-            take_counters(b->rep[r]);
-        }
-        return 0;
-    };
+    if (!b->tables_valid)
+        if (int e = batch_tables(b)) return e;
+    // every replica's cell phase (exchange, borders, binning), each kernel
+    // one launch over all replicas; then one neighbour build over every
+    // replica's segments
+    int max_cap = 0, max_ncells = 0;
     for (int r = 0; r < R; r++) {
         pairs_hip_ctx *c = b->rep[r];
-        hipStream_t rs = rstream(r);
-        launch_set_counters(rs, c->d.counters, C_IMG_OVF, 1, 0);
-        launch_exchange(rs, c->d, c->geom, c->cfg.periodic);
-        launch_borders(rs, c->d, c->geom, c->cfg.periodic);
-        launch_bin_sort(rs, c->d, c->geom);
+        RepDesc &rd = b->h_rdesc[r];
+        rd.d = c->d;
+        rd.g = c->geom;
+        for (int k = 0; k < 3; k++) rd.periodic[k] = c->cfg.periodic[k] ? 1 : 0;
+        max_cap = std::max(max_cap, c->d.cap);
+        max_ncells = std::max(max_ncells, c->geom.ncells);
+        c->xy_valid[0] = c->xy_valid[1] = false;
     }
-    if (int e = fetch_counters()) return e;
+    BATCH_OK(hipMemcpyAsync(b->d_rdesc, b->h_rdesc, R * sizeof(RepDesc), hipMemcpyHostToDevice, s));
+    b->rdesc_valid = true;
+    launch_cells_batch(s, b->d_rdesc, R, max_cap, max_ncells);
+    for (pairs_hip_ctx *c : b->rep) c->d.cur = 1 - c->d.cur;  // the permutation moved every replica
+    size_t lds = 0;
+    for (int r = 0; r < R; r++) {
+        pairs_hip_ctx *c = b->rep[r];
+        b->h_bdesc[r] = build_args(c->d, c->geom, 0, c->cfg.cell_spacing * c->cfg.cell_spacing);
+        lds = std::max(lds, build_lds_bytes(c->d, c->geom, 0));
+    }
+    BATCH_OK(hipMemcpyAsync(b->d_bdesc, b->h_bdesc, R * sizeof(BuildArgs), hipMemcpyHostToDevice, s));
+    launch_build_batch(s, b->d_bdesc, b->d_rep, b->d_base, b->nblocks, lds, 0);
+    BATCH_OK(hipGetLastError());
+    // every replica's counters gathered on the device, one copy to the host
+    for (int r = 0; r < R; r++) b->h_cptr[r] = b->rep[r]->d.counters;
+    BATCH_OK(hipMemcpyAsync(b->d_cptr, b->h_cptr, R * sizeof(int *), hipMemcpyHostToDevice, s));
+    launch_gather_counters(s, b->d_cptr, R, b->d_call);
+    BATCH_OK(hipMemcpyAsync(b->h_cnt, b->d_call, (size_t) R * C_COUNT * sizeof(int), hipMemcpyDeviceToHost, s));
+    BATCH_OK(hipStreamSynchronize(s));
+    for (int r = 0; r < R; r++) {
+        std::memcpy(b->rep[r]->h_counters, b->h_cnt + (size_t) r * C_COUNT, sizeof(int) * C_COUNT);
+        take_counters(b->rep[r]);
+    }
     std::vector<char> fallback(R, 0);
     for (int r = 0; r < R; r++) {
         pairs_hip_ctx *c = b->rep[r];
         c->d.img_on = c->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
-        if (take_overflow(c)) fallback[r] = 1;
-        else {
-            launch_set_counters(rstream(r), c->d.counters, C_TILE_MAX, 1, 0);
-            launch_build_neighbors(rstream(r), c->d, c->geom, c->cfg.half, c->cfg.cell_spacing * c->cfg.cell_spacing);
-        }
-    }
-    if (int e = fetch_counters()) return e;
-    for (int r = 0; r < R; r++) {
-        pairs_hip_ctx *c = b->rep[r];
-        if (!fallback[r]) {
-            if (c->h_counters[C_OVF] == OVF_TILE || c->h_counters[C_REQ_NEIGH] > 0) {
-                launch_set_counters(s, c->d.counters, C_OVF, 1, 0);
-                launch_set_counters(s, c->d.counters, C_REQ_NEIGH, 3, 0);
-                fallback[r] = 1;
-            } else {
-                set_tile_cap(c);
-            }
+        if (c->h_counters[C_OVF] != 0 || c->h_counters[C_REQ_NEIGH] > 0) {
+            // an overflow anywhere: the replica repeats its rebuild on its
+            // own, growing what it needs
+            launch_set_counters(s, c->d.counters, C_OVF, 1, 0);
+            launch_set_counters(s, c->d.counters, C_REQ_NEIGH, 3, 0);
+            fallback[r] = 1;
+        } else {
+            set_tile_cap(c);
         }
         if (fallback[r]) {
             drop_graphs(c);
+            b->tables_valid = false;  // the replica's segments may change
             const int code = rebuild(c);
             if (code) { b->err = "replica " + std::to_string(r) + ": " + c->err; return code; }
         }
     }
-    int e = batch_tables(b);
+    int e = b->tables_valid ? 0 : batch_tables(b);
     if (!e) e = batch_descs(b);
     return e;
 }
@@ -1445,13 +1466,16 @@ int pairs_hip_batch_create(pairs_hip_batch **out, int nreplicas, const pairs_hip
         pairs_hip_batch_destroy(b);
         return PAIRS_HIP_ERR_HIP;
     }
-    for (int k = 0; k < pairs_hip_batch::NRS; k++)
-        if (hipStreamCreateWithFlags(&b->rs[k], hipStreamNonBlocking) != hipSuccess) {
-            pairs_hip_batch_destroy(b);
-            return PAIRS_HIP_ERR_HIP;
-        }
-    if (hipEventCreateWithFlags(&b->ev_main, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc((void **) &b->h_cnt, R * C_COUNT * sizeof(int)) != hipSuccess) {
+    if (hipHostMalloc((void **) &b->h_cnt, R * C_COUNT * sizeof(int)) != hipSuccess ||
+        hipHostMalloc((void **) &b->h_rdesc, R * sizeof(RepDesc)) != hipSuccess ||
+        hipMalloc((void **) &b->d_rdesc, R * sizeof(RepDesc)) != hipSuccess ||
+        hipHostMalloc((void **) &b->h_thermo, 5 * R * sizeof(double)) != hipSuccess ||
+        hipMalloc((void **) &b->d_thermo, 5 * R * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void **) &b->h_bdesc, R * sizeof(BuildArgs)) != hipSuccess ||
+        hipMalloc((void **) &b->d_bdesc, R * sizeof(BuildArgs)) != hipSuccess ||
+        hipHostMalloc((void **) &b->h_cptr, R * sizeof(int *)) != hipSuccess ||
+        hipMalloc((void **) &b->d_cptr, R * sizeof(int *)) != hipSuccess ||
+        hipMalloc((void **) &b->d_call, R * C_COUNT * sizeof(int)) != hipSuccess) {
         pairs_hip_batch_destroy(b);
         return PAIRS_HIP_ERR_HIP;
     }
@@ -1469,10 +1493,16 @@ int pairs_hip_batch_destroy(pairs_hip_batch *b) {
     for (int p = 0; p < 3; p++) if (b->d_desc[p]) hipFree(b->d_desc[p]);
     if (b->d_rep) hipFree(b->d_rep);
     if (b->d_base) hipFree(b->d_base);
-    for (int k = 0; k < pairs_hip_batch::NRS; k++)
-        if (b->rs[k]) { hipStreamSynchronize(b->rs[k]); hipStreamDestroy(b->rs[k]); }
-    if (b->ev_main) hipEventDestroy(b->ev_main);
     if (b->h_cnt) hipHostFree(b->h_cnt);
+    if (b->h_rdesc) hipHostFree(b->h_rdesc);
+    if (b->d_rdesc) hipFree(b->d_rdesc);
+    if (b->h_thermo) hipHostFree(b->h_thermo);
+    if (b->d_thermo) hipFree(b->d_thermo);
+    if (b->h_bdesc) hipHostFree(b->h_bdesc);
+    if (b->d_bdesc) hipFree(b->d_bdesc);
+    if (b->h_cptr) hipHostFree(b->h_cptr);
+    if (b->d_cptr) hipFree(b->d_cptr);
+    if (b->d_call) hipFree(b->d_call);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return 0;
@@ -1539,10 +1569,18 @@ int pairs_hip_batch_run(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps
             BATCH_OK(hipStreamSynchronize(s));  // h_desc is reused by the next descriptor build
             launch_lj_full_batch(s, b->d_desc[2], b->d_rep, b->d_base, b->nblocks, b->max_tile, b->ntypes1, T ? 1 : 0);
             if (T) {
-                for (int r = 0; r < R; r++) {
-                    pairs_hip_ctx *c = b->rep[r];
-                    launch_thermo_reduce(s, c->d, c->geom.nsegs, 3, c->geom, c->d.thermo);
+                if (!b->rdesc_valid) {  // the replicas' arrays and sizes for the batched reduction
+                    BATCH_OK(hipStreamSynchronize(s));
+                    for (int r = 0; r < R; r++) {
+                        pairs_hip_ctx *c = b->rep[r];
+                        b->h_rdesc[r].d = c->d;
+                        b->h_rdesc[r].g = c->geom;
+                        for (int k = 0; k < 3; k++) b->h_rdesc[r].periodic[k] = c->cfg.periodic[k] ? 1 : 0;
+                    }
+                    BATCH_OK(hipMemcpyAsync(b->d_rdesc, b->h_rdesc, R * sizeof(RepDesc), hipMemcpyHostToDevice, s));
+                    b->rdesc_valid = true;
                 }
+                launch_thermo_batch(s, b->d_rdesc, R, b->d_thermo);
             }
         }
         {
@@ -1550,9 +1588,9 @@ int pairs_hip_batch_run(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps
             if (e_ != hipSuccess) { b->err = std::string("kernel launch: ") + hipGetErrorString(e_); return PAIRS_HIP_ERR_HIP; }
         }
         if (T) {
+            BATCH_OK(hipMemcpyAsync(b->h_thermo, b->d_thermo, 5 * (size_t) R * sizeof(double), hipMemcpyDeviceToHost, s));
             BATCH_OK(hipStreamSynchronize(s));
-            for (int r = 0; r < R; r++)
-                BATCH_OK(hipMemcpy(&b->thermo[5 * r], b->rep[r]->d.thermo, 5 * sizeof(double), hipMemcpyDeviceToHost));
+            std::memcpy(b->thermo.data(), b->h_thermo, 5 * (size_t) R * sizeof(double));
         }
         if (vv1) {
             for (pairs_hip_ctx *c : b->rep) c->d.cur = 1 - c->d.cur;
@@ -1573,12 +1611,14 @@ int pairs_hip_batch_run_each(pairs_hip_batch *b, int ts_begin, int ts_end, int n
         if (n > 0) for (int k = 0; k < 5; k++) b->thermo[5 * r + k] = th[5 * (std::min(n, 16) - 1) + k];
     }
     b->nblocks = 0;  // per-replica runs may have re-segmented: rebuild tables on the next batched run
+    b->rdesc_valid = false;
     return 0;
 }
 
 int pairs_hip_batch_set_home(pairs_hip_batch *b, int home) {
     if (!b || home < 8) return PAIRS_HIP_ERR_ARG;
     b->pending_home = home;
+    b->rdesc_valid = false;
     return 0;
 }
 

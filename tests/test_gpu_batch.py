@@ -42,14 +42,19 @@ def _same(a, b, tag):
 def test_batched_equals_single_systems():
     B = _batch().BatchEnsemble(SIZES, RHO, TEMP)
     singles = [_single(B, r) for r in range(len(SIZES))]
+    rows = []
     for a, e in ((0, 1), (1, 45), (45, 122)):
         B.run(a, e, 300)
-        for s in singles:
-            s.run(a, e, 300)
+        rows = [s.run(a, e, 300) for s in singles]
         for r, s in enumerate(singles):
             _same(B.replicas[r], s, (r, e))
+    # the batched thermo reduction (one workgroup per replica) gives each
+    # replica's own row of ts = 99
     th = B.thermo()
     assert np.all(th[:, 0] > 0)
+    for r, rr in enumerate(rows):
+        t, p = rr[-1][0], rr[-1][1]
+        assert abs(th[r, 0] - t) <= 1e-12 * t and abs(th[r, 1] - p) <= 1e-12 * abs(p), r
 
 
 def test_batched_equals_oracle():
