@@ -1403,7 +1403,12 @@ static int batch_rebuild(pairs_hip_batch *b) {
         if (fallback[r]) {
             drop_graphs(c);
             b->tables_valid = false;  // the replica's segments may change
+            // launched directly: a replica's own rebuild graphs are never
+            // replayed in batched runs
+            const bool ug = c->use_graphs;
+            c->use_graphs = false;
             const int code = rebuild(c);
+            c->use_graphs = ug;
             if (code) { b->err = "replica " + std::to_string(r) + ": " + c->err; return code; }
         }
     }
