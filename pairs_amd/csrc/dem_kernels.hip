@@ -585,77 +585,106 @@ __device__ __forceinline__ void walk_candidates(const DemPtrs &p, const DemGeom 
     }
 }
 
-// inclusive sum over the 32 lanes of each half-wave
-__device__ __forceinline__ int half_scan(int x) {
-    const int l = threadIdx.x & 31;
+// inclusive sum over each group of W lanes
+template <int W>
+__device__ __forceinline__ int group_scan(int x) {
+    const int l = threadIdx.x & (W - 1);
 #pragma unroll
-    for (int off = 1; off < 32; off <<= 1) {
-        const int y = __shfl_up(x, off, 32);
+    for (int off = 1; off < W; off <<= 1) {
+        const int y = __shfl_up(x, off, W);
         if (l >= off) x += y;
     }
     return x;
 }
 
-// Contact search, 32 lanes per local particle: lane 0 takes cell 0, lane
-// 1 + s stencil cell s.  Each lane counts the touching spheres and
-// half-spaces of its cell; prefix sums over the lanes give every hit its
-// place in the reference's visiting order (the spheres of cell 0 and of the
-// stencil cells, then the half-spaces of the same cells; cell_lists.py
-// partitions each cell spheres-first), and a second walk over the same
-// candidates stores them as cand[t][i] (half-spaces as ~j).  cand_n[i] =
-// DCAND + 1 when more than DCAND touch (the force pass then walks the cells
-// itself).  Only geometry is read: no history is touched.
+// Contact search, LPS lanes per local particle.  The visiting order's 28
+// cells (cell 0, then the 27 stencil cells) are dealt to the lanes in
+// contiguous runs of CPL = ceil(28 / LPS).  Each lane counts the touching
+// spheres and half-spaces of its cells; prefix sums over the lanes give every
+// hit its place in the reference's visiting order (the spheres of cell 0 and
+// of the stencil cells, then the half-spaces of the same cells;
+// cell_lists.py partitions each cell spheres-first), and a second walk stores
+// them as cand[t][i] (half-spaces as ~j), from a bit mask of the lane's
+// first 32 candidates (beyond them the geometry test is repeated).
+// cand_n[i] = DCAND + 1 when more than DCAND touch (the force pass then walks
+// the cells itself).  Only geometry is read: no history is touched.
+template <int LPS>
 __global__ __launch_bounds__(256) void k_contact_find(DemPtrs p, const int *__restrict__ c, DemGeom g) {
+    constexpr int CPL = (28 + LPS - 1) / LPS;
     if (c[D_OVF]) return;
     const int nl = c[D_NLOCAL];
-    const int i = (blockIdx.x * 256 + threadIdx.x) >> 5;
-    const int l = threadIdx.x & 31;
+    const int i = (blockIdx.x * 256 + threadIdx.x) / LPS;
+    const int l = threadIdx.x & (LPS - 1);
     const bool act = i < nl;
     const int ii = act ? i : 0;
     const bool fixed = !act || (p.flags[ii] & DFLAG_FIXED);
-    int st = 0, nsph = 0, nhs = 0;
-    if (!fixed && l < 28) {
-        const int cell = l == 0 ? 0 : p.particle_cell[ii] + g.stencil[l - 1];
-        if (l == 0 || (cell > 0 && cell < g.ncells)) {
-            st = p.cell_start[cell];
-            const int2 ns = reinterpret_cast<const int2 *>(p.nshapes)[cell];
-            nsph = ns.x;
-            nhs = ns.y;
+    // this lane's cells: start, sphere count, candidate count, and the
+    // offset of each cell's candidates in the lane's sequence
+    int cst[CPL], csph[CPL], cn[CPL], cbase[CPL + 1];
+    const int ci = fixed ? 0 : p.particle_cell[ii];
+    cbase[0] = 0;
+#pragma unroll
+    for (int u = 0; u < CPL; u++) {
+        const int k = l * CPL + u;
+        cst[u] = 0; csph[u] = 0; cn[u] = 0;
+        if (!fixed && k < 28) {
+            const int cell = k == 0 ? 0 : ci + g.stencil[k - 1];
+            if (k == 0 || (cell > 0 && cell < g.ncells)) {
+                cst[u] = p.cell_start[cell];
+                const int2 ns = reinterpret_cast<const int2 *>(p.nshapes)[cell];
+                csph[u] = ns.x;
+                cn[u] = ns.x + ns.y;
+            }
         }
+        cbase[u + 1] = cbase[u] + cn[u];
     }
     const double ri = p.r[ii];
     const double xi[3] = {p.x[ii], p.y[ii], p.z[ii]};
-    // hits of the first 32 candidates of the cell kept as a bit mask, so the
-    // store walk repeats the geometry test only beyond them
     int hs = 0, hh = 0;
     unsigned hit = 0u;
-    for (int q = 0; q < nsph + nhs; q++) {
-        const int j = p.cell_particles[st + q];
-        double nv[3], cpt[3], delta;
-        if (j != ii && contact_geom(p, xi, ri, j, q < nsph ? 0 : 1, nv, cpt, delta)) {
-            if (q < nsph) hs++; else hh++;
-            if (q < 32) hit |= 1u << q;
+#pragma unroll
+    for (int u = 0; u < CPL; u++)
+        for (int q = 0; q < cn[u]; q++) {
+            const int j = p.cell_particles[cst[u] + q];
+            const bool sph = q < csph[u];
+            double nv[3], cpt[3], delta;
+            if (j != ii && contact_geom(p, xi, ri, j, sph ? 0 : 1, nv, cpt, delta)) {
+                if (sph) hs++; else hh++;
+                const int b = cbase[u] + q;
+                if (b < 32) hit |= 1u << b;
+            }
         }
-    }
-    const int ps = half_scan(hs), ph = half_scan(hh);
-    const int tot_s = __shfl(ps, 31, 32), tot_h = __shfl(ph, 31, 32);
+    const int ps = group_scan<LPS>(hs), ph = group_scan<LPS>(hh);
+    const int tot_s = __shfl(ps, LPS - 1, LPS), tot_h = __shfl(ph, LPS - 1, LPS);
     const int n = tot_s + tot_h;
     if (n <= DCAND) {
         int os = ps - hs, oh = tot_s + ph - hh;
         while (hit) {
-            const int q = __builtin_ctz(hit);
+            const int b = __builtin_ctz(hit);
             hit &= hit - 1u;
+            int u = 0;
+#pragma unroll
+            for (int v = 1; v < CPL; v++) u += b >= cbase[v] ? 1 : 0;
+            int st = cst[0], q = b, sp = csph[0];
+#pragma unroll
+            for (int v = 1; v < CPL; v++)
+                if (u == v) { st = cst[v]; q = b - cbase[v]; sp = csph[v]; }
             const int j = p.cell_particles[st + q];
-            const int t = q < nsph ? os++ : oh++;
-            p.cand[(size_t) t * p.cap + ii] = q < nsph ? j : ~j;
+            const int t = q < sp ? os++ : oh++;
+            p.cand[(size_t) t * p.cap + ii] = q < sp ? j : ~j;
         }
-        for (int q = 32; q < nsph + nhs; q++) {
-            const int j = p.cell_particles[st + q];
-            double nv[3], cpt[3], delta;
-            if (j != ii && contact_geom(p, xi, ri, j, q < nsph ? 0 : 1, nv, cpt, delta)) {
-                const int t = q < nsph ? os++ : oh++;
-                p.cand[(size_t) t * p.cap + ii] = q < nsph ? j : ~j;
-            }
+        if (cbase[CPL] > 32) {
+#pragma unroll
+            for (int u = 0; u < CPL; u++)
+                for (int q = max(0, 32 - cbase[u]); q < cn[u]; q++) {
+                    const int j = p.cell_particles[cst[u] + q];
+                    const bool sph = q < csph[u];
+                    double nv[3], cpt[3], delta;
+                    if (j != ii && contact_geom(p, xi, ri, j, sph ? 0 : 1, nv, cpt, delta)) {
+                        const int t = sph ? os++ : oh++;
+                        p.cand[(size_t) t * p.cap + ii] = sph ? j : ~j;
+                    }
+                }
         }
     }
     if (act && l == 0) p.cand_n[i] = n <= DCAND ? n : DCAND + 1;
@@ -924,7 +953,7 @@ void launch_prepare(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g) {
 // forces + history bookkeeping, then integration
 void launch_contacts(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, const DemParams &prm,
                      const hipEvent_t *between) {
-    hipLaunchKernelGGL(k_contact_find, dim3(grid_of(p.cap, 8)), dim3(256), 0, s, p, c, g);
+    hipLaunchKernelGGL(k_contact_find<DEM_LPS>, dim3(grid_of(p.cap, 256 / DEM_LPS)), dim3(256), 0, s, p, c, g);
     if (between) hipEventRecord(between[0], s);
     hipLaunchKernelGGL(k_contact_slots, dim3(grid_of(p.cap, 256)), dim3(256), 0, s, p, c);
     if (between) hipEventRecord(between[1], s);

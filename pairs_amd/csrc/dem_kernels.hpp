@@ -28,6 +28,10 @@ enum {
 // touching partners recorded per particle by the contact search; a particle
 // with more walks its cells again in the force pass
 constexpr int DCAND = 16;
+// lanes per sphere of the contact search (each a run of the 28 cells)
+#ifndef DEM_LPS
+#define DEM_LPS 16
+#endif
 
 enum { DOVF_NONE = 0, DOVF_PART = 1, DOVF_CONTACT = 2, DOVF_SEND = 3, DOVF_CELLS = 4 };
 
