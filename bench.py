@@ -260,7 +260,7 @@ def main():
                          "calibrates FETCH_SIZE / WRITE_SIZE counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: pairs_amd has no multi-GPU path "
               "(the single-domain hot path; sharding is a design note)", file=sys.stderr)
         sys.exit(2)
 

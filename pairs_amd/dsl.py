@@ -1,4 +1,4 @@
-"""The reference's Python DSL as a front end of this build (a HIP target).
+"""The reference's Python DSL as a front end of the HIP library pairs_amd (a GPU target).
 
 ``import pairs_amd.dsl as pairs`` (or run a script through ``python -m
 pairs_amd.dsl script.py gpu``, which installs this module as ``pairs``) gives
@@ -16,7 +16,7 @@ them to the hand-written HIP kernels behind the C ABI.  A module is traced
 symbolically -- its source is parsed, ``and`` / ``or`` / ``not`` / ``if``
 rewritten into recording calls, and the body run over proxy objects -- into
 an expression graph with hash-consed nodes, which is compared with the graph
-of this build's own definition of each supported module (Lennard-Jones with
+of the library's own definition of each supported module (Lennard-Jones with
 feature tables, the velocity-Verlet pair, the DEM linear spring-dashpot with
 tangential history, gravity, the Euler integrator with rotation, and the DEM
 mass / inertia set-up).  Operation order is part of the graph, so a module
@@ -54,7 +54,7 @@ def target_cpu(parallel=False):
 
 
 def target_gpu():
-    """The GPU target: on this build, the MI355X HIP kernels."""
+    """The GPU target: the MI355X HIP kernels of pairs_amd."""
     return Target("gpu")
 
 
@@ -91,7 +91,7 @@ def regular_domain_partitioner_xy(): return _REGULAR_XY
 
 
 class UnsupportedModule(RuntimeError):
-    """A user module that is not one of the hot-path modules this build runs."""
+    """A user module that is not one of the hot-path modules the HIP kernels implement."""
 
 
 # ---------------------------------------------------------------- symbolic tracing of modules
@@ -303,7 +303,7 @@ def _canonical(stmts):
     return "\n".join(lines + out)
 
 
-# ---------------------------------------------------------------- this build's modules (the kernels' math)
+# ---------------------------------------------------------------- the library's modules (the kernels' math)
 #
 # Each definition is what a hand-written kernel computes, written in the
 # DSL with its operation order (the traced graph fixes the order):
@@ -435,8 +435,8 @@ def recognise(fn, symbols=()):
     for k, f in _FORMS.items():
         if f == form:
             return k
-    raise UnsupportedModule(f"module {getattr(fn, '__name__', fn)} is not one of the hot-path modules this "
-                            f"build runs ({', '.join(_BUILTIN_SRC)}) with their operation order")
+    raise UnsupportedModule(f"module {getattr(fn, '__name__', fn)} is not one of the hot-path modules the HIP "
+                            f"kernels implement ({', '.join(_BUILTIN_SRC)}) with their operation order")
 
 
 # ---------------------------------------------------------------- the simulation object (sim/simulation.py)
@@ -448,9 +448,9 @@ class Simulation:
     def __init__(self, ref, shapes, dims=3, timesteps=100, double_prec=False, use_contact_history=False,
                  particle_capacity=800000, neighbor_capacity=100, debug=False):
         if dims != 3:
-            raise UnsupportedModule("this build runs three-dimensional simulations")
+            raise UnsupportedModule("pairs_amd runs three-dimensional simulations")
         if not double_prec:
-            raise UnsupportedModule("this build runs the double-precision program (double_prec=True)")
+            raise UnsupportedModule("pairs_amd runs the double-precision program (double_prec=True)")
         self.ref, self.shapes, self.timesteps = ref, list(shapes), timesteps
         self.use_contact_history = use_contact_history
         self.particle_capacity, self.neighbor_capacity = particle_capacity, neighbor_capacity
@@ -542,7 +542,7 @@ class Simulation:
     # The reference's current Simulation defines none of these calls; they
     # are thin aliases onto the calls above, so such scripts configure.
     # Their modules are traced and recognised like any other: modules
-    # outside this build's kernels fail at generate() with the reason.
+    # outside the library's kernels fail at generate() with the reason.
     def add_real_property(self, name, value=0.0, vol=False):
         self.add_property(name, _REAL, value, vol)
         return _Prop(name)
@@ -583,7 +583,7 @@ class Simulation:
     # ---- lowering onto the C ABI
     def _plan(self):
         if self._target is not None and self._target.kind != "gpu":
-            raise UnsupportedModule("this build has the GPU target only (pairs.target_gpu())")
+            raise UnsupportedModule("pairs_amd has the GPU target only (pairs.target_gpu())")
         names = [recognise(m["fn"], m["symbols"]) for m in self.modules]
         setups = [recognise(f, s) for f, s in self.setup_modules]
         md = ["initial_integrate", "lennard_jones", "final_integrate"]
@@ -600,7 +600,7 @@ class Simulation:
             return "md", setups
         if names == dem and setups == ["update_mass_and_inertia"]:
             return "dem", setups
-        raise UnsupportedModule(f"unsupported module sequence {names} (setup {setups}); this build runs "
+        raise UnsupportedModule(f"unsupported module sequence {names} (setup {setups}); pairs_amd runs "
                                 f"{md} or setup update_mass_and_inertia + {dem}")
 
     def _md_system(self):

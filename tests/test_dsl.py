@@ -64,7 +64,7 @@ def test_rejects_unsupported_configurations(tmp_path):
         sim.generate(run=False)  # no integrators: not the md.py program
     with pytest.raises(dsl.UnsupportedModule):
         dsl.simulation("y", [dsl.point_mass()], double_prec=False)
-    with pytest.raises(dsl.UnsupportedModule):  # target_cpu(): this build has the GPU target only
+    with pytest.raises(dsl.UnsupportedModule):  # target_cpu(): pairs_amd has the GPU target only
         dsl.run_script(os.path.join(HERE, "dsl", "lj_fcc.py"), ["cpu"], run=False)
 
 
@@ -85,7 +85,7 @@ def test_emitted_host_program_compiles(tmp_path, script):
 def test_older_api_aliases():
     """The older calls configure a simulation (properties, file input, cells,
     neighbour lists, periodicity); its modules are traced with rsq / delta as
-    bare names, and generate() names the module this build has no kernel for
+    bare names, and generate() names the module the library has no kernel for
     (Euler-integrated LJ with the older operation order)."""
     script = os.path.join(HERE, "dsl", "older_api.py")
     data = os.path.join(HERE, "golden", "minimd_setup_4x4x4_onetype.input")

@@ -1,11 +1,11 @@
 """Parity of the HIP LJ path (libpairs_hip.so through its C ABI) with the CPU
 oracle and with the golden fixtures of the reference's generated program.
 
-Full neighbour lists: every value the reference computes is bit-identical
-(positions, velocities, forces, ghosts, neighbour lists, cells, send maps).
-Half lists: the mirror forces are FP64 atomics whose order is not fixed, so
-FP64 state agrees within the north-star tolerance of 1e-10 relative while the
-integer state (lists, cells, maps) stays identical over 100 steps.
+Full and half neighbour lists: every value the reference computes is
+bit-identical (positions, velocities, forces, ghosts, neighbour lists, cells,
+send maps).  For half lists the mirror forces are summed from the receiving
+particle's side in the reference's loop order, so no FP64 atomics reorder
+them.
 """
 import os
 
