@@ -863,7 +863,7 @@ static void enqueue_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
     launch_exchange(s, ctx->d, ctx->geom, ctx->cfg.periodic);
     launch_borders(s, ctx->d, ctx->geom, ctx->cfg.periodic);
     launch_bin_sort(s, ctx->d, ctx->geom);  // flips d.cur
-    // the build runs right behind the cell phase; every rebuild kernel
+    // the neighbour build runs right behind the cell phase; every rebuild kernel
     // returns at once when an earlier one has set an overflow
     enqueue_build(ctx, s);
 }
