@@ -174,6 +174,27 @@ def test_c2_size_properties_and_oracle_100_steps():
     assert g.nghost == o.nghost
 
 
+def test_c2_half_lists_oracle_100_steps():
+    """131072 atoms with half lists: the lists equal the oracle's (a local
+    keeps its later-index neighbours and every ghost neighbour, so the counts
+    of the perfect lattice add up to 78 N / 2 plus half the local-ghost
+    pairs); 100 fused steps, mirror parts included, bit-identical to the
+    oracle's sequential half-list loop."""
+    md = _md()
+    g = md.fcc_system(32, half=True)
+    g.run(0, 1, 1000)
+    o = po.fcc_system(32, half=True)
+    o.step(0)
+    _compare_lists(g, o, tag="c2-half ts=0")
+    nn = g.numneighs()
+    n_ghost_pairs = 2 * int(nn.sum()) - 78 * len(nn)
+    assert nn.max() == 78 and 0 < n_ghost_pairs < 78 * len(nn)
+    assert np.array_equal(g.forces(), o.forces())
+    g.run(1, 101, 1000)
+    o.run(1, 101)
+    _compare_state(g, o, tag="c2-half")
+
+
 def test_vtk_output_of_lj_system(tmp_path):
     from pairs_amd import md, io
     s = md.fcc_system(4)
