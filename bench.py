@@ -20,6 +20,7 @@ larger domain, each as its own object of the same JSON line.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -212,6 +213,53 @@ def run_dem(steps):
     }
 
 
+def run_dem_settling(steps):
+    """The DEM settling workload defined for this path's measurement: about
+    200 000 spheres (d ~ U[1.8, 2.2] mm on a 2.45 mm SC grid, 2500 kg/m3, at
+    rest) in a 0.1 x 0.1 x 0.3 m box, periodic in x and y, a floor plane at
+    z = 0, restitution 0.6, friction 0.5 with history, dt 1e-6 s.  The contact
+    model is examples/dem.py's spring-dashpot, whose stiffness follows from the
+    collision time: t_c is chosen so that k_n = 1e4 N/m at the mean effective
+    mass (k_n = m_eff (pi^2 + ln^2 e) / t_c^2)."""
+    from pairs_amd import dem
+
+    d, rho, e, kn = 0.002, 2500.0, 0.6, 1.0e4
+    meff = 0.5 * rho * math.pi / 6.0 * d ** 3
+    tc = math.sqrt(meff * (math.pi ** 2 + math.log(e) ** 2) / kn)
+    params = dict(diameter=d, gravity=9.81, density_fluid=0.0, density_particle=rho, spacing=0.00245,
+                  initial_velocity=0.0, dt=1e-6, friction=0.5, restitution=e, collision_time=tc,
+                  friction_static=0.0)
+    box = (0.1, 0.1, 0.3)
+    s = dem.DEMSystem(box, periodic=(True, True, False), particle_capacity=320000, send_capacity=80000,
+                      params=params)
+    s.sc_grid(box)
+    s.add_halfspace(100000, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    s.setup()
+    n = s.nlocal
+    s.set_timing(50)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(0, steps)
+    s.sync()
+    t = time.perf_counter() - t0
+    phases = s.phase_times()
+    nc = s.get("num_contacts")
+    s.close()
+    return {
+        "value": n * steps / t,
+        "unit": "particle-steps/s",
+        "ms_per_step": t * 1e3 / steps,
+        "phase_ms": dict(zip(("prepare", "contact_search", "history_slots", "spring_dashpot", "sums", "euler"),
+                             phases)),
+        "spheres": n,
+        "steps": steps,
+        "contacts_at_end": int(nc.sum()),
+        "collision_time_s": tc,
+        "workload": "DEM settling: d ~ U[1.8, 2.2] mm, 2500 kg/m3, 0.1 x 0.1 x 0.3 m, periodic x/y, floor plane, "
+                    "k_n 1e4 N/m at the mean effective mass, e 0.6, mu 0.5 with history, dt 1e-6 s, FP64",
+    }
+
+
 def run_cpu_baseline(sample_steps, threads):
     """The oracle (C restatement of the reference's generated CPU program) on a
     bounded sample of the same workload, with the given number of OpenMP
@@ -327,6 +375,7 @@ def main():
         rec["mlp"]["vs_plain_lj"] = rec["mlp"]["value"] / value
     if not args.no_dem:
         rec["dem"] = run_dem(args.dem_steps)
+        rec["dem_settling"] = run_dem_settling(args.dem_steps)
     if not args.no_cpu:
         v1, secs1 = run_cpu_baseline(args.cpu_steps, 1)
         nt = host_threads()
