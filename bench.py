@@ -260,6 +260,26 @@ def run_dem_settling(steps):
     }
 
 
+def measured_copy_gbps(mib=1024, reps=10):
+    """A device-to-device copy of mib MiB timed with HIP events: the HBM rate
+    this device reaches on plain streaming (bytes read + bytes written)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9
+
+
 def run_cpu_baseline(sample_steps, threads):
     """The oracle (C restatement of the reference's generated CPU program) on a
     bounded sample of the same workload, with the given number of OpenMP
@@ -320,6 +340,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    copy_gbps = measured_copy_gbps()
     full = run_gpu(args.steps, args.warmup, False, args.time_every)
     value = NATOMS * args.steps / full["seconds"]
     fused_bytes = force_kernel_bytes(full["nlocal"], full["nghost"], full["nneigh"], True)
@@ -349,6 +370,8 @@ def main():
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": profiled_traffic(),
+            "measured_copy_GBps": copy_gbps,
+            "frac_of_measured_copy": (achieved / copy_gbps) if (achieved and copy_gbps) else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a "
                               "512 MiB copy (profiles/lj_fcc32_force_pmc.md)",
             "kernel": "k_lj_full<8,1,1,false,true,false> (LJ force + final/initial integrate + next packed positions, fused)",
