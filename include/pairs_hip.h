@@ -143,6 +143,13 @@ int pairs_hip_lj(pairs_hip_ctx *ctx);
 /* initial_integrate / final_integrate (examples/md.py:11-17) */
 int pairs_hip_vv_initial(pairs_hip_ctx *ctx);
 int pairs_hip_vv_final(pairs_hip_ctx *ctx);
+/* The modules of the older API (examples/lj_onetype.py:5-13):
+ * reset_volatile_properties + `lj`, whose pair force is
+ * delta * 48.0 * sr6 * (sr6 - 0.5) * sr2 * epsilon evaluated left to right
+ * (full lists only), and `euler`: velocity += dt * force / mass, then
+ * position += dt * velocity, for the non-fixed locals. */
+int pairs_hip_lj_older(pairs_hip_ctx *ctx);
+int pairs_hip_euler(pairs_hip_ctx *ctx);
 /* compute_thermo (runtime/thermo.hpp:11-51): out[0] = T, out[1] = kinetic P,
  * out[2] = potential energy, out[3] = virial W, out[4] = P with virial
  * (out[2..4] have no reference counterpart: U = sum 4eps[(s/r)^12-(s/r)^6]

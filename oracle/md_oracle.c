@@ -516,6 +516,56 @@ void omd_rebuild(omd_t *s) {
 }
 
 void omd_force(omd_t *s) { lennard_jones(s); }
+
+/* The older API's modules (examples/lj_onetype.py:5-13), full lists:
+   reset_volatile_properties and `lj`, whose pair force
+   delta * 48.0 * sr6 * (sr6 - 0.5) * sr2 * epsilon is evaluated left to right
+   per component, then `euler`: v += (dt * f) / m, x += dt * v. */
+void omd_lj_older(omd_t *s) {
+    const double rc2 = s->cutoff * s->cutoff;
+    const int nt = s->ntypes;
+    for (int i = 0; i < s->nlocal; i++) { s->f[3 * i] = 0.0; s->f[3 * i + 1] = 0.0; s->f[3 * i + 2] = 0.0; }
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < s->nlocal; i++) {
+        if (s->flags[i] & FLAG_FIXED) continue;
+        double t[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < s->numneighs[i]; k++) {
+            const int j = s->neighborlists[(size_t) i * s->ncap + k];
+            double d[3];
+            for (int e = 0; e < 3; e++) d[e] = s->x[3 * i + e] - s->x[3 * j + e];
+            double rsq = d[0] * d[0] + d[1] * d[1];
+            rsq = rsq + d[2] * d[2];
+            if (rsq < rc2) {
+                const int tk = s->type[i] * nt + s->type[j];
+                const double sr2 = 1.0 / rsq;
+                double sr6 = sr2 * sr2;
+                sr6 = sr6 * sr2;
+                sr6 = sr6 * s->sig6[tk];
+                for (int e = 0; e < 3; e++) {
+                    double c = d[e] * 48.0;
+                    c = c * sr6;
+                    c = c * (sr6 - 0.5);
+                    c = c * sr2;
+                    c = c * s->eps[tk];
+                    t[e] = t[e] + c;
+                }
+            }
+        }
+        for (int e = 0; e < 3; e++) s->f[3 * i + e] = s->f[3 * i + e] + t[e];
+    }
+}
+
+void omd_euler(omd_t *s) {
+    for (int i = 0; i < s->nlocal; i++) {
+        if (s->flags[i] & FLAG_FIXED) continue;
+        for (int e = 0; e < 3; e++) {
+            double a = s->dt * s->f[3 * i + e];
+            a = a / s->m[i];
+            s->v[3 * i + e] = s->v[3 * i + e] + a;
+        }
+        for (int e = 0; e < 3; e++) s->x[3 * i + e] = s->x[3 * i + e] + s->dt * s->v[3 * i + e];
+    }
+}
 void omd_sync(omd_t *s) { synchronize(s); }
 
 static int every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 || ts == 0); }

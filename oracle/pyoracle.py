@@ -45,7 +45,7 @@ def lib():
         L.omd_temperature.argtypes = [ctypes.c_void_p]
         L.omd_temperature.restype = ctypes.c_double
         L.omd_set_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _I, _I]
-        for fn in ("omd_setup_cells", "omd_rebuild", "omd_force", "omd_sync"):
+        for fn in ("omd_setup_cells", "omd_rebuild", "omd_force", "omd_sync", "omd_lj_older", "omd_euler"):
             getattr(L, fn).argtypes = [ctypes.c_void_p]
         L.omd_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.omd_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
@@ -144,6 +144,13 @@ class OracleMD:
 
     def sync(self):
         self.L.omd_sync(self.h)
+
+    # the older API's modules (examples/lj_onetype.py:5-13)
+    def lj_older(self):
+        self.L.omd_lj_older(self.h)
+
+    def euler(self):
+        self.L.omd_euler(self.h)
 
     def temperature(self):
         return self.L.omd_temperature(self.h)

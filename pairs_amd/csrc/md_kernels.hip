@@ -718,7 +718,10 @@ __global__ __launch_bounds__(256) void k_expand_neighbors(const int *__restrict_
 // == t exactly for every other t, so the masked pairs leave the sum
 // bit-identical to the reference's conditional add; without branches the U
 // divisions of a group overlap.
-template <bool ENERGY>
+// OLDER: the pair module of the older API (examples/lj_onetype.py:5-8),
+// force[i] += delta * 48.0 * sr6 * (sr6 - 0.5) * sr2 * epsilon evaluated left
+// to right, so each component is ((((d * 48) * sr6) * (sr6 - 0.5)) * sr2) * eps.
+template <bool ENERGY, bool OLDER = false>
 __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double xj, double yj, double zj, bool valid,
                                         double rc2, double sg6, double ep, double &g0, double &g1, double &g2,
                                         double &u, double &w) {
@@ -730,12 +733,18 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
     double sr6 = sr2 * sr2;
     sr6 = sr6 * sr2;
     sr6 = sr6 * sg6;
-    double s = 48.0 * sr6;
-    s = s * (sr6 - 0.5);
-    s = s * sr2;
-    s = s * ep;
-    s = in ? s : 0.0;
-    g0 = dx * s; g1 = dy * s; g2 = dz * s;
+    if (OLDER) {
+        const double h = sr6 - 0.5;
+        auto term = [&](double d) { return in ? (((d * 48.0) * sr6) * h) * sr2 * ep : 0.0; };
+        g0 = term(dx); g1 = term(dy); g2 = term(dz);
+    } else {
+        double s = 48.0 * sr6;
+        s = s * (sr6 - 0.5);
+        s = s * sr2;
+        s = s * ep;
+        s = in ? s : 0.0;
+        g0 = dx * s; g1 = dy * s; g2 = dz * s;
+    }
     if (ENERGY) {
         u = in ? 4.0 * ep * sr6 * (sr6 - 1.0) : 0.0;
         w = in ? 48.0 * sr6 * (sr6 - 0.5) * ep : 0.0;
@@ -762,7 +771,7 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
 // i's side -- the mirror pair's force recomputed from i is exactly the
 // negated force j computes -- so the result is the reference's bit for bit,
 // without atomics.
-template <int U, int NT, int MODE, bool RED, bool PXY = false, bool HALF = false>
+template <int U, int NT, int MODE, bool RED, bool PXY = false, bool HALF = false, bool OLDER = false>
 __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
     static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
     const int *__restrict__ counters = A.counters;
@@ -838,7 +847,8 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
                         double sg6 = sig0, ep = eps0;
                         if (NT != 1) { const int kt = ti * nt + tj[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
                         double g0, g1, g2, uu, ww;
-                        lj_pair<RED>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < cnt, rc2, sg6, ep, g0, g1, g2, uu, ww);
+                        lj_pair<RED, OLDER>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < cnt, rc2, sg6, ep, g0, g1, g2, uu,
+                                            ww);
                         a0 = a0 + g0; a1 = a1 + g1; a2 = a2 + g2;
                         if (RED && !HALF) { u += uu; w += ww; }
                         if (RED && HALF && !mirror) {
@@ -909,9 +919,45 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
 
 
 // one segment per workgroup of one system
-template <int U, int NT, int MODE, bool RED, bool PXY = true, bool HALF = false>
+template <int U, int NT, int MODE, bool RED, bool PXY = true, bool HALF = false, bool OLDER = false>
 __global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
-    lj_full_segment<U, NT, MODE, RED, PXY, HALF>(A, xcd_block(blockIdx.x, gridDim.x));
+    lj_full_segment<U, NT, MODE, RED, PXY, HALF, OLDER>(A, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// euler of the older API (examples/lj_onetype.py:11-13) for the non-fixed
+// locals: velocity += dt * force / mass, then position += dt * velocity, in
+// the current position buffer
+__global__ __launch_bounds__(256) void k_euler_md(double *__restrict__ X, double *__restrict__ Y,
+                                                  double *__restrict__ Z, double *__restrict__ vx,
+                                                  double *__restrict__ vy, double *__restrict__ vz,
+                                                  const double *__restrict__ fx, const double *__restrict__ fy,
+                                                  const double *__restrict__ fz, const double *__restrict__ mass,
+                                                  const int *__restrict__ flags, int nlocal, double dt) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nlocal || (flags[i] & FLAG_FIXED)) return;
+    const double m = mass[i];
+    const double v0 = vx[i] + (dt * fx[i]) / m, v1 = vy[i] + (dt * fy[i]) / m, v2 = vz[i] + (dt * fz[i]) / m;
+    vx[i] = v0; vy[i] = v1; vz[i] = v2;
+    X[i] = X[i] + dt * v0; Y[i] = Y[i] + dt * v1; Z[i] = Z[i] + dt * v2;
+}
+
+void launch_euler_md(hipStream_t s, const DevState &d, double dt) {
+    if (d.nlocal <= 0) return;
+    const int c = d.cur;
+    hipLaunchKernelGGL(k_euler_md, dim3((d.nlocal + 255) / 256), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.vx, d.vy,
+                       d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, dt);
+}
+
+// forces only with the older API's pair expression (full lists)
+void launch_lj_older(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm) {
+    if (d.nlocal <= 0) return;
+    launch_pack_xy(s, d, d.cur);
+    const size_t lds = (size_t) d.tile_cap * sizeof(int);
+    const LJArgs A = lj_args(d, g, prm, d.cur, d.cur, 0, 0, 0, nullptr);
+    if (prm.ntypes == 1)
+        hipLaunchKernelGGL((k_lj_full<8, 1, 0, false, true, false, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    else
+        hipLaunchKernelGGL((k_lj_full<8, 0, 0, false, true, false, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
 }
 
 // the packed (x, y) copy the single-system force kernel gathers from

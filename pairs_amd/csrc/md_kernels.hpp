@@ -133,6 +133,10 @@ struct DevState {
 void launch_fill_i32(hipStream_t s, int *p, int v, int n);
 // XY[cur] = (X[cur], Y[cur]) for locals and ghosts
 void launch_pack_xy(hipStream_t s, const DevState &d, int cur);
+// the older API's modules (examples/lj_onetype.py): forces with its pair
+// expression (full lists), and its Euler update in the current buffer
+void launch_lj_older(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm);
+void launch_euler_md(hipStream_t s, const DevState &d, double dt);
 void launch_fill_f64(hipStream_t s, double *p, double v, int n);
 void launch_set_counters(hipStream_t s, int *c, int first, int count, int v);
 // dst[r * C_COUNT + k] = src[r][k] for r < n (device array of counter pointers)

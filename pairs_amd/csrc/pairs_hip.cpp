@@ -791,6 +791,26 @@ int pairs_hip_vv_final(pairs_hip_ctx *ctx) {
     return 0;
 }
 
+int pairs_hip_lj_older(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    if (ctx->cfg.half) {
+        ctx->err = "the older API's lj module runs on full neighbour lists (cfg.half = 0)";
+        return PAIRS_HIP_ERR_STATE;
+    }
+    launch_lj_older(ctx->stream, ctx->d, ctx->geom, ctx->prm);
+    ctx->xy_valid[ctx->d.cur] = true;  // packed by the launch
+    CHECK_LAUNCH();
+    return 0;
+}
+
+int pairs_hip_euler(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    ctx->xy_valid[0] = ctx->xy_valid[1] = false;
+    launch_euler_md(ctx->stream, ctx->d, ctx->prm.dt);
+    CHECK_LAUNCH();
+    return 0;
+}
+
 static int ensure_eatom(pairs_hip_ctx *ctx) {
     if (ctx->eatom_cap < ctx->d.cap) {
         dfree(ctx->d_eatom);

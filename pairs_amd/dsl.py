@@ -313,7 +313,23 @@ def _canonical(stmts):
 #   euler               -> k_euler                    (dem_kernels.hip)
 #   update_mass_and_inertia -> k_mass_inertia         (dem_kernels.hip)
 
+# the older API's scalars (examples/lj_onetype.py passes them as symbols)
+_BUILTIN_SYMS = {"lj_older": ("sigma6", "epsilon")}
+
 _BUILTIN_SRC = {
+    # the older API's modules (examples/lj_onetype.py:5-13): its pair force
+    # evaluated left to right, and an Euler update
+    "lj_older": '''
+def lj_older(i, j):
+    inv = 1.0 / rsq
+    p6 = inv * inv * inv * sigma6
+    force[i] = force[i] + delta * 48.0 * p6 * (p6 - 0.5) * inv * epsilon
+''',
+    "euler_md": '''
+def euler_md(i):
+    velocity[i] = velocity[i] + dt * force[i] / mass[i]
+    position[i] = position[i] + dt * velocity[i]
+''',
     "lennard_jones": '''
 def lennard_jones(i, j):
     inv_rsq = 1.0 / squared_distance(i, j)
@@ -409,7 +425,7 @@ def linear_spring_dashpot(i, j):
 ''',
 }
 
-_MD_PROPS = ("position", "mass", "linear_velocity", "force", "epsilon", "sigma6", "type")
+_MD_PROPS = ("position", "mass", "linear_velocity", "velocity", "force", "epsilon", "sigma6", "type")
 _DEM_PROPS = ("position", "mass", "linear_velocity", "angular_velocity", "force", "torque", "radius", "normal",
               "inv_inertia", "rotation_matrix", "rotation_quat", "friction_static", "friction_dynamic",
               "is_sticking", "tangential_spring_displacement", "impact_velocity_magnitude", "type")
@@ -419,7 +435,7 @@ _SYMS = ("dt", "pi", "kappa", "lnDryResCoeff", "collisionTime_SI", "densityParti
 
 def _builtin_forms():
     props = tuple(sorted(set(_MD_PROPS) | set(_DEM_PROPS)))
-    return {k: trace(src, props, _SYMS, k) for k, src in _BUILTIN_SRC.items()}
+    return {k: trace(src, props, _SYMS + _BUILTIN_SYMS.get(k, ()), k) for k, src in _BUILTIN_SRC.items()}
 
 
 _FORMS = None
@@ -588,6 +604,15 @@ class Simulation:
         setups = [recognise(f, s) for f, s in self.setup_modules]
         md = ["initial_integrate", "lennard_jones", "final_integrate"]
         dem = ["gravity", "linear_spring_dashpot", "euler"]
+        older = ["lj_older", "euler_md"]
+        if names == older and not setups:
+            if self.modules[0]["cutoff"] is None or self.neighbor_radius is None:
+                raise UnsupportedModule("the pair module needs a cutoff and build_neighbor_lists")
+            if self.domain is None:
+                raise UnsupportedModule("a program that reads its particles from a file needs set_domain(): the "
+                                        "older API leaves the box implicit and no rule for it survives in the "
+                                        "reference")
+            return "md_euler", setups
         if names == md:
             pre, lj, fin = self.modules
             if not (pre["pre_step"] and pre["skip_first"] and fin["skip_first"] and not fin["pre_step"]):
@@ -624,6 +649,30 @@ class Simulation:
                           periodic=self.pbc_flags)
         sim.copper_fcc_lattice(nx, ny, nz, rho, nt)
         sim.adjust_thermo(temp)
+        sim._call("pairs_hip_setup_cells")
+        return sim
+
+    def _md_euler_system(self):
+        """The older API's LJ program: particles from one from_file call,
+        full lists, the module lj with its own pair expression, Euler."""
+        from . import io, md
+        reads = [s for s in self.setups if s[0] == "read"]
+        if len(reads) != 1 or len(self.setups) != 1:
+            raise UnsupportedModule("the older API's LJ program is set up by one from_file call")
+        _, path, names, _ = reads[0]
+        kinds = {"mass": io.PROP_REAL, "position": io.PROP_VECTOR, "velocity": io.PROP_VECTOR}
+        data = io.read_particle_data(path, names, self.domain, kinds=kinds)
+        x, v = data["position"], data["velocity"]
+        m = data.get("mass")
+        n = len(x)
+        lj, eu = self.modules
+        sim = md.LJSystem(self.domain, particle_capacity=max(self.particle_capacity, 4 * n),
+                          neighbor_capacity=self.neighbor_capacity, send_capacity=max(200000, 2 * n),
+                          cutoff=lj["cutoff"], skin=self.neighbor_radius - lj["cutoff"], dt=eu["symbols"]["dt"],
+                          half=False, ntypes=1, epsilon=[builtins.float(lj["symbols"]["epsilon"])],
+                          sigma6=[builtins.float(lj["symbols"]["sigma6"])], reneighbor_every=self.reneighbor,
+                          thermo_every=0, periodic=self.pbc_flags)
+        sim.upload(x, v, m)
         sim._call("pairs_hip_setup_cells")
         return sim
 
@@ -668,6 +717,8 @@ class Simulation:
         emit_cpp = emit_cpp or _SCRIPT["emit_cpp"]
         quiet = quiet or _SCRIPT["quiet"]
         kind, _ = self._plan()
+        if emit_cpp and kind == "md_euler":
+            raise UnsupportedModule("emit_cpp writes the md.py and dem.py programs")
         if emit_cpp:
             with open(emit_cpp, "w") as f:
                 f.write(self._cpp(kind))
@@ -675,6 +726,23 @@ class Simulation:
             return None
         from . import io
         n = self.timesteps
+        if kind == "md_euler":
+            # the reference's timestep loop (src/pairs/sim/timestep.py:36-61)
+            # over the older program's two modules, module by module
+            self.system = s = self._md_euler_system()
+            for ts in range(n + 1):
+                if self.reneighbor > 0 and ((ts + 1) % self.reneighbor == 0 or ts == 0):
+                    s.exchange()
+                    s.borders()
+                    s.build_cell_lists()
+                    s.build_neighbor_lists()
+                else:
+                    s.synchronize()
+                s.lennard_jones_older()
+                s.euler()
+                if self.vtk_file and self.vtk_frequency and ts % self.vtk_frequency == 0:
+                    io.vtk_write(s, self.vtk_file + "_local", ts, self.vtk_frequency)
+            return s
         if kind == "md":
             self.system = s = self._md_system()
             stops = sorted({0, n + 1} | ({t + 1 for t in range(0, n + 1, self.vtk_frequency)}

@@ -175,6 +175,13 @@ class LJSystem:
     def final_integrate(self):
         self._call("pairs_hip_vv_final")
 
+    # the older API's modules (examples/lj_onetype.py:5-13)
+    def lennard_jones_older(self):
+        self._call("pairs_hip_lj_older")
+
+    def euler(self):
+        self._call("pairs_hip_euler")
+
     def compute_thermo(self):
         """(T, P_kinetic, U, W, P) -- the first two as runtime/thermo.hpp:11-51."""
         out = np.zeros(5)

@@ -78,6 +78,8 @@ _SIGNATURES = {
     "pairs_hip_lj": (_I, [_P]),
     "pairs_hip_vv_initial": (_I, [_P]),
     "pairs_hip_vv_final": (_I, [_P]),
+    "pairs_hip_lj_older": (_I, [_P]),
+    "pairs_hip_euler": (_I, [_P]),
     "pairs_hip_thermo": (_I, [_P, _DP]),
     "pairs_hip_energy": (_I, [_P, _DP, _DP]),
     "pairs_hip_step": (_I, [_P, _I, _I]),

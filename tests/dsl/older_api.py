@@ -2,7 +2,8 @@
 lj_embedded.py that the reference's current Simulation no longer defines):
 properties declared one kind at a time, particles from a file, a pair module
 reading the bare names rsq and delta, and an Euler update.  argv[1]: the
-particle file (mass, position, velocity columns)."""
+particle file (mass, position, velocity columns); argv[2], if given: the edge
+of the cubic periodic box (the older API names none)."""
 import sys
 
 import pairs
@@ -27,6 +28,9 @@ sim.add_position("position")
 sim.add_vector_property("velocity")
 sim.add_vector_property("force", vol=True)
 sim.from_file(sys.argv[1], ["mass", "position", "velocity"])
+if len(sys.argv) > 2:
+    edge = float(sys.argv[2])
+    sim.set_domain([0.0, 0.0, 0.0, edge, edge, edge])
 sim.create_cell_lists(2.8, 2.5)
 sim.create_neighbor_lists()
 sim.periodic(2.8)

@@ -85,12 +85,13 @@ def test_emitted_host_program_compiles(tmp_path, script):
 def test_older_api_aliases():
     """The older calls configure a simulation (properties, file input, cells,
     neighbour lists, periodicity); its modules are traced with rsq / delta as
-    bare names, and generate() names the module the library has no kernel for
-    (Euler-integrated LJ with the older operation order)."""
+    bare names and recognised as the older LJ program (its own pair
+    expression, Euler), which needs the box the older API leaves implicit."""
     script = os.path.join(HERE, "dsl", "older_api.py")
     data = os.path.join(HERE, "golden", "minimd_setup_4x4x4_onetype.input")
-    with pytest.raises(dsl.UnsupportedModule, match="module lj is not one of the hot-path modules"):
+    with pytest.raises(dsl.UnsupportedModule, match="set_domain"):
         dsl.run_script(script, [data], run=False)
+    assert [dsl.recognise(m["fn"], m["symbols"]) for m in dsl._CREATED[-1].modules] == ["lj_older", "euler_md"]
     sim = dsl._CREATED[-1]
     assert set(sim.props) == {"mass", "position", "velocity", "force"}
     assert sim.props["force"][2] is True and sim.position_name == "position"

@@ -101,6 +101,49 @@ def test_two_body_force_closed_form():
             assert abs(f[0, 0]) < 1e-12
 
 
+def test_older_api_modules():
+    """The older API's pair force (examples/lj_onetype.py:5-8) is the same
+    LJ force as md.py's with another rounding order: both agree to a few
+    ulps, each equal to the two-body closed form; its Euler update moves a
+    particle beyond the cutoff (zero force) by exactly dt * v per step."""
+    L = 20.0
+    for r in (0.95, 1.3, 2.2):
+        s = po.OracleMD([(0.0, L)] * 3, 64, reneighbor_every=1)
+        x = np.array([[5.0, 5.0, 5.0], [5.0 + r, 5.0, 5.0]])
+        s.set_particles(x, np.zeros_like(x))
+        s.setup_cells()
+        s.rebuild()
+        s.force()
+        f_md = s.forces().copy()
+        s.lj_older()
+        f_old = s.forces()
+        fa = 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7)
+        assert abs(f_old[0, 0] + fa) <= 1e-12 * max(1.0, abs(fa))
+        assert np.allclose(f_old, f_md, rtol=1e-14, atol=1e-300)
+        assert np.all(f_old[0] + f_old[1] == 0.0)
+    # a many-body configuration: still the same forces to rounding
+    o = po.fcc_system(4)
+    o.run(0, 30)
+    o.force()
+    f_md = o.forces().copy()
+    o.lj_older()
+    assert np.allclose(o.forces(), f_md, rtol=0, atol=1e-12 * np.abs(f_md).max())
+    # Euler on two particles 5 apart (cutoff 2.5): x_k = x_{k-1} + dt v, v fixed
+    s = po.OracleMD([(0.0, L)] * 3, 64, reneighbor_every=1, dt=0.005)
+    x = np.array([[5.0, 5.0, 5.0], [10.0, 5.0, 5.0]])
+    v = np.array([[0.3, -0.2, 0.1], [-0.1, 0.25, 0.05]])
+    s.set_particles(x, v)
+    s.setup_cells()
+    xr = x.copy()
+    for _ in range(7):
+        s.rebuild()
+        s.lj_older()
+        s.euler()
+        xr = xr + 0.005 * v
+    assert np.array_equal(s.velocities(), v)
+    assert np.array_equal(s.positions(), xr)
+
+
 def _brute_images(x, L, rlist):
     """All (i, j, image) pairs with |x_i - (x_j + image L)| < rlist, images in {-1,0,1}^3."""
     n = len(x)
