@@ -374,7 +374,7 @@ def main():
             "frac_of_measured_copy": (achieved / copy_gbps) if (achieved and copy_gbps) else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a "
                               "512 MiB copy (profiles/lj_fcc32_force_pmc.md)",
-            "kernel": "k_lj_full<8,1,1,false,true,false> (LJ force + final/initial integrate + next packed positions, fused)",
+            "kernel": "k_lj_full<8,1,1,false,true,false,false> (LJ force + final/initial integrate + next packed positions, fused)",
             "bytes_per_launch": fused_bytes,
             "lj_only_bytes_per_launch": lj_bytes,
             "avg_launch_ms": full["force_ms"],
