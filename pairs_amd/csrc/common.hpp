@@ -24,24 +24,3 @@ __device__ __forceinline__ T wave_sum(T v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
-
-// 1 / x rounded to nearest, for x in [2^-700, 2^700]: the hardware
-// reciprocal estimate, two Newton-Raphson refinements e = 1 - x r,
-// r <- r + r e, and Markstein's final correction r + r (1 - x r) (P. Markstein,
-// "IA-64 and Elementary Functions", 2000, ch. 8).  These are the steps of the
-// IEEE double division that hipcc emits for 1.0 / x; its v_div_scale /
-// v_div_fmas / v_div_fixup wrappers only rescale operands near the ends of the
-// exponent range or patch special values, and are the identity in this range,
-// so the result is the same correctly rounded quotient bit for bit in four
-// fewer instructions.  Outside the range (x = 0 included) the result is
-// meaningless: callers use it only where x is a squared distance below a
-// cutoff, or discard it.
-__device__ __forceinline__ double recip_rn(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(e, r, r);
-}

@@ -729,7 +729,7 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
     double rsq = dx * dx + dy * dy;
     rsq = rsq + dz * dz;
     const bool in = valid && rsq < rc2;
-    const double sr2 = recip_rn(rsq);  // == 1.0 / rsq wherever the pair counts
+    const double sr2 = 1.0 / rsq;
     double sr6 = sr2 * sr2;
     sr6 = sr6 * sr2;
     sr6 = sr6 * sg6;
@@ -803,11 +803,9 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
         seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
         const int T = r_off[NRUNS];
         const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
-        // the tile as byte offsets of the slots in a double array: a gather
-        // is then an SGPR base plus a 32-bit lane offset (slots < 2^28)
-        const uint32_t *toff = (const uint32_t *) smem;
+        int *tphys = (int *) smem;
         if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
-        if (fits) stage_slots<3>(r_start, r_off, T, (int *) smem);
+        if (fits) stage_slots(r_start, r_off, T, tphys);
         __syncthreads();
         const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
         const int nt = tab.nt;
@@ -828,21 +826,21 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
             // half the own part's sum (ue)
             auto walk = [&](const uint16_t *rows, int cnt, int cmax, bool mirror, double &a0, double &a1, double &a2) {
                 list_walk<U>(rows, ii, nl_stride, cnt, cmax, T, [&](int k, const int *on) {
-                    uint32_t jo[U];  // 8 x slot
+                    int jp[U];
 #pragma unroll
-                    for (int q = 0; q < U; q++) jo[q] = toff[on[q]];
+                    for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
                     double xj[U], yj[U], zj[U];
                     int tj[U];
 #pragma unroll
                     for (int q = 0; q < U; q++) {
                         if (PXY) {
-                            const double2 p = at(A.XY, jo[q] << 1);
+                            const double2 p = A.XY[jp[q]];
                             xj[q] = p.x; yj[q] = p.y;
                         } else {
-                            xj[q] = at(X, jo[q]); yj[q] = at(Y, jo[q]);
+                            xj[q] = X[jp[q]]; yj[q] = Y[jp[q]];
                         }
-                        zj[q] = at(Z, jo[q]);
-                        if (NT != 1) tj[q] = at(type, jo[q] >> 1);
+                        zj[q] = Z[jp[q]];
+                        if (NT != 1) tj[q] = type[jp[q]];
                     }
 #pragma unroll
                     for (int q = 0; q < U; q++) {
@@ -854,7 +852,7 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
                         a0 = a0 + g0; a1 = a1 + g1; a2 = a2 + g2;
                         if (RED && !HALF) { u += uu; w += ww; }
                         if (RED && HALF && !mirror) {
-                            const double wt = jo[q] < 8u * (uint32_t) nlocal ? 2.0 : 1.0;
+                            const double wt = jp[q] < nlocal ? 2.0 : 1.0;
                             u += wt * uu; w += wt * ww; ue += uu;
                         }
                     }
@@ -920,16 +918,9 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
 }
 
 
-// Waves per SIMD the register allocation must allow: 5 (at most 96 VGPRs)
-// for the one-type variants without energy sums, which fit that without
-// spilling; the others keep the compiler's choice (4), since holding them to
-// 96 VGPRs spills.
-template <int NT, bool RED, bool HALF>
-constexpr int lj_min_waves() { return NT == 1 && !RED && !HALF ? 5 : 1; }
-
 // one segment per workgroup of one system
 template <int U, int NT, int MODE, bool RED, bool PXY = true, bool HALF = false, bool OLDER = false>
-__global__ __launch_bounds__(FB, (lj_min_waves<NT, RED, HALF>())) void k_lj_full(LJArgs A) {
+__global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
     lj_full_segment<U, NT, MODE, RED, PXY, HALF, OLDER>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 

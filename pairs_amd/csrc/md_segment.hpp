@@ -83,20 +83,10 @@ __device__ __forceinline__ int run_of(const int *r_off, int p) {
 
 constexpr int FB = 64;   // threads per force workgroup (one wave)
 
-// base[bytes / sizeof(T)] for a 32-bit byte offset (a zero-extended lane
-// offset from a scalar base: one global load, no 64-bit address arithmetic)
-template <typename T>
-__device__ __forceinline__ T at(const T *base, uint32_t bytes) {
-    return *(const T *) ((const char *) base + bytes);
-}
-
-// tile entry p -> its slot, shifted left by SH (SH = 3: the byte offset of
-// the slot in an array of doubles)
-template <int SH = 0>
 __device__ __forceinline__ void stage_slots(const int *r_start, const int *r_off, int T, int *tslot) {
     for (int p = threadIdx.x; p < T; p += blockDim.x) {
         const int r = run_of(r_off, p);
-        tslot[p] = (r_start[r] + (p - r_off[r])) << SH;
+        tslot[p] = r_start[r] + (p - r_off[r]);
     }
 }
 

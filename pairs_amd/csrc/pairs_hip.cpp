@@ -1335,9 +1335,16 @@ static int batch_tables(pairs_hip_batch *b) {
     return 0;
 }
 
-// descriptors of both phases (steady steps: final and initial integrate on)
+// descriptors of both phases (steady steps: final and initial integrate on).
+// Each descriptor's tcap is its replica's tile_cap, which a rebuild may have
+// grown (set_tile_cap), and the force kernel stages any tile of up to tcap
+// entries; the batched launch sizes its LDS from max_tile, so max_tile is
+// taken here from the same tile_caps.
 static int batch_descs(pairs_hip_batch *b) {
     const int R = (int) b->rep.size();
+    int mt = 0;
+    for (int r = 0; r < R; r++) mt = std::max(mt, b->rep[r]->d.tile_cap);
+    b->max_tile = mt;
     b->base_cur.resize(R);
     for (int p = 0; p < 2; p++)
         for (int r = 0; r < R; r++) {
