@@ -4,7 +4,7 @@
  * This is the drop-in boundary for the modules that the reference's code
  * generator emits as CUDA kernels for the LJ molecular-dynamics path
  * (examples/md.py).  A host program (hand written, or emitted by the HIP code
- * generation target in pairs_amd/codegen) calls these functions in the order of
+ * generation target in pairs_amd/dsl.py, emit_cpp) calls these functions in the order of
  * the reference's timestep (src/pairs/sim/simulation.py:382-452):
  *
  *   pre-step  initial_integrate                  -> pairs_hip_vv_initial
@@ -75,7 +75,8 @@ enum {
 enum {
     PAIRS_HIP_NLOCAL = 0, PAIRS_HIP_NGHOST = 1, PAIRS_HIP_NCELLS = 2, PAIRS_HIP_NSEND_ALL = 3,
     PAIRS_HIP_NEIGHBOR_CAPACITY = 4, PAIRS_HIP_PARTICLE_CAPACITY = 5, PAIRS_HIP_SEND_CAPACITY = 6,
-    PAIRS_HIP_NEIGHBOR_TOTAL = 7
+    PAIRS_HIP_NEIGHBOR_TOTAL = 7,
+    PAIRS_HIP_TILE_CAPACITY = 8   /* cell-tile entries a force workgroup stages (grows at rebuilds) */
 };
 
 typedef struct pairs_hip_md_config {

@@ -61,10 +61,18 @@ def replica_state(natoms, rho, temp, seed):
 
 class BatchEnsemble:
     def __init__(self, sizes, densities, temps, seed=20241202, cutoff=2.5, skin=0.3, dt=0.005, reneighbor_every=20,
-                 thermo_every=100, neighbor_capacity=128):
+                 thermo_every=100, neighbor_capacity=128, particle_capacity=None, send_capacity=None):
+        """particle_capacity / send_capacity: None (8 N_r + 4096 each), one
+        value for every replica, or one per replica; a replica that outgrows
+        one grows it at its next rebuild (x2 the need)."""
         self.L = nat.lib()
         R = len(sizes)
         self.sizes = [int(n) for n in sizes]
+
+        def per_replica(v, r):
+            if v is None:
+                return 8 * self.sizes[r] + 4096
+            return int(v[r]) if hasattr(v, "__len__") else int(v)
         cfgs = (nat.MDConfig * R)()
         self._keep = []
         boxes, states = [], []
@@ -76,9 +84,9 @@ class BatchEnsemble:
             c = cfgs[r]
             n = self.sizes[r]
             dims = math.ceil((L + 2 * s) / s) + 1
-            c.particle_capacity = 8 * n + 4096
+            c.particle_capacity = per_replica(particle_capacity, r)
             c.neighbor_capacity = neighbor_capacity
-            c.send_capacity = 8 * n + 4096
+            c.send_capacity = per_replica(send_capacity, r)
             c.ncells_capacity = dims ** 3 + 64
             for d in range(3):
                 c.box_lo[d] = 0.0

@@ -803,6 +803,9 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
         seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
         const int T = r_off[NRUNS];
         const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
+        // a converged build keeps every tile within tcap; a segment that is
+        // not is left unwritten and reported (the host stops the run)
+        if (!fits && threadIdx.x == 0) atomicMax(&A.counters_rw[C_FORCE_SKIP], T);
         int *tphys = (int *) smem;
         if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
         if (fits) stage_slots(r_start, r_off, T, tphys);
@@ -1760,6 +1763,7 @@ LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, i
                int fused, double *eatom, bool packed) {
     LJArgs A;
     A.counters = d.counters;
+    A.counters_rw = d.counters;
     A.g = g;
     A.lstart = d.lstart; A.gstart = d.gstart;
     A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];

@@ -45,7 +45,8 @@ enum {
     C_TILE_MAX = 16,  // largest segment tile of the current lists
     C_UNSUPPORTED = 17,
     C_IMG_OVF = 18,   // a local has more periodic images than IMG_MAX
-    C_COUNT = 19
+    C_FORCE_SKIP = 19,  // a force kernel met a tile above its staged capacity (largest such tile)
+    C_COUNT = 20
 };
 enum { OVF_NONE = 0, OVF_PART = 1, OVF_NEIGH = 2, OVF_SEND = 3, OVF_TILE = 4 };
 
@@ -205,6 +206,7 @@ int force_blocks(const DevState &d, const Geom &g);
 // every argument of the full-list force kernel (one system)
 struct LJArgs {
     const int *counters;
+    int *counters_rw;    // the same counters (C_FORCE_SKIP)
     Geom g;
     const int *lstart, *gstart;
     const double *X, *Y, *Z;

@@ -230,6 +230,7 @@ constexpr float MLP_PIRC = 3.14159265358979323846f / (float) MLP_RC;
 
 struct MLPSegArgs {
     const int *counters;
+    int *counters_rw;    // the same counters (C_FORCE_SKIP)
     Geom g;
     const int *lstart, *gstart;
     const double *X, *Y, *Z;
@@ -263,6 +264,7 @@ __global__ __launch_bounds__(FB) void k_mlp_segment(MLPSegArgs A) {
         seg_runs(si, g, A.lstart, A.gstart, nlocal, r_start, r_len, r_off, col_lo);
         const int T = r_off[NRUNS];
         const bool fits = T <= A.tcap;
+        if (!fits && threadIdx.x == 0) atomicMax(&A.counters_rw[C_FORCE_SKIP], T);
         int *tphys = (int *) smem;
         if (fits) stage_slots(r_start, r_off, T, tphys);
         __syncthreads();
@@ -617,6 +619,7 @@ __global__ __launch_bounds__(256) void k_force_diff(const double *__restrict__ a
 static MLPSegArgs seg_args(const DevState &d, const Geom &g, int cur) {
     MLPSegArgs A{};
     A.counters = d.counters;
+    A.counters_rw = d.counters;
     A.g = g;
     A.lstart = d.lstart;
     A.gstart = d.gstart;

@@ -535,6 +535,7 @@ int pairs_hip_count(pairs_hip_ctx *ctx, int what) {
         case PAIRS_HIP_NEIGHBOR_CAPACITY: return ctx->d.ncap;
         case PAIRS_HIP_PARTICLE_CAPACITY: return ctx->d.cap;
         case PAIRS_HIP_SEND_CAPACITY: return ctx->d.send_cap;
+        case PAIRS_HIP_TILE_CAPACITY: return ctx->d.tile_cap;
         case PAIRS_HIP_NEIGHBOR_TOTAL: {
             const int n = ctx->d.nlocal;
             std::vector<int> nn(n);
@@ -923,13 +924,34 @@ static int launch_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
     return 0;
 }
 
+// An overflow of the send map or of the particle arrays stops the cell phase
+// before the permutation that writes the other position buffer: the state to
+// retry from is still in the buffer the rebuild started on (the exchange
+// dimensions already done moved particles there in place, and repeating them
+// moves nothing).  Later overflows (tile, neighbour rows) come after the
+// permutation, whose buffer then holds the state.
+static bool overflow_before_permute(const int *h) { return h[C_OVF] == OVF_SEND || h[C_OVF] == OVF_PART; }
+
+// a force kernel found a segment tile larger than the LDS it was launched
+// with and left that segment's particles unwritten
+static int check_force_skip(pairs_hip_ctx *ctx) {
+    if (ctx->h_counters[C_FORCE_SKIP] == 0) return 0;
+    ctx->err = "a force kernel skipped a segment whose tile (" + std::to_string(ctx->h_counters[C_FORCE_SKIP]) +
+               " entries) exceeds the staged tile capacity (" + std::to_string(ctx->d.tile_cap) +
+               "): forces and positions of that step are invalid";
+    return PAIRS_HIP_ERR_STATE;
+}
+
 static int rebuild(pairs_hip_ctx *ctx) {
     if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     for (int attempt = 0; attempt < 8; attempt++) {
         // one counter read for the whole rebuild: after an overflow it is
         // repeated with grown capacities
+        const int cur0 = ctx->d.cur;
         if (int r = launch_rebuild(ctx, ctx->stream)) return r;
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+        if (int r = check_force_skip(ctx)) return r;
+        if (overflow_before_permute(ctx->h_counters)) ctx->d.cur = cur0;
         // images refresh the ghosts from the position kernels unless some
         // local has too many of them (then a separate refresh runs)
         ctx->d.img_on = ctx->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
@@ -1117,6 +1139,8 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         if (vv1) { d.cur = other; prev_in_other = true; }
     }
     HIP_OK(hipStreamSynchronize(s));
+    if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
+    if (int r = check_force_skip(ctx)) return r;
     if (thermo_out && nthermo > 0) {
         const int n = std::min(nthermo, thermo_capacity);
         HIP_OK(hipMemcpy(thermo_out, d.thermo, 5 * n * sizeof(double), hipMemcpyDeviceToHost));
@@ -1335,6 +1359,25 @@ static int batch_tables(pairs_hip_batch *b) {
     return 0;
 }
 
+// Before a batched force launch: the launch allocates max_tile entries of LDS
+// per workgroup, and each replica's kernel stages up to its tcap entries, so
+// every tcap must be <= max_tile; and every replica's largest tile of its
+// current lists (C_TILE_MAX of its last rebuild) must be <= its tcap, or the
+// kernel would skip that segment.
+static int batch_check_tiles(pairs_hip_batch *b, const LJArgs *desc) {
+    for (size_t r = 0; r < b->rep.size(); r++) {
+        const pairs_hip_ctx *c = b->rep[r];
+        const int tmax = c->h_counters[C_TILE_MAX];
+        if (desc[r].tcap > b->max_tile || tmax > desc[r].tcap || desc[r].tcap != c->d.tile_cap) {
+            b->err = "replica " + std::to_string(r) + ": tile capacity " + std::to_string(desc[r].tcap) +
+                     " does not cover its largest tile " + std::to_string(tmax) + " within the launch's " +
+                     std::to_string(b->max_tile) + " LDS entries";
+            return PAIRS_HIP_ERR_STATE;
+        }
+    }
+    return 0;
+}
+
 // descriptors of both phases (steady steps: final and initial integrate on).
 // Each descriptor's tcap is its replica's tile_cap, which a rebuild may have
 // grown (set_tile_cap), and the force kernel stages any tile of up to tcap
@@ -1353,6 +1396,8 @@ static int batch_descs(pairs_hip_batch *b) {
             b->h_desc[p * R + r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, 1, 1, 1, nullptr, false);
         }
     for (int r = 0; r < R; r++) b->base_cur[r] = b->rep[r]->d.cur;
+    for (int p = 0; p < 2; p++)
+        if (int e = batch_check_tiles(b, b->h_desc + p * R)) return e;
     for (int p = 0; p < 2; p++)
         BATCH_OK(hipMemcpyAsync(b->d_desc[p], b->h_desc + p * R, R * sizeof(LJArgs), hipMemcpyHostToDevice, b->stream));
     BATCH_OK(hipStreamSynchronize(b->stream));
@@ -1394,7 +1439,11 @@ static int batch_rebuild(pairs_hip_batch *b) {
     BATCH_OK(hipMemcpyAsync(b->d_rdesc, b->h_rdesc, R * sizeof(RepDesc), hipMemcpyHostToDevice, s));
     b->rdesc_valid = true;
     launch_cells_batch(s, b->d_rdesc, R, max_cap, max_ncells);
-    for (pairs_hip_ctx *c : b->rep) c->d.cur = 1 - c->d.cur;  // the permutation moved every replica
+    std::vector<int> cur0(R);
+    for (int r = 0; r < R; r++) {
+        cur0[r] = b->rep[r]->d.cur;
+        b->rep[r]->d.cur = 1 - cur0[r];  // the permutation moved every replica
+    }
     size_t lds = 0;
     for (int r = 0; r < R; r++) {
         pairs_hip_ctx *c = b->rep[r];
@@ -1417,10 +1466,17 @@ static int batch_rebuild(pairs_hip_batch *b) {
     std::vector<char> fallback(R, 0);
     for (int r = 0; r < R; r++) {
         pairs_hip_ctx *c = b->rep[r];
+        if (check_force_skip(c)) { b->err = "replica " + std::to_string(r) + ": " + c->err; return PAIRS_HIP_ERR_STATE; }
         c->d.img_on = c->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
         if (c->h_counters[C_OVF] != 0 || c->h_counters[C_REQ_NEIGH] > 0) {
             // an overflow anywhere: the replica repeats its rebuild on its
-            // own, growing what it needs
+            // own, growing what it needs, from the buffer that holds its
+            // state (the one it started on when the cell phase stopped
+            // before the permutation)
+            if (overflow_before_permute(c->h_counters)) c->d.cur = cur0[r];
+            // its arrays, segments or tile capacity may change: the batched
+            // thermo descriptors must be rebuilt before their next use
+            b->rdesc_valid = false;
             launch_set_counters(s, c->d.counters, C_OVF, 1, 0);
             launch_set_counters(s, c->d.counters, C_REQ_NEIGH, 3, 0);
             fallback[r] = 1;
@@ -1597,6 +1653,7 @@ int pairs_hip_batch_run(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps
                 const int cur = b->base_cur[r] ^ phase;
                 b->h_desc[r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, vv2, vv1, 1, nullptr, false);
             }
+            if (int e = batch_check_tiles(b, b->h_desc)) return e;
             BATCH_OK(hipMemcpyAsync(b->d_desc[2], b->h_desc, R * sizeof(LJArgs), hipMemcpyHostToDevice, s));
             BATCH_OK(hipStreamSynchronize(s));  // h_desc is reused by the next descriptor build
             launch_lj_full_batch(s, b->d_desc[2], b->d_rep, b->d_base, b->nblocks, b->max_tile, b->ntypes1, T ? 1 : 0);

@@ -18,7 +18,8 @@ ERR_HIP, ERR_ARG, ERR_STATE = -1, -2, -3
 # download selectors
 POSITION, VELOCITY, FORCE, MASS, TYPE, FLAGS, UID, NUMNEIGHS, NEIGHBORLISTS, PARTICLE_CELL, \
     CELL_SIZES, CELL_PARTICLES, SEND_MAP, SEND_MULT, NSEND, ATOM_ENERGY = range(16)
-NLOCAL, NGHOST, NCELLS, NSEND_ALL, NEIGHBOR_CAPACITY, PARTICLE_CAPACITY, SEND_CAPACITY, NEIGHBOR_TOTAL = range(8)
+NLOCAL, NGHOST, NCELLS, NSEND_ALL, NEIGHBOR_CAPACITY, PARTICLE_CAPACITY, SEND_CAPACITY, NEIGHBOR_TOTAL, TILE_CAPACITY = \
+    range(9)
 
 FLAG_INFINITE, FLAG_GHOST, FLAG_FIXED, FLAG_GLOBAL = 1, 2, 4, 8
 

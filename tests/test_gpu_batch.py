@@ -85,3 +85,57 @@ def test_run_each_and_autotune_keep_trajectories():
     C.run_each(0, 121, 400)
     for r in range(len(SIZES)):
         _same(A.replicas[r], C.replicas[r], r)
+
+
+def _oracle_replica(B, r, ts_end):
+    L = B.boxes[r]
+    n = B.sizes[r]
+    o = po.OracleMD([(0.0, L)] * 3, 8 * n + 4096, ncap=128, send_cap=8 * n + 4096)
+    x, v = B.initial[r]
+    o.set_particles(x, v)
+    o.setup_cells()
+    o.run(0, ts_end)
+    return o
+
+
+def test_batched_overflow_retry_and_tile_growth():
+    """Replicas whose send map (replica 0) or particle arrays (replica 1)
+    overflow at the rebuild of ts = 19, while the other buffer holds ts = 18,
+    retry on their own from the current state; a larger segment size set
+    mid-run makes every replica's tiles outgrow their staged capacity at the
+    rebuild of ts = 59, so the batched launch runs with grown tile_caps.  All
+    three replicas stay bit-identical to the oracle."""
+    from pairs_amd import native as nat
+    sizes, rho, temp = [500, 864, 1372], [0.55, 0.6, 0.8442], [1.44, 2.0, 1.44]
+    bm = _batch()
+    probe = bm.BatchEnsemble(sizes, rho, temp)
+    need = []
+    for r in range(3):
+        o = _oracle_replica(probe, r, 1)
+        need.append((o.nsend_all, o.nlocal + o.nghost))
+        o.run(1, 20)
+        assert r == 2 or (o.nsend_all > need[r][0] and o.nlocal + o.nghost > need[r][1])
+    probe.close()
+    send = [need[0][0], 8 * sizes[1] + 4096, 8 * sizes[2] + 4096]
+    part = [8 * sizes[0] + 4096, need[1][1], 8 * sizes[2] + 4096]
+    B = bm.BatchEnsemble(sizes, rho, temp, send_capacity=send, particle_capacity=part)
+    B.run(0, 41, 300)
+    assert B.replicas[0].count(nat.SEND_CAPACITY) > need[0][0]
+    assert B.replicas[1].count(nat.PARTICLE_CAPACITY) > need[1][1]
+    for r in range(3):
+        o = _oracle_replica(B, r, 41)
+        g = B.replicas[r]
+        assert np.array_equal(g.positions(True), o.positions(True)), r
+        assert np.array_equal(g.velocities(), o.velocities()), r
+        assert np.array_equal(g.forces(), o.forces()), r
+    caps = [B.replicas[r].count(nat.TILE_CAPACITY) for r in range(3)]
+    B.set_home(400)
+    B.run(41, 101, 300)
+    grown = [B.replicas[r].count(nat.TILE_CAPACITY) for r in range(3)]
+    assert any(gc > c for gc, c in zip(grown, caps)), (caps, grown)
+    for r in range(3):
+        o = _oracle_replica(B, r, 101)
+        g = B.replicas[r]
+        assert np.array_equal(g.positions(True), o.positions(True)), r
+        assert np.array_equal(g.velocities(), o.velocities()), r
+        assert np.array_equal(g.forces(), o.forces()), r

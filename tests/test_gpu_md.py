@@ -135,6 +135,45 @@ def test_resize_protocol_small_capacities():
     assert g.count(6) >= g.nsend_all and g.count(4) > 8
 
 
+@pytest.mark.parametrize("which", ["send", "particles"])
+def test_overflow_at_a_later_rebuild(which):
+    """A send-map or particle overflow at the rebuild of ts = 19, when the
+    other position buffer holds the positions of ts = 18: the retry must start
+    from the current state.  At density 0.55 the FCC 4^3 lattice sends 843
+    particles at ts = 0 and more at ts = 19 (the oracle counts them), so a
+    capacity that fits ts = 0 exactly overflows at ts = 19."""
+    md = _md()
+    o = po.fcc_system(4, rho=0.55)
+    o.step(0)
+    nsend0, nall0 = o.nsend_all, o.nlocal + o.nghost
+    for ts in range(1, 20):
+        o.step(ts)
+    assert o.nsend_all > nsend0 and o.nlocal + o.nghost > nall0
+    kw = dict(send_capacity=nsend0) if which == "send" else dict(particle_capacity=nall0)
+    g = md.fcc_system(4, rho=0.55, **kw)
+    g.run(0, 1, 60)
+    from pairs_amd import native as nat
+    kind = nat.SEND_CAPACITY if which == "send" else nat.PARTICLE_CAPACITY
+    cap0 = g.count(kind)
+    assert cap0 == (nsend0 if which == "send" else nall0)  # ts = 0 fitted
+    g.run(1, 61, 60)
+    assert g.count(kind) > cap0  # grown at a later rebuild
+    o = po.fcc_system(4, rho=0.55)
+    o.run(0, 61)
+    _compare_state(g, o, tag=f"overflow {which}")
+    _compare_lists(g, o, tag=f"overflow {which}")
+
+
+def test_force_kernel_reports_no_skipped_segment():
+    """The force kernel never meets a tile above its staged capacity after a
+    converged build; the run checks the device flag and would fail otherwise."""
+    md = _md()
+    g = md.fcc_system(8)
+    g.run(0, 41, 40)
+    from pairs_amd import native as nat
+    assert g.count(nat.TILE_CAPACITY) >= 64  # staged entries, grown from the builds
+
+
 @pytest.mark.parametrize("half", [False, True])
 def test_energy_and_virial_vs_oracle(half):
     md = _md()
