@@ -40,11 +40,11 @@ def force_kernel_bytes(n, g, s, fused):
     indices + 24N force write.  The fused kernel of the timed loop also does
     final_integrate + initial_integrate of its own atoms and writes the next
     positions of its atoms and of their ghost images: + 8N mass + 48N velocity
-    read/write + 24(N+G) position write + 16(N+G) packed (x, y) write.
+    read/write + 24(N+G) position write + 32(N+G) packed (x, y, z, 0) write.
     """
     b = 24 * (n + g) + 4 * n + 4 * s + 24 * n
     if fused:
-        b += 8 * n + 48 * n + 24 * (n + g) + 16 * (n + g)
+        b += 8 * n + 48 * n + 24 * (n + g) + 32 * (n + g)
     return b
 
 
@@ -65,18 +65,25 @@ def profiled_traffic():
     return None
 
 
-def run_gpu(steps, warmup, half, time_every):
+def run_gpu(steps, warmup, half, time_every, time_steps):
+    """warmup steps, graph capture (set-up), exactly `steps` timed steps with
+    nothing else in the window, then a separate pass of `time_steps` steps
+    that times every `time_every`-th force launch with HIP events"""
     from pairs_amd import md
 
     sim = md.fcc_system(NX, half=half)
-    nsteps = warmup + steps
-    sim.run(0, warmup, nsteps)
+    nsteps = warmup + steps + time_steps
+    if warmup > 0:
+        sim.run(0, warmup, nsteps)
+        sim.prepare_graphs()  # set-up: both buffers' step and rebuild graphs
     sim.sync()
-    sim.set_force_timing(time_every)
     t0 = time.perf_counter()
-    thermo = sim.run(warmup, nsteps, nsteps)
+    thermo = sim.run(warmup, warmup + steps, nsteps)
     sim.sync()
     t1 = time.perf_counter()
+    sim.set_force_timing(time_every)
+    sim.run(warmup + steps, nsteps, nsteps)
+    sim.sync()
     ms, launches = sim.force_kernel_time()
     n = sim.nlocal
     g = sim.nghost
@@ -214,26 +221,24 @@ def run_dem(steps):
 
 
 def run_dem_settling(steps):
-    """The DEM settling workload defined for this path's measurement: about
-    200 000 spheres (d ~ U[1.8, 2.2] mm on a 2.45 mm SC grid, 2500 kg/m3, at
-    rest) in a 0.1 x 0.1 x 0.3 m box, periodic in x and y, a floor plane at
-    z = 0, restitution 0.6, friction 0.5 with history, dt 1e-6 s.  The contact
-    model is examples/dem.py's spring-dashpot, whose stiffness follows from the
-    collision time: t_c is chosen so that k_n = 1e4 N/m at the mean effective
-    mass (k_n = m_eff (pi^2 + ln^2 e) / t_c^2)."""
+    """The DEM settling workload: 200 000 spheres, radii uniform in
+    [0.9, 1.1] mm, 2500 kg/m3, at rest at seeded random non-overlapping
+    positions (a 40 x 40 x 125 grid of pitch 2.5 x 2.5 x 2.4 mm, each
+    coordinate jittered uniformly by +-0.1 mm, MT19937-64 seed 7) in a
+    0.1 x 0.1 x 0.3 m box, periodic in x and y, a floor plane at z = 0,
+    gravity -9.81 z, linear spring-dashpot with fixed k_n = 1e4 N/m and the
+    damping of restitution 0.6, Coulomb friction 0.5 with per-contact history,
+    dt 1e-6 s."""
     from pairs_amd import dem
 
-    d, rho, e, kn = 0.002, 2500.0, 0.6, 1.0e4
-    meff = 0.5 * rho * math.pi / 6.0 * d ** 3
-    tc = math.sqrt(meff * (math.pi ** 2 + math.log(e) ** 2) / kn)
-    params = dict(diameter=d, gravity=9.81, density_fluid=0.0, density_particle=rho, spacing=0.00245,
-                  initial_velocity=0.0, dt=1e-6, friction=0.5, restitution=e, collision_time=tc,
-                  friction_static=0.0)
+    params = dict(diameter=0.002, gravity=9.81, density_fluid=0.0, density_particle=2500.0, dt=1e-6,
+                  friction=0.5, restitution=0.6, friction_static=0.0, stiffness_norm=1e4)
     box = (0.1, 0.1, 0.3)
     s = dem.DEMSystem(box, periodic=(True, True, False), particle_capacity=320000, send_capacity=80000,
                       params=params)
-    s.sc_grid(box)
-    s.add_halfspace(100000, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    s.jittered_grid((40, 40, 125), (0.00125, 0.00125, 0.0012), (0.0025, 0.0025, 0.0024), 1e-4, 0.9e-3, 1.1e-3,
+                    seed=7)
+    s.add_halfspace(1000000, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
     s.setup()
     n = s.nlocal
     s.set_timing(50)
@@ -251,12 +256,12 @@ def run_dem_settling(steps):
         "ms_per_step": t * 1e3 / steps,
         "phase_ms": dict(zip(("prepare", "contact_search", "history_slots", "spring_dashpot", "sums", "euler"),
                              phases)),
-        "spheres": n,
+        "particles": n,
         "steps": steps,
         "contacts_at_end": int(nc.sum()),
-        "collision_time_s": tc,
-        "workload": "DEM settling: d ~ U[1.8, 2.2] mm, 2500 kg/m3, 0.1 x 0.1 x 0.3 m, periodic x/y, floor plane, "
-                    "k_n 1e4 N/m at the mean effective mass, e 0.6, mu 0.5 with history, dt 1e-6 s, FP64",
+        "workload": "DEM settling: 200000 spheres r ~ U[0.9, 1.1] mm, 2500 kg/m3, jittered grid (seed 7), "
+                    "0.1 x 0.1 x 0.3 m, periodic x/y, floor plane, k_n 1e4 N/m, e 0.6, mu 0.5 with history, "
+                    "dt 1e-6 s, FP64",
     }
 
 
@@ -314,7 +319,9 @@ def main():
     ap.add_argument("--no-half", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=200)
-    ap.add_argument("--time-every", type=int, default=10)
+    ap.add_argument("--time-every", type=int, default=5,
+                    help="in the force-timing pass after the timed steps, time every n-th force launch")
+    ap.add_argument("--time-steps", type=int, default=200, help="steps of the force-timing pass")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--no-dem", action="store_true")
@@ -341,7 +348,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     copy_gbps = measured_copy_gbps()
-    full = run_gpu(args.steps, args.warmup, False, args.time_every)
+    full = run_gpu(args.steps, args.warmup, False, args.time_every, args.time_steps)
     value = NATOMS * args.steps / full["seconds"]
     fused_bytes = force_kernel_bytes(full["nlocal"], full["nghost"], full["nneigh"], True)
     lj_bytes = force_kernel_bytes(full["nlocal"], full["nghost"], full["nneigh"], False)
@@ -374,7 +381,10 @@ def main():
             "frac_of_measured_copy": (achieved / copy_gbps) if (achieved and copy_gbps) else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a "
                               "512 MiB copy (profiles/lj_fcc32_force_pmc.md)",
-            "kernel": "k_lj_full<8,1,1,false,true,false,false> (LJ force + final/initial integrate + next packed positions, fused)",
+            "kernel": "k_lj<1,1,false,false,false,true> (LJ force + final/initial integrate + next packed positions, "
+                      "fused; one lane per local, 64 locals per wave)",
+            "timing": "HIP events around every 5th force launch of a separate pass of 200 steps after the timed "
+                      "steps (eager launches on the context stream)",
             "bytes_per_launch": fused_bytes,
             "lj_only_bytes_per_launch": lj_bytes,
             "avg_launch_ms": full["force_ms"],
@@ -382,7 +392,7 @@ def main():
         },
     }
     if not args.no_half:
-        half = run_gpu(args.steps, args.warmup, True, args.time_every)
+        half = run_gpu(args.steps, args.warmup, True, args.time_every, args.time_steps)
         hb = force_kernel_bytes(half["nlocal"], half["nghost"], half["nneigh"], False)
         rec["half_list"] = {
             "value": NATOMS * args.steps / half["seconds"],

@@ -71,6 +71,13 @@ typedef struct pairs_hip_dem_config {
     double kappa;           /* tangential / normal stiffness ratio                    */
     double ln_restitution;  /* lnDryResCoeff                                          */
     double collision_time;
+    /* > 0: a fixed normal stiffness k_n (N/m) instead of the one the collision
+       time gives, with the damping that keeps the restitution coefficient,
+       gamma_n = -2 ln(e) sqrt(m_eff k_n) / sqrt(pi^2 + ln(e)^2) -- the same
+       spring-dashpot kernel re-parameterised (no reference counterpart:
+       examples/dem.py:18-74 derives both from collision_time); 0: from
+       collision_time as the reference does */
+    double stiffness_norm;
     int ntypes;
     double friction_static[16], friction_dynamic[16]; /* [t_i * ntypes + t_j] */
     void *stream;           /* hipStream_t or NULL (the context creates one)         */
@@ -89,6 +96,18 @@ int pairs_hip_dem_last_required(const pairs_hip_dem_ctx *ctx, int which_overflow
 int pairs_hip_dem_sc_grid(pairs_hip_dem_ctx *ctx, double xmax, double ymax, double zmax, double spacing,
                           double diameter, double min_diameter, double max_diameter, double initial_velocity,
                           double particle_density, int ntypes);
+/* Spheres at seeded random non-overlapping positions (no reference
+   counterpart; a build-defined initial state): a grid of nx * ny * nz sites
+   origin + (i, j, k) * spacing (x fastest), each moved by a uniform jitter in
+   [-jitter, jitter] per coordinate, radii uniform in [r_min, r_max], mass
+   4/3 pi r^3 density, at rest, uid from 1 in site order, type 0.  Uniform
+   numbers are (g() >> 11) * 2^-53 of std::mt19937_64(seed), four per site in
+   the order x, y, z jitter then radius.  Non-overlapping when every spacing
+   exceeds 2 r_max + 2 jitter.  Sites outside the domain are dropped (as
+   read_particle_data drops rows).  Returns the number of locals, or < 0. */
+int pairs_hip_dem_jittered_grid(pairs_hip_dem_ctx *ctx, int nx, int ny, int nz, const double origin[3],
+                                const double spacing[3], double jitter, double r_min, double r_max,
+                                double density, unsigned long long seed);
 /* One row of pairs::read_particle_data (runtime/read_from_file.hpp:33-115) for a
    half-space with attributes uid, type, mass, position, normal, flags. */
 int pairs_hip_dem_add_halfspace(pairs_hip_dem_ctx *ctx, int uid, int type, double mass, const double position[3],

@@ -17,6 +17,14 @@
  *     tests/golden/dem_sc168.npz (bit-exact), the closed-form sphere mass and
  *     inertia; the grid generator's std::mt19937 is checked against the
  *     published 10000th output of the default-seeded generator.
+ *   - odem_jittered_grid (build-defined initial state, no reference
+ *     counterpart): its MT19937-64 against the C++ standard's check value
+ *     (10000th output of the default seed 5489 is 9981545732273789042); site
+ *     count, bounds and non-overlap of the spheres it places.
+ *   - the fixed-stiffness contact form (odem_set_stiffness; the reference
+ *     derives k_n from a collision time): equal, to rounding, to the
+ *     collision-time form at the collision time that gives the same k_n for
+ *     the pair's effective mass; a sphere resting on a plane sinks W / k_n.
  *   - odem_step (exchange with contact-history migration, borders, cells with
  *     shape partition, gravity, contacts, Euler, history compaction): the
  *     golden trajectory of the reference's generated program (positions,
@@ -67,6 +75,7 @@ typedef struct odem {
     double *send_buf;
     /* parameters (examples/dem.py:113-140) */
     double dt, rho_p, rho_f, grav, pi, kappa, ln_e, t_c;
+    double kn_fixed, sqrt_pl; /* fixed normal stiffness (> 0) and sqrt(pi^2 + ln^2 e) */
     int ntypes;
     double mu_s[16], mu_d[16];
     int overflow;
@@ -149,6 +158,47 @@ static uint32_t mt_next(odem_t *s) {
 
 uint32_t odem_mt_next(odem_t *s) { return mt_next(s); }
 
+/* MT19937-64 (Nishimura 2000; the generator std::mt19937_64 defines): state
+   of 312 words, seeded by x_i = 6364136223846793005 (x_{i-1} ^ (x_{i-1} >> 62)) + i */
+typedef struct { uint64_t mt[312]; int mti; } mt64_t;
+
+static void mt64_seed(mt64_t *g, uint64_t seed) {
+    g->mt[0] = seed;
+    for (int i = 1; i < 312; i++) g->mt[i] = 6364136223846793005ull * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t) i;
+    g->mti = 312;
+}
+
+static uint64_t mt64_next(mt64_t *g) {
+    if (g->mti >= 312) {
+        for (int k = 0; k < 312; k++) {
+            const uint64_t y = (g->mt[k] & 0xFFFFFFFF80000000ull) | (g->mt[(k + 1) % 312] & 0x7FFFFFFFull);
+            g->mt[k] = g->mt[(k + 156) % 312] ^ (y >> 1) ^ ((y & 1ull) ? 0xB5026F5AA96619E9ull : 0ull);
+        }
+        g->mti = 0;
+    }
+    uint64_t x = g->mt[g->mti++];
+    x ^= (x >> 29) & 0x5555555555555555ull;
+    x ^= (x << 17) & 0x71D67FFFEDA60000ull;
+    x ^= (x << 37) & 0xFFF7EEE000000000ull;
+    x ^= x >> 43;
+    return x;
+}
+
+/* the n-th output (n >= 1) of MT19937-64 seeded with seed (for checks) */
+uint64_t odem_mt64_nth(uint64_t seed, int n) {
+    mt64_t g;
+    mt64_seed(&g, seed);
+    uint64_t x = 0;
+    for (int k = 0; k < n; k++) x = mt64_next(&g);
+    return x;
+}
+
+/* fixed normal stiffness for the contact model (0 = from the collision time) */
+void odem_set_stiffness(odem_t *s, double kn) {
+    s->kn_fixed = kn > 0.0 ? kn : 0.0;
+    s->sqrt_pl = sqrt(s->pi * s->pi + s->ln_e * s->ln_e);
+}
+
 /* std::uniform_real_distribution<double>(a, b) over std::mt19937 as libstdc++
    (GCC 11) defines it: generate_canonical<double, 53> takes two 32-bit draws,
    sum = u0 + u1 * 2^32, r = sum / 2^64 (r >= 1 -> nextafter(1, 0)), then
@@ -216,6 +266,44 @@ int odem_sc_grid(odem_t *s, double xmax, double ymax, double zmax, double spacin
         }
         last_uid++;
     }
+    s->nlocal = np;
+    return np;
+}
+
+/* Spheres at seeded random non-overlapping positions, the build-defined
+   initial state of pairs_hip_dem_jittered_grid (no reference counterpart):
+   sites origin + (i, j, k) * spacing, x fastest, each coordinate moved by
+   (2u - 1) * jitter, radius r_min + (r_max - r_min) u, u = (g() >> 11) 2^-53
+   of MT19937-64(seed), four per site (x, y, z, radius); at rest; uid from 1
+   in site order; sites outside the domain dropped. */
+int odem_jittered_grid(odem_t *s, int nx, int ny, int nz, const double *origin, const double *spacing,
+                       double jitter, double r_min, double r_max, double density, uint64_t seed) {
+    mt64_t g;
+    mt64_seed(&g, seed);
+    int uid = 1, np = s->nlocal;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int site[3] = {i, j, k};
+                double q[3];
+                for (int d = 0; d < 3; d++) {
+                    const double u = (double) (mt64_next(&g) >> 11) * 0x1.0p-53;
+                    q[d] = (origin[d] + site[d] * spacing[d]) + (2.0 * u - 1.0) * jitter;
+                }
+                const double u = (double) (mt64_next(&g) >> 11) * 0x1.0p-53;
+                const double rad = r_min + (r_max - r_min) * u;
+                const int id = uid++;
+                if (!within_subdomain(s, q)) continue;
+                if (np >= s->cap) { s->overflow |= 1; s->nlocal = np; return np; }
+                s->uid[np] = id;
+                s->r[np] = rad;
+                s->m[np] = (((4.0 / 3.0) * 3.141592653589793) * rad * rad * rad) * density;
+                for (int d = 0; d < 3; d++) { s->x[3 * np + d] = q[d]; s->v[3 * np + d] = 0.0; s->w[3 * np + d] = 0.0; }
+                s->type[np] = 0;
+                s->flags[np] = 0;
+                s->shape[np] = SHAPE_SPHERE;
+                np++;
+            }
     s->nlocal = np;
     return np;
 }
@@ -576,7 +664,9 @@ static void spring_dashpot(odem_t *s, int i, int j, int slot, const double *n, d
     const double relt_len = sqrt((relt[0] * relt[0] + relt[1] * relt[1]) + relt[2] * relt[2]);
     const int cond_a = sticking && relt_len < 1e-08;
     const double meff = 1.0 / (1.0 / s->m[i] + 1.0 / s->m[j]);
-    const double kn = (meff * (s->pi * s->pi + s->ln_e * s->ln_e)) / (s->t_c * s->t_c);
+    /* stiffness from the collision time (examples/dem.py:34-35), or fixed */
+    const double kn = s->kn_fixed > 0.0 ? s->kn_fixed
+                                        : (meff * (s->pi * s->pi + s->ln_e * s->ln_e)) / (s->t_c * s->t_c);
     const double kt = s->kappa * kn;
     /* rotate the spring into the tangent plane, keep its length */
     const double *h = s->tsd + 3 * a;
@@ -587,7 +677,11 @@ static void spring_dashpot(odem_t *s, int i, int j, int slot, const double *n, d
     const double h2 = (h[0] * h[0] + h[1] * h[1]) + h[2] * h[2];
     const double scale = sqrt(h2 / rot2);
     for (int d = 0; d < 3; d++) ntsd[d] = s->dt * relt[d] + ((rot2 <= 0.0) ? 0.0 : rot[d] * scale);
-    const double dn = ((-(2.0) * meff) * s->ln_e) / s->t_c;
+    /* damping for the restitution coefficient (examples/dem.py:36), or for
+       the fixed stiffness: -2 ln(e) sqrt(m_eff k_n) / sqrt(pi^2 + ln^2 e),
+       which is the same value at the collision time that gives k_n */
+    const double dn = s->kn_fixed > 0.0 ? ((-(2.0) * s->ln_e) * sqrt(meff * kn)) / s->sqrt_pl
+                                        : ((-(2.0) * meff) * s->ln_e) / s->t_c;
     const double dt_ = sqrt(s->kappa) * dn;
     double ftls[3];
     for (int d = 0; d < 3; d++) ftls[d] = kt * ntsd[d] + dt_ * relt[d];

@@ -283,6 +283,13 @@ def _dem_lib():
                                   _D, _D]
     L.odem_mt_next.restype = ctypes.c_uint32
     L.odem_mt_next.argtypes = [ctypes.c_void_p]
+    L.odem_mt64_nth.restype = ctypes.c_uint64
+    L.odem_mt64_nth.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    L.odem_set_stiffness.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L.odem_jittered_grid.restype = ctypes.c_int
+    L.odem_jittered_grid.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_uint64]
     L._dem_ready = True
     return L
 
@@ -301,7 +308,7 @@ class OracleDEM:
     """The CPU oracle's DEM system (spheres + half-spaces, contact history)."""
 
     def __init__(self, domain, periodic=(1, 1, 0), cap=4096, ccap=20, cellcap=64, send_cap=4096,
-                 ncells_cap=200000, cfg=DEM_REF, ntypes=1, mu_s=None, mu_d=None):
+                 ncells_cap=200000, cfg=DEM_REF, ntypes=1, mu_s=None, mu_d=None, stiffness_norm=0.0):
         L = _dem_lib()
         p = dem_params(cfg)
         lo = (ctypes.c_double * 3)(0.0, 0.0, 0.0)
@@ -316,6 +323,7 @@ class OracleDEM:
                                ntypes, ms, md)
         self.L = L
         self.cap, self.ccap = cap, ccap
+        L.odem_set_stiffness(self.h, stiffness_norm)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -327,6 +335,12 @@ class OracleDEM:
         d = diameter if diameter is not None else c["diameter"]
         return self.L.odem_sc_grid(self.h, domain[0], domain[1], domain[2], spacing or c["spacing"], d, d * 0.9,
                                    d * 1.1, c["v0"] if v0 is None else v0, c["rho_p"], self.ntypes)
+
+    def jittered_grid(self, n, origin, spacing, jitter, r_min, r_max, density=None, seed=7):
+        o = (ctypes.c_double * 3)(*origin)
+        sp = (ctypes.c_double * 3)(*spacing)
+        return self.L.odem_jittered_grid(self.h, n[0], n[1], n[2], o, sp, jitter, r_min, r_max,
+                                         self.cfg["rho_p"] if density is None else density, seed)
 
     def add_halfspace(self, uid, pos, normal, flags=13, type_=0, mass=1.0):
         p = (ctypes.c_double * 3)(*pos)
@@ -405,3 +419,8 @@ def dem_reference_system(domain=(0.04, 0.015, 0.04), **kw):
     o.add_reference_planes()
     o.setup()
     return o
+
+
+def mt64_nth(seed, n):
+    """n-th output (n >= 1) of the oracle's MT19937-64 seeded with seed."""
+    return _dem_lib().odem_mt64_nth(seed, n)

@@ -140,6 +140,8 @@ int pairs_hip_dem_create(pairs_hip_dem_ctx **out, const pairs_hip_dem_config *cf
     prm.sqrt_kappa = std::sqrt(cfg->kappa);
     prm.ln_e = cfg->ln_restitution;
     prm.t_c = cfg->collision_time;
+    prm.kn_fixed = cfg->stiffness_norm > 0.0 ? cfg->stiffness_norm : 0.0;
+    prm.sqrt_pl = std::sqrt(prm.pi * prm.pi + prm.ln_e * prm.ln_e);
     prm.drho = cfg->density_particle - cfg->density_fluid;
     prm.grav = cfg->gravity;
     prm.ntypes = cfg->ntypes;
@@ -299,6 +301,37 @@ int pairs_hip_dem_sc_grid(pairs_hip_dem_ctx *ctx, double xmax, double ymax, doub
         }
         last_uid++;
     }
+    const int e = upload_rows(ctx, (int) uid.size(), uid, type, flags, shape, r, m, pos, vel, nrm);
+    return e ? e : ctx->nlocal;
+}
+
+int pairs_hip_dem_jittered_grid(pairs_hip_dem_ctx *ctx, int nx, int ny, int nz, const double origin[3],
+                                const double spacing[3], double jitter, double r_min, double r_max,
+                                double density, unsigned long long seed) {
+    if (!ctx || !origin || !spacing || nx < 0 || ny < 0 || nz < 0 || r_min <= 0.0 || r_max < r_min || jitter < 0.0)
+        return -2;
+    std::mt19937_64 gen(seed);
+    auto unit = [&]() { return (double) (gen() >> 11) * 0x1.0p-53; };
+    std::vector<int> uid, type, flags, shape;
+    std::vector<double> r, m, pos, vel, nrm;
+    int next_uid = 1;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int site[3] = {i, j, k};
+                double q[3];
+                for (int d = 0; d < 3; d++) q[d] = (origin[d] + site[d] * spacing[d]) + (2.0 * unit() - 1.0) * jitter;
+                const double rad = r_min + (r_max - r_min) * unit();
+                const int u = next_uid++;
+                if (!within_subdomain(ctx, q)) continue;
+                uid.push_back(u);
+                r.push_back(rad);
+                m.push_back(((4.0 / 3.0) * M_PI) * rad * rad * rad * density);
+                for (int d = 0; d < 3; d++) { pos.push_back(q[d]); nrm.push_back(0.0); vel.push_back(0.0); }
+                type.push_back(0);
+                flags.push_back(0);
+                shape.push_back(PAIRS_HIP_DEM_SPHERE);
+            }
     const int e = upload_rows(ctx, (int) uid.size(), uid, type, flags, shape, r, m, pos, vel, nrm);
     return e ? e : ctx->nlocal;
 }

@@ -452,7 +452,9 @@ __device__ __forceinline__ void spring_dashpot(const DemPtrs &p, const DemParams
     const double relt_len = sqrt((relt[0] * relt[0] + relt[1] * relt[1]) + relt[2] * relt[2]);
     const bool cond_a = sticking && relt_len < 1e-08;
     const double meff = 1.0 / (1.0 / p.m[i] + 1.0 / p.m[j]);
-    const double kn = (meff * (prm.pi * prm.pi + prm.ln_e * prm.ln_e)) / (prm.t_c * prm.t_c);
+    // stiffness from the collision time (examples/dem.py:34-35), or fixed
+    const double kn = prm.kn_fixed > 0.0 ? prm.kn_fixed
+                                         : (meff * (prm.pi * prm.pi + prm.ln_e * prm.ln_e)) / (prm.t_c * prm.t_c);
     const double kt = prm.kappa * kn;
     const double h[3] = {p.c_tsd[3 * (size_t) k * p.cap + i], p.c_tsd[(3 * (size_t) k + 1) * p.cap + i],
                          p.c_tsd[(3 * (size_t) k + 2) * p.cap + i]};
@@ -463,7 +465,10 @@ __device__ __forceinline__ void spring_dashpot(const DemPtrs &p, const DemParams
     const double h2 = (h[0] * h[0] + h[1] * h[1]) + h[2] * h[2];
     const double scale = sqrt(h2 / rot2);
     for (int d = 0; d < 3; d++) ntsd[d] = prm.dt * relt[d] + ((rot2 <= 0.0) ? 0.0 : rot[d] * scale);
-    const double dn = ((-(2.0) * meff) * prm.ln_e) / prm.t_c;
+    // damping for the restitution coefficient: from the collision time
+    // (examples/dem.py:36), or for the fixed stiffness
+    const double dn = prm.kn_fixed > 0.0 ? ((-(2.0) * prm.ln_e) * sqrt(meff * kn)) / prm.sqrt_pl
+                                         : ((-(2.0) * meff) * prm.ln_e) / prm.t_c;
     const double dtan = prm.sqrt_kappa * dn;
     double ftls[3];
     for (int d = 0; d < 3; d++) ftls[d] = kt * ntsd[d] + dtan * relt[d];

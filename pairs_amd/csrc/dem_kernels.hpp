@@ -46,6 +46,8 @@ struct DemGeom {
 
 struct DemParams {
     double dt, pi, kappa, sqrt_kappa, ln_e, t_c, drho, grav;
+    double kn_fixed;  // > 0: fixed normal stiffness (else from the collision time)
+    double sqrt_pl;   // sqrt(pi^2 + ln(e)^2) for the fixed-stiffness damping
     int ntypes;
     double mu_s[16], mu_d[16];
 };

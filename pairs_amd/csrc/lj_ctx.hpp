@@ -34,7 +34,6 @@ struct pairs_hip_ctx {
     // position buffer
     hipGraphExec_t reb_graph[2] = {nullptr, nullptr};
     int graph_ghost_grid[2] = {0, 0};
-    int graph_tile_cap[2] = {0, 0};
     int graph_img[2] = {0, 0};
     // force-kernel timing
     bool time_force = false;
@@ -49,7 +48,7 @@ struct pairs_hip_ctx {
     size_t expand_cap = 0;
     // MLP force field (include/pairs_hip_mlp.h): when on, every force
     // evaluation of the run uses the attached model instead of LJ
-    // XY[b] (packed x, y) matches X[b], Y[b] for every particle
+    // P[b] (packed x, y, z) matches X[b], Y[b], Z[b] for every particle
     bool xy_valid[2] = {false, false};
     bool mlp_on = false;
     MLPBind mlp{};

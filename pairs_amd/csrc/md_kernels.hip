@@ -361,7 +361,7 @@ __device__ __forceinline__ void write_images(const Images &im, int i, int nlocal
                                              const double *__restrict__ X, const double *__restrict__ Y,
                                              const double *__restrict__ Z, double *__restrict__ Xn,
                                              double *__restrict__ Yn, double *__restrict__ Zn, const Geom &g,
-                                             double2 *__restrict__ XYn = nullptr) {
+                                             double4 *__restrict__ Pn = nullptr) {
     const int n = min(im.cnt[i], IMG_MAX);
     for (int k = 0; k < n; k++) {
         const int q = im.slot[(size_t) k * im.cap + i];
@@ -370,10 +370,11 @@ __device__ __forceinline__ void write_images(const Images &im, int i, int nlocal
         const double bx = own ? px : X[src], by = own ? py : Y[src], bz = own ? pz : Z[src];
         const double x = bx + (double) im.mult[3 * q + 0] * g.len[0];
         const double y = by + (double) im.mult[3 * q + 1] * g.len[1];
+        const double z = bz + (double) im.mult[3 * q + 2] * g.len[2];
         Xn[nlocal + q] = x;
         Yn[nlocal + q] = y;
-        Zn[nlocal + q] = bz + (double) im.mult[3 * q + 2] * g.len[2];
-        if (XYn) XYn[nlocal + q] = make_double2(x, y);
+        Zn[nlocal + q] = z;
+        if (Pn) Pn[nlocal + q] = make_double4(x, y, z, 0.0);
     }
 }
 
@@ -425,7 +426,7 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
     const double *__restrict__ X = A.X, *__restrict__ Y = A.Y, *__restrict__ Z = A.Z;
     const int *__restrict__ ref = A.ref, *__restrict__ flags = A.flags;
     const double rc2 = A.rc2;
-    uint16_t *__restrict__ nl = A.nl;
+    uint32_t *__restrict__ nl = A.nl;
     const int nl_stride = A.nl_stride, ncap = A.ncap, lrows = A.lrows, tcap = A.tcap, zt = A.zt, rev0 = A.rev0;
     int *__restrict__ numneighs = A.numneighs, *__restrict__ ctr = A.counters_rw, *__restrict__ nrev = A.nrev;
     extern __shared__ char smem[];
@@ -563,7 +564,7 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
                         const int k = cnt + lane_rank(m);
                         if (acc && k < ncap) {
                             if (k < lr_own && idx < HCAP) lbuf[k * LBS + idx] = (uint16_t) o;
-                            else nl[nl_at(k, h0 + idx, nl_stride)] = (uint16_t) o;
+                            else nl[nl_at(k, h0 + idx, nl_stride)] = (uint32_t) tslot[o];
                         }
                         cnt += __popcll(m);
                         if (HALF) {
@@ -610,7 +611,7 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
                         for (int q = 0; q < cntr; q++) rank += __builtin_amdgcn_readlane(re, q) < re ? 1 : 0;
                         if (lane < cntr && rank < KR) {
                             if (rank < lr_mir && idx < HCAP) lbuf[(lr_own + rank) * LBS + idx] = (uint16_t) oe;
-                            else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint16_t) oe;
+                            else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint32_t) tslot[oe];
                         }
                     } else {
                         for (int e = lane; e < cntr; e += 64) {
@@ -620,7 +621,7 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
                             for (int q = 0; q < cntr; q++) rank += (tref[rbuf[q]] & REF_MASK) < re ? 1 : 0;
                             if (rank < KR) {
                                 if (rank < lr_mir && idx < HCAP) lbuf[(lr_own + rank) * LBS + idx] = (uint16_t) oe;
-                                else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint16_t) oe;
+                                else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint32_t) tslot[oe];
                             }
                         }
                     }
@@ -646,16 +647,17 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
         }
     }
     __syncthreads();
-    // buffered rows leave as packed groups of four (lane order = physical order)
+    // buffered rows leave as packed groups of four physical slots (lane
+    // order = physical order); rows of the last group past a particle's count
+    // hold earlier particles' entries or nothing: clamped into the tile
     const int nhome = min(h1 - h0, HCAP);
-    uint64_t *nl4 = (uint64_t *) nl;
+    uint4 *nl4 = (uint4 *) nl;
+    auto slot_of = [&](int r, int a) { return (uint32_t) tslot[min((int) lbuf[r * LBS + a], T - 1)]; };
     auto flush = [&](int a, int row0, int nrows, int grp0) {
         for (int gq = 0; gq < (nrows + 3) >> 2; gq++) {
             const int r = row0 + 4 * gq;
-            const uint64_t v = (uint64_t) lbuf[r * LBS + a] | ((uint64_t) lbuf[(r + 1) * LBS + a] << 16) |
-                               ((uint64_t) lbuf[(r + 2) * LBS + a] << 32) |
-                               ((uint64_t) lbuf[(r + 3) * LBS + a] << 48);
-            nl4[(size_t) (grp0 + gq) * nl_stride + h0 + a] = v;
+            nl4[(size_t) (grp0 + gq) * nl_stride + h0 + a] =
+                make_uint4(slot_of(r, a), slot_of(r + 1, a), slot_of(r + 2, a), slot_of(r + 3, a));
         }
     };
     for (int a = threadIdx.x; a < nhome; a += BB) {
@@ -680,34 +682,38 @@ __global__ __launch_bounds__(BB) void k_build_neighbors_batch(const BuildArgs *_
     build_segment<HALF>(desc[r], b - seg_base[r]);
 }
 
-// Expand the 16-bit tile-offset lists into reference indices (for download):
+// Expand the lists into reference indices (for download):
 // out[physical local][k] = reference index of neighbour k.
-__global__ __launch_bounds__(256) void k_expand_neighbors(const int *__restrict__ counters, Geom g,
-                                                          const int *__restrict__ lstart,
-                                                          const int *__restrict__ gstart,
+__global__ __launch_bounds__(256) void k_expand_neighbors(const int *__restrict__ counters,
                                                           const int *__restrict__ ref,
-                                                          const uint16_t *__restrict__ nl, int nl_stride,
+                                                          const uint32_t *__restrict__ nl, int nl_stride,
                                                           const int *__restrict__ numneighs,
                                                           int *__restrict__ out, int ncap_out) {
-    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     const int nlocal = counters[C_NLOCAL];
-    const int seg = blockIdx.x;
-    if (seg >= g.nsegs) return;
-    const SegInfo si = seg_info(seg, g);
-    const int h0 = lstart[si.ca], h1 = lstart[si.cb];
-    if (h1 <= h0) return;
-    seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
-    for (int i = h0 + threadIdx.x; i < h1; i += 256) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nlocal; i += gridDim.x * 256) {
         const int n = numneighs[i];
-        for (int k = 0; k < n && k < ncap_out; k++) {
-            const int o = nl[nl_at(k, i, nl_stride)];
-            const int r = run_of(r_off, o);
-            out[(size_t) i * ncap_out + k] = ref[r_start[r] + (o - r_off[r])];
-        }
+        for (int k = 0; k < n && k < ncap_out; k++) out[(size_t) i * ncap_out + k] = ref[nl[nl_at(k, i, nl_stride)]];
     }
 }
 
 // ---------------------------------------------------------------- Lennard-Jones
+
+// 1 / b rounded to nearest, for a finite b of magnitude in [2^-1000, 2^1000]
+// (squared pair distances): Newton-Raphson from the hardware estimate, two
+// iterations, then Markstein's final correction q = y + (1 - b y) y with
+// both steps fused.  This is the division sequence the compiler emits for
+// 1.0 / b with its operand-scaling and special-case fix-up steps left out,
+// which are identities in this range, so the bits are the same (and those
+// of the correctly rounded quotient the reference's CPU division gives).
+__device__ __forceinline__ double recip_rn(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double r = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(r, y, y);
+}
 
 // One pair of examples/md.py:5-8 as the reference emits it: delta,
 // rsq = dx*dx + dy*dy + dz*dz, sr2 = 1/rsq, sr6 = sr2*sr2*sr2*sigma6,
@@ -716,12 +722,14 @@ __global__ __launch_bounds__(256) void k_expand_neighbors(const int *__restrict_
 // (past the lane's list, or rsq >= rc2) gets s = 0, so its contribution is a
 // signed zero.  A running sum that starts at +0.0 is never -0.0, and t + (+-0)
 // == t exactly for every other t, so the masked pairs leave the sum
-// bit-identical to the reference's conditional add; without branches the U
-// divisions of a group overlap.
+// bit-identical to the reference's conditional add; without branches the
+// reciprocals of a step overlap.
+// UNIT: sigma6 = epsilon = 1 (one type with the reference's defaults); the
+// multiplications by them are then exact identities and are left out.
 // OLDER: the pair module of the older API (examples/lj_onetype.py:5-8),
 // force[i] += delta * 48.0 * sr6 * (sr6 - 0.5) * sr2 * epsilon evaluated left
 // to right, so each component is ((((d * 48) * sr6) * (sr6 - 0.5)) * sr2) * eps.
-template <bool ENERGY, bool OLDER = false>
+template <bool ENERGY, bool OLDER = false, bool UNIT = false>
 __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double xj, double yj, double zj, bool valid,
                                         double rc2, double sg6, double ep, double &g0, double &g1, double &g2,
                                         double &u, double &w) {
@@ -729,10 +737,10 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
     double rsq = dx * dx + dy * dy;
     rsq = rsq + dz * dz;
     const bool in = valid && rsq < rc2;
-    const double sr2 = 1.0 / rsq;
+    const double sr2 = recip_rn(rsq);
     double sr6 = sr2 * sr2;
     sr6 = sr6 * sr2;
-    sr6 = sr6 * sg6;
+    if (!UNIT) sr6 = sr6 * sg6;
     if (OLDER) {
         const double h = sr6 - 0.5;
         auto term = [&](double d) { return in ? (((d * 48.0) * sr6) * h) * sr2 * ep : 0.0; };
@@ -741,7 +749,7 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
         double s = 48.0 * sr6;
         s = s * (sr6 - 0.5);
         s = s * sr2;
-        s = s * ep;
+        if (!UNIT) s = s * ep;
         s = in ? s : 0.0;
         g0 = dx * s; g1 = dy * s; g2 = dz * s;
     }
@@ -752,16 +760,16 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
     return in;
 }
 
-// LJ force over the 16-bit tile-offset lists of one segment per block.
+// LJ force of the locals of one wave: lanes take 64 consecutive physical
+// locals (cell-sorted, so their neighbours are close in memory), each lane
+// walks its own list in order and adds its pair forces in list order, so the
+// sum is bit-identical to the reference's.  Neighbour positions are gathered
+// from the packed copy P (x, y, z, 0 at 32 bytes per particle): one 16-byte
+// and one 8-byte load per entry at a 32-bit byte offset.
 // MODE 0: forces only (lennard_jones with reset_volatile_properties);
 // MODE 1: fused F(ts) + final_integrate(ts) + [thermo] + initial_integrate(ts+1)
 //         writing the next positions into the other buffer;
 // MODE 2: energy and virial only (no writes but the partials).
-// One lane per home particle; the list is walked in groups of U entries to a
-// wave-uniform depth (entries past a lane's own count are masked), the next
-// group's offsets loading while the current one computes; each lane adds its
-// pair forces in list order, so the sum is bit-identical to the reference's.
-// PXY: gather (x, y) as one 16-byte load from the packed copy A.XY
 // HALF: the system uses half lists.  lennard_jones then adds, for local i,
 // each pair of its half list into its own sum T_i and the mirror force of
 // every earlier-index local j whose half list holds i into force[i] before
@@ -771,108 +779,92 @@ __device__ __forceinline__ bool lj_pair(double xi, double yi, double zi, double 
 // i's side -- the mirror pair's force recomputed from i is exactly the
 // negated force j computes -- so the result is the reference's bit for bit,
 // without atomics.
-template <int U, int NT, int MODE, bool RED, bool PXY = false, bool HALF = false, bool OLDER = false>
-__device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
-    static_assert(U == 4 || U == 8, "the list walk yields four or eight entries per step");
+template <int NT, int MODE, bool RED, bool HALF = false, bool OLDER = false, bool UNIT = false>
+__device__ __forceinline__ void lj_wave(const LJArgs &A, int wv) {
+    const int lane = threadIdx.x & 63;
     const int *__restrict__ counters = A.counters;
-    const Geom &g = A.g;
-    const int *__restrict__ lstart = A.lstart, *__restrict__ gstart = A.gstart;
+    const int nlocal = counters[C_NLOCAL];
+    const uint32_t jmax = (uint32_t) max(nlocal + counters[C_NGHOST] - 1, 0);
     const double *__restrict__ X = A.X, *__restrict__ Y = A.Y, *__restrict__ Z = A.Z;
     double *__restrict__ Xn = A.Xn, *__restrict__ Yn = A.Yn, *__restrict__ Zn = A.Zn;
     const int *__restrict__ type = A.type;
-    const uint16_t *__restrict__ nl = A.nl;
     const int nl_stride = A.nl_stride;
     const int *__restrict__ numneighs = A.numneighs, *__restrict__ flags = A.flags;
     double *__restrict__ fx = A.fx, *__restrict__ fy = A.fy, *__restrict__ fz = A.fz;
     double *__restrict__ vx = A.vx, *__restrict__ vy = A.vy, *__restrict__ vz = A.vz;
     const double *__restrict__ mass = A.mass;
+    const char *__restrict__ P = (const char *) A.P;
     const double rc2 = A.rc2, dt = A.dt, hdt = A.hdt;
     const LJTab &tab = A.tab;
-    const int do_vv2 = A.do_vv2, do_vv1 = A.do_vv1, pstride = A.pstride, tcap = A.tcap;
+    const int do_vv2 = A.do_vv2, do_vv1 = A.do_vv1, pstride = A.pstride;
     double *__restrict__ partials = A.partials;
     double *__restrict__ eatom = A.eatom;
     const Images &im = A.im;
-    extern __shared__ char smem[];
-    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     __shared__ double s_eps[16], s_sig6[16];
-    const int nlocal = counters[C_NLOCAL];
-    const SegInfo si = seg_info(seg, g);
-    const int h0 = lstart[si.ca], h1 = lstart[si.cb];
-    double ke = 0.0, pe = 0.0, vir = 0.0;
-    if (h1 > h0) {
-        seg_runs(si, g, lstart, gstart, nlocal, r_start, r_len, r_off, col_lo);
-        const int T = r_off[NRUNS];
-        const bool fits = T <= tcap;  // the launch sized LDS for tcap entries
-        // a converged build keeps every tile within tcap; a segment that is
-        // not is left unwritten and reported (the host stops the run)
-        if (!fits && threadIdx.x == 0) atomicMax(&A.counters_rw[C_FORCE_SKIP], T);
-        int *tphys = (int *) smem;
-        if (NT != 1 && threadIdx.x < 16) { s_eps[threadIdx.x] = tab.eps[threadIdx.x]; s_sig6[threadIdx.x] = tab.sig6[threadIdx.x]; }
-        if (fits) stage_slots(r_start, r_off, T, tphys);
+    if (NT != 1) {
+        if (lane < 16) { s_eps[lane] = tab.eps[lane]; s_sig6[lane] = tab.sig6[lane]; }
         __syncthreads();
+    }
+    const int i0 = wv * 64;
+    double ke = 0.0, pe = 0.0, vir = 0.0;
+    if (i0 < nlocal) {  // wave-uniform
+        const int i = i0 + lane;
+        const bool live = i < nlocal;
+        const int ii = live ? i : i0;
+        const int fl = live ? flags[ii] : FLAG_FIXED;
+        const bool moves = !(fl & FLAG_FIXED);
+        const int n = moves ? numneighs[ii] : 0;
+        const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
+        const double xi = X[ii], yi = Y[ii], zi = Z[ii];
+        const int ti = NT != 1 ? type[ii] : 0;
         const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
         const int nt = tab.nt;
-        for (int base = h0; fits && base < h1; base += FB) {
-            const int i = base + threadIdx.x;
-            const bool live = i < h1;
-            const int ii = live ? i : h0;
-            const int fl = live ? flags[ii] : FLAG_FIXED;
-            const int n = (live && !(fl & FLAG_FIXED)) ? numneighs[ii] : 0;
-            const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
-            const double xi = X[ii], yi = Y[ii], zi = Z[ii];
-            const int ti = NT != 1 ? type[ii] : 0;
-            double t0 = 0.0, t1 = 0.0, t2 = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0, u = 0.0, w = 0.0, ue = 0.0;
-            // one list part walked in order into (a0, a1, a2).  Energy and
-            // virial: a full list weighs every pair 1/2 (u, w are halved
-            // below); a half list weighs a local pair 1 and a local-ghost
-            // pair 1/2 over its own part only, and its per-atom energy is
-            // half the own part's sum (ue)
-            auto walk = [&](const uint16_t *rows, int cnt, int cmax, bool mirror, double &a0, double &a1, double &a2) {
-                list_walk<U>(rows, ii, nl_stride, cnt, cmax, T, [&](int k, const int *on) {
-                    int jp[U];
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0, u = 0.0, w = 0.0, ue = 0.0;
+        // one list part walked in order into (a0, a1, a2).  Energy and
+        // virial: a full list weighs every pair 1/2 (u, w are halved below);
+        // a half list weighs a local pair 1 and a local-ghost pair 1/2 over
+        // its own part only, and its per-atom energy is half the own part's
+        // sum (ue)
+        auto walk = [&](int row0, int cnt, int cmax, bool mirror, double &a0, double &a1, double &a2) {
+            list_walk32<8>(A.nl, row0, ii, nl_stride, cnt, cmax, jmax, [&](int k, const uint32_t *j) {
+                double xj[8], yj[8], zj[8];
+                int tj[8];
 #pragma unroll
-                    for (int q = 0; q < U; q++) jp[q] = tphys[on[q]];
-                    double xj[U], yj[U], zj[U];
-                    int tj[U];
-#pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        if (PXY) {
-                            const double2 p = A.XY[jp[q]];
-                            xj[q] = p.x; yj[q] = p.y;
-                        } else {
-                            xj[q] = X[jp[q]]; yj[q] = Y[jp[q]];
-                        }
-                        zj[q] = Z[jp[q]];
-                        if (NT != 1) tj[q] = type[jp[q]];
-                    }
-#pragma unroll
-                    for (int q = 0; q < U; q++) {
-                        double sg6 = sig0, ep = eps0;
-                        if (NT != 1) { const int kt = ti * nt + tj[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
-                        double g0, g1, g2, uu, ww;
-                        lj_pair<RED, OLDER>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < cnt, rc2, sg6, ep, g0, g1, g2, uu,
-                                            ww);
-                        a0 = a0 + g0; a1 = a1 + g1; a2 = a2 + g2;
-                        if (RED && !HALF) { u += uu; w += ww; }
-                        if (RED && HALF && !mirror) {
-                            const double wt = jp[q] < nlocal ? 2.0 : 1.0;
-                            u += wt * uu; w += wt * ww; ue += uu;
-                        }
-                    }
-                });
-            };
-            walk(nl, n, nmax, false, t0, t1, t2);
-            if (HALF) {
-                const int nr = (live && !(fl & FLAG_FIXED)) ? A.nrev[ii] : 0;
-                const int nrmax = __builtin_amdgcn_readfirstlane(wave_max(nr));
-                walk(nl + (size_t) A.rev0 * nl_stride, nr, nrmax, true, r0, r1, r2);
-            }
-            if (live) {
-                if (RED) { pe += 0.5 * u; vir += 0.5 * w; }
-                if (MODE == 2) {
-                    if (eatom) eatom[i] = 0.5 * (HALF ? ue : u);
-                    continue;
+                for (int q = 0; q < 8; q++) {
+                    const uint32_t off = j[q] << 5;
+                    const double2 p = *(const double2 *) (P + off);
+                    xj[q] = p.x;
+                    yj[q] = p.y;
+                    zj[q] = *(const double *) (P + off + 16);
+                    if (NT != 1) tj[q] = type[j[q]];
                 }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    double sg6 = sig0, ep = eps0;
+                    if (NT != 1) { const int kt = ti * nt + tj[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
+                    double g0, g1, g2, uu, ww;
+                    lj_pair<RED, OLDER, UNIT>(xi, yi, zi, xj[q], yj[q], zj[q], k + q < cnt, rc2, sg6, ep, g0, g1,
+                                              g2, uu, ww);
+                    a0 = a0 + g0; a1 = a1 + g1; a2 = a2 + g2;
+                    if (RED && !HALF) { u += uu; w += ww; }
+                    if (RED && HALF && !mirror) {
+                        const double wt = j[q] < (uint32_t) nlocal ? 2.0 : 1.0;
+                        u += wt * uu; w += wt * ww; ue += uu;
+                    }
+                }
+            });
+        };
+        walk(0, n, nmax, false, t0, t1, t2);
+        if (HALF) {
+            const int nr = moves ? A.nrev[ii] : 0;
+            const int nrmax = __builtin_amdgcn_readfirstlane(wave_max(nr));
+            walk(A.rev0, nr, nrmax, true, r0, r1, r2);
+        }
+        if (live) {
+            if (RED) { pe = 0.5 * u; vir = 0.5 * w; }
+            if (MODE == 2) {
+                if (eatom) eatom[i] = 0.5 * (HALF ? ue : u);
+            } else {
                 // reset_volatile_properties, the mirror contributions (half
                 // lists), then force[i] = force[i] + reduction
                 const double f0 = r0 + t0, f1 = r1 + t1, f2 = r2 + t2;
@@ -881,13 +873,13 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
                     const double m = mass[i];
                     double v0 = vx[i], v1 = vy[i], v2 = vz[i];
                     double px = xi, py = yi, pz = zi;
-                    if (!(fl & FLAG_FIXED) && do_vv2) {  // final_integrate(ts)
+                    if (moves && do_vv2) {  // final_integrate(ts)
                         double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
                         a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
                         v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
                     }
-                    if (RED) ke += m * (v0 * v0 + v1 * v1 + v2 * v2);
-                    if (!(fl & FLAG_FIXED)) {
+                    if (RED) ke = m * (v0 * v0 + v1 * v1 + v2 * v2);
+                    if (moves) {
                         if (do_vv1) {  // initial_integrate(ts+1)
                             double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
                             a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
@@ -898,33 +890,38 @@ __device__ __forceinline__ void lj_full_segment(const LJArgs &A, int seg) {
                     }
                     if (do_vv1) {
                         Xn[i] = px; Yn[i] = py; Zn[i] = pz;
-                        if (A.XYn) A.XYn[i] = make_double2(px, py);
-                        if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, g, A.XYn);
+                        if (A.Pn) A.Pn[i] = make_double4(px, py, pz, 0.0);
+                        if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, A.g, A.Pn);
                     }
                 }
             }
         }
     }
     if (RED) {
-        __shared__ double red[3][FB / 64];
+        // one partial per wave (kinetic row only from the fused kernel)
         const double a = wave_sum(ke), bb = wave_sum(pe), cc = wave_sum(vir);
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) { red[0][wave] = a; red[1][wave] = bb; red[2][wave] = cc; }
-        __syncthreads();
-        if (threadIdx.x < 3) {
-            const int r = threadIdx.x;
-            double s = 0.0;
-            for (int k = 0; k < FB / 64; k++) s += red[r][k];
-            if (MODE == 1 || r > 0) partials[(size_t) r * pstride + seg] = s;
+        if (lane == 0) {
+            if (MODE == 1) partials[wv] = a;
+            partials[(size_t) pstride + wv] = bb;
+            partials[(size_t) 2 * pstride + wv] = cc;
         }
     }
 }
 
+// one wave per workgroup, waves of one system
+template <int NT, int MODE, bool RED, bool HALF = false, bool OLDER = false, bool UNIT = false>
+__global__ __launch_bounds__(64) void k_lj(LJArgs A) {
+    lj_wave<NT, MODE, RED, HALF, OLDER, UNIT>(A, xcd_block(blockIdx.x, gridDim.x));
+}
 
-// one segment per workgroup of one system
-template <int U, int NT, int MODE, bool RED, bool PXY = true, bool HALF = false, bool OLDER = false>
-__global__ __launch_bounds__(FB) void k_lj_full(LJArgs A) {
-    lj_full_segment<U, NT, MODE, RED, PXY, HALF, OLDER>(A, xcd_block(blockIdx.x, gridDim.x));
+// the waves of many independent systems in one launch: block b belongs to
+// system rep[b] and is its wave b - wave_base[rep[b]]
+template <int NT, int MODE, bool RED, bool UNIT>
+__global__ __launch_bounds__(64) void k_lj_batch(const LJArgs *__restrict__ desc, const int *__restrict__ rep,
+                                                 const int *__restrict__ wave_base) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int r = rep[b];
+    lj_wave<NT, MODE, RED, false, false, UNIT>(desc[r], b - wave_base[r]);
 }
 
 // euler of the older API (examples/lj_onetype.py:11-13) for the non-fixed
@@ -954,32 +951,20 @@ void launch_euler_md(hipStream_t s, const DevState &d, double dt) {
 // forces only with the older API's pair expression (full lists)
 void launch_lj_older(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm) {
     if (d.nlocal <= 0) return;
-    launch_pack_xy(s, d, d.cur);
-    const size_t lds = (size_t) d.tile_cap * sizeof(int);
+    launch_pack_p(s, d, d.cur);
     const LJArgs A = lj_args(d, g, prm, d.cur, d.cur, 0, 0, 0, nullptr);
-    if (prm.ntypes == 1)
-        hipLaunchKernelGGL((k_lj_full<8, 1, 0, false, true, false, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
-    else
-        hipLaunchKernelGGL((k_lj_full<8, 0, 0, false, true, false, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    const dim3 grid(force_waves(d));
+    if (prm.ntypes == 1) hipLaunchKernelGGL((k_lj<1, 0, false, false, true>), grid, dim3(64), 0, s, A);
+    else hipLaunchKernelGGL((k_lj<0, 0, false, false, true>), grid, dim3(64), 0, s, A);
 }
 
-// the packed (x, y) copy the single-system force kernel gathers from
-__global__ __launch_bounds__(256) void k_pack_xy(const int *__restrict__ counters, const double *__restrict__ X,
-                                                 const double *__restrict__ Y, double2 *__restrict__ XY) {
+// the packed (x, y, z, 0) copy the force kernels gather from
+__global__ __launch_bounds__(256) void k_pack_p(const int *__restrict__ counters, const double *__restrict__ X,
+                                                const double *__restrict__ Y, const double *__restrict__ Z,
+                                                double4 *__restrict__ P) {
     const int n = counters[C_NLOCAL] + counters[C_NGHOST];
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) XY[i] = make_double2(X[i], Y[i]);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) P[i] = make_double4(X[i], Y[i], Z[i], 0.0);
 }
-
-// the segments of many independent systems in one launch: block b belongs to
-// system rep[b] and is its segment b - seg_base[rep[b]]
-template <int U, int NT, int MODE, bool RED>
-__global__ __launch_bounds__(FB) void k_lj_full_batch(const LJArgs *__restrict__ desc, const int *__restrict__ rep,
-                                                      const int *__restrict__ seg_base) {
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int r = rep[b];
-    lj_full_segment<U, NT, MODE, RED>(desc[r], b - seg_base[r]);
-}
-
 
 // ---------------------------------------------------------------- integration
 
@@ -1107,8 +1092,9 @@ __global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict_
 // (kinetic energy per segment): out[5 r + k], one workgroup per system
 __global__ __launch_bounds__(256) void k_thermo_reduce_batch(const RepDesc *__restrict__ D, double *__restrict__ out) {
     const RepDesc &R = D[blockIdx.x];
-    thermo_reduce_body(R.d.partials, R.d.pstride, R.g.nsegs, R.g.nsegs, 3, R.d.nlocal, R.g.len[0], R.g.len[1],
-                       R.g.len[2], out + 5 * (size_t) blockIdx.x);
+    const int nw = (R.d.nlocal + 63) / 64;  // force waves of the system
+    thermo_reduce_body(R.d.partials, R.d.pstride, nw, nw, 3, R.d.nlocal, R.g.len[0], R.g.len[1], R.g.len[2],
+                       out + 5 * (size_t) blockIdx.x);
 }
 
 void launch_thermo_batch(hipStream_t s, const RepDesc *D, int n, double *out) {
@@ -1375,7 +1361,7 @@ __global__ __launch_bounds__(256) void k_sync_pp(double *__restrict__ Xc, double
                                                  const double *__restrict__ Yp, const double *__restrict__ Zp,
                                                  const int *__restrict__ counters, const int *__restrict__ sync_src,
                                                  const int *__restrict__ sync_mult, Geom g,
-                                                 double2 *__restrict__ XYc) {
+                                                 double4 *__restrict__ Pc) {
     const int n = counters[C_NSEND_ALL];
     const int nlocal = counters[C_NLOCAL];
     for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) {
@@ -1386,10 +1372,11 @@ __global__ __launch_bounds__(256) void k_sync_pp(double *__restrict__ Xc, double
         const double z = loc ? Zc[src] : Zp[src];
         const double xg = x + (double) sync_mult[3 * q + 0] * g.len[0];
         const double yg = y + (double) sync_mult[3 * q + 1] * g.len[1];
+        const double zg = z + (double) sync_mult[3 * q + 2] * g.len[2];
         Xc[nlocal + q] = xg;
         Yc[nlocal + q] = yg;
-        Zc[nlocal + q] = z + (double) sync_mult[3 * q + 2] * g.len[2];
-        XYc[nlocal + q] = make_double2(xg, yg);
+        Zc[nlocal + q] = zg;
+        Pc[nlocal + q] = make_double4(xg, yg, zg, 0.0);
     }
 }
 
@@ -1681,8 +1668,9 @@ void launch_build_batch(hipStream_t s, const BuildArgs *desc, const int *rep, co
 }
 
 void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, int *out_ref, int ncap_out) {
-    hipLaunchKernelGGL(k_expand_neighbors, dim3(g.nsegs), dim3(256), 0, s, d.counters, g, d.lstart, d.gstart, d.ref,
-                       d.nl, d.nl_stride, d.numneighs, out_ref, ncap_out);
+    (void) g;
+    hipLaunchKernelGGL(k_expand_neighbors, dim3(std::max(1, std::min((d.nlocal + 255) / 256, 2048))), dim3(256), 0,
+                       s, d.counters, d.ref, d.nl, d.nl_stride, d.numneighs, out_ref, ncap_out);
 }
 
 static void launch_select_dim(hipStream_t s, const DevState &d, const Geom &g, int dim, int exch) {
@@ -1724,7 +1712,7 @@ void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *
 
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid) {
     hipLaunchKernelGGL(k_sync_pp, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[prev], d.Y[prev],
-                       d.Z[prev], d.counters, d.sync_src, d.sync_mult, g, d.XY[cur]);
+                       d.Z[prev], d.counters, d.sync_src, d.sync_mult, g, d.P[cur]);
 }
 
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel) {
@@ -1742,12 +1730,11 @@ static LJTab make_tab(const LJParams &prm) {
         t.eps[k] = k < prm.ntypes * prm.ntypes ? prm.eps[k] : 0.0;
         t.sig6[k] = k < prm.ntypes * prm.ntypes ? prm.sig6[k] : 0.0;
     }
+    t.unit = (prm.ntypes == 1 && prm.eps[0] == 1.0 && prm.sig6[0] == 1.0) ? 1 : 0;
     return t;
 }
 
-constexpr int LJ_UNROLL = 8;
-
-int force_blocks(const DevState &d, const Geom &g) { return g.nsegs; }
+int force_waves(const DevState &d) { return (d.nlocal + 63) / 64; }
 
 // image refresh only when the kernel writes next positions into the other
 // buffer and every local's images fit the table
@@ -1763,78 +1750,91 @@ LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, i
                int fused, double *eatom, bool packed) {
     LJArgs A;
     A.counters = d.counters;
-    A.counters_rw = d.counters;
     A.g = g;
     A.lstart = d.lstart; A.gstart = d.gstart;
     A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];
-    A.XY = d.XY[cur];
-    A.XYn = (packed && fused && do_vv1 && next != cur) ? d.XY[next] : nullptr;
+    A.P = d.P[cur];
+    A.Pn = (packed && fused && do_vv1 && next != cur) ? d.P[next] : nullptr;
     A.Xn = d.X[next]; A.Yn = d.Y[next]; A.Zn = d.Z[next];
     A.type = d.type; A.nl = d.nl; A.nl_stride = d.nl_stride; A.numneighs = d.numneighs; A.flags = d.flags;
     A.nrev = d.nrev; A.rev0 = d.rev0;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz; A.vx = d.vx; A.vy = d.vy; A.vz = d.vz; A.mass = d.mass;
     A.rc2 = prm.rc2; A.tab = make_tab(prm); A.dt = prm.dt; A.hdt = prm.hdt;
     A.do_vv2 = do_vv2; A.do_vv1 = do_vv1;
-    A.partials = d.partials; A.pstride = d.pstride; A.eatom = eatom; A.tcap = d.tile_cap;
+    A.partials = d.partials; A.pstride = d.pstride; A.eatom = eatom;
     A.im = images_of(d, fused && do_vv1 && next != cur);
     return A;
 }
 
-template <int NT, int MODE, bool RED>
-static void lj_full_launch(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
-                           int do_vv2, int do_vv1, double *eatom) {
-    const size_t lds = (size_t) d.tile_cap * sizeof(int);
+template <int NT, int MODE, bool RED, bool UNIT>
+static void lj_launch(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
+                      int do_vv2, int do_vv1, double *eatom) {
     const LJArgs A = lj_args(d, g, prm, cur, next, do_vv2, do_vv1, MODE == 1, eatom);
-    if (d.rev0 > 0)
-        hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED, true, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
-    else
-        hipLaunchKernelGGL((k_lj_full<LJ_UNROLL, NT, MODE, RED, true, false>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    const dim3 grid(force_waves(d));
+    if (d.rev0 > 0) hipLaunchKernelGGL((k_lj<NT, MODE, RED, true, false, UNIT>), grid, dim3(64), 0, s, A);
+    else hipLaunchKernelGGL((k_lj<NT, MODE, RED, false, false, UNIT>), grid, dim3(64), 0, s, A);
 }
 
-void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,
-                          int max_tile_cap, int ntypes1, int reduce) {
-    const size_t lds = (size_t) max_tile_cap * sizeof(int);
-    if (ntypes1) {
-        if (reduce) hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 1, 1, true>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
-        else hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 1, 1, false>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
-    } else {
-        if (reduce) hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 0, 1, true>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
-        else hipLaunchKernelGGL((k_lj_full_batch<LJ_UNROLL, 0, 1, false>), dim3(nblocks), dim3(FB), lds, s, desc, rep, seg_base);
-    }
+void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *wave_base, int nblocks,
+                          int ntypes1, int unit, int reduce) {
+    if (nblocks <= 0) return;
+    const dim3 grid(nblocks), blk(64);
+#define LJB(NT, RED, UNIT) hipLaunchKernelGGL((k_lj_batch<NT, 1, RED, UNIT>), grid, blk, 0, s, desc, rep, wave_base)
+    if (ntypes1 && unit) { if (reduce) LJB(1, true, true); else LJB(1, false, true); }
+    else if (ntypes1) { if (reduce) LJB(1, true, false); else LJB(1, false, false); }
+    else { if (reduce) LJB(0, true, false); else LJB(0, false, false); }
+#undef LJB
 }
 
-template <int NT>
-static void lj_full_nt(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
-                       int fuse, int do_vv2, int do_vv1, int reduce) {
+template <int NT, bool UNIT>
+static void lj_nt(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
+                  int do_vv2, int do_vv1, int reduce) {
     if (fuse) {
-        if (reduce) lj_full_launch<NT, 1, true>(s, d, g, prm, cur, next, do_vv2, do_vv1, nullptr);
-        else lj_full_launch<NT, 1, false>(s, d, g, prm, cur, next, do_vv2, do_vv1, nullptr);
+        if (reduce) lj_launch<NT, 1, true, UNIT>(s, d, g, prm, cur, next, do_vv2, do_vv1, nullptr);
+        else lj_launch<NT, 1, false, UNIT>(s, d, g, prm, cur, next, do_vv2, do_vv1, nullptr);
     } else {
-        if (reduce) lj_full_launch<NT, 0, true>(s, d, g, prm, cur, next, 0, 0, nullptr);
-        else lj_full_launch<NT, 0, false>(s, d, g, prm, cur, next, 0, 0, nullptr);
+        if (reduce) lj_launch<NT, 0, true, UNIT>(s, d, g, prm, cur, next, 0, 0, nullptr);
+        else lj_launch<NT, 0, false, UNIT>(s, d, g, prm, cur, next, 0, 0, nullptr);
     }
 }
 
-void launch_pack_xy(hipStream_t s, const DevState &d, int cur) {
-    hipLaunchKernelGGL(k_pack_xy, dim3(std::max(1, std::min((d.cap + 255) / 256, 1024))), dim3(256), 0, s,
-                       d.counters, d.X[cur], d.Y[cur], d.XY[cur]);
+// the packed copies of many systems, one launch: system blockIdx.y packs
+// the position buffer its force descriptor reads
+__global__ __launch_bounds__(256) void k_pack_p_batch(const LJArgs *__restrict__ desc) {
+    const LJArgs &A = desc[blockIdx.y];
+    const int n = A.counters[C_NLOCAL] + A.counters[C_NGHOST];
+    double4 *P = (double4 *) A.P;  // the system's own packed buffer
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        P[i] = make_double4(A.X[i], A.Y[i], A.Z[i], 0.0);
+}
+
+void launch_pack_p_batch(hipStream_t s, const LJArgs *desc, int n, int max_cap) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_pack_p_batch, dim3(std::max(1, std::min((max_cap + 255) / 256, 256)), n), dim3(256), 0, s,
+                       desc);
+}
+
+void launch_pack_p(hipStream_t s, const DevState &d, int cur) {
+    hipLaunchKernelGGL(k_pack_p, dim3(std::max(1, std::min((d.cap + 255) / 256, 1024))), dim3(256), 0, s,
+                       d.counters, d.X[cur], d.Y[cur], d.Z[cur], d.P[cur]);
 }
 
 void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
                     int do_vv2, int do_vv1, int reduce, int pack, hipEvent_t after_pack) {
     if (d.nlocal <= 0) return;
-    if (pack) launch_pack_xy(s, d, cur);
+    if (pack) launch_pack_p(s, d, cur);
     if (after_pack) hipEventRecord(after_pack, s);
-    if (prm.ntypes == 1) lj_full_nt<1>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
-    else lj_full_nt<0>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
+    if (prm.ntypes != 1) lj_nt<0, false>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
+    else if (make_tab(prm).unit) lj_nt<1, true>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
+    else lj_nt<1, false>(s, d, g, prm, cur, next, fuse, do_vv2, do_vv1, reduce);
 }
 
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom) {
     if (d.nlocal <= 0) return;
     (void) half;  // half lists: the kernel adds the mirror part (d.rev0 > 0)
-    launch_pack_xy(s, d, d.cur);
-    if (prm.ntypes == 1) lj_full_launch<1, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
-    else lj_full_launch<0, 2, true>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
+    launch_pack_p(s, d, d.cur);
+    if (prm.ntypes == 1) lj_launch<1, 2, true, false>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
+    else lj_launch<0, 2, true, false>(s, d, g, prm, d.cur, d.cur, 0, 0, eatom);
 }
 
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm) {
@@ -1865,8 +1865,8 @@ void launch_ke_partial(hipStream_t s, const DevState &d) {
 }
 
 void launch_thermo_reduce(hipStream_t s, const DevState &d, int nb_ke, int which_mask, const Geom &g, double *out) {
-    hipLaunchKernelGGL(k_thermo_reduce, dim3(1), dim3(256), 0, s, d.partials, d.pstride, nb_ke, g.nsegs, which_mask,
-                       d.nlocal, g.len[0], g.len[1], g.len[2], out);
+    hipLaunchKernelGGL(k_thermo_reduce, dim3(1), dim3(256), 0, s, d.partials, d.pstride, nb_ke, force_waves(d),
+                       which_mask, d.nlocal, g.len[0], g.len[1], g.len[2], out);
 }
 
 }  // namespace pk

@@ -11,13 +11,15 @@
 // double-buffered (X[2], Y[2], Z[2]) so that the fused force/integrate kernel
 // never overwrites a position another thread gathers.
 //
-// Work decomposition: a "segment" is ZC consecutive cells of one z-column of
-// the cell grid.  Its home particles are one contiguous range of physical
-// local slots, and the cells any of them can see (cell 0 plus the 3x3 columns
-// around it, z range widened by one) are 20 contiguous physical runs: the
-// segment's tile, staged in LDS once and shared by all its home particles.
-// Neighbour lists hold 16-bit offsets into the tile, in the reference's
-// candidate order.
+// Work decomposition.  The neighbour build works on "segments": ZC
+// consecutive cells of one z-column of the cell grid, whose home particles
+// are one contiguous range of physical local slots, and whose candidate cells
+// (cell 0 plus the 3x3 columns around it, z range widened by one) are 20
+// contiguous physical runs: the segment's tile, staged in LDS once and shared
+// by all its home particles.  Neighbour lists hold physical slots (32 bits),
+// in the reference's candidate order.  The force kernels work on "waves": 64
+// consecutive physical locals, one per lane, gathering neighbour positions
+// from a packed (x, y, z, 0) copy of the position buffer.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -45,8 +47,7 @@ enum {
     C_TILE_MAX = 16,  // largest segment tile of the current lists
     C_UNSUPPORTED = 17,
     C_IMG_OVF = 18,   // a local has more periodic images than IMG_MAX
-    C_FORCE_SKIP = 19,  // a force kernel met a tile above its staged capacity (largest such tile)
-    C_COUNT = 20
+    C_COUNT = 19
 };
 enum { OVF_NONE = 0, OVF_PART = 1, OVF_NEIGH = 2, OVF_SEND = 3, OVF_TILE = 4 };
 
@@ -65,7 +66,7 @@ struct Geom {
 };
 
 // Neighbour lists: entry k of physical local i.  Rows come in groups of four
-// so that one 8-byte load gives a lane four consecutive entries; a group
+// so that one 16-byte load gives a lane four consecutive entries; a group
 // holds, for every particle, its four entries side by side.
 __host__ __device__ __forceinline__ size_t nl_at(int k, int i, int nl_stride) {
     return ((size_t) (k >> 2) * nl_stride + i) * 4 + (k & 3);
@@ -81,6 +82,7 @@ struct Images {
 
 struct LJTab {
     int nt;
+    int unit;  // one type with epsilon = sigma6 = 1
     double eps[16], sig6[16];  // ntypes <= 4
 };
 
@@ -94,11 +96,11 @@ struct DevState {
     int cap, ncap, send_cap, ncells_cap, nl_stride;
     int nlocal;   // host mirror (constant between uploads on one domain)
     int cur;      // which position buffer holds the current state
-    int tile_cap; // LDS tile entries the force kernels may stage
+    int tile_cap; // LDS tile entries the neighbour build may stage
     int pstride;  // row stride of the partials array
     int max_lds;  // shared memory a workgroup may allocate
     double *X[2], *Y[2], *Z[2];
-    double2 *XY[2];      // (x, y) of each position buffer, packed for one 16-byte gather
+    double4 *P[2];       // (x, y, z, 0) of each position buffer, packed for the force kernels' gathers
     double *vx, *vy, *vz, *fx, *fy, *fz, *mass;
     int *type, *flags, *uid, *shape, *ref;
     int *phys;    // physical slot of each reference index
@@ -106,7 +108,7 @@ struct DevState {
     double *alt_vx, *alt_vy, *alt_vz, *alt_mass;
     int *alt_type, *alt_flags, *alt_uid, *alt_shape, *alt_ref, *alt_cell;
     int *numneighs;
-    uint16_t *nl;  // entry k of physical local i at nl_at(k, i, nl_stride): tile offset
+    uint32_t *nl;  // entry k of physical local i at nl_at(k, i, nl_stride): physical slot
     // half lists: rows [rev0, 2 rev0) hold each local's mirror part (the
     // earlier-index locals whose half list holds it, by reference index);
     // rev0 = 0 for full lists
@@ -127,13 +129,15 @@ struct DevState {
     int *counters;
     int *scratch_i;      // 4 * send_cap
     double *stage;       // 10 * send_cap doubles
-    double *partials;    // 3 per force block
+    double *partials;    // 3 rows of pstride: per force wave (kinetic, energy, virial)
     double *thermo;      // 5 per thermo record
 };
 
 void launch_fill_i32(hipStream_t s, int *p, int v, int n);
-// XY[cur] = (X[cur], Y[cur]) for locals and ghosts
-void launch_pack_xy(hipStream_t s, const DevState &d, int cur);
+// P[cur] = (X[cur], Y[cur], Z[cur], 0) for locals and ghosts
+void launch_pack_p(hipStream_t s, const DevState &d, int cur);
+// waves of the force kernels: 64 locals each
+int force_waves(const DevState &d);
 // the older API's modules (examples/lj_onetype.py): forces with its pair
 // expression (full lists), and its Euler update in the current buffer
 void launch_lj_older(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm);
@@ -174,7 +178,7 @@ struct BuildArgs {
     const double *X, *Y, *Z;
     const int *ref, *flags;
     double rc2;
-    uint16_t *nl;
+    uint32_t *nl;
     int nl_stride, ncap, lrows, tcap, zt, rev0;
     int *numneighs, *nrev;
 };
@@ -189,10 +193,10 @@ void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, in
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid);
 void launch_sync_inplace(hipStream_t s, const DevState &d, const Geom &g, int grid, int with_vel);
 // mode: 0 force only, 1 force + integrators (fused), 2 energy only (no writes)
-// The full-list force kernel gathers (x, y) from XY[cur]: with pack, a pass
-// first packs it from X, Y; after_pack, if given, is recorded between that
-// pass and the force kernel.  With fuse and do_vv1 the kernel also writes
-// XY[next] for every local and (images on) every ghost.
+// The force kernel gathers positions from P[cur]: with pack, a pass first
+// packs it from X, Y, Z; after_pack, if given, is recorded between that pass
+// and the force kernel.  With fuse and do_vv1 the kernel also writes P[next]
+// for every local and (images on) every ghost.
 void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int fuse,
                     int do_vv2, int do_vv1, int reduce, int pack = 1, hipEvent_t after_pack = nullptr);
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom);
@@ -201,20 +205,18 @@ void launch_vv_post(hipStream_t s, const DevState &d, const Geom &g, const LJPar
                     int do_vv2, int do_vv1, int reduce);
 void launch_ke_partial(hipStream_t s, const DevState &d);
 void launch_thermo_reduce(hipStream_t s, const DevState &d, int nblocks, int which_mask, const Geom &g, double *out);
-int force_blocks(const DevState &d, const Geom &g);
 
 // every argument of the full-list force kernel (one system)
 struct LJArgs {
     const int *counters;
-    int *counters_rw;    // the same counters (C_FORCE_SKIP)
     Geom g;
     const int *lstart, *gstart;
     const double *X, *Y, *Z;
-    const double2 *XY;   // packed (x, y) of X, Y (single-system launches)
-    double2 *XYn;        // packed (x, y) of the next positions, or null
+    const double4 *P;    // packed (x, y, z, 0) of X, Y, Z
+    double4 *Pn;         // packed copy of the next positions, or null
     double *Xn, *Yn, *Zn;
     const int *type;
-    const uint16_t *nl;
+    const uint32_t *nl;
     int nl_stride;
     const int *numneighs, *flags;
     const int *nrev;     // half lists: length of each particle's mirror part
@@ -223,15 +225,18 @@ struct LJArgs {
     const double *mass;
     double rc2, dt, hdt;
     LJTab tab;
-    int do_vv2, do_vv1, pstride, tcap;
+    int do_vv2, do_vv1, pstride;
     double *partials, *eatom;
     Images im;
 };
 // the fused force + integrate arguments of one system (cur -> next buffer)
 LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, int next, int do_vv2, int do_vv1,
                int fused, double *eatom, bool packed = true);
-// fused force + integrate for many systems: block b is segment b - seg_base[rep[b]] of system rep[b]
-void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *seg_base, int nblocks,
-                          int max_tile_cap, int ntypes1, int reduce);
+// the packed copy (P) of the buffer each of n descriptors reads (max_cap:
+// the largest system's particle capacity)
+void launch_pack_p_batch(hipStream_t s, const LJArgs *desc, int n, int max_cap);
+// fused force + integrate for many systems: block b is wave b - wave_base[rep[b]] of system rep[b]
+void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *wave_base, int nblocks,
+                          int ntypes1, int unit, int reduce);
 
 }  // namespace pk

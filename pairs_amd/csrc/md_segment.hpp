@@ -90,68 +90,63 @@ __device__ __forceinline__ void stage_slots(const int *r_start, const int *r_off
     }
 }
 
-// Walk a lane's neighbour list eight tile offsets per step, to the
-// wave-uniform depth nmax: body(k, on) gets entries k .. k+7.  A step
-// consumes two packed groups of four entries; the loads of the groups two
-// steps ahead are issued first, in a ring of three register pairs written
-// out (no register moves), so a list load never stalls the step that issues
-// it.  Groups past the list re-read its last group (in bounds, no branch);
-// offsets are clamped into the tile.
+// Walk rows [row0, row0 + cnt) of a lane's list to the wave-uniform depth
+// cmax, U (4 or 8) entries per step: body(k, j) gets entries k .. k+U-1 as
+// physical slots.  A step takes U/4 packed groups of four entries (one
+// 16-byte load each); the groups two steps ahead are loaded first, in a ring
+// of three register sets written out (no register moves), so a list load
+// never stalls the step that issues it.  Groups past the list re-read its
+// last group.  Entries past cnt are masked by the body; every entry is
+// clamped to jmax (the last particle slot), so no gather leaves the particle
+// arrays whatever a row past a particle's count holds.
 template <int U, typename Body>
-__device__ __forceinline__ void list_walk(const uint16_t *nl, int i, int nl_stride, int n, int nmax, int T,
-                                          Body &&body) {
+__device__ __forceinline__ void list_walk32(const uint32_t *__restrict__ nl, int row0, int i, int stride, int cnt,
+                                            int cmax, uint32_t jmax, Body &&body) {
     static_assert(U == 4 || U == 8, "a step takes one or two packed groups");
-    const uint64_t *lp = (const uint64_t *) nl + i;
-    const int gl = max((n + 3) / 4 - 1, 0), tlast = T - 1;
+    constexpr int G = U / 4;  // groups per step
+    const uint4 *lp = (const uint4 *) nl + (size_t) (row0 >> 2) * stride + i;
+    const int gl = max((cnt + 3) / 4 - 1, 0);
     // a relaxed atomic load is an ordered memory access to the compiler: it
     // stays where it is issued instead of being sunk next to its use
-    auto ld = [&](int gq) {
-        return __hip_atomic_load(lp + (size_t) min(gq, gl) * nl_stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    auto ld1 = [&](int gq) {
+        const uint32_t *p = (const uint32_t *) (lp + (size_t) min(gq, gl) * stride);
+        uint4 v;
+        v.x = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        v.y = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        v.z = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        v.w = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        return v;
     };
-    auto step = [&](int k, uint64_t a, uint64_t b) {
-        int on[8];
+    struct Grp { uint4 v[G]; };
+    auto ld = [&](int step) {
+        Grp r;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            on[q] = min((int) ((a >> (16 * q)) & 0xffff), tlast);
-            on[4 + q] = min((int) ((b >> (16 * q)) & 0xffff), tlast);
-        }
-        body(k, on);
+        for (int q = 0; q < G; q++) r.v[q] = ld1(step * G + q);
+        return r;
     };
-    if (U == 4) {
-        // one group per step, same ring of three
-        auto step4 = [&](int k, uint64_t a) {
-            int on[8];
+    auto run = [&](int k, const Grp &r) {
+        uint32_t j[U];
 #pragma unroll
-            for (int q = 0; q < 4; q++) on[q] = min((int) ((a >> (16 * q)) & 0xffff), tlast);
-            body(k, on);
-        };
-        uint64_t r0 = ld(0), r1 = ld(1), r2;
-        for (int k = 0; k < nmax; k += 12) {
-            const int g = k >> 2;
-            r2 = ld(g + 2);
-            step4(k, r0);
-            if (k + 4 >= nmax) break;
-            r0 = ld(g + 3);
-            step4(k + 4, r1);
-            if (k + 8 >= nmax) break;
-            r1 = ld(g + 4);
-            step4(k + 8, r2);
+        for (int q = 0; q < G; q++) {
+            j[4 * q + 0] = min(r.v[q].x, jmax);
+            j[4 * q + 1] = min(r.v[q].y, jmax);
+            j[4 * q + 2] = min(r.v[q].z, jmax);
+            j[4 * q + 3] = min(r.v[q].w, jmax);
         }
-        return;
-    }
-    uint64_t r0a = ld(0), r0b = ld(1), r1a = ld(2), r1b = ld(3), r2a, r2b;
-    for (int k = 0; k < nmax; k += 24) {
-        const int g = k >> 2;
-        r2a = ld(g + 4); r2b = ld(g + 5);
-        step(k, r0a, r0b);
-        if (k + 8 >= nmax) break;
-        r0a = ld(g + 6); r0b = ld(g + 7);
-        step(k + 8, r1a, r1b);
-        if (k + 16 >= nmax) break;
-        r1a = ld(g + 8); r1b = ld(g + 9);
-        step(k + 16, r2a, r2b);
+        body(k, j);
+    };
+    Grp r0 = ld(0), r1 = ld(1), r2;
+    for (int k = 0; k < cmax; k += 3 * U) {
+        const int st = k / U;
+        r2 = ld(st + 2);
+        run(k, r0);
+        if (k + U >= cmax) break;
+        r0 = ld(st + 3);
+        run(k + U, r1);
+        if (k + 2 * U >= cmax) break;
+        r1 = ld(st + 4);
+        run(k + 2 * U, r2);
     }
 }
-
 
 }  // namespace pk

@@ -230,48 +230,37 @@ constexpr float MLP_PIRC = 3.14159265358979323846f / (float) MLP_RC;
 
 struct MLPSegArgs {
     const int *counters;
-    int *counters_rw;    // the same counters (C_FORCE_SKIP)
-    Geom g;
-    const int *lstart, *gstart;
     const double *X, *Y, *Z;
-    const double2 *XY;   // (x, y) of X, Y packed: one 16-byte gather
-    const uint16_t *nl;
+    const double4 *P;    // (x, y, z, 0) of X, Y, Z packed: one 16-byte and one 8-byte gather
+    const uint32_t *nl;
     int nl_stride;
     const int *numneighs;
-    int tcap, pstride;
+    int pstride;
     const float *D, *E;
     float *G;
     double *fx, *fy, *fz;
     double *partials;
 };
 
-// G of every home particle of one segment (MODE 0), or its force, energy
-// and virial (MODE 1).  One lane per home particle, the full list walked four
-// entries per step as in the LJ kernel; pairs at r >= 2.5 and entries past a
-// lane's own count are masked.
+// G of the 64 locals of one wave (MODE 0), or their force, energy and
+// virial (MODE 1).  One lane per local, the full list walked four entries per
+// step as in the LJ kernel; pairs at r >= 2.5 and entries past a lane's own
+// count are masked.
 template <int MODE, bool RED>
-__global__ __launch_bounds__(FB) void k_mlp_segment(MLPSegArgs A) {
-    const int seg = xcd_block(blockIdx.x, gridDim.x);
-    const Geom &g = A.g;
-    extern __shared__ char smem[];
-    __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
+__global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
+    const int wv = xcd_block(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
     const int nlocal = A.counters[C_NLOCAL];
-    const SegInfo si = seg_info(seg, g);
-    const int h0 = A.lstart[si.ca], h1 = A.lstart[si.cb];
+    const uint32_t jmax = (uint32_t) max(nlocal + A.counters[C_NGHOST] - 1, 0);
+    const char *__restrict__ P = (const char *) A.P;
     const double rc2 = MLP_RC * MLP_RC;
     double pe = 0.0, vir = 0.0;
-    if (h1 > h0) {
-        seg_runs(si, g, A.lstart, A.gstart, nlocal, r_start, r_len, r_off, col_lo);
-        const int T = r_off[NRUNS];
-        const bool fits = T <= A.tcap;
-        if (!fits && threadIdx.x == 0) atomicMax(&A.counters_rw[C_FORCE_SKIP], T);
-        int *tphys = (int *) smem;
-        if (fits) stage_slots(r_start, r_off, T, tphys);
-        __syncthreads();
-        for (int base = h0; fits && base < h1; base += FB) {
-            const int i = base + threadIdx.x;
-            const bool live = i < h1;
-            const int ii = live ? i : h0;
+    const int i0 = wv * 64;
+    if (i0 < nlocal) {  // wave-uniform
+        {
+            const int i = i0 + lane;
+            const bool live = i < nlocal;
+            const int ii = live ? i : i0;
             const int n = live ? A.numneighs[ii] : 0;
             const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
             const double xi = A.X[ii], yi = A.Y[ii], zi = A.Z[ii];
@@ -288,16 +277,14 @@ __global__ __launch_bounds__(FB) void k_mlp_segment(MLPSegArgs A) {
             }
             double f0 = 0.0, f1 = 0.0, f2 = 0.0, w = 0.0;
             const float gb = exp2f(-2.0f * MLP_DMU * MLP_DMU * MLP_C);  // a_{k+1} / a_k
-            list_walk<4>(A.nl, ii, A.nl_stride, n, nmax, T, [&](int k, const int *on) {
-                int jp[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) jp[q] = tphys[on[q]];
+            list_walk32<4>(A.nl, 0, ii, A.nl_stride, n, nmax, jmax, [&](int k, const uint32_t *jp) {
                 double xj[4], yj[4], zj[4];
                 float4 dj[4][4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const double2 pxy = A.XY[jp[q]];
-                    xj[q] = pxy.x; yj[q] = pxy.y; zj[q] = A.Z[jp[q]];
+                    const uint32_t off = jp[q] << 5;
+                    const double2 pxy = *(const double2 *) (P + off);
+                    xj[q] = pxy.x; yj[q] = pxy.y; zj[q] = *(const double *) (P + off + 16);
                     if (MODE == 1) {
                         const float4 *dp = (const float4 *) (A.D + (size_t) jp[q] * MLP_ND);
 #pragma unroll
@@ -370,15 +357,11 @@ __global__ __launch_bounds__(FB) void k_mlp_segment(MLPSegArgs A) {
         }
     }
     if (MODE == 1 && RED) {
-        __shared__ double red[2][FB / 64];
+        // one partial per wave (energy, virial rows)
         const double a = wave_sum(pe), b = wave_sum(vir);
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) { red[0][wave] = a; red[1][wave] = b; }
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            double s = 0.0;
-            for (int k = 0; k < FB / 64; k++) s += red[threadIdx.x][k];
-            A.partials[(size_t) (1 + threadIdx.x) * A.pstride + seg] = s;
+        if (lane == 0) {
+            A.partials[(size_t) A.pstride + wv] = a;
+            A.partials[(size_t) 2 * A.pstride + wv] = b;
         }
     }
 }
@@ -616,29 +599,25 @@ __global__ __launch_bounds__(256) void k_force_diff(const double *__restrict__ a
 
 // ---------------------------------------------------------------- launchers
 
-static MLPSegArgs seg_args(const DevState &d, const Geom &g, int cur) {
+static MLPSegArgs seg_args(const DevState &d, int cur) {
     MLPSegArgs A{};
     A.counters = d.counters;
-    A.counters_rw = d.counters;
-    A.g = g;
-    A.lstart = d.lstart;
-    A.gstart = d.gstart;
     A.X = d.X[cur]; A.Y = d.Y[cur]; A.Z = d.Z[cur];
-    A.XY = d.XY[cur];
+    A.P = d.P[cur];
     A.nl = d.nl;
     A.nl_stride = d.nl_stride;
     A.numneighs = d.numneighs;
-    A.tcap = d.tile_cap;
     A.pstride = d.pstride;
     return A;
 }
 
 void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int cur, float *G) {
     if (d.nlocal <= 0) return;
-    launch_pack_xy(s, d, cur);  // both passes gather (x, y) packed
-    MLPSegArgs A = seg_args(d, g, cur);
+    (void) g;
+    launch_pack_p(s, d, cur);  // both passes gather packed positions
+    MLPSegArgs A = seg_args(d, cur);
     A.G = G;
-    hipLaunchKernelGGL((k_mlp_segment<0, false>), dim3(g.nsegs), dim3(FB), (size_t) d.tile_cap * sizeof(int), s, A);
+    hipLaunchKernelGGL((k_mlp_wave<0, false>), dim3(force_waves(d)), dim3(64), 0, s, A);
 }
 
 static int eval_grid(int n) {
@@ -658,14 +637,13 @@ void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const ML
     launch_mlp_descriptors(s, d, g, cur, b.G);
     launch_mlp_eval(s, b.p, b.sc, b.G, d.nlocal, b.E, b.D);
     hipLaunchKernelGGL(k_mlp_ghost_d, dim3(std::max(ghost_grid, 1)), dim3(256), 0, s, d.counters, d.sync_src, b.D);
-    MLPSegArgs A = seg_args(d, g, cur);
+    MLPSegArgs A = seg_args(d, cur);
     A.D = b.D;
     A.E = b.E;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz;
     A.partials = d.partials;
-    const size_t lds = (size_t) d.tile_cap * sizeof(int);
-    if (reduce) hipLaunchKernelGGL((k_mlp_segment<1, true>), dim3(g.nsegs), dim3(FB), lds, s, A);
-    else hipLaunchKernelGGL((k_mlp_segment<1, false>), dim3(g.nsegs), dim3(FB), lds, s, A);
+    if (reduce) hipLaunchKernelGGL((k_mlp_wave<1, true>), dim3(force_waves(d)), dim3(64), 0, s, A);
+    else hipLaunchKernelGGL((k_mlp_wave<1, false>), dim3(force_waves(d)), dim3(64), 0, s, A);
 }
 
 void launch_mlp_grad(hipStream_t s, const float *p, const float *sc, const MLPTrain &t, uint64_t seed, int step) {

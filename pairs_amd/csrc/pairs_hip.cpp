@@ -50,13 +50,16 @@ static void dfree(void *p) {
     if (p) hipFree(p);
 }
 
+// rows of the per-wave partials (force waves of 64 locals; blocks of 256 of
+// the stand-alone integrator)
 static size_t pstride_of(const DevState &d, const Geom &g) {
-    return (size_t) std::max(g.dim_cells[0] * g.dim_cells[1] * g.dim_cells[2], d.cap / 256 + 1) + 8;
+    (void) g;
+    return (size_t) d.cap / 64 + 8;
 }
 
 static void free_particle_arrays(DevState &d) {
     for (int b = 0; b < 2; b++) { dfree(d.X[b]); dfree(d.Y[b]); dfree(d.Z[b]); d.X[b] = d.Y[b] = d.Z[b] = nullptr; }
-    for (int b = 0; b < 2; b++) { dfree(d.XY[b]); d.XY[b] = nullptr; }
+    for (int b = 0; b < 2; b++) { dfree(d.P[b]); d.P[b] = nullptr; }
     double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz,
                      &d.alt_mass, &d.partials};
     for (auto p : dd) { dfree(*p); *p = nullptr; }
@@ -69,8 +72,8 @@ static void free_particle_arrays(DevState &d) {
 static int alloc_particle_arrays(pairs_hip_ctx *ctx, DevState &d, int cap) {
     int r = 0;
     for (int b = 0; b < 2; b++) { r |= dalloc(ctx, &d.X[b], cap); r |= dalloc(ctx, &d.Y[b], cap); r |= dalloc(ctx, &d.Z[b], cap); }
-    r |= dalloc(ctx, &d.XY[0], cap);
-    r |= dalloc(ctx, &d.XY[1], cap);
+    r |= dalloc(ctx, &d.P[0], cap);
+    r |= dalloc(ctx, &d.P[1], cap);
     double **dd[] = {&d.vx, &d.vy, &d.vz, &d.fx, &d.fy, &d.fz, &d.mass, &d.alt_vx, &d.alt_vy, &d.alt_vz, &d.alt_mass};
     for (auto p : dd) r |= dalloc(ctx, p, cap);
     int **ii[] = {&d.type, &d.flags, &d.uid, &d.shape, &d.ref, &d.phys, &d.alt_type, &d.alt_flags, &d.alt_uid,
@@ -700,8 +703,8 @@ int pairs_hip_bin_cells(pairs_hip_ctx *ctx) {
     return finish_module(ctx);
 }
 
-// LDS entries the kernels stage per segment tile: the largest tile of the
-// last build plus 1/8 and 64 entries of headroom (tiles change only at a
+// LDS entries the neighbour build stages per segment tile: the largest tile
+// of the last build plus 1/8 and 64 entries of headroom (tiles change only at a
 // rebuild), never above what a workgroup's LDS holds; grows only, so captured
 // graphs stay valid while tiles fluctuate
 static void set_tile_cap(pairs_hip_ctx *ctx) {
@@ -893,20 +896,17 @@ static void enqueue_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
 // capacities, and their counts live on the device: they are replayed from a
 // graph (one per position buffer, captured on the context's stream), so they
 // run back to back and cost one launch on the host.
-static int launch_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
-    const int cur = ctx->d.cur;
-    if (!ctx->use_graphs) {
-        enqueue_rebuild(ctx, s);
-        CHECK_LAUNCH();
-        return 0;
-    }
+// capture (without running) the rebuild that starts on buffer cur when missing
+static int capture_rebuild_graph(pairs_hip_ctx *ctx, int cur) {
     hipGraphExec_t *slot = &ctx->reb_graph[cur];
     if (!*slot) {
+        const int cur0 = ctx->d.cur;
+        ctx->d.cur = cur;
         hipGraph_t g;
         HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         enqueue_rebuild(ctx, ctx->stream);
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-        ctx->d.cur = cur;
+        ctx->d.cur = cur0;
         if (ec != hipSuccess) {
             ctx->err = std::string("rebuild capture: ") + hipGetErrorString(ec);
             return PAIRS_HIP_ERR_HIP;
@@ -919,7 +919,18 @@ static int launch_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
             return PAIRS_HIP_ERR_HIP;
         }
     }
-    HIP_OK(hipGraphLaunch(*slot, s));
+    return 0;
+}
+
+static int launch_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
+    const int cur = ctx->d.cur;
+    if (!ctx->use_graphs) {
+        enqueue_rebuild(ctx, s);
+        CHECK_LAUNCH();
+        return 0;
+    }
+    if (int r = capture_rebuild_graph(ctx, cur)) return r;
+    HIP_OK(hipGraphLaunch(ctx->reb_graph[cur], s));
     ctx->d.cur = 1 - cur;
     return 0;
 }
@@ -932,16 +943,6 @@ static int launch_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
 // permutation, whose buffer then holds the state.
 static bool overflow_before_permute(const int *h) { return h[C_OVF] == OVF_SEND || h[C_OVF] == OVF_PART; }
 
-// a force kernel found a segment tile larger than the LDS it was launched
-// with and left that segment's particles unwritten
-static int check_force_skip(pairs_hip_ctx *ctx) {
-    if (ctx->h_counters[C_FORCE_SKIP] == 0) return 0;
-    ctx->err = "a force kernel skipped a segment whose tile (" + std::to_string(ctx->h_counters[C_FORCE_SKIP]) +
-               " entries) exceeds the staged tile capacity (" + std::to_string(ctx->d.tile_cap) +
-               "): forces and positions of that step are invalid";
-    return PAIRS_HIP_ERR_STATE;
-}
-
 static int rebuild(pairs_hip_ctx *ctx) {
     if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
     for (int attempt = 0; attempt < 8; attempt++) {
@@ -950,7 +951,6 @@ static int rebuild(pairs_hip_ctx *ctx) {
         const int cur0 = ctx->d.cur;
         if (int r = launch_rebuild(ctx, ctx->stream)) return r;
         if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
-        if (int r = check_force_skip(ctx)) return r;
         if (overflow_before_permute(ctx->h_counters)) ctx->d.cur = cur0;
         // images refresh the ghosts from the position kernels unless some
         // local has too many of them (then a separate refresh runs)
@@ -1061,9 +1061,13 @@ int pairs_hip_force_kernel_time(pairs_hip_ctx *ctx, double *avg_ms, int *launche
     return 0;
 }
 
-static int graph_step(pairs_hip_ctx *ctx, int cur) {
-    if (!ctx->step_graph[cur] || ctx->graph_ghost_grid[cur] < ctx->ghost_grid ||
-        ctx->graph_tile_cap[cur] != ctx->d.tile_cap || ctx->graph_img[cur] != ctx->d.img_on) {
+static bool step_graph_stale(const pairs_hip_ctx *ctx, int cur) {
+    return !ctx->step_graph[cur] || ctx->graph_ghost_grid[cur] < ctx->ghost_grid || ctx->graph_img[cur] != ctx->d.img_on;
+}
+
+// capture (without running) the steady step of buffer cur when missing or stale
+static int capture_step_graph(pairs_hip_ctx *ctx, int cur) {
+    if (step_graph_stale(ctx, cur)) {
         if (ctx->step_graph[cur]) hipGraphExecDestroy(ctx->step_graph[cur]);
         ctx->step_graph[cur] = nullptr;
         // capture with headroom so that later rebuilds with a few more ghosts reuse it
@@ -1076,7 +1080,6 @@ static int graph_step(pairs_hip_ctx *ctx, int cur) {
         hipError_t e = hipGraphInstantiate(&ctx->step_graph[cur], g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         ctx->graph_ghost_grid[cur] = ctx->ghost_grid;
-        ctx->graph_tile_cap[cur] = ctx->d.tile_cap;
         ctx->graph_img[cur] = ctx->d.img_on;
         ctx->ghost_grid = saved;
         if (e != hipSuccess) {
@@ -1085,7 +1088,24 @@ static int graph_step(pairs_hip_ctx *ctx, int cur) {
             return PAIRS_HIP_ERR_HIP;
         }
     }
+    return 0;
+}
+
+static int graph_step(pairs_hip_ctx *ctx, int cur) {
+    if (int r = capture_step_graph(ctx, cur)) return r;
     HIP_OK(hipGraphLaunch(ctx->step_graph[cur], ctx->stream));
+    return 0;
+}
+
+int pairs_hip_prepare_graphs(pairs_hip_ctx *ctx) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    if (!ctx->use_graphs) return 0;
+    if (ctx->time_force && ensure_events(ctx, 4096)) return PAIRS_HIP_ERR_HIP;
+    for (int b = 0; b < 2; b++) {
+        if (int r = capture_rebuild_graph(ctx, b)) return r;
+        if (ctx->h_counters[C_NLOCAL] > 0 && ctx->d.nl && !ctx->mlp_on)
+            if (int r = capture_step_graph(ctx, b)) return r;
+    }
     return 0;
 }
 
@@ -1131,7 +1151,7 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         }
         if (timed) ctx->ev_used++;
         if (T && nthermo < 4096) {
-            const int nb_ke = ctx->mlp_on ? (d.nlocal + 255) / 256 : ctx->geom.nsegs;
+            const int nb_ke = ctx->mlp_on ? (d.nlocal + 255) / 256 : force_waves(d);
             launch_thermo_reduce(s, d, nb_ke, 3, ctx->geom, d.thermo + 5 * nthermo);
             nthermo++;
         }
@@ -1139,8 +1159,6 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         if (vv1) { d.cur = other; prev_in_other = true; }
     }
     HIP_OK(hipStreamSynchronize(s));
-    if (read_counters(ctx)) return PAIRS_HIP_ERR_HIP;
-    if (int r = check_force_skip(ctx)) return r;
     if (thermo_out && nthermo > 0) {
         const int n = std::min(nthermo, thermo_capacity);
         HIP_OK(hipMemcpy(thermo_out, d.thermo, 5 * n * sizeof(double), hipMemcpyDeviceToHost));
@@ -1299,11 +1317,12 @@ struct pairs_hip_batch {
     LJArgs *h_desc = nullptr;          // pinned staging, 3 * R
     LJArgs *d_desc[3] = {nullptr, nullptr, nullptr};  // phase 0, phase 1, one-off
     int *d_rep = nullptr, *d_base = nullptr;
-    int nblocks = 0, blocks_cap = 0, max_tile = 0;
+    int nblocks = 0, blocks_cap = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_blocks[2] = {0, 0}, graph_tile[2] = {0, 0};
+    int graph_blocks[2] = {0, 0};
     std::vector<double> thermo;        // last thermo row per replica
     bool ntypes1 = true;
+    bool unit = true;                  // every replica: one type, epsilon = sigma6 = 1
     int pending_home = 0;              // segment size to apply at the next rebuild
     int *h_cnt = nullptr;              // pinned, [R][C_COUNT] counters of every replica
     RepDesc *h_rdesc = nullptr, *d_rdesc = nullptr;    // every replica's state for the batched kernels
@@ -1335,11 +1354,10 @@ static void batch_drop_graphs(pairs_hip_batch *b) {
 static int batch_tables(pairs_hip_batch *b) {
     const int R = (int) b->rep.size();
     std::vector<int> base(R + 1);
-    int tot = 0, mt = 0;
+    int tot = 0;
     for (int r = 0; r < R; r++) {
         base[r] = tot;
-        tot += b->rep[r]->geom.nsegs;
-        mt = std::max(mt, b->rep[r]->d.tile_cap);
+        tot += force_waves(b->rep[r]->d);
     }
     base[R] = tot;
     if (tot > b->blocks_cap) {
@@ -1354,50 +1372,21 @@ static int batch_tables(pairs_hip_batch *b) {
     BATCH_OK(hipMemcpyAsync(b->d_base, base.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
     BATCH_OK(hipStreamSynchronize(b->stream));
     b->nblocks = tot;
-    b->max_tile = mt;
     b->tables_valid = true;
     return 0;
 }
 
-// Before a batched force launch: the launch allocates max_tile entries of LDS
-// per workgroup, and each replica's kernel stages up to its tcap entries, so
-// every tcap must be <= max_tile; and every replica's largest tile of its
-// current lists (C_TILE_MAX of its last rebuild) must be <= its tcap, or the
-// kernel would skip that segment.
-static int batch_check_tiles(pairs_hip_batch *b, const LJArgs *desc) {
-    for (size_t r = 0; r < b->rep.size(); r++) {
-        const pairs_hip_ctx *c = b->rep[r];
-        const int tmax = c->h_counters[C_TILE_MAX];
-        if (desc[r].tcap > b->max_tile || tmax > desc[r].tcap || desc[r].tcap != c->d.tile_cap) {
-            b->err = "replica " + std::to_string(r) + ": tile capacity " + std::to_string(desc[r].tcap) +
-                     " does not cover its largest tile " + std::to_string(tmax) + " within the launch's " +
-                     std::to_string(b->max_tile) + " LDS entries";
-            return PAIRS_HIP_ERR_STATE;
-        }
-    }
-    return 0;
-}
-
-// descriptors of both phases (steady steps: final and initial integrate on).
-// Each descriptor's tcap is its replica's tile_cap, which a rebuild may have
-// grown (set_tile_cap), and the force kernel stages any tile of up to tcap
-// entries; the batched launch sizes its LDS from max_tile, so max_tile is
-// taken here from the same tile_caps.
+// descriptors of both phases (steady steps: final and initial integrate on)
 static int batch_descs(pairs_hip_batch *b) {
     const int R = (int) b->rep.size();
-    int mt = 0;
-    for (int r = 0; r < R; r++) mt = std::max(mt, b->rep[r]->d.tile_cap);
-    b->max_tile = mt;
     b->base_cur.resize(R);
     for (int p = 0; p < 2; p++)
         for (int r = 0; r < R; r++) {
             pairs_hip_ctx *c = b->rep[r];
             const int cur = c->d.cur ^ p;
-            b->h_desc[p * R + r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, 1, 1, 1, nullptr, false);
+            b->h_desc[p * R + r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, 1, 1, 1, nullptr, true);
         }
     for (int r = 0; r < R; r++) b->base_cur[r] = b->rep[r]->d.cur;
-    for (int p = 0; p < 2; p++)
-        if (int e = batch_check_tiles(b, b->h_desc + p * R)) return e;
     for (int p = 0; p < 2; p++)
         BATCH_OK(hipMemcpyAsync(b->d_desc[p], b->h_desc + p * R, R * sizeof(LJArgs), hipMemcpyHostToDevice, b->stream));
     BATCH_OK(hipStreamSynchronize(b->stream));
@@ -1466,7 +1455,6 @@ static int batch_rebuild(pairs_hip_batch *b) {
     std::vector<char> fallback(R, 0);
     for (int r = 0; r < R; r++) {
         pairs_hip_ctx *c = b->rep[r];
-        if (check_force_skip(c)) { b->err = "replica " + std::to_string(r) + ": " + c->err; return PAIRS_HIP_ERR_STATE; }
         c->d.img_on = c->h_counters[C_IMG_OVF] == 0 ? 1 : 0;
         if (c->h_counters[C_OVF] != 0 || c->h_counters[C_REQ_NEIGH] > 0) {
             // an overflow anywhere: the replica repeats its rebuild on its
@@ -1501,7 +1489,7 @@ static int batch_rebuild(pairs_hip_batch *b) {
 }
 
 static int batch_graph_launch(pairs_hip_batch *b, int phase) {
-    if (!b->graph[phase] || b->graph_blocks[phase] != b->nblocks || b->graph_tile[phase] != b->max_tile) {
+    if (!b->graph[phase] || b->graph_blocks[phase] != b->nblocks) {
         if (b->graph[phase]) hipGraphExecDestroy(b->graph[phase]);
         b->graph[phase] = nullptr;
         hipGraph_t g;
@@ -1513,13 +1501,12 @@ static int batch_graph_launch(pairs_hip_batch *b, int phase) {
                 launch_sync_pp(b->stream, c->d, c->geom, cur, 1 - cur, c->ghost_grid);
             }
         }
-        launch_lj_full_batch(b->stream, b->d_desc[phase], b->d_rep, b->d_base, b->nblocks, b->max_tile, b->ntypes1, 0);
+        launch_lj_full_batch(b->stream, b->d_desc[phase], b->d_rep, b->d_base, b->nblocks, b->ntypes1, b->unit, 0);
         BATCH_OK(hipStreamEndCapture(b->stream, &g));
         const hipError_t e = hipGraphInstantiate(&b->graph[phase], g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) { b->graph[phase] = nullptr; b->err = "batch graph instantiation failed"; return PAIRS_HIP_ERR_HIP; }
         b->graph_blocks[phase] = b->nblocks;
-        b->graph_tile[phase] = b->max_tile;
     }
     BATCH_OK(hipGraphLaunch(b->graph[phase], b->stream));
     return 0;
@@ -1544,6 +1531,7 @@ int pairs_hip_batch_create(pairs_hip_batch **out, int nreplicas, const pairs_hip
         const int e = pairs_hip_create(&ctx, &c);
         if (e) { pairs_hip_batch_destroy(b); return e; }
         b->rep.push_back(ctx);
+        b->unit = b->unit && b->ntypes1 && ctx->prm.eps[0] == 1.0 && ctx->prm.sig6[0] == 1.0;
     }
     const size_t R = nreplicas;
     if (hipHostMalloc((void **) &b->h_desc, 2 * R * sizeof(LJArgs)) != hipSuccess ||
@@ -1648,15 +1636,20 @@ int pairs_hip_batch_run(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps
             const int e = batch_graph_launch(b, phase);
             if (e) return e;
         } else {
+            int max_cap = 0;
             for (int r = 0; r < R; r++) {
                 pairs_hip_ctx *c = b->rep[r];
                 const int cur = b->base_cur[r] ^ phase;
-                b->h_desc[r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, vv2, vv1, 1, nullptr, false);
+                b->h_desc[r] = lj_args(c->d, c->geom, c->prm, cur, 1 - cur, vv2, vv1, 1, nullptr, true);
+                max_cap = std::max(max_cap, c->d.cap);
             }
-            if (int e = batch_check_tiles(b, b->h_desc)) return e;
             BATCH_OK(hipMemcpyAsync(b->d_desc[2], b->h_desc, R * sizeof(LJArgs), hipMemcpyHostToDevice, s));
             BATCH_OK(hipStreamSynchronize(s));  // h_desc is reused by the next descriptor build
-            launch_lj_full_batch(s, b->d_desc[2], b->d_rep, b->d_base, b->nblocks, b->max_tile, b->ntypes1, T ? 1 : 0);
+            // positions may have moved without their packed copy (a rebuild,
+            // an in-place ghost refresh or initial integrate): pack them;
+            // this launch writes the next buffer's copy for the steps after
+            launch_pack_p_batch(s, b->d_desc[2], R, max_cap);
+            launch_lj_full_batch(s, b->d_desc[2], b->d_rep, b->d_base, b->nblocks, b->ntypes1, b->unit, T ? 1 : 0);
             if (T) {
                 if (!b->rdesc_valid) {  // the replicas' arrays and sizes for the batched reduction
                     BATCH_OK(hipStreamSynchronize(s));

@@ -32,7 +32,7 @@ class DEMConfig(ctypes.Structure):
         ("particle_capacity", _I), ("contact_capacity", _I), ("send_capacity", _I), ("ncells_capacity", _I),
         ("box_lo", _D * 3), ("box_hi", _D * 3), ("periodic", _I * 3), ("cell_width", _D),
         ("dt", _D), ("density_particle", _D), ("density_fluid", _D), ("gravity", _D),
-        ("kappa", _D), ("ln_restitution", _D), ("collision_time", _D), ("ntypes", _I),
+        ("kappa", _D), ("ln_restitution", _D), ("collision_time", _D), ("stiffness_norm", _D), ("ntypes", _I),
         ("friction_static", _D * 16), ("friction_dynamic", _D * 16), ("stream", _P),
     ]
 
@@ -43,6 +43,7 @@ native.register({
     "pairs_hip_dem_last_error": (ctypes.c_char_p, [_P]),
     "pairs_hip_dem_last_required": (_I, [_P, _I]),
     "pairs_hip_dem_sc_grid": (_I, [_P, _D, _D, _D, _D, _D, _D, _D, _D, _D, _I]),
+    "pairs_hip_dem_jittered_grid": (_I, [_P, _I, _I, _I, _DP, _DP, _D, _D, _D, _D, ctypes.c_ulonglong]),
     "pairs_hip_dem_add_halfspace": (_I, [_P, _I, _I, _D, _DP, _DP, _I]),
     "pairs_hip_dem_add_spheres": (_I, [_P, _I, _IP, _IP, _DP, _DP, _DP, _DP, _IP]),
     "pairs_hip_dem_setup": (_I, [_P]),
@@ -91,6 +92,7 @@ class DEMSystem:
         cfg.density_particle, cfg.density_fluid = p["density_particle"], p["density_fluid"]
         lnr = math.log(p["restitution"]) if ln_restitution is None else ln_restitution
         cfg.kappa, cfg.ln_restitution, cfg.collision_time = kappa, lnr, p["collision_time"]
+        cfg.stiffness_norm = p.get("stiffness_norm", 0.0) or 0.0
         cfg.ntypes = ntypes
         for k in range(ntypes * ntypes):
             cfg.friction_static[k] = p["friction_static"]
@@ -135,6 +137,15 @@ class DEMSystem:
             self.ctx, dom[0], dom[1], dom[2], spacing or p["spacing"], d, dmin, dmax,
             p["initial_velocity"] if initial_velocity is None else initial_velocity,
             density or p["density_particle"], self.ntypes), "sc_grid")
+
+    def jittered_grid(self, n, origin, spacing, jitter, r_min, r_max, density=None, seed=7):
+        """Spheres at seeded random non-overlapping positions
+        (pairs_hip_dem_jittered_grid): n = (nx, ny, nz) sites."""
+        o = (_D * 3)(*origin)
+        sp = (_D * 3)(*spacing)
+        return self._check(self.L.pairs_hip_dem_jittered_grid(
+            self.ctx, n[0], n[1], n[2], o, sp, jitter, r_min, r_max,
+            density or self.params["density_particle"], seed), "jittered_grid")
 
     def add_halfspace(self, uid, position, normal, flags=13, type_=0, mass=1.0):
         pos = (_D * 3)(*position)

@@ -206,6 +206,10 @@ class LJSystem:
     def set_graphs(self, enable):
         self._call("pairs_hip_set_graphs", int(bool(enable)))
 
+    def prepare_graphs(self):
+        """Capture the step and rebuild graphs of both buffers now (set-up)."""
+        self._call("pairs_hip_prepare_graphs")
+
     def set_force_timing(self, every_n):
         self._call("pairs_hip_set_force_timing", every_n)
 

@@ -87,6 +87,7 @@ _SIGNATURES = {
     "pairs_hip_run": (_I, [_P, _I, _I, _I, _DP, _I]),
     "pairs_hip_sync": (_I, [_P]),
     "pairs_hip_set_graphs": (_I, [_P, _I]),
+    "pairs_hip_prepare_graphs": (_I, [_P]),
     "pairs_hip_set_force_timing": (_I, [_P, _I]),
     "pairs_hip_force_kernel_time": (_I, [_P, _DP, _IP]),
 }

@@ -116,3 +116,46 @@ def test_planes_read_from_csv_equal_builtin_planes(tmp_path):
     text = open(str(tmp_path / "dem_60.vtk")).read()
     nsph = int(np.sum(b.get("shape") == 0))
     assert f"POINTS {nsph} double" in text  # the two planes are Infinite
+
+
+# the settling workload's contact model and particles: k_n 1e4 N/m (fixed),
+# e 0.6, mu 0.5 with history, dt 1e-6 s, r ~ U[0.9, 1.1] mm, 2500 kg/m^3
+C3_GPU = dict(diameter=0.002, gravity=9.81, density_fluid=0.0, density_particle=2500.0, dt=1e-6, friction=0.5,
+              restitution=0.6, friction_static=0.0, stiffness_norm=1e4)
+C3_ORACLE = dict(po.DEM_REF, diameter=0.002, gravity=9.81, rho_f=0.0, rho_p=2500.0, dt=1e-6, mu=0.5, restitution=0.6)
+
+
+def test_settling_parameters_vs_oracle():
+    """6000 spheres with the settling workload's parameters on a
+    0.05 x 0.05 x 0.04 m domain (periodic x/y, floor plane): the jittered grid
+    with a z pitch of 2.25 mm, so that some neighbours start in contact and
+    fly apart -- sphere-sphere and floor contacts with history, periodic
+    crossings -- 3000 steps against the oracle."""
+    dom = (0.05, 0.05, 0.04)
+    n = (20, 20, 15)
+    origin, pitch, jitter = (0.00125, 0.00125, 0.001), (0.0025, 0.0025, 0.00225), 1e-4
+    g = _dem().DEMSystem(dom, particle_capacity=16384, send_capacity=8192, params=C3_GPU)
+    assert g.jittered_grid(n, origin, pitch, jitter, 0.9e-3, 1.1e-3) == 6000
+    g.add_halfspace(100000, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    g.setup()
+    o = po.OracleDEM(dom, cap=16384, send_cap=8192, cfg=C3_ORACLE, stiffness_norm=1e4, ncells_cap=400000)
+    assert o.jittered_grid(n, origin, pitch, jitter, 0.9e-3, 1.1e-3) == 6000
+    o.add_halfspace(100000, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    o.setup()
+    for k in ("position", "radius", "mass", "uid"):
+        assert np.array_equal(g.get(k), o.get(k)), k
+    for a, b in ((0, 1000), (1000, 3000)):
+        g.run(a, b)
+        o.run(a, b)
+        assert g.nlocal == o.nlocal and g.nghost == o.nghost, b
+        assert np.array_equal(g.get("uid"), o.get("uid")), b
+        for k in ("position", "linear_velocity", "angular_velocity", "force", "torque"):
+            assert _rel(g.get(k), o.get(k)) <= REL, (b, k, _rel(g.get(k), o.get(k)))
+        nc, cl, sk, tsd, iv = g.contacts()
+        onc, ocl, osk, otsd, oiv = o.contacts()
+        assert np.array_equal(nc, onc), b
+        gu, gs, gt, gi = _flat_contacts(nc, cl, sk, tsd, iv)
+        ou, os_, ot, oi = _flat_contacts(onc, ocl, osk, otsd, oiv)
+        assert np.array_equal(gu, ou) and np.array_equal(gs, os_), b
+        assert _rel(gt, ot) <= REL and _rel(gi, oi) <= REL, b
+    assert int(nc.sum()) > 50

@@ -164,16 +164,6 @@ def test_overflow_at_a_later_rebuild(which):
     _compare_lists(g, o, tag=f"overflow {which}")
 
 
-def test_force_kernel_reports_no_skipped_segment():
-    """The force kernel never meets a tile above its staged capacity after a
-    converged build; the run checks the device flag and would fail otherwise."""
-    md = _md()
-    g = md.fcc_system(8)
-    g.run(0, 41, 40)
-    from pairs_amd import native as nat
-    assert g.count(nat.TILE_CAPACITY) >= 64  # staged entries, grown from the builds
-
-
 @pytest.mark.parametrize("half", [False, True])
 def test_energy_and_virial_vs_oracle(half):
     md = _md()

@@ -139,3 +139,73 @@ def test_head_on_collision_restitution():
     # after t_c with relative speed e (continuous time)
     assert abs(e - 0.5) < 0.01
     assert abs(v1 + v2) < 1e-12  # momentum of two equal spheres is conserved
+
+
+# ---------------------------------------------------------------- the settling workload (fixed k_n, jittered grid)
+
+C3 = dict(po.DEM_REF, gravity=9.81, rho_f=0.0, rho_p=2500.0, dt=1e-6, mu=0.5, restitution=0.6)
+
+
+def test_mt19937_64_check_value():
+    # [rand.predef]: the 10000th consecutive output of a default-constructed
+    # std::mt19937_64 is 9981545732273789042
+    assert po.mt64_nth(5489, 10000) == 9981545732273789042
+
+
+def test_jittered_grid_places_non_overlapping_spheres():
+    o = po.OracleDEM((0.05, 0.05, 0.04), cap=8192, cfg=C3, stiffness_norm=1e4)
+    n = o.jittered_grid((20, 20, 15), (0.00125, 0.00125, 0.0012), (0.0025, 0.0025, 0.0024), 1e-4, 0.9e-3, 1.1e-3)
+    assert n == 6000
+    x, r = o.get("position"), o.get("radius")
+    assert r.min() >= 0.9e-3 and r.max() <= 1.1e-3 and r.std() > 0.05e-3
+    site = np.stack(np.meshgrid(np.arange(15), np.arange(20), np.arange(20), indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+    grid = np.array([0.00125, 0.00125, 0.0012]) + site * np.array([0.0025, 0.0025, 0.0024])
+    off = x - grid
+    assert np.all(np.abs(off) <= 1e-4) and np.abs(off).max() > 0.9e-4  # jitter used, within bounds
+    assert np.array_equal(o.get("uid"), np.arange(1, 6001))
+    # no two spheres overlap, across the periodic x/y boundaries too
+    d = x[:, None, :] - x[None, :, :]
+    d[..., :2] -= 0.05 * np.round(d[..., :2] / 0.05)
+    dist = np.sqrt((d * d).sum(-1))
+    np.fill_diagonal(dist, np.inf)
+    assert np.all(dist > r[:, None] + r[None, :])
+
+
+def test_fixed_stiffness_equals_collision_time_form():
+    """A fixed k_n with the restitution-preserving damping is the collision-time
+    model at the t_c that gives the same k_n for the pair's effective mass:
+    two equal spheres colliding head-on follow the same trajectory (to
+    rounding), and separate with the relative speed e (dt = t_c / 400: the
+    explicit integrator resolves the contact)."""
+    r, v, kn = 1.0e-3, 0.2, 1e4
+    m = 4.0 / 3.0 * math.pi * r ** 3 * 2500.0
+    meff = 0.5 * m
+    ln_e = math.log(0.6)
+    tc = math.sqrt(meff * (math.pi ** 2 + ln_e ** 2) / kn)
+    runs = []
+    for fixed in (True, False):
+        cfg = dict(C3, gravity=0.0, t_c=tc, dt=tc / 400)
+        o = po.OracleDEM((0.1, 0.1, 0.1), periodic=(1, 1, 0), cfg=cfg, stiffness_norm=kn if fixed else 0.0)
+        o.add_sphere(1, r, (0.05 - 1.02 * r, 0.05, 0.05), (v, 0.0, 0.0))
+        o.add_sphere(2, r, (0.05 + 1.02 * r, 0.05, 0.05), (-v, 0.0, 0.0))
+        o.setup()
+        o.run(0, 1600)
+        uid = o.get("uid")
+        runs.append(o.get("linear_velocity")[np.argsort(uid)])
+    assert np.allclose(runs[0], runs[1], rtol=1e-12, atol=1e-15)
+    e = (runs[0][1][0] - runs[0][0][0]) / (2 * v)
+    assert abs(e - 0.6) < 0.01
+
+
+def test_fixed_stiffness_resting_penetration():
+    o = po.OracleDEM((0.1, 0.1, 0.1), periodic=(1, 1, 0), cfg=C3, stiffness_norm=1e4)
+    r = 1.0e-3
+    o.add_sphere(1, r, (0.05, 0.05, r), (0.0, 0.0, 0.0))
+    o.add_halfspace(100000, (0.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+    o.setup()
+    o.run(0, 20000)  # about 40 oscillation periods of sqrt(m / k_n)
+    m = o.get("mass")[0]
+    w = 9.81 * m
+    assert abs(o.get("force")[0][2]) < 1e-8 * w
+    delta = r - o.get("position")[0][2]
+    assert abs(delta - w / 1e4) < 1e-6 * (w / 1e4)

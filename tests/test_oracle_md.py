@@ -265,3 +265,52 @@ def test_energy_two_body_closed_form():
     assert abs(pe - u) < 1e-13 and abs(e.sum() - u) < 1e-13
     fr = 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7)
     assert abs(w - r * fr) < 1e-12
+
+
+def _energy_of(x, v, L, half):
+    t = po.OracleMD([(0.0, L)] * 3, 8192, half=half, reneighbor_every=1)
+    t.set_particles(x, v)
+    t.setup_cells()
+    t.rebuild()
+    t.force()
+    return t.energy()
+
+
+def test_half_list_energy_and_virial_equal_full_list():
+    """The half-list weighting (a local-local pair counted once with weight 1,
+    a local-ghost pair seen from the local side with weight 1/2) gives the
+    full list's PE and virial (every ordered pair with weight 1/2) on the same
+    state, to rounding; the per-atom energies sum to the PE on both."""
+    s = po.fcc_system(5)
+    s.run(0, 60)
+    x, v = s.positions(), s.velocities()
+    L = po.fcc_box_length(5, 0.8442)
+    pe_f, w_f, e_f = _energy_of(x, v, L, False)
+    pe_h, w_h, e_h = _energy_of(x, v, L, True)
+    assert abs(pe_h - pe_f) <= 1e-12 * abs(pe_f)
+    assert abs(w_h - w_f) <= 1e-12 * abs(w_f)
+    assert abs(e_f.sum() - pe_f) <= 1e-12 * abs(pe_f)
+    # a dropped local-ghost pair or a doubled local-local pair moves the sum
+    # by far more than rounding: one pair's energy is about 1e-2 of |pe| / N
+    assert abs(pe_f) > 100.0
+
+
+@pytest.mark.parametrize("half", [False, True])
+@pytest.mark.parametrize("across", [False, True])
+def test_energy_two_body_closed_form_full_and_half(half, across):
+    """Two particles at distance 1.2, either inside the box (a local-local
+    pair) or across the periodic boundary in x (each local sees the other's
+    ghost: two local-ghost entries): U = 4 (r^-12 - r^-6) and
+    W = r F(r) with F = 24 (2 r^-13 - r^-7), for full and half lists."""
+    r, L = 1.2, 10.0
+    if across:
+        x = np.array([[0.5, 5.0, 5.0], [L - (r - 0.5), 5.0, 5.0]])
+    else:
+        x = np.array([[5.0, 5.0, 5.0], [5.0, 5.0 + r, 5.0]])
+    pe, w, e = _energy_of(x, np.zeros_like(x), L, half)
+    u = 4.0 * (r ** -12 - r ** -6)
+    fr = 24.0 * (2.0 / r ** 13 - 1.0 / r ** 7)
+    assert abs(pe - u) < 1e-13, (pe, u)
+    assert abs(w - r * fr) < 1e-12, (w, r * fr)
+    if not half:
+        assert abs(e.sum() - u) < 1e-13

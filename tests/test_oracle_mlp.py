@@ -104,3 +104,19 @@ def test_bf16_rounding_and_budget():
     Eb, Db = mo.forward_bf16(p, sc, G)
     assert np.sqrt(np.mean((E - Eb) ** 2)) < 0.01 * E.std()
     assert np.sqrt(np.mean((D - Db) ** 2) / np.mean(D ** 2)) < 0.01
+
+
+def test_virial_is_minus_scaling_derivative():
+    """W = sum over pairs of r_ij . F_ij equals -dE/dlambda at lambda = 1 for
+    the uniformly scaled system (x -> lambda x, box -> lambda L): the energy
+    depends on the pair distances only, so dE/dlambda = sum_pairs (dE/dr) r.
+    A wrong sign, a doubled pair or a dropped c_j term fails it."""
+    x, L = _cluster()
+    p, sc = mo.xavier(7), _scale()
+    W = mo.virial(p, sc, x, L)
+    h = 1e-6
+    ep = mo.energy(p, sc, x * (1 + h), L * (1 + h))
+    em = mo.energy(p, sc, x * (1 - h), L * (1 - h))
+    dEdl = (ep - em) / (2 * h)
+    assert abs(W + dEdl) <= 1e-6 * max(1.0, abs(W)), (W, -dEdl)
+    assert abs(W) > 1e-3  # a non-trivial check
