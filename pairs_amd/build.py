@@ -43,6 +43,26 @@ def build(verbose=False):
     return LIB
 
 
+def build_host_program(cpp, exe, verbose=False):
+    """Compile a host program emitted by the DSL's HIP target (pairs_amd.dsl,
+    emit_cpp) against the library: g++, the C ABI headers of include/, and
+    libpairs_hip.so found at run time next to the package (rpath relative to
+    the executable's directory when it lies inside the tree)."""
+    if not os.path.exists(LIB):
+        raise RuntimeError("libpairs_hip.so is not built: run pairs_amd.build.build() first")
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    exe = os.path.abspath(exe)
+    os.makedirs(os.path.dirname(exe), exist_ok=True)
+    rel = os.path.relpath(HERE, os.path.dirname(exe))
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(HERE, "..", "include"), "-o", exe + ".tmp", cpp,
+           "-L" + HERE, "-lpairs_hip", "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath-link," + os.path.join(rocm, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
     print(LIB)

@@ -717,8 +717,6 @@ class Simulation:
         emit_cpp = emit_cpp or _SCRIPT["emit_cpp"]
         quiet = quiet or _SCRIPT["quiet"]
         kind, _ = self._plan()
-        if emit_cpp and kind == "md_euler":
-            raise UnsupportedModule("emit_cpp writes the md.py and dem.py programs")
         if emit_cpp:
             with open(emit_cpp, "w") as f:
                 f.write(self._cpp(kind))
@@ -767,9 +765,126 @@ class Simulation:
                     io.vtk_write(s, self.vtk_file + "_local", b - 1, self.vtk_frequency)
         return s
 
+    def _cpp_md_euler(self):
+        """The older API's LJ program (particles from one file, full lists,
+        its own pair module, Euler) as a host program on the C ABI: the
+        modules called one by one in the reference's timestep order, each
+        retried with grown capacities on overflow (the generated resize loop,
+        src/pairs/transformations/modules.py:159-205)."""
+        n = self.timesteps
+        reads = [s for s in self.setups if s[0] == "read"]
+        _, path, names, _ = reads[0]
+        width = {"mass": 1, "position": 3, "velocity": 3}
+        kinds = {"mass": "PAIRS_HIP_PROP_REAL", "position": "PAIRS_HIP_PROP_VECTOR",
+                 "velocity": "PAIRS_HIP_PROP_VECTOR"}
+        for nm in names:
+            if nm not in width:
+                raise UnsupportedModule(f"from_file property {nm}: the older program reads mass, position, velocity")
+        cols, off = {}, 0
+        for nm in names:
+            cols[nm] = off
+            off += width[nm]
+        lj, eu = self.modules
+        lo = [d[0] for d in self.domain]
+        hi = [d[1] for d in self.domain]
+        mcol = f"rows[(size_t) i * {off} + {cols['mass']}]" if "mass" in cols else "1.0"
+        return f'''// host program of "{self.ref}" (the older API) for the HIP target (include/pairs_hip.h)
+// usage: prog [state.bin [particles.csv]]  (state.bin: nlocal, then positions,
+// velocities and forces of the final state in the reference's particle order)
+#include <cstdio>
+#include <vector>
+#include "pairs_hip.h"
+#include "pairs_hip_io.h"
+
+// a module call with the resize protocol: grow the capacity it outgrew (x2 the need) and repeat
+static int module(pairs_hip_ctx *ctx, int (*fn)(pairs_hip_ctx *)) {{
+    for (int attempt = 0; attempt < 8; attempt++) {{
+        const int r = fn(ctx);
+        if (r <= 0) return r;
+        const int need = pairs_hip_last_required(ctx, r);
+        int grow[3] = {{0, 0, 0}};
+        grow[r == PAIRS_HIP_OVERFLOW_PARTICLES ? 0 : r == PAIRS_HIP_OVERFLOW_NEIGHBORS ? 1 : 2] = 2 * need;
+        if (pairs_hip_reserve(ctx, grow[0], grow[1], grow[2])) return -1;
+    }}
+    return -1;
+}}
+
+static int dump(pairs_hip_ctx *ctx, const char *path) {{
+    const int n = pairs_hip_count(ctx, PAIRS_HIP_NLOCAL);
+    if (n < 0) return 1;
+    std::vector<double> x(3 * (size_t) n), v(3 * (size_t) n), f(3 * (size_t) n);
+    if (pairs_hip_download(ctx, PAIRS_HIP_POSITION, x.data(), n) != n) return 1;
+    if (pairs_hip_download(ctx, PAIRS_HIP_VELOCITY, v.data(), n) != n) return 1;
+    if (pairs_hip_download(ctx, PAIRS_HIP_FORCE, f.data(), n) != n) return 1;
+    FILE *o = std::fopen(path, "wb");
+    if (!o) return 1;
+    std::fwrite(&n, sizeof(int), 1, o);
+    std::fwrite(x.data(), sizeof(double), x.size(), o);
+    std::fwrite(v.data(), sizeof(double), v.size(), o);
+    std::fwrite(f.data(), sizeof(double), f.size(), o);
+    return std::fclose(o) != 0;
+}}
+
+int main(int argc, char **argv) {{
+    const char *input = argc > 2 ? argv[2] : {_c_string(path)};
+    const int kinds[{len(names)}] = {{{", ".join(kinds[nm] for nm in names)}}};
+    const double lo[3] = {{{lo[0]!r}, {lo[1]!r}, {lo[2]!r}}}, hi[3] = {{{hi[0]!r}, {hi[1]!r}, {hi[2]!r}}};
+    int need = 0;
+    pairs_hip_read_particle_csv(input, {len(names)}, kinds, {names.index("position")}, -1, lo, hi, nullptr, 0, &need);
+    std::vector<double> rows((size_t) {off} * (need > 0 ? need : 1));
+    const int np = pairs_hip_read_particle_csv(input, {len(names)}, kinds, {names.index("position")}, -1, lo, hi,
+                                               rows.data(), need, &need);
+    if (np < 0) {{ std::fprintf(stderr, "cannot read %s\\n", input); return 1; }}
+    std::vector<double> x(3 * (size_t) np), v(3 * (size_t) np), m(np);
+    for (int i = 0; i < np; i++) {{
+        for (int d = 0; d < 3; d++) {{
+            x[3 * i + d] = rows[(size_t) i * {off} + {cols["position"]} + d];
+            v[3 * i + d] = rows[(size_t) i * {off} + {cols["velocity"]} + d];
+        }}
+        m[i] = {mcol};
+    }}
+    static double eps[1] = {{{float_(lj["symbols"]["epsilon"])!r}}};
+    static double sig6[1] = {{{float_(lj["symbols"]["sigma6"])!r}}};
+    pairs_hip_md_config c = {{}};
+    c.particle_capacity = {max(self.particle_capacity, 16)};
+    c.neighbor_capacity = {self.neighbor_capacity};
+    c.send_capacity = 200000;
+    c.ncells_capacity = 100000;
+    for (int d = 0; d < 3; d++) {{ c.box_lo[d] = lo[d]; c.box_hi[d] = hi[d]; }}
+    c.periodic[0] = {int(self.pbc_flags[0])}; c.periodic[1] = {int(self.pbc_flags[1])}; c.periodic[2] = {int(self.pbc_flags[2])};
+    c.cell_spacing = {self.neighbor_radius!r};
+    c.cutoff = {lj["cutoff"]!r};
+    c.dt = {eu["symbols"]["dt"]!r};
+    c.half = 0;
+    c.ntypes = 1;
+    c.epsilon = eps;
+    c.sigma6 = sig6;
+    c.reneighbor_every = {self.reneighbor};
+    c.thermo_every = 0;
+    pairs_hip_ctx *ctx;
+    if (pairs_hip_create(&ctx, &c)) return 1;
+    int bad = pairs_hip_upload(ctx, np, x.data(), v.data(), m.data(), nullptr, nullptr, nullptr) != 0;
+    if (!bad) bad = pairs_hip_setup_cells(ctx) != 0;
+    for (int ts = 0; !bad && ts <= {n}; ts++) {{
+        if ({self.reneighbor} > 0 && ((ts + 1) % {max(self.reneighbor, 1)} == 0 || ts == 0)) {{
+            bad = module(ctx, pairs_hip_exchange) || module(ctx, pairs_hip_borders) ||
+                  module(ctx, pairs_hip_bin_cells) || module(ctx, pairs_hip_build_neighbors);
+        }} else {{
+            bad = pairs_hip_sync_ghosts(ctx) != 0;
+        }}
+        if (!bad) bad = pairs_hip_lj_older(ctx) != 0 || pairs_hip_euler(ctx) != 0;
+    }}
+    if (!bad && argc > 1) bad = dump(ctx, argv[1]);
+    pairs_hip_destroy(ctx);
+    return bad;
+}}
+'''
+
     def _cpp(self, kind):
         """The host C++ program of this simulation on the C ABI (the HIP
         target's generated main)."""
+        if kind == "md_euler":
+            return self._cpp_md_euler()
         n = self.timesteps
         if kind == "md":
             (_, nx, ny, nz, rho, temp, nt) = self.setups[0]
@@ -778,10 +893,27 @@ class Simulation:
             sg6 = self.feature_props.get("sigma6", (0, 0, [1.0] * nt * nt))[2]
             a = (4.0 / rho) ** (1.0 / 3.0)
             return f'''// host program of "{self.ref}" for the HIP target (include/pairs_hip.h)
+// usage: prog [state.bin]  (state.bin: nlocal, then positions and velocities
+// of the final state in the reference's particle order, as doubles)
 #include <cstdio>
+#include <vector>
 #include "pairs_hip.h"
 
-int main() {{
+static int dump(pairs_hip_ctx *ctx, const char *path) {{
+    const int n = pairs_hip_count(ctx, PAIRS_HIP_NLOCAL);
+    if (n < 0) return 1;
+    std::vector<double> x(3 * (size_t) n), v(3 * (size_t) n);
+    if (pairs_hip_download(ctx, PAIRS_HIP_POSITION, x.data(), n) != n) return 1;
+    if (pairs_hip_download(ctx, PAIRS_HIP_VELOCITY, v.data(), n) != n) return 1;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return 1;
+    std::fwrite(&n, sizeof(int), 1, f);
+    std::fwrite(x.data(), sizeof(double), x.size(), f);
+    std::fwrite(v.data(), sizeof(double), v.size(), f);
+    return std::fclose(f) != 0;
+}}
+
+int main(int argc, char **argv) {{
     static double eps[{nt * nt}] = {{{", ".join(repr(float_(x)) for x in eps)}}};
     static double sig6[{nt * nt}] = {{{", ".join(repr(float_(x)) for x in sg6)}}};
     pairs_hip_md_config c = {{}};
@@ -808,8 +940,10 @@ int main() {{
     double th[5 * 4096];
     const int rows = pairs_hip_run(ctx, 0, {n + 1}, {n}, th, 4096);
     for (int r = 0; r < rows; r++) std::printf("%g\\t%g\\n", th[5 * r], th[5 * r + 1]);
+    int bad = rows < 0;
+    if (!bad && argc > 1) bad = dump(ctx, argv[1]);
     pairs_hip_destroy(ctx);
-    return rows < 0;
+    return bad;
 }}
 '''
         grav, con, eul = self.modules
@@ -827,9 +961,30 @@ int main() {{
         fs = self.feature_props.get("friction_static", (0, 0, [0.0] * ntypes * ntypes))[2]
         fd = self.feature_props.get("friction_dynamic", (0, 0, [0.5] * ntypes * ntypes))[2]
         lines = [f'''// host program of "{self.ref}" for the HIP target (include/pairs_hip_dem.h)
+// usage: prog [state.bin]  (state.bin: nlocal, then positions, linear
+// velocities and uids of the final state's locals)
+#include <cstdio>
+#include <vector>
 #include "pairs_hip_dem.h"
 
-int main() {{
+static int dump(pairs_hip_dem_ctx *d, const char *path) {{
+    const int n = pairs_hip_dem_count(d, PAIRS_HIP_DEM_NLOCAL);
+    if (n < 0) return 1;
+    std::vector<double> x(3 * (size_t) n), v(3 * (size_t) n);
+    std::vector<int> uid(n);
+    if (pairs_hip_dem_download(d, PAIRS_HIP_DEM_POSITION, x.data(), n) != n) return 1;
+    if (pairs_hip_dem_download(d, PAIRS_HIP_DEM_LINEAR_VELOCITY, v.data(), n) != n) return 1;
+    if (pairs_hip_dem_download(d, PAIRS_HIP_DEM_UID, uid.data(), n) != n) return 1;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return 1;
+    std::fwrite(&n, sizeof(int), 1, f);
+    std::fwrite(x.data(), sizeof(double), x.size(), f);
+    std::fwrite(v.data(), sizeof(double), v.size(), f);
+    std::fwrite(uid.data(), sizeof(int), uid.size(), f);
+    return std::fclose(f) != 0;
+}}
+
+int main(int argc, char **argv) {{
     pairs_hip_dem_config c = {{}};
     c.particle_capacity = {min(self.particle_capacity, 4 << 20)};
     c.contact_capacity = {self.neighbor_capacity};
@@ -856,9 +1011,10 @@ int main() {{
                          f"{{{nrm[0]!r}, {nrm[1]!r}, {nrm[2]!r}}}; "
                          f"pairs_hip_dem_add_halfspace(d, {uid}, {typ}, {mass!r}, p, n, {fl}); }}")
         lines.append(f'''    pairs_hip_dem_setup(d);
-    const int r = pairs_hip_dem_run(d, 0, {n + 1});
+    int bad = pairs_hip_dem_run(d, 0, {n + 1}) != 0;
+    if (!bad && argc > 1) bad = dump(d, argv[1]);
     pairs_hip_dem_destroy(d);
-    return r != 0;
+    return bad;
 }}
 ''')
         return "\n".join(lines)
@@ -866,6 +1022,19 @@ int main() {{
 
 def float_(x):
     return builtins.float(x)
+
+
+def _c_string(text):
+    """a C string literal of text (backslashes, quotes and control characters escaped)"""
+    out = []
+    for ch in text:
+        if ch in '\\"':
+            out.append("\\" + ch)
+        elif ord(ch) < 32 or ord(ch) > 126:
+            out.append("\\%03o" % ord(ch))
+        else:
+            out.append(ch)
+    return '"' + "".join(out) + '"'
 
 
 # how generate() behaves inside run_script, and the simulations it made

@@ -1,6 +1,7 @@
 """The DSL front end (pairs_amd/dsl.py): module recognition by traced
 expression graphs, lowering to the C ABI, and the emitted host C++ program
 (on the host; the GPU runs are in tests/test_gpu_dsl.py)."""
+import json
 import os
 import shutil
 import subprocess
@@ -32,15 +33,32 @@ def lennard_jones(i, j):
 """
 
 
-def test_recognises_the_example_modules():
-    sims = dsl.run_script(os.path.join(HERE, "dsl", "lj_fcc.py"), ["gpu"], run=False)
-    assert [dsl.recognise(m["fn"], m["symbols"]) for m in sims[0].modules] == \
-        ["initial_integrate", "lennard_jones", "final_integrate"]
-    sims = dsl.run_script(os.path.join(HERE, "dsl", "dem_settle.py"), ["gpu"], run=False)
-    s = sims[0]
-    assert [dsl.recognise(m["fn"], m["symbols"]) for m in s.modules] == ["gravity", "linear_spring_dashpot",
-                                                                           "euler"]
-    assert [dsl.recognise(f, sy) for f, sy in s.setup_modules] == ["update_mass_and_inertia"]
+def _fixture(name):
+    with open(os.path.join(HERE, "golden", f"dsl_{name}.json")) as f:
+        return json.load(f)
+
+
+def test_example_scripts_were_recognised():
+    """The reference's example scripts, run through the front end by
+    tests/golden/make_dsl_fixtures.py: their modules were recognised as the
+    kernels' modules and lowered with the scripts' own configuration."""
+    md = _fixture("md_fcc4_full")
+    assert md["script"] == "examples/md.py" and md["plan"] == "md"
+    assert [m["name"] for m in md["modules"]] == ["initial_integrate", "lennard_jones", "final_integrate"]
+    ii, lj, fi = md["modules"]
+    assert ii["pre_step"] and ii["skip_first"] and fi["skip_first"] and not fi["pre_step"]
+    assert lj["cutoff"] == 2.5 and md["neighbor_radius"] == 2.8 and ii["symbols"]["dt"] == 0.005
+    assert md["reneighbor_every"] == 20 and md["thermo_every"] == 100 and not md["half"]
+    assert _fixture("md_fcc4_half")["half"]
+    dem = _fixture("dem_sc168")
+    assert dem["script"] == "examples/dem.py" and dem["plan"] == "dem"
+    assert [m["name"] for m in dem["modules"]] == ["gravity", "linear_spring_dashpot", "euler"]
+    assert dem["setup_modules"] == ["update_mass_and_inertia"]
+    assert dem["pbc"] == [True, True, False] and dem["timesteps"] == 500
+    old = _fixture("lj_onetype")
+    assert old["script"] == "examples/lj_onetype.py" and old["plan"] == "md_euler"
+    assert [m["name"] for m in old["modules"]] == ["lj_older", "euler_md"]
+    assert old["reneighbor_every"] == 1 and old["neighbor_radius"] == 2.8
 
 
 def test_operation_order_is_part_of_the_module(tmp_path):
@@ -64,35 +82,64 @@ def test_rejects_unsupported_configurations(tmp_path):
         sim.generate(run=False)  # no integrators: not the md.py program
     with pytest.raises(dsl.UnsupportedModule):
         dsl.simulation("y", [dsl.point_mass()], double_prec=False)
-    with pytest.raises(dsl.UnsupportedModule):  # target_cpu(): pairs_amd has the GPU target only
-        dsl.run_script(os.path.join(HERE, "dsl", "lj_fcc.py"), ["cpu"], run=False)
+    cpu = dsl.simulation("z", [dsl.point_mass()], timesteps=1, double_prec=True)
+    cpu.target(dsl.target_cpu())
+    with pytest.raises(dsl.UnsupportedModule, match="GPU target only"):
+        cpu.generate(run=False)
 
 
-@pytest.mark.parametrize("script", ["lj_fcc.py", "dem_settle.py"])
-def test_emitted_host_program_compiles(tmp_path, script):
-    out = str(tmp_path / (script[:-3] + ".cpp"))
-    dsl.run_script(os.path.join(HERE, "dsl", script), ["gpu"], run=False, emit_cpp=out)
-    text = open(out).read()
-    assert "int main()" in text and ("pairs_hip_run(" in text or "pairs_hip_dem_run(" in text)
+@pytest.mark.parametrize("name", ["md_fcc4_full", "md_fcc4_half", "dem_sc168", "lj_onetype"])
+def test_emitted_host_program_compiles(name):
+    """The committed emitted programs compile against the C ABI headers (the
+    GPU tests build, run and check them)."""
+    src = os.path.join(HERE, "golden", f"dsl_{name}.cpp")
+    text = open(src).read()
+    assert "int main(int argc, char **argv)" in text
+    assert any(call in text for call in ("pairs_hip_run(", "pairs_hip_dem_run(", "pairs_hip_lj_older("))
     cxx = shutil.which("g++")
     if cxx is None:
         pytest.skip("no host compiler")
-    r = subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), out],
+    r = subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
 
 
-def test_older_api_aliases():
+# the older API's modules: bare rsq / delta in the pair module, whose
+# expression tree must be the older kernel's (examples/lj_onetype.py:5-13)
+OLDER_PAIR = """
+def pair(i, j):
+    inv = 1.0 / rsq
+    s6 = inv * inv * inv * sigma6
+    force[i] += delta * 48.0 * s6 * (s6 - 0.5) * inv * epsilon
+"""
+OLDER_STEP = """
+def step(i):
+    velocity[i] += dt * force[i] / mass[i]
+    position[i] += dt * velocity[i]
+"""
+
+
+def test_older_api_aliases(tmp_path):
     """The older calls configure a simulation (properties, file input, cells,
     neighbour lists, periodicity); its modules are traced with rsq / delta as
     bare names and recognised as the older LJ program (its own pair
     expression, Euler), which needs the box the older API leaves implicit."""
-    script = os.path.join(HERE, "dsl", "older_api.py")
     data = os.path.join(HERE, "golden", "minimd_setup_4x4x4_onetype.input")
+    sim = dsl.simulation("older_api")
+    sim.add_real_property("mass", 1.0)
+    sim.add_position("position")
+    sim.add_vector_property("velocity")
+    sim.add_vector_property("force", vol=True)
+    sim.from_file(data, ["mass", "position", "velocity"])
+    sim.create_cell_lists(2.8, 2.5)
+    sim.create_neighbor_lists()
+    sim.periodic(2.8)
+    sim.compute(_module(tmp_path, OLDER_PAIR, "pair"), 2.5, {"sigma6": 1.0, "epsilon": 1.0})
+    sim.compute(_module(tmp_path, OLDER_STEP, "step"), symbols={"dt": 0.005})
+    sim.target(dsl.target_gpu())
     with pytest.raises(dsl.UnsupportedModule, match="set_domain"):
-        dsl.run_script(script, [data], run=False)
-    assert [dsl.recognise(m["fn"], m["symbols"]) for m in dsl._CREATED[-1].modules] == ["lj_older", "euler_md"]
-    sim = dsl._CREATED[-1]
+        sim.generate(run=False)
+    assert [dsl.recognise(m["fn"], m["symbols"]) for m in sim.modules] == ["lj_older", "euler_md"]
     assert set(sim.props) == {"mass", "position", "velocity", "force"}
     assert sim.props["force"][2] is True and sim.position_name == "position"
     assert sim.setups == [("read", data, ["mass", "position", "velocity"], dsl.point_mass())]
@@ -105,5 +152,9 @@ def test_older_api_aliases():
     assert f"div(const:1.0,{rsq})" in ids.values()
     assert f"mul({dlt},const:48.0)" in ids.values()  # the older order: delta * 48.0 first
     assert form.splitlines()[-1].startswith("set[](name:force,i,")
+    # another rounding order of the same pair force is not the older kernel
+    with pytest.raises(dsl.UnsupportedModule):
+        dsl.recognise(_module(tmp_path, OLDER_PAIR.replace("delta * 48.0", "48.0 * delta"), "pair"),
+                      {"sigma6": 1.0, "epsilon": 1.0})
     with pytest.raises(dsl.UnsupportedModule, match="loop-style kernels"):
         sim.particle_pairs(2.5, None)
