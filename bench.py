@@ -132,14 +132,27 @@ def run_batched(steps, warmup, nreplicas):
     }
 
 
-def run_mlp(steps, warmup, train_steps):
-    """The MLP potential at 131072 atoms: GPU training on LJ snapshots, the
-    force check that decides MLP or LJ fallback, and the MLP MD throughput."""
+MLP_TEMPS = (0.8, 1.1, 1.4, 1.7, 2.0)
+MLP_CONFIG_NX = 8  # force-training systems: FCC 8^3 = 2048 atoms each
+
+
+def run_mlp(steps, warmup, train_steps, force_steps):
+    """The MLP potential at 131072 atoms: GPU training on LJ data, the force
+    check that decides MLP or LJ fallback, and the MLP MD throughput.
+
+    Training, all on the GPU: (1) energy rows -- every atom's descriptor and
+    per-atom LJ energy from 10 snapshots of the 131072-atom system (five
+    temperatures in [0.8, 2.0], two snapshots 50 steps apart each), Adam on
+    minibatches of 4096 rows; (2) configurations with forces -- five plain-LJ
+    systems of 2048 atoms at the same temperatures each advance 50 steps
+    between Adam steps, and every step takes the gradient of the energy term
+    plus the relative squared force error of all five configurations
+    (force_steps x 5 snapshots sampled every 50 steps)."""
     from pairs_amd import md, mlp
 
     m = mlp.MLPModel(seed=20241202, dataset_capacity=12 * NATOMS)
     t0 = time.perf_counter()
-    rows = mlp.collect_lj_snapshots(m, lambda T: md.fcc_system(NX, temp=T), (0.8, 1.1, 1.4, 1.7, 2.0),
+    rows = mlp.collect_lj_snapshots(m, lambda T: md.fcc_system(NX, temp=T), MLP_TEMPS,
                                     equilibrate=200, every=50, per_temp=2)
     t_collect = time.perf_counter() - t0
     m.fit_scaling()
@@ -152,6 +165,22 @@ def run_mlp(steps, warmup, train_steps):
     # held-out state: plain LJ at T = 1.44 after 300 steps (the last a rebuild)
     sim = md.fcc_system(NX)
     sim.run(0, 300, 100000)
+    rel_energy_only = mlp.force_error(sim, m)
+    t0 = time.perf_counter()
+    hist = []
+    if force_steps > 0:
+        sims = []
+        for T in MLP_TEMPS:
+            s = md.fcc_system(MLP_CONFIG_NX, temp=T)
+            s.run(0, 200, 10 ** 9)
+            sims.append(s)
+        _, hist = mlp.train_on_configurations(m, sims, force_steps, every=50, ts0=200, lr=1e-3, w_energy=1.0,
+                                              w_force=1.0, total_steps=10 ** 9,
+                                              record_every=max(force_steps // 10, 1))
+        for s in sims:
+            s.close()
+    t_force = time.perf_counter() - t0
+    rmse2 = m.rmse()
     rel = mlp.force_error(sim, m)
     attached, _ = mlp.attach(sim, m, 0.02)
     # MLP MD throughput (attached regardless of the check, to time the path)
@@ -172,17 +201,25 @@ def run_mlp(steps, warmup, train_steps):
         "ms_per_step": t_md * 1e3 / steps,
         "force_eval_ms": ms,
         "force_rel_rms_vs_lj": rel,
+        "force_rel_rms_vs_lj_energy_training_only": rel_energy_only,
         "fallback_to_lj": not attached,
         "fallback_rule": "MLP used only if relative RMS force error vs LJ <= 0.02",
         "energy_rmse_train_before": rmse0,
         "energy_rmse_train_after": rmse1,
-        "training": {"rows": rows, "steps": train_steps, "batch": batch,
+        "energy_rmse_train_after_force_training": rmse2,
+        "training": {"energy_rows": rows, "energy_steps": train_steps, "batch": batch,
                      "samples_per_s": train_steps * batch / t_train, "seconds": t_train,
                      "final_loss": float(loss[-50:].mean()) if len(loss) else None,
-                     "snapshot_collection_s": t_collect},
+                     "snapshot_collection_s": t_collect,
+                     "force_steps": force_steps,
+                     "force_configurations": force_steps * len(MLP_TEMPS),
+                     "force_configuration_atoms": 4 * MLP_CONFIG_NX ** 3,
+                     "force_seconds": t_force,
+                     "force_loss_history": [{"step": k, "energy_term": a, "force_term": b} for k, a, b in hist]},
         "workload": "FCC 32^3 = 131072 atoms, rho 0.8442, T 1.44, descriptor 16 Gaussians rc 2.5, "
-                    "MLP 16-64-64-1 SiLU BF16 MFMA / FP32, FP64 integration; training on LJ snapshots "
-                    "(T 0.8..2.0, every 50 steps)",
+                    "MLP 16-64-64-1 SiLU BF16 MFMA / FP32, FP64 integration; trained on LJ energies (10 snapshots "
+                    "of 131072 atoms) and on energies + forces of 2048-atom LJ configurations sampled every 50 "
+                    "steps at T 0.8..2.0",
     }
 
 
@@ -311,6 +348,11 @@ def host_threads():
         return os.cpu_count() or 1
 
 
+def progress(msg):
+    """one line per finished section on stderr (the JSON line stays alone on stdout)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +371,8 @@ def main():
     ap.add_argument("--side-steps", type=int, default=200,
                     help="timed steps of the batched, MLP and DEM lines")
     ap.add_argument("--mlp-train-steps", type=int, default=12000)
+    ap.add_argument("--mlp-force-steps", type=int, default=10000,
+                    help="Adam steps on configurations with forces (5 configurations each)")
     ap.add_argument("--dem-steps", type=int, default=10000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once: a kernel of known HBM traffic that "
@@ -349,6 +393,7 @@ def main():
         del a, b
     copy_gbps = measured_copy_gbps()
     full = run_gpu(args.steps, args.warmup, False, args.time_every, args.time_steps)
+    progress(f"full list: {full['seconds'] * 1e3 / args.steps:.4f} ms/step, force kernel {full['force_ms']:.4f} ms")
     value = NATOMS * args.steps / full["seconds"]
     fused_bytes = force_kernel_bytes(full["nlocal"], full["nghost"], full["nneigh"], True)
     lj_bytes = force_kernel_bytes(full["nlocal"], full["nghost"], full["nneigh"], False)
@@ -393,6 +438,7 @@ def main():
     }
     if not args.no_half:
         half = run_gpu(args.steps, args.warmup, True, args.time_every, args.time_steps)
+        progress(f"half list: {half['seconds'] * 1e3 / args.steps:.4f} ms/step")
         hb = force_kernel_bytes(half["nlocal"], half["nghost"], half["nneigh"], False)
         rec["half_list"] = {
             "value": NATOMS * args.steps / half["seconds"],
@@ -403,14 +449,19 @@ def main():
         }
     if not args.no_batched:
         rec["batched"] = run_batched(args.side_steps, 40, args.replicas)
+        progress(f"batched: {rec['batched']['ms_per_step']:.4f} ms/step")
     if not args.no_mlp:
-        rec["mlp"] = run_mlp(args.side_steps, 20, args.mlp_train_steps)
+        rec["mlp"] = run_mlp(args.side_steps, 20, args.mlp_train_steps, args.mlp_force_steps)
         rec["mlp"]["vs_plain_lj"] = rec["mlp"]["value"] / value
+        progress(f"mlp: force error {rec['mlp']['force_rel_rms_vs_lj']:.4f}")
     if not args.no_dem:
         rec["dem"] = run_dem(args.dem_steps)
+        progress(f"dem: {rec['dem']['ms_per_step']:.4f} ms/step")
         rec["dem_settling"] = run_dem_settling(args.dem_steps)
+        progress(f"dem settling: {rec['dem_settling']['ms_per_step']:.4f} ms/step")
     if not args.no_cpu:
         v1, secs1 = run_cpu_baseline(args.cpu_steps, 1)
+        progress(f"cpu baseline, 1 thread: {secs1:.1f} s")
         nt = host_threads()
         vp, secsp = run_cpu_baseline(args.cpu_steps, nt) if nt > 1 else (v1, secs1)
         sample = f"same workload, iterations ts=0..{args.cpu_steps - 1} (includes {args.cpu_steps // 20} rebuilds)"

@@ -19,11 +19,15 @@
  * (W2[out][in]), b2[64], w3[64], b3.  Scaling (PAIRS_HIP_MLP_NSCALE):
  * g_mean[16], g_istd[16], e_mean, e_std.
  *
- * Training labels are per-atom LJ energies (pairs_hip_energy's per-atom
- * split, 0.5 * sum_j 4 eps ((s/r)^12 - (s/r)^6) over r < cutoff) of snapshots
- * of LJ runs.  A model is used by a context only if its forces on that
- * context's current state are within a relative RMS tolerance of the LJ
- * forces; otherwise the context keeps LJ (pairs_hip_mlp_attach).
+ * Training labels come from LJ runs: per-atom LJ energies (pairs_hip_energy's
+ * per-atom split, 0.5 * sum_j 4 eps ((s/r)^12 - (s/r)^6) over r < cutoff) and
+ * LJ forces.  Two training forms: energy rows in a device-resident set
+ * (pairs_hip_mlp_add_snapshot / _add_rows, pairs_hip_mlp_train), and whole
+ * configurations of LJ contexts with energy and force terms
+ * (pairs_hip_mlp_config_grad, pairs_hip_mlp_apply_grad).  A model is used by
+ * a context only if its forces on that context's current state are within a
+ * relative RMS tolerance of the LJ forces; otherwise the context keeps LJ
+ * (pairs_hip_mlp_attach).
  *
  * Full neighbour lists only (cfg.half == 0).  Return codes as pairs_hip.h.
  */
@@ -71,6 +75,31 @@ int pairs_hip_mlp_fit_scaling(pairs_hip_mlp *model, int rows);
    receives each step's mean squared error of the standardised energy. */
 int pairs_hip_mlp_train(pairs_hip_mlp *model, int rows, int steps, int batch, float lr, unsigned long long seed,
                         float *loss_out);
+/* Append n rows given on the host (descriptors [n][16], per-atom energy
+   labels [n]); returns n, or a negative error. */
+int pairs_hip_mlp_add_rows(pairs_hip_mlp *model, int n, const float *G, const float *label);
+/* The gradient (grad[PAIRS_HIP_MLP_NPARAMS]) and loss of one minibatch of
+   pairs_hip_mlp_train (same row draw for the same seed and step), without
+   updating the model. */
+int pairs_hip_mlp_energy_grad(pairs_hip_mlp *model, int rows, int batch, unsigned long long seed, int step,
+                              float *grad, float *loss);
+/* Gradient over the parameters of the loss of the context's current
+   configuration, labelled by LJ:
+     L = w_energy * mean_i (y_i - (E_lj_i - e_mean) / e_std)^2
+       + w_force * sum_i |F_mlp_i - F_lj_i|^2 / sum_i |F_lj_i|^2
+   over its locals (F_mlp = -dE/dx of the model, so the force term is
+   differentiated through the network's input gradient).  The gradient is
+   added to the model's accumulator (pairs_hip_mlp_apply_grad); grad (may be
+   NULL) receives this configuration's gradient alone, loss (may be NULL) the
+   two unweighted terms {mean squared standardised energy error, relative
+   squared force error}.  The context must not have a model attached; its
+   force array is left holding the LJ forces. */
+int pairs_hip_mlp_config_grad(pairs_hip_ctx *ctx, pairs_hip_mlp *model, double w_energy, double w_force,
+                              float *grad, double *loss);
+/* One Adam step (as pairs_hip_mlp_train's) on the mean of the gradients
+   accumulated by pairs_hip_mlp_config_grad since the last call; clears the
+   accumulator.  No-op when nothing was accumulated. */
+int pairs_hip_mlp_apply_grad(pairs_hip_mlp *model, float lr);
 /* Root mean squared energy error (LJ units) over training rows [begin, end). */
 int pairs_hip_mlp_rmse(pairs_hip_mlp *model, int begin, int end, double *rmse);
 

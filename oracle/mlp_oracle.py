@@ -189,3 +189,106 @@ def lj_atom_energy(x, box_len, rc=RCUT):
     E = np.zeros(len(x))
     np.add.at(E, i, 0.5 * u)
     return E
+
+
+def lj_forces(x, box_len, rc=RCUT):
+    """LJ forces sum_j 48 (r^-14 - r^-7 / 2) (x_i - x_j) over r < rc (the force labels)."""
+    i, _, d, r = pairs_within(x, box_len, rc)
+    s = 48.0 * (r ** -14 - 0.5 * r ** -8)
+    F = np.zeros((len(x), 3))
+    np.add.at(F, i, s[:, None] * d)
+    return F
+
+
+# ---------------------------------------------------------------- training gradients
+
+def silu2(z):
+    """SiLU h(z) = z sigma(z), h'(z) and h''(z) = sigma (1 - sigma) (2 + z (1 - 2 sigma))."""
+    s = 1.0 / (1.0 + np.exp(-z))
+    return z * s, s * (1.0 + z * (1.0 - s)), s * (1.0 - s) * (2.0 + z * (1.0 - 2.0 * s))
+
+
+def pack(W1, b1, W2, b2, w3, b3):
+    return np.concatenate([np.ravel(W1), b1, np.ravel(W2), b2, w3, [b3]])
+
+
+def param_grad(p, sc, G, eps, u):
+    """Gradient over the parameters of sum_a [eps_a y_a + (dy_a/ds) . u_a]
+    with eps [n] and u [n, 16] (directions in standardised-descriptor space)
+    held fixed: forward, tangent along u, then reverse mode through both."""
+    W1, b1, W2, b2, w3, b3 = unpack(p)
+    sc = np.asarray(sc, np.float64)
+    gm, gi = sc[0:16], sc[16:32]
+    s = (np.asarray(G, np.float64) - gm) * gi
+    u = np.asarray(u, np.float64)
+    eps = np.asarray(eps, np.float64)
+    z1 = s @ W1.T + b1
+    h1, d1, e1 = silu2(z1)
+    z2 = h1 @ W2.T + b2
+    h2, d2, e2 = silu2(z2)
+    # tangent: dz1 = W1 u, dh1 = h1' dz1, dz2 = W2 dh1, dh2 = h2' dz2 (dy . u = w3 . dh2)
+    t1 = u @ W1.T
+    th1 = d1 * t1
+    t2 = th1 @ W2.T
+    th2 = d2 * t2
+    # reverse
+    bt2 = w3 * d2                               # d/d(dz2)
+    bz2 = eps[:, None] * w3 * d2 + w3 * e2 * t2  # d/dz2
+    bh1 = bz2 @ W2                              # d/dh1
+    bth1 = bt2 @ W2                             # d/d(dh1)
+    bt1 = bth1 * d1                             # d/d(dz1)
+    bz1 = bh1 * d1 + bth1 * e1 * t1             # d/dz1
+    gW2 = bz2.T @ h1 + bt2.T @ th1
+    gW1 = bz1.T @ s + bt1.T @ u
+    return pack(gW1, bz1.sum(0), gW2, bz2.sum(0), (eps[:, None] * h2 + th2).sum(0), eps.sum())
+
+
+def energy_loss_grad(p, sc, G, label):
+    """L = mean_b (y_b - (label_b - e_mean) / e_std)^2 over rows G and dL/dp."""
+    sc = np.asarray(sc, np.float64)
+    em, es = sc[32], sc[33]
+    E, _ = forward(p, sc, G)
+    err = (E - em) / es - (np.asarray(label, np.float64) - em) / es
+    n = len(err)
+    return (err * err).mean(), param_grad(p, sc, G, 2.0 * err / n, np.zeros((n, NDESC)))
+
+
+def config_loss(p, sc, x, box_len, E_label, F_label, w_energy, w_force):
+    """Loss of one periodic configuration:
+    w_energy * mean_a (y_a - (E*_a - e_mean) / e_std)^2
+      + w_force * sum_i |F_i - F*_i|^2 / sum_i |F*_i|^2."""
+    sc = np.asarray(sc, np.float64)
+    em, es = sc[32], sc[33]
+    F, E = forces(p, sc, x, box_len)
+    t = (E - np.asarray(E_label, np.float64)) / es
+    Fl = np.asarray(F_label, np.float64)
+    le = (t * t).mean()
+    lf = ((F - Fl) ** 2).sum() / (Fl ** 2).sum()
+    return w_energy * le + w_force * lf, le, lf
+
+
+def config_loss_grad(p, sc, x, box_len, E_label, F_label, w_energy, w_force):
+    """config_loss and its gradient over the parameters.  With rho_i = dL/dF_i
+    and w_aj,k = dphi_k(r_aj) (x_a - x_j) / r_aj, dL/dD_a,k =
+    -sum_j w_aj,k . (rho_a - rho_j) (F_i = -sum_j w_ij . (D_i + D_j)); the
+    network then sees the direction u_a = e_std g_istd dL/dD_a in standardised
+    space (D_a = e_std g_istd dy_a/ds)."""
+    sc = np.asarray(sc, np.float64)
+    em, es, gi = sc[32], sc[33], sc[16:32]
+    n = len(x)
+    G = descriptors(x, box_len)
+    F, E = forces(p, sc, x, box_len)
+    Fl = np.asarray(F_label, np.float64)
+    S = (Fl ** 2).sum()
+    t = (E - np.asarray(E_label, np.float64)) / es
+    le = (t * t).mean()
+    lf = ((F - Fl) ** 2).sum() / S
+    rho = 2.0 * w_force * (F - Fl) / S
+    i, j, d, r = pairs_within(x, box_len)
+    _, dphi = basis(r)
+    proj = ((d / r[:, None]) * (rho[i] - rho[j])).sum(-1)
+    gD = np.zeros((n, NDESC))
+    np.add.at(gD, i, -dphi * proj[:, None])
+    u = es * gi * gD
+    eps = 2.0 * w_energy * t / n
+    return w_energy * le + w_force * lf, le, lf, param_grad(p, sc, G, eps, u)

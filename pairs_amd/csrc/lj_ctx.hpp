@@ -56,6 +56,10 @@ struct pairs_hip_ctx {
     double *mlp_f = nullptr;                                      // [3][cap] saved forces
     double *mlp_red = nullptr;                                    // reduction partials
     int mlp_cap = 0;
+    // training on this context's configurations (pairs_hip_mlp_config_grad)
+    float4 *mlp_rho = nullptr;                                    // [cap] dL/dF of every slot
+    float *mlp_U = nullptr;                                       // [cap][16] back-projected directions
+    int mlp_train_cap = 0;
 };
 
 #define HIP_OK(expr)                                                                                   \

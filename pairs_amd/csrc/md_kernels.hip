@@ -813,7 +813,9 @@ __device__ __forceinline__ void lj_wave(const LJArgs &A, int wv) {
         const int ii = live ? i : i0;
         const int fl = live ? flags[ii] : FLAG_FIXED;
         const bool moves = !(fl & FLAG_FIXED);
-        const int n = moves ? numneighs[ii] : 0;
+        // counts past the list rows (a build never writes them) are clamped,
+        // so no list load leaves the allocation
+        const int n = moves ? min(numneighs[ii], A.ncap) : 0;
         const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
         const double xi = X[ii], yi = Y[ii], zi = Z[ii];
         const int ti = NT != 1 ? type[ii] : 0;
@@ -856,7 +858,7 @@ __device__ __forceinline__ void lj_wave(const LJArgs &A, int wv) {
         };
         walk(0, n, nmax, false, t0, t1, t2);
         if (HALF) {
-            const int nr = moves ? A.nrev[ii] : 0;
+            const int nr = moves ? min(A.nrev[ii], A.ncap) : 0;
             const int nrmax = __builtin_amdgcn_readfirstlane(wave_max(nr));
             walk(A.rev0, nr, nrmax, true, r0, r1, r2);
         }
@@ -1757,7 +1759,7 @@ LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, i
     A.Pn = (packed && fused && do_vv1 && next != cur) ? d.P[next] : nullptr;
     A.Xn = d.X[next]; A.Yn = d.Y[next]; A.Zn = d.Z[next];
     A.type = d.type; A.nl = d.nl; A.nl_stride = d.nl_stride; A.numneighs = d.numneighs; A.flags = d.flags;
-    A.nrev = d.nrev; A.rev0 = d.rev0;
+    A.nrev = d.nrev; A.rev0 = d.rev0; A.ncap = d.ncap;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz; A.vx = d.vx; A.vy = d.vy; A.vz = d.vz; A.mass = d.mass;
     A.rc2 = prm.rc2; A.tab = make_tab(prm); A.dt = prm.dt; A.hdt = prm.hdt;
     A.do_vv2 = do_vv2; A.do_vv1 = do_vv1;

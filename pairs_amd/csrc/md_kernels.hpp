@@ -221,6 +221,7 @@ struct LJArgs {
     const int *numneighs, *flags;
     const int *nrev;     // half lists: length of each particle's mirror part
     int rev0;            // half lists: first list row of the mirror part
+    int ncap;            // list rows of each part: counts are clamped to it
     double *fx, *fy, *fz, *vx, *vy, *vz;
     const double *mass;
     double rc2, dt, hdt;

@@ -61,6 +61,9 @@ int pairs_hip_mlp_create(pairs_hip_mlp **out, unsigned long long seed, int datas
     MHIP(hipMalloc((void **) &m->ds_E, cap * sizeof(float)));
     m->ds_cap = (int) cap;
     MHIP(hipMalloc((void **) &m->partial, (size_t) m->nwaves * MLP_NP * sizeof(float)));
+    MHIP(hipMalloc((void **) &m->gacc, MLP_NP * sizeof(float)));
+    MHIP(hipMalloc((void **) &m->gout, MLP_NP * sizeof(float)));
+    MHIP(hipMemset(m->gacc, 0, MLP_NP * sizeof(float)));
     MHIP(hipMalloc((void **) &m->loss, m->nwaves * sizeof(float)));
     // Xavier-uniform weights, zero biases, identity scaling
     std::vector<float> p(MLP_NP, 0.0f), sc(MLP_NS, 0.0f);
@@ -84,7 +87,7 @@ int pairs_hip_mlp_destroy(pairs_hip_mlp *m) {
     if (!m) return PAIRS_HIP_ERR_ARG;
     if (m->stream) hipStreamSynchronize(m->stream);
     void *bufs[] = {m->p, m->sc, m->m, m->v, m->ds_G, m->ds_label, m->ds_E, m->partial, m->loss,
-                    m->loss_hist, m->red, m->io_G, m->io_E, m->io_D};
+                    m->loss_hist, m->red, m->io_G, m->io_E, m->io_D, m->gacc, m->gout};
     for (void *b : bufs)
         if (b) hipFree(b);
     if (m->stream) hipStreamDestroy(m->stream);
@@ -193,6 +196,67 @@ int pairs_hip_mlp_train(pairs_hip_mlp *m, int rows, int steps, int batch, float 
         MHIP(hipMemcpyAsync(loss_out, m->loss_hist, (size_t) steps * sizeof(float), hipMemcpyDeviceToHost,
                             m->stream));
     MHIP(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+int pairs_hip_mlp_add_rows(pairs_hip_mlp *m, int n, const float *G, const float *label) {
+    if (!m || n < 0 || (n > 0 && (!G || !label))) return PAIRS_HIP_ERR_ARG;
+    if (m->ds_rows + n > m->ds_cap) {
+        m->err = "the model's training set is full";
+        return PAIRS_HIP_ERR_STATE;
+    }
+    if (n == 0) return 0;
+    MHIP(hipMemcpyAsync(m->ds_G + (size_t) m->ds_rows * MLP_ND, G, (size_t) n * MLP_ND * sizeof(float),
+                        hipMemcpyHostToDevice, m->stream));
+    MHIP(hipMemcpyAsync(m->ds_label + m->ds_rows, label, (size_t) n * sizeof(float), hipMemcpyHostToDevice,
+                        m->stream));
+    MHIP(hipStreamSynchronize(m->stream));
+    m->ds_rows += n;
+    return n;
+}
+
+int pairs_hip_mlp_energy_grad(pairs_hip_mlp *m, int rows, int batch, unsigned long long seed, int step, float *grad,
+                              float *loss) {
+    if (!m || rows < 1 || rows > m->ds_rows || batch <= 0 || batch % (16 * m->nwaves) != 0 || step < 0 || !grad)
+        return PAIRS_HIP_ERR_ARG;
+    MLPTrain t;
+    t.G = m->ds_G;
+    t.label = m->ds_label;
+    t.rows = rows;
+    t.batch = batch;
+    t.nwaves = m->nwaves;
+    t.partial = m->partial;
+    t.loss = m->loss;
+    launch_mlp_grad(m->stream, m->p, m->sc, t, (uint64_t) seed, step);
+    launch_mlp_accum(m->stream, m->partial, m->nwaves, nullptr, m->gout);
+    MLAUNCH();
+    MHIP(hipMemcpyAsync(grad, m->gout, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    std::vector<float> wl(m->nwaves);
+    MHIP(hipMemcpyAsync(wl.data(), m->loss, wl.size() * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MHIP(hipStreamSynchronize(m->stream));
+    if (loss) {
+        double s = 0.0;
+        for (float x : wl) s += x;
+        *loss = (float) (s / batch);
+    }
+    return 0;
+}
+
+int pairs_hip_mlp_apply_grad(pairs_hip_mlp *m, float lr) {
+    if (!m || !(lr > 0.0f)) return PAIRS_HIP_ERR_ARG;
+    if (m->gcount == 0) return 0;
+    MLPTrain t{};
+    t.partial = m->gacc;
+    t.loss = m->loss;
+    t.nwaves = 1;
+    t.batch = 1;
+    t.gscale = 1.0f / (float) m->gcount;
+    m->adam_t++;
+    launch_mlp_adam(m->stream, m->p, m->m, m->v, t, lr, m->adam_t, nullptr);
+    MHIP(hipMemsetAsync(m->gacc, 0, MLP_NP * sizeof(float), m->stream));
+    MLAUNCH();
+    MHIP(hipStreamSynchronize(m->stream));
+    m->gcount = 0;
     return 0;
 }
 

@@ -235,17 +235,26 @@ struct MLPSegArgs {
     const uint32_t *nl;
     int nl_stride;
     const int *numneighs;
+    int ncap;            // list rows: counts are clamped to it
     int pstride;
     const float *D, *E;
     float *G;
     double *fx, *fy, *fz;
     double *partials;
+    const float4 *rho;   // MODE 2: dL/dF of every slot (a ghost holds its root's)
+    const float *sc;     // MODE 2: standardisation (e_std, g_istd)
+    float *U;            // MODE 2: [cap][16] back-projected directions
 };
 
 // G of the 64 locals of one wave (MODE 0), or their force, energy and
-// virial (MODE 1).  One lane per local, the full list walked four entries per
-// step as in the LJ kernel; pairs at r >= 2.5 and entries past a lane's own
-// count are masked.
+// virial (MODE 1), or (MODE 2) the back-projection of force residuals onto
+// the descriptor gradients: with rho_i = dL/dF_i and F_i = -sum_j w_ij .
+// (D_i + D_j), w_ij,k = dphi_k(r_ij) (x_i - x_j) / r_ij,
+//   dL/dD_a,k = -sum_{j in list(a)} w_aj,k . (rho_a - rho_j),
+// written as U_a,k = e_std g_istd,k dL/dD_a,k (the direction the network's
+// input gradient is differentiated along).  One lane per local, the full list
+// walked four entries per step as in the LJ kernel; pairs at r >= 2.5 and
+// entries past a lane's own count are masked.
 template <int MODE, bool RED>
 __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
     const int wv = xcd_block(blockIdx.x, gridDim.x);
@@ -261,7 +270,7 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
             const int i = i0 + lane;
             const bool live = i < nlocal;
             const int ii = live ? i : i0;
-            const int n = live ? A.numneighs[ii] : 0;
+            const int n = live ? min(A.numneighs[ii], A.ncap) : 0;
             const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
             const double xi = A.X[ii], yi = A.Y[ii], zi = A.Z[ii];
             float acc[MLP_ND], di[MLP_ND];
@@ -275,11 +284,12 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                     di[4 * k4] = d.x; di[4 * k4 + 1] = d.y; di[4 * k4 + 2] = d.z; di[4 * k4 + 3] = d.w;
                 }
             }
+            const float4 rhoi = MODE == 2 ? A.rho[ii] : make_float4(0.f, 0.f, 0.f, 0.f);
             double f0 = 0.0, f1 = 0.0, f2 = 0.0, w = 0.0;
             const float gb = exp2f(-2.0f * MLP_DMU * MLP_DMU * MLP_C);  // a_{k+1} / a_k
             list_walk32<4>(A.nl, 0, ii, A.nl_stride, n, nmax, jmax, [&](int k, const uint32_t *jp) {
                 double xj[4], yj[4], zj[4];
-                float4 dj[4][4];
+                float4 dj[4][4], rj[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const uint32_t off = jp[q] << 5;
@@ -290,6 +300,7 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
 #pragma unroll
                         for (int k4 = 0; k4 < 4; k4++) dj[q][k4] = dp[k4];
                     }
+                    if (MODE == 2) rj[q] = A.rho[jp[q]];
                 }
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -317,6 +328,23 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                         for (int kk = 0; kk < MLP_ND; kk++) {
                             if (kk % 8 == 0) anchor(kk, gk, ak);
                             acc[kk] = fmaf(gk, fc, acc[kk]);
+                            gk *= ak;
+                            ak *= gb;
+                        }
+                    } else if (MODE == 2) {
+                        float sn, cs;
+                        sincosf(MLP_PIRC * r, &sn, &cs);
+                        const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
+                        // (x_a - x_j) / r . (rho_a - rho_j), zero for a masked entry
+                        float proj = (float) dx * (rhoi.x - rj[q].x) + (float) dy * (rhoi.y - rj[q].y);
+                        proj = proj + (float) dz * (rhoi.z - rj[q].z);
+                        proj = in ? -proj / r : 0.0f;
+#pragma unroll
+                        for (int kk = 0; kk < MLP_ND; kk++) {
+                            if (kk % 8 == 0) anchor(kk, gk, ak);
+                            const float t = t0 - (float) kk * MLP_DMU;
+                            const float dphi = gk * fmaf(-t * MLP_IW2, fc, dfc);
+                            acc[kk] = fmaf(dphi, proj, acc[kk]);
                             gk *= ak;
                             ak *= gb;
                         }
@@ -349,6 +377,15 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
 #pragma unroll
                     for (int k4 = 0; k4 < 4; k4++)
                         gp[k4] = make_float4(acc[4 * k4], acc[4 * k4 + 1], acc[4 * k4 + 2], acc[4 * k4 + 3]);
+                } else if (MODE == 2) {
+                    const float es = A.sc[MLP_ESTD];
+                    float4 *up = (float4 *) (A.U + (size_t) i * MLP_ND);
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; k4++) {
+                        const float *gi = A.sc + MLP_GISTD + 4 * k4;
+                        up[k4] = make_float4(es * gi[0] * acc[4 * k4], es * gi[1] * acc[4 * k4 + 1],
+                                             es * gi[2] * acc[4 * k4 + 2], es * gi[3] * acc[4 * k4 + 3]);
+                    }
                 } else {
                     A.fx[i] = f0; A.fy[i] = f1; A.fz[i] = f2;
                     if (RED) { pe += (double) A.E[i]; vir += 0.5 * w; }
@@ -508,15 +545,268 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ p, c
     if (lane == 0) { out[MLP_B3] = gb3; t.loss[wave] = lsum; }
 }
 
+// SiLU with its first and second derivatives:
+// h = z s, h' = s (1 + z (1 - s)), h'' = s (1 - s) (2 + z (1 - 2 s)), s = sigmoid(z)
+__device__ __forceinline__ void silu3(float z, float &h, float &dh, float &ddh) {
+    const float sg = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+    h = z * sg;
+    dh = sg * fmaf(z, 1.0f - sg, 1.0f);
+    ddh = sg * (1.0f - sg) * fmaf(z, 1.0f - 2.0f * sg, 2.0f);
+}
+
+// Gradient over the parameters of one configuration's loss (rows [0, n), one
+// per local): sum_a [eps_a y_a + (dy_a/ds) . u_a], eps_a = eps_scale *
+// (y_a - (E*_a - e_mean) / e_std) and u_a = U row a held fixed.  Per
+// 16-particle tile: the forward pass (h1, h2 and both SiLU derivatives), the
+// tangent along u (dz1 = W1 u, dh1 = h1' dz1, dz2 = W2 dh1), then reverse mode
+// through both:
+//   bar dz2 = w3 h2',  bar z2 = w3 (eps h2' + h2'' dz2),
+//   bar dz1 = (W2^T bar dz2) h1',  bar z1 = (W2^T bar z2) h1' + (W2^T bar dz2) h1'' dz1,
+//   dW2 = bar z2 h1^T + bar dz2 dh1^T,  dW1 = bar z1 s^T + bar dz1 u^T,
+//   db1 = bar z1, db2 = bar z2, dw3 = eps h2 + h2' dz2, db3 = eps.
+// Layer products and weight gradients are MFMAs in the layout of k_mlp_grad
+// (BF16 operands, FP32 accumulation; s and u enter the first layer as hi + lo
+// pairs).  Every wave runs the same number of tile iterations (tiles past the
+// end have eps = u = 0 and add exact zeros), so the block barriers are uniform.
+__global__ __launch_bounds__(256) void k_mlp_cgrad(const float *__restrict__ p, const float *__restrict__ sc,
+                                                   MLPCfgGrad t) {
+    __shared__ short lds[4][6 * MLP_H * 16 + 2 * MLP_ND * 16];
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15, wl = threadIdx.x >> 6;
+    const int wave = blockIdx.x * 4 + wl, nw = gridDim.x * 4;
+    short *sZ2 = lds[wl], *sH1 = sZ2 + MLP_H * 16, *sT2 = sH1 + MLP_H * 16, *sTH1 = sT2 + MLP_H * 16;
+    short *sZ1 = sTH1 + MLP_H * 16, *sT1 = sZ1 + MLP_H * 16, *sS = sT1 + MLP_H * 16, *sU = sS + MLP_ND * 16;
+    MLPRegs R;
+    mlp_load<true>(R, p, sc, lane);
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4 gW2[4][4], gW1[4];
+    float gb1[4][4], gb2[4][4], gw3[4][4], gb3 = 0.f, lsum = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        gW1[a] = zero;
+#pragma unroll
+        for (int b = 0; b < 4; b++) { gW2[a][b] = zero; gb1[a][b] = 0.f; gb2[a][b] = 0.f; gw3[a][b] = 0.f; }
+    }
+    const float inv_es = 1.0f / R.es;
+    const int ntiles = (t.n + 15) >> 4;
+    const int iters = (ntiles + nw - 1) / nw;
+    for (int it = 0; it < iters; it++) {
+        const int row = (wave + it * nw) * 16 + col;
+        const bool ok = row < t.n;
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f), uu = g;
+        double lab = 0.0;
+        if (ok) {
+            g = *(const float4 *) (t.G + (size_t) row * MLP_ND + 4 * q);
+            uu = *(const float4 *) (t.U + (size_t) row * MLP_ND + 4 * q);
+            lab = t.label[row];
+        }
+        bf16x4 b0, b0l, u0, u0l;
+        split4(mlp_input(R, g), b0, b0l);
+        split4(uu.x, uu.y, uu.z, uu.w, u0, u0l);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            sS[(4 * q + v) * 16 + col] = b0[v];
+            sU[(4 * q + v) * 16 + col] = u0[v];
+        }
+        // forward, keeping h, h' and h'' of both layers
+        f32x4 h1[4], d1[4], e1[4], h2[4], d2[4], e2[4];
+        bf16x4 bh1[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f32x4 z = mfma16(R.a1l[c], b0, zero);
+            z = mfma16(R.a1[c], b0l, z);
+            z = mfma16(R.a1[c], b0, z);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                float h, dh, ddh;
+                silu3(z[v] + R.b1[c][v], h, dh, ddh);
+                h1[c][v] = h; d1[c][v] = dh; e1[c][v] = ddh;
+            }
+            bh1[c] = pack4(h1[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f32x4 z = zero;
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) z = mfma16(R.a2[c][kc], bh1[kc], z);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                float h, dh, ddh;
+                silu3(z[v] + R.b2[c][v], h, dh, ddh);
+                h2[c][v] = h; d2[c][v] = dh; e2[c][v] = ddh;
+            }
+        }
+        float y = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) y = fmaf(R.w3[c][v], h2[c][v], y);
+        y += __shfl_xor(y, 16, 64);
+        y += __shfl_xor(y, 32, 64);
+        y += R.b3;
+        const float err = ok ? y - ((float) lab - R.em) * inv_es : 0.0f;
+        const float e = t.eps_scale * err;
+        if (q == 0) { lsum = fmaf(err, err, lsum); gb3 += e; }
+        // tangent along u
+        f32x4 t1[4];
+        bf16x4 bth1[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f32x4 z = mfma16(R.a1l[c], u0, zero);
+            z = mfma16(R.a1[c], u0l, z);
+            t1[c] = mfma16(R.a1[c], u0, z);
+            bth1[c] = pack4(d1[c][0] * t1[c][0], d1[c][1] * t1[c][1], d1[c][2] * t1[c][2], d1[c][3] * t1[c][3]);
+        }
+        bf16x4 bz2[4], bt2[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f32x4 t2 = zero;
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) t2 = mfma16(R.a2[c][kc], bth1[kc], t2);
+            f32x4 z2b, t2b;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float w = R.w3[c][v];
+                gw3[c][v] = fmaf(e, h2[c][v], fmaf(d2[c][v], t2[v], gw3[c][v]));
+                t2b[v] = w * d2[c][v];
+                z2b[v] = w * fmaf(e, d2[c][v], e2[c][v] * t2[v]);
+                gb2[c][v] += z2b[v];
+            }
+            bz2[c] = pack4(z2b);
+            bt2[c] = pack4(t2b);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int f = (16 * c + 4 * q + v) * 16 + col;
+                sZ2[f] = bz2[c][v];
+                sT2[f] = bt2[c][v];
+                sH1[f] = bh1[c][v];
+                sTH1[f] = bth1[c][v];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            f32x4 a = zero, at = zero;
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) {
+                a = mfma16(R.a2t[c][kc], bz2[kc], a);
+                at = mfma16(R.a2t[c][kc], bt2[kc], at);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const float z1b = fmaf(at[v] * e1[c][v], t1[c][v], a[v] * d1[c][v]);
+                gb1[c][v] += z1b;
+                const int f = (16 * c + 4 * q + v) * 16 + col;
+                sZ1[f] = f2bf(z1b);
+                sT1[f] = f2bf(at[v] * d1[c][v]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c2 = 0; c2 < 4; c2++) {
+            const bf16x4 az = *(const bf16x4 *) (sZ2 + (16 * c2 + col) * 16 + 4 * q);
+            const bf16x4 at = *(const bf16x4 *) (sT2 + (16 * c2 + col) * 16 + 4 * q);
+#pragma unroll
+            for (int c1 = 0; c1 < 4; c1++) {
+                const bf16x4 bh = *(const bf16x4 *) (sH1 + (16 * c1 + col) * 16 + 4 * q);
+                const bf16x4 bt = *(const bf16x4 *) (sTH1 + (16 * c1 + col) * 16 + 4 * q);
+                gW2[c2][c1] = mfma16(az, bh, gW2[c2][c1]);
+                gW2[c2][c1] = mfma16(at, bt, gW2[c2][c1]);
+            }
+        }
+        const bf16x4 bs = *(const bf16x4 *) (sS + col * 16 + 4 * q);
+        const bf16x4 bu = *(const bf16x4 *) (sU + col * 16 + 4 * q);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const bf16x4 az = *(const bf16x4 *) (sZ1 + (16 * c + col) * 16 + 4 * q);
+            const bf16x4 at = *(const bf16x4 *) (sT1 + (16 * c + col) * 16 + 4 * q);
+            gW1[c] = mfma16(az, bs, gW1[c]);
+            gW1[c] = mfma16(at, bu, gW1[c]);
+        }
+        __syncthreads();
+    }
+    float *out = t.partial + (size_t) wave * MLP_NP;
+#pragma unroll
+    for (int c2 = 0; c2 < 4; c2++)
+#pragma unroll
+        for (int c1 = 0; c1 < 4; c1++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) out[MLP_W2 + (16 * c2 + 4 * q + v) * MLP_H + 16 * c1 + col] = gW2[c2][c1][v];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) out[MLP_W1 + (16 * c + 4 * q + v) * MLP_ND + col] = gW1[c][v];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            float a = gb1[c][v], b = gb2[c][v], w = gw3[c][v];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                a += __shfl_xor(a, o, 64); b += __shfl_xor(b, o, 64); w += __shfl_xor(w, o, 64);
+            }
+            if (col == 0) {
+                const int hh = 16 * c + 4 * q + v;
+                out[MLP_B1 + hh] = a; out[MLP_B2 + hh] = b; out[MLP_W3 + hh] = w;
+            }
+        }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) { gb3 += __shfl_xor(gb3, o, 64); lsum += __shfl_xor(lsum, o, 64); }
+    if (lane == 0) { out[MLP_B3] = gb3; t.loss[wave] = lsum; }
+}
+
+// rho_i = 2 w_force (F_i - F*_i) / sum_j |F*_j|^2 for the locals; the sum is
+// taken from k_force_diff's per-block partials (rows 2b + 1)
+__global__ __launch_bounds__(256) void k_mlp_rho(const int *__restrict__ counters, const double *__restrict__ fx,
+                                                 const double *__restrict__ fy, const double *__restrict__ fz,
+                                                 const double *__restrict__ lx, const double *__restrict__ ly,
+                                                 const double *__restrict__ lz, const double *__restrict__ partials,
+                                                 int nb, double w_force, float4 *__restrict__ rho) {
+    __shared__ double s_scale;
+    if (threadIdx.x == 0) {
+        double den = 0.0;
+        for (int b = 0; b < nb; b++) den += partials[2 * b + 1];
+        s_scale = den > 0.0 ? 2.0 * w_force / den : 0.0;
+    }
+    __syncthreads();
+    const double sc = s_scale;
+    const int n = counters[C_NLOCAL];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        rho[i] = make_float4((float) (sc * (fx[i] - lx[i])), (float) (sc * (fy[i] - ly[i])),
+                             (float) (sc * (fz[i] - lz[i])), 0.0f);
+}
+
+// rho of every ghost = that of the local at the end of its source chain
+__global__ __launch_bounds__(256) void k_mlp_ghost_rho(const int *__restrict__ counters,
+                                                       const int *__restrict__ sync_src, float4 *__restrict__ rho) {
+    const int nlocal = counters[C_NLOCAL], ng = counters[C_NGHOST];
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < ng; q += gridDim.x * 256) {
+        int root = sync_src[q];
+        for (int hop = 0; root >= nlocal && hop < 64; hop++) root = sync_src[root - nlocal];
+        rho[nlocal + q] = root < nlocal ? rho[root] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// acc += sum over waves of the gradient partials (in wave order); with out,
+// the sum itself is also written there
+__global__ __launch_bounds__(256) void k_mlp_accum(const float *__restrict__ partial, int nwaves,
+                                                   float *__restrict__ acc, float *__restrict__ out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= MLP_NP) return;
+    float s = 0.f;
+    for (int w = 0; w < nwaves; w++) s += partial[(size_t) w * MLP_NP + k];
+    if (acc) acc[k] += s;
+    if (out) out[k] = s;
+}
+
 // Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the summed wave partials
 __global__ __launch_bounds__(256) void k_mlp_adam(float *__restrict__ p, float *__restrict__ m, float *__restrict__ v,
                                                   const float *__restrict__ partial, const float *__restrict__ loss,
                                                   int nwaves, int batch, float lr, float bc1, float bc2,
-                                                  float *__restrict__ loss_out) {
+                                                  float gscale, float *__restrict__ loss_out) {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k < MLP_NP) {
         float gsum = 0.f;
         for (int w = 0; w < nwaves; w++) gsum += partial[(size_t) w * MLP_NP + k];
+        gsum *= gscale;
         const float mm = 0.9f * m[k] + 0.1f * gsum;
         const float vv = 0.999f * v[k] + 0.001f * gsum * gsum;
         m[k] = mm;
@@ -607,6 +897,7 @@ static MLPSegArgs seg_args(const DevState &d, int cur) {
     A.nl = d.nl;
     A.nl_stride = d.nl_stride;
     A.numneighs = d.numneighs;
+    A.ncap = d.ncap;
     A.pstride = d.pstride;
     return A;
 }
@@ -654,7 +945,38 @@ void launch_mlp_adam(hipStream_t s, float *p, float *m, float *v, const MLPTrain
                      float *loss_out) {
     const float bc1 = 1.0f - powf(0.9f, (float) step), bc2 = 1.0f - powf(0.999f, (float) step);
     hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NP + 255) / 256), dim3(256), 0, s, p, m, v, t.partial, t.loss,
-                       t.nwaves, t.batch, lr, bc1, bc2, loss_out);
+                       t.nwaves, t.batch, lr, bc1, bc2, t.gscale, loss_out);
+}
+
+void launch_mlp_config_grad(hipStream_t s, const DevState &d, const Geom &g, const MLPBind &b, int cur,
+                            const double *lx, const double *ly, const double *lz, const double *eatom,
+                            double w_energy, double w_force, float4 *rho, float *U, double *red, int nb,
+                            int ghost_grid, const MLPCfgGrad &t) {
+    const int n = d.nlocal;
+    if (n <= 0) return;
+    // the network's forces (descriptors, E, dE/dG of locals and ghosts) into d.f*
+    launch_mlp_forces(s, d, g, b, cur, 0, ghost_grid);
+    launch_force_diff(s, d.fx, d.fy, d.fz, lx, ly, lz, n, red, nb);
+    hipLaunchKernelGGL(k_mlp_rho, dim3(std::min((n + 255) / 256, 1024)), dim3(256), 0, s, d.counters, d.fx, d.fy,
+                       d.fz, lx, ly, lz, red, nb, w_force, rho);
+    hipLaunchKernelGGL(k_mlp_ghost_rho, dim3(std::max(ghost_grid, 1)), dim3(256), 0, s, d.counters, d.sync_src,
+                       rho);
+    MLPSegArgs A = seg_args(d, cur);
+    A.rho = rho;
+    A.sc = b.sc;
+    A.U = U;
+    hipLaunchKernelGGL((k_mlp_wave<2, false>), dim3(force_waves(d)), dim3(64), 0, s, A);
+    MLPCfgGrad tt = t;
+    tt.G = b.G;
+    tt.label = eatom;
+    tt.U = U;
+    tt.n = n;
+    tt.eps_scale = (float) (2.0 * w_energy / n);
+    hipLaunchKernelGGL(k_mlp_cgrad, dim3(tt.nwaves / 4), dim3(256), 0, s, b.p, b.sc, tt);
+}
+
+void launch_mlp_accum(hipStream_t s, const float *partial, int nwaves, float *acc, float *out) {
+    hipLaunchKernelGGL(k_mlp_accum, dim3((MLP_NP + 255) / 256), dim3(256), 0, s, partial, nwaves, acc, out);
 }
 
 void launch_mlp_moments(hipStream_t s, const float *G, const float *label, int n, double *partials, int nblocks) {

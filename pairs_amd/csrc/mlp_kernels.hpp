@@ -65,6 +65,20 @@ struct MLPTrain {
     int nwaves;          // waves of the gradient launch
     float *partial;      // [nwaves][MLP_NP]
     float *loss;         // [nwaves] partial squared errors
+    float gscale = 1.0f; // Adam takes gscale * (sum of the partials)
+};
+
+// one configuration's loss gradient (k_mlp_cgrad): rows [0, n) of G, their
+// per-atom energy labels and back-projected force directions U
+struct MLPCfgGrad {
+    const float *G;
+    const double *label;
+    const float *U;
+    int n;
+    float eps_scale;     // 2 w_energy / n
+    int nwaves;          // waves of the launch (a multiple of 4)
+    float *partial;      // [nwaves][MLP_NP]
+    float *loss;         // [nwaves] partial sums of the squared standardised energy error
 };
 // gradient of the mean squared error of the standardised energy over one
 // minibatch (rows chosen by a counter hash of seed and step)
@@ -72,6 +86,17 @@ void launch_mlp_grad(hipStream_t s, const float *p, const float *sc, const MLPTr
 // sum the wave partials and take one Adam step (t = 1, 2, ...)
 void launch_mlp_adam(hipStream_t s, float *p, float *m, float *v, const MLPTrain &t, float lr, int step,
                      float *loss_out);
+// Gradient of one configuration's loss on buffer `cur` (lists current):
+//   w_energy mean_a (y_a - (E*_a - e_mean) / e_std)^2 + w_force sum |F - F*|^2 / sum |F*|^2
+// with labels F* = (lx, ly, lz) and E* = eatom (physical slots).  Leaves the
+// network's forces in d.f*, the residual/force-norm partials in red ([nb][2])
+// and the per-wave gradient and loss partials in t.partial / t.loss.
+void launch_mlp_config_grad(hipStream_t s, const DevState &d, const Geom &g, const MLPBind &b, int cur,
+                            const double *lx, const double *ly, const double *lz, const double *eatom,
+                            double w_energy, double w_force, float4 *rho, float *U, double *red, int nb,
+                            int ghost_grid, const MLPCfgGrad &t);
+// acc += sum of nwaves partials (acc may be null); out (may be null) = that sum
+void launch_mlp_accum(hipStream_t s, const float *partial, int nwaves, float *acc, float *out);
 // column sums and sums of squares of rows [0, n): G (16 columns) and labels
 void launch_mlp_moments(hipStream_t s, const float *G, const float *label, int n, double *partials, int nblocks);
 // sum over rows [0, n) of (E - label)^2 into per-block partials

@@ -20,4 +20,7 @@ struct pairs_hip_mlp {
     int red_cap = 0;
     float *io_G = nullptr, *io_E = nullptr, *io_D = nullptr;  // pairs_hip_mlp_eval staging
     int io_cap = 0;
+    float *gacc = nullptr;               // accumulated configuration gradients
+    float *gout = nullptr;               // one gradient for the host
+    int gcount = 0;                      // configurations in gacc
 };

@@ -314,6 +314,7 @@ int pairs_hip_destroy(pairs_hip_ctx *ctx) {
     dfree(d.lcount); dfree(d.gcount); dfree(d.lstart); dfree(d.gstart); dfree(d.counters); dfree(d.thermo);
     dfree(ctx->d_eatom); dfree(ctx->d_expand);
     dfree(ctx->mlp_G); dfree(ctx->mlp_D); dfree(ctx->mlp_E); dfree(ctx->mlp_f); dfree(ctx->mlp_red);
+    dfree(ctx->mlp_rho); dfree(ctx->mlp_U);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -1271,6 +1272,71 @@ int pairs_hip_mlp_force_error(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double *rel_
     return 0;
 }
 
+int pairs_hip_mlp_config_grad(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double w_energy, double w_force, float *grad,
+                              double *loss) {
+    if (!ctx || !m || !(w_energy >= 0.0) || !(w_force >= 0.0)) return PAIRS_HIP_ERR_ARG;
+    if (ctx->mlp_on) {
+        ctx->err = "pairs_hip_mlp_config_grad labels the configuration with LJ: detach the model first";
+        return PAIRS_HIP_ERR_STATE;
+    }
+    if (int r = mlp_prepare(ctx, m)) return r;
+    if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
+    DevState &d = ctx->d;
+    const int n = d.nlocal;
+    if (n <= 0) {
+        ctx->err = "no particles";
+        return PAIRS_HIP_ERR_STATE;
+    }
+    if (ctx->mlp_train_cap < d.cap) {
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        dfree(ctx->mlp_rho); dfree(ctx->mlp_U);
+        ctx->mlp_rho = nullptr;
+        ctx->mlp_U = nullptr;
+        if (dalloc(ctx, &ctx->mlp_rho, d.cap) || dalloc(ctx, &ctx->mlp_U, (size_t) d.cap * MLP_ND))
+            return PAIRS_HIP_ERR_HIP;
+        ctx->mlp_train_cap = d.cap;
+    }
+    hipStream_t s = ctx->stream;
+    const size_t cap = (size_t) ctx->mlp_cap;
+    double *lx = ctx->mlp_f, *ly = ctx->mlp_f + cap, *lz = ctx->mlp_f + 2 * cap;
+    // labels: LJ forces and per-atom LJ energies of this configuration
+    launch_lj_full(s, d, ctx->geom, ctx->prm, d.cur, d.cur, 0, 0, 0, 0, !ctx->xy_valid[d.cur]);
+    ctx->xy_valid[d.cur] = true;
+    HIP_OK(hipMemcpyAsync(lx, d.fx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(ly, d.fy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(lz, d.fz, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    launch_energy(s, d, ctx->geom, ctx->prm, 0, ctx->d_eatom);
+    const int nb = std::min((n + 255) / 256, 1024);
+    MLPCfgGrad t{};
+    t.nwaves = m->nwaves;
+    t.partial = m->partial;
+    t.loss = m->loss;
+    launch_mlp_config_grad(s, d, ctx->geom, mlp_bind(ctx, m), d.cur, lx, ly, lz, ctx->d_eatom, w_energy, w_force,
+                           ctx->mlp_rho, ctx->mlp_U, ctx->mlp_red, nb, ctx->ghost_grid, t);
+    launch_mlp_accum(s, m->partial, m->nwaves, m->gacc, grad ? m->gout : nullptr);
+    // leave the LJ forces in the force array, as after pairs_hip_lj
+    HIP_OK(hipMemcpyAsync(d.fx, lx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(d.fy, ly, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipMemcpyAsync(d.fz, lz, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    CHECK_LAUNCH();
+    m->gcount++;
+    if (grad) HIP_OK(hipMemcpyAsync(grad, m->gout, MLP_NP * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (loss) {
+        std::vector<double> part(2 * nb);
+        std::vector<float> wl(m->nwaves);
+        HIP_OK(hipMemcpyAsync(part.data(), ctx->mlp_red, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(wl.data(), m->loss, wl.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        double num = 0.0, den = 0.0, le = 0.0;
+        for (int b = 0; b < nb; b++) { num += part[2 * b]; den += part[2 * b + 1]; }
+        for (float x : wl) le += x;
+        loss[0] = le / n;
+        loss[1] = den > 0.0 ? num / den : num;
+    }
+    HIP_OK(hipStreamSynchronize(s));
+    return 0;
+}
+
 int pairs_hip_mlp_attach(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double max_rel_rms, double *rel_rms) {
     if (!ctx || !m) return PAIRS_HIP_ERR_ARG;
     if (int r = mlp_prepare(ctx, m)) return r;
@@ -1316,8 +1382,13 @@ struct pairs_hip_batch {
     std::vector<int> base_cur;
     LJArgs *h_desc = nullptr;          // pinned staging, 3 * R
     LJArgs *d_desc[3] = {nullptr, nullptr, nullptr};  // phase 0, phase 1, one-off
+    // block tables of the force kernel (one block per wave of 64 locals):
+    // block k belongs to replica d_rep[k] and is its wave k - d_base[replica]
     int *d_rep = nullptr, *d_base = nullptr;
     int nblocks = 0, blocks_cap = 0;
+    // the same for the neighbour build (one block per segment)
+    int *d_srep = nullptr, *d_sbase = nullptr;
+    int nsblocks = 0, sblocks_cap = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_blocks[2] = {0, 0};
     std::vector<double> thermo;        // last thermo row per replica
@@ -1331,7 +1402,7 @@ struct pairs_hip_batch {
     BuildArgs *h_bdesc = nullptr, *d_bdesc = nullptr;  // every replica's neighbour-build arguments
     const int **h_cptr = nullptr, **d_cptr = nullptr;  // every replica's device counters
     int *d_call = nullptr;             // [R][C_COUNT] gathered counters
-    bool tables_valid = false;         // d_rep / d_base match the replicas' segments
+    bool tables_valid = false;         // the block tables match the replicas' waves and segments
 };
 
 static void batch_drop_graphs(pairs_hip_batch *b) {
@@ -1351,27 +1422,41 @@ static void batch_drop_graphs(pairs_hip_batch *b) {
     } while (0)
 
 // block -> replica tables after (re)segmentation
-static int batch_tables(pairs_hip_batch *b) {
+// one block table: replica r owns blocks [base[r], base[r + 1]), count(r) of them
+template <typename Count>
+static int batch_table(pairs_hip_batch *b, Count count, int *&d_rep, int *d_base, int &cap, int &nblocks) {
     const int R = (int) b->rep.size();
     std::vector<int> base(R + 1);
     int tot = 0;
     for (int r = 0; r < R; r++) {
         base[r] = tot;
-        tot += force_waves(b->rep[r]->d);
+        tot += count(b->rep[r]);
     }
     base[R] = tot;
-    if (tot > b->blocks_cap) {
-        if (b->d_rep) hipFree(b->d_rep);
-        b->blocks_cap = tot + tot / 4 + 64;
-        BATCH_OK(hipMalloc((void **) &b->d_rep, b->blocks_cap * sizeof(int)));
+    if (tot > cap) {
+        if (d_rep) hipFree(d_rep);
+        d_rep = nullptr;
+        cap = tot + tot / 4 + 64;
+        BATCH_OK(hipMalloc((void **) &d_rep, cap * sizeof(int)));
     }
     std::vector<int> rep(std::max(tot, 1));
     for (int r = 0; r < R; r++)
         for (int k = base[r]; k < base[r + 1]; k++) rep[k] = r;
-    BATCH_OK(hipMemcpyAsync(b->d_rep, rep.data(), tot * sizeof(int), hipMemcpyHostToDevice, b->stream));
-    BATCH_OK(hipMemcpyAsync(b->d_base, base.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BATCH_OK(hipMemcpyAsync(d_rep, rep.data(), tot * sizeof(int), hipMemcpyHostToDevice, b->stream));
+    BATCH_OK(hipMemcpyAsync(d_base, base.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice, b->stream));
     BATCH_OK(hipStreamSynchronize(b->stream));
-    b->nblocks = tot;
+    nblocks = tot;
+    return 0;
+}
+
+// the force kernel's wave table and the neighbour build's segment table
+static int batch_tables(pairs_hip_batch *b) {
+    if (int e = batch_table(b, [](pairs_hip_ctx *c) { return force_waves(c->d); }, b->d_rep, b->d_base,
+                            b->blocks_cap, b->nblocks))
+        return e;
+    if (int e = batch_table(b, [](pairs_hip_ctx *c) { return c->geom.nsegs; }, b->d_srep, b->d_sbase,
+                            b->sblocks_cap, b->nsblocks))
+        return e;
     b->tables_valid = true;
     return 0;
 }
@@ -1440,7 +1525,7 @@ static int batch_rebuild(pairs_hip_batch *b) {
         lds = std::max(lds, build_lds_bytes(c->d, c->geom, 0));
     }
     BATCH_OK(hipMemcpyAsync(b->d_bdesc, b->h_bdesc, R * sizeof(BuildArgs), hipMemcpyHostToDevice, s));
-    launch_build_batch(s, b->d_bdesc, b->d_rep, b->d_base, b->nblocks, lds, 0);
+    launch_build_batch(s, b->d_bdesc, b->d_srep, b->d_sbase, b->nsblocks, lds, 0);
     BATCH_OK(hipGetLastError());
     // every replica's counters gathered on the device, one copy to the host
     for (int r = 0; r < R; r++) b->h_cptr[r] = b->rep[r]->d.counters;
@@ -1538,7 +1623,8 @@ int pairs_hip_batch_create(pairs_hip_batch **out, int nreplicas, const pairs_hip
         hipMalloc((void **) &b->d_desc[0], R * sizeof(LJArgs)) != hipSuccess ||
         hipMalloc((void **) &b->d_desc[1], R * sizeof(LJArgs)) != hipSuccess ||
         hipMalloc((void **) &b->d_desc[2], R * sizeof(LJArgs)) != hipSuccess ||
-        hipMalloc((void **) &b->d_base, (R + 1) * sizeof(int)) != hipSuccess) {
+        hipMalloc((void **) &b->d_base, (R + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **) &b->d_sbase, (R + 1) * sizeof(int)) != hipSuccess) {
         pairs_hip_batch_destroy(b);
         return PAIRS_HIP_ERR_HIP;
     }
@@ -1569,6 +1655,8 @@ int pairs_hip_batch_destroy(pairs_hip_batch *b) {
     for (int p = 0; p < 3; p++) if (b->d_desc[p]) hipFree(b->d_desc[p]);
     if (b->d_rep) hipFree(b->d_rep);
     if (b->d_base) hipFree(b->d_base);
+    if (b->d_srep) hipFree(b->d_srep);
+    if (b->d_sbase) hipFree(b->d_sbase);
     if (b->h_cnt) hipHostFree(b->h_cnt);
     if (b->h_rdesc) hipHostFree(b->h_rdesc);
     if (b->d_rdesc) hipFree(b->d_rdesc);

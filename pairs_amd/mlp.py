@@ -9,9 +9,14 @@ system, and is attached only if its forces on the system's current state are
 within a relative RMS tolerance (default 2 %) of the LJ forces -- otherwise
 the system keeps LJ.
 
-Training data are snapshots of LJ runs: each snapshot appends every atom's
-descriptor and its per-atom LJ energy to the model's device-resident training
-set; ``MLPModel.train`` runs Adam on the GPU.
+Training data come from LJ runs, in two forms.  Energy rows: each snapshot
+appends every atom's descriptor and its per-atom LJ energy to the model's
+device-resident training set, and ``MLPModel.train`` runs Adam on minibatches
+of rows.  Configurations with forces: ``train_on_configurations`` advances
+plain-LJ systems and, every ``every`` steps, takes the gradient of each
+system's configuration loss (standardised energy error plus relative squared
+force error against LJ, the force term differentiated through the network's
+input gradient) and one Adam step on their mean -- all on the GPU.
 """
 import ctypes
 
@@ -37,6 +42,10 @@ nat.register({
     "pairs_hip_mlp_fit_scaling": (_I, [_P, _I]),
     "pairs_hip_mlp_train": (_I, [_P, _I, _I, _I, _F, _U64, _FP]),
     "pairs_hip_mlp_rmse": (_I, [_P, _I, _I, _DP]),
+    "pairs_hip_mlp_add_rows": (_I, [_P, _I, _FP, _FP]),
+    "pairs_hip_mlp_energy_grad": (_I, [_P, _I, _I, _U64, _I, _FP, _FP]),
+    "pairs_hip_mlp_config_grad": (_I, [_P, _P, _D, _D, _FP, _DP]),
+    "pairs_hip_mlp_apply_grad": (_I, [_P, _F]),
     "pairs_hip_mlp_forces": (_I, [_P, _P, _DP, _DP]),
     "pairs_hip_mlp_force_error": (_I, [_P, _P, _DP]),
     "pairs_hip_mlp_attach": (_I, [_P, _P, _D, _DP]),
@@ -117,6 +126,38 @@ class MLPModel:
         self._check(self.L.pairs_hip_mlp_train(self.h, rows, steps, batch, lr, seed, _fp(loss)), "train")
         return loss[:steps]
 
+    def add_rows(self, G, labels):
+        """Append descriptor rows [n, 16] with per-atom energy labels [n]."""
+        G = np.ascontiguousarray(G, np.float32)
+        lab = np.ascontiguousarray(labels, np.float32)
+        if G.ndim != 2 or G.shape[1] != NDESC or lab.shape != (G.shape[0],):
+            raise ValueError("rows are [n, 16] descriptors and [n] labels")
+        return self._check(self.L.pairs_hip_mlp_add_rows(self.h, G.shape[0], _fp(G), _fp(lab)), "add_rows")
+
+    def energy_grad(self, step, *, batch=4096, seed=7, rows=None):
+        """Gradient [NPARAMS] and loss of train()'s minibatch `step` (no update)."""
+        rows = self.dataset_size if rows is None else rows
+        g = np.zeros(NPARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.L.pairs_hip_mlp_energy_grad(self.h, rows, batch, seed, step, _fp(g), _fp(loss)),
+                    "energy_grad")
+        return g, float(loss[0])
+
+    def config_grad(self, sim, w_energy=1.0, w_force=1.0, *, want_grad=True):
+        """Gradient of the loss of the system's current configuration (added
+        to the accumulator of apply_grad); returns (gradient or None,
+        (energy term, force term))."""
+        g = np.zeros(NPARAMS, np.float32) if want_grad else None
+        loss = np.zeros(2)
+        nat.check(sim.h, self.L.pairs_hip_mlp_config_grad(sim.h, self.h, w_energy, w_force,
+                                                          None if g is None else _fp(g),
+                                                          loss.ctypes.data_as(_DP)), "pairs_hip_mlp_config_grad")
+        return g, (float(loss[0]), float(loss[1]))
+
+    def apply_grad(self, lr):
+        """One Adam step on the mean of the accumulated configuration gradients."""
+        self._check(self.L.pairs_hip_mlp_apply_grad(self.h, lr), "apply_grad")
+
     def rmse(self, begin=0, end=None):
         end = self.dataset_size if end is None else end
         out = ctypes.c_double()
@@ -181,3 +222,34 @@ def collect_lj_snapshots(model, make_system, temps, *, equilibrate=200, every=50
             added += model.add_snapshot(sim)
         sim.close()
     return added
+
+
+def train_on_configurations(model, sims, steps, *, every=50, ts0=0, lr=2e-3, lr_end_frac=0.05, w_energy=1.0,
+                            w_force=1.0, total_steps=None, record_every=0):
+    """Adam on configurations of plain-LJ runs: before each Adam step every
+    system advances `every` timesteps of its own LJ run, then the gradient of
+    its configuration loss is accumulated; the step uses their mean.  The
+    learning rate decays linearly from lr to lr * lr_end_frac.  Returns the
+    next timestep of the systems and, every `record_every` steps (0: never),
+    (step, mean energy term, mean force term) over the systems."""
+    ts = ts0
+    total = total_steps if total_steps is not None else ts0 + every * steps
+    hist = []
+    for k in range(steps):
+        rec = record_every > 0 and k % record_every == 0
+        le = lf = 0.0
+        for sim in sims:
+            sim.run(ts, ts + every, total)
+            _, (a, b) = model.config_grad(sim, w_energy, w_force, want_grad=False) if rec else \
+                (None, (0.0, 0.0))
+            if not rec:
+                nat.check(sim.h, model.L.pairs_hip_mlp_config_grad(sim.h, model.h, w_energy, w_force, None, None),
+                          "pairs_hip_mlp_config_grad")
+            le += a
+            lf += b
+        ts += every
+        frac = k / (steps - 1) if steps > 1 else 0.0
+        model.apply_grad(lr * (1.0 - (1.0 - lr_end_frac) * frac))
+        if rec:
+            hist.append((k, le / len(sims), lf / len(sims)))
+    return ts, hist

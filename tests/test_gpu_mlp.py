@@ -186,3 +186,129 @@ def test_training_and_ablation():
     used, rel2 = mlp.attach(s, m, 0.02)
     print(f"force error vs LJ: {rel:.4f} -> {'MLP' if used else 'LJ fallback'}")
     assert rel == rel2 and used == (rel <= 0.02) and mlp.active(s) == used
+
+
+# ---------------------------------------------------------------- training gradients
+#
+# Tolerances for the parameter gradients against the FP64 oracle
+# (oracle/mlp_oracle.py energy_loss_grad / config_loss_grad, pinned by finite
+# differences): emulating the kernels' BF16 rounding points in FP64 moves the
+# gradient by 0.1-0.2 % (relative L2, every block); the forward pass's own
+# BF16 error enters through the energy residuals and the force residuals.
+# Bar: relative L2 error <= 3 % over all 5313 parameters and <= 5 % in each
+# block (W1, b1, W2, b2, w3, b3); a transposed or mis-indexed block gives
+# errors of order 1.
+
+_BLOCKS = {"W1": slice(0, 1024), "b1": slice(1024, 1088), "W2": slice(1088, 5184), "b2": slice(5184, 5248),
+           "w3": slice(5248, 5312), "b3": slice(5312, 5313)}
+
+
+def _grad_errors(g, ref):
+    out = {"all": float(np.linalg.norm(g - ref) / np.linalg.norm(ref))}
+    for k, sl in _BLOCKS.items():
+        nr = np.linalg.norm(ref[sl])
+        out[k] = float(np.linalg.norm(g[sl] - ref[sl]) / nr) if nr > 0 else float(np.abs(g[sl]).max())
+    return out
+
+
+def _check_grad(g, ref, what):
+    e = _grad_errors(np.asarray(g, np.float64), ref)
+    print(what, {k: round(v, 5) for k, v in e.items()})
+    assert e["all"] <= 3e-2, (what, e)
+    for k in _BLOCKS:
+        assert e[k] <= 5e-2, (what, k, e)
+
+
+def _splitmix64(x):
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+def test_energy_grad_vs_oracle():
+    """One minibatch of k_mlp_grad (the energy-row training step) against the
+    oracle's gradient on the same rows; the row draw is the counter hash of
+    seed, step and position in the batch."""
+    _, mlp = _mods()
+    rng = np.random.default_rng(4)
+    rows, batch, seed, step = 3000, 1024, 11, 5
+    G = rng.uniform(0.0, 4.0, (rows, 16)).astype(np.float32)
+    lab = rng.normal(-5.0, 0.8, rows).astype(np.float32)
+    p = mo.xavier(5).astype(np.float32)
+    sc = _scale().astype(np.float32)
+    m = mlp.MLPModel(seed=1, dataset_capacity=rows)
+    m.set_params(p, sc)
+    assert m.add_rows(G, lab) == rows
+    g, loss = m.energy_grad(step, batch=batch, seed=seed)
+    idx = np.array([_splitmix64(seed ^ (step << 32) ^ j) % rows for j in range(batch)])
+    Lr, ref = mo.energy_loss_grad(p.astype(np.float64), sc.astype(np.float64), G[idx].astype(np.float64),
+                                  lab[idx].astype(np.float64))
+    assert abs(loss - Lr) <= 2e-2 * Lr, (loss, Lr)
+    _check_grad(g, ref, "energy minibatch")
+    # another step draws other rows
+    g2, _ = m.energy_grad(step + 1, batch=batch, seed=seed)
+    assert np.linalg.norm(g2 - g) > 1e-3 * np.linalg.norm(g)
+
+
+@pytest.mark.parametrize("w_energy, w_force", [(1.0, 0.0), (0.0, 1.0), (0.5, 1.0)])
+def test_config_grad_vs_oracle(w_energy, w_force):
+    """Gradient of one configuration's energy + force loss (back-projection of
+    the force residuals over the full lists and ghosts, then the network's
+    second derivative on MFMA) against the oracle's on the same positions."""
+    _, mlp = _mods()
+    s, L = _state(5, 40)
+    x = s.positions()
+    El, Fl = mo.lj_atom_energy(x, L), mo.lj_forces(x, L)
+    G = mo.descriptors(x, L)
+    sc = np.concatenate([G.mean(0), 1.0 / G.std(0), [El.mean(), El.std()]]).astype(np.float32)
+    p = mo.xavier(5).astype(np.float32)
+    m = mlp.MLPModel(seed=1, dataset_capacity=16)
+    m.set_params(p, sc)
+    f_before = s.forces()
+    g, (le, lf) = m.config_grad(s, w_energy, w_force)
+    Lo, le_r, lf_r, ref = mo.config_loss_grad(p.astype(np.float64), sc.astype(np.float64), x, L, El, Fl,
+                                              w_energy, w_force)
+    print(f"loss terms: energy {le:.5f} (oracle {le_r:.5f}), force {lf:.5f} (oracle {lf_r:.5f})")
+    assert abs(le - le_r) <= 3e-2 * le_r and abs(lf - lf_r) <= 3e-2 * lf_r
+    _check_grad(g, ref, f"configuration wE={w_energy} wF={w_force}")
+    # the force array holds the LJ forces afterwards
+    assert np.allclose(s.forces(), Fl, rtol=1e-10, atol=1e-10)
+    assert np.array_equal(s.forces(), f_before) or np.allclose(s.forces(), f_before, rtol=1e-12, atol=1e-12)
+
+
+def test_training_on_forces():
+    """Adam on configurations of plain-LJ runs at four temperatures, energy
+    and force terms, against the same number of steps on the energy term
+    alone: the force term lowers the held-out force error."""
+    md, mlp = _mods()
+    temps = (0.8, 1.2, 1.6, 2.0)
+
+    def systems():
+        out = []
+        for T in temps:
+            s = md.fcc_system(8, temp=T)
+            s.run(0, 100, 100000)
+            out.append(s)
+        return out
+
+    held = md.fcc_system(8, temp=1.44)
+    held.run(0, 300, 1000)
+    errs = {}
+    for name, wE, wF in (("energy", 1.0, 0.0), ("energy+force", 1.0, 1.0)):
+        m = mlp.MLPModel(seed=3, dataset_capacity=4 * 2048)
+        sims = systems()
+        for s in sims:
+            m.add_snapshot(s)
+        m.fit_scaling()
+        e0 = mlp.force_error(held, m)
+        ts, hist = mlp.train_on_configurations(m, sims, 400, every=10, ts0=100, lr=3e-3, w_energy=wE,
+                                               w_force=wF, total_steps=100000, record_every=100)
+        errs[name] = (e0, mlp.force_error(held, m), hist)
+        for s in sims:
+            s.close()
+    for k, v in errs.items():
+        print(f"{k}: force error {v[0]:.4f} -> {v[1]:.4f}; losses {[(a, round(b, 4), round(c, 4)) for a, b, c in v[2]]}")
+    assert errs["energy+force"][1] < 0.5 * errs["energy+force"][0]
+    assert errs["energy+force"][1] < errs["energy"][1]

@@ -120,3 +120,81 @@ def test_virial_is_minus_scaling_derivative():
     dEdl = (ep - em) / (2 * h)
     assert abs(W + dEdl) <= 1e-6 * max(1.0, abs(W)), (W, -dEdl)
     assert abs(W) > 1e-3  # a non-trivial check
+
+
+def test_silu_derivatives_match_finite_difference():
+    z = np.linspace(-6.0, 6.0, 49)
+    h = 1e-6
+    _, d1, d2 = mo.silu2(z)
+    hp, dp, _ = mo.silu2(z + h)
+    hm, dm, _ = mo.silu2(z - h)
+    assert np.allclose(d1, (hp - hm) / (2 * h), rtol=1e-8, atol=1e-10)
+    assert np.allclose(d2, (dp - dm) / (2 * h), rtol=1e-7, atol=1e-9)
+
+
+# one parameter of each block at its ends and inside (W1, b1, W2, b2, w3, b3):
+# a transposed weight block or a dropped bias term fails one of them
+_PARAM_PROBES = [0, 17, 1023, 1024, 1050, 1087, 1088, 1100, 3000, 5183, 5184, 5230, 5247, 5248, 5290, 5311, 5312]
+
+
+def _fd(loss, p, idx, h=1e-6):
+    out = []
+    for k in idx:
+        pp = p.copy()
+        pp[k] += h
+        a = loss(pp)
+        pp[k] -= 2 * h
+        out.append((a - loss(pp)) / (2 * h))
+    return np.array(out)
+
+
+def test_energy_loss_gradient_matches_finite_difference():
+    p, sc = mo.xavier(5), _scale()
+    rng = np.random.default_rng(4)
+    G = rng.uniform(0.0, 4.0, (37, 16))
+    lab = rng.normal(-5.0, 0.8, 37)
+    L0, g = mo.energy_loss_grad(p, sc, G, lab)
+    assert g.shape == (mo.NPARAMS,)
+    fd = _fd(lambda q: mo.energy_loss_grad(q, sc, G, lab)[0], p, _PARAM_PROBES)
+    assert np.allclose(g[_PARAM_PROBES], fd, rtol=1e-6, atol=1e-9 * np.abs(fd).max())
+    v = rng.standard_normal(mo.NPARAMS)
+    h = 1e-6
+    dd = (mo.energy_loss_grad(p + h * v, sc, G, lab)[0] - mo.energy_loss_grad(p - h * v, sc, G, lab)[0]) / (2 * h)
+    assert abs(dd - g @ v) <= 1e-6 * abs(dd)
+
+
+def test_lj_force_labels_are_minus_energy_gradient():
+    x, L = _cluster(30, 6.2)
+    F = mo.lj_forces(x, L)
+    h = 1e-6
+    for a in (0, 7, 29):
+        for c in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[a, c] += h
+            xm[a, c] -= h
+            dE = (mo.lj_atom_energy(xp, L).sum() - mo.lj_atom_energy(xm, L).sum()) / (2 * h)
+            assert abs(F[a, c] + dE) <= 1e-6 * max(1.0, abs(F[a, c]))
+    assert np.abs(F.sum(0)).max() < 1e-10  # Newton's third law
+
+
+def test_config_loss_gradient_matches_finite_difference():
+    """Energy term, force term and both: the force term's gradient goes
+    through the input gradient of the network (a second derivative) and the
+    pair back-projection of the force residuals, including through particles'
+    periodic images."""
+    x, L = _cluster(40, 5.6)
+    p, sc = mo.xavier(5), _scale()
+    G = mo.descriptors(x, L)
+    sc[0:16], sc[16:32] = G.mean(0), 1.0 / G.std(0)
+    El, Fl = mo.lj_atom_energy(x, L), mo.lj_forces(x, L)
+    rng = np.random.default_rng(9)
+    for wE, wF in ((1.0, 0.0), (0.0, 1.0), (0.7, 1.3)):
+        Lo, le, lf, g = mo.config_loss_grad(p, sc, x, L, El, Fl, wE, wF)
+        assert abs(Lo - mo.config_loss(p, sc, x, L, El, Fl, wE, wF)[0]) <= 1e-12 * Lo
+        fd = _fd(lambda q: mo.config_loss(q, sc, x, L, El, Fl, wE, wF)[0], p, _PARAM_PROBES)
+        assert np.allclose(g[_PARAM_PROBES], fd, rtol=1e-5, atol=1e-8 * np.abs(fd).max()), (wE, wF)
+        v = rng.standard_normal(mo.NPARAMS)
+        h = 1e-6
+        dd = (mo.config_loss(p + h * v, sc, x, L, El, Fl, wE, wF)[0]
+              - mo.config_loss(p - h * v, sc, x, L, El, Fl, wE, wF)[0]) / (2 * h)
+        assert abs(dd - g @ v) <= 1e-6 * abs(dd), (wE, wF)
