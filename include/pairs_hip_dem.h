@@ -15,12 +15,21 @@
  *   clear_unused_contact_history                             /
  *
  * Conventions are those of pairs_hip.h: extern "C", return 0 on success, a
- * positive PAIRS_HIP_DEM_OVERFLOW_* code when a capacity would be exceeded
- * (the required size from pairs_hip_dem_last_required; the step's results are
- * then not valid and the caller re-creates the context with larger
- * capacities), a negative code on a HIP or argument error (text from
+ * negative code on a HIP or argument error (text from
  * pairs_hip_dem_last_error); work is ordered on the context's stream;
  * downloads synchronise; one context per host thread.
+ *
+ * Resize protocol (src/pairs/transformations/modules.py:159-205): the step
+ * functions checkpoint the locals' state at the start of every chunk of
+ * steps (pairs_hip_dem_set_check_every, default 100) and read the device
+ * counters at its end; a chunk in which a particle, contact or send capacity
+ * would have been exceeded is run again from its checkpoint with that
+ * capacity grown to twice the required size, so a run continues as if the
+ * capacities had been large enough from the start.  A positive
+ * PAIRS_HIP_DEM_OVERFLOW_* code (required size from
+ * pairs_hip_dem_last_required) is returned only by set-up calls that exceed
+ * a capacity, or when growing cannot resolve an overflow (the cell grid is
+ * fixed by the domain).
  */
 #ifndef PAIRS_HIP_DEM_H
 #define PAIRS_HIP_DEM_H
@@ -56,7 +65,16 @@ enum {
     PAIRS_HIP_DEM_SHAPE = 12,           /* i32 [n]                             */
     PAIRS_HIP_DEM_NUM_CONTACTS = 13     /* i32 [nlocal]                        */
 };
-enum { PAIRS_HIP_DEM_NLOCAL = 0, PAIRS_HIP_DEM_NGHOST = 1, PAIRS_HIP_DEM_NCELLS = 2, PAIRS_HIP_DEM_CONTACTS = 3 };
+enum {
+    PAIRS_HIP_DEM_NLOCAL = 0,
+    PAIRS_HIP_DEM_NGHOST = 1,
+    PAIRS_HIP_DEM_NCELLS = 2,
+    PAIRS_HIP_DEM_CONTACTS = 3,
+    PAIRS_HIP_DEM_PARTICLE_CAPACITY = 4, /* current capacities (they grow on overflow) */
+    PAIRS_HIP_DEM_CONTACT_CAPACITY = 5,
+    PAIRS_HIP_DEM_SEND_CAPACITY = 6,
+    PAIRS_HIP_DEM_RESIZES = 7            /* capacity growths so far                   */
+};
 
 typedef struct pairs_hip_dem_config {
     int particle_capacity;  /* locals + ghosts (particle_capacity, examples/dem.py:154) */
@@ -120,15 +138,23 @@ int pairs_hip_dem_add_spheres(pairs_hip_dem_ctx *ctx, int n, const int *uid, con
 /* update_mass_and_inertia (examples/dem.py:6-15) + build_cell_lists_stencil. */
 int pairs_hip_dem_setup(pairs_hip_dem_ctx *ctx);
 
-/* One timestep (the generated loop body); timesteps [ts_begin, ts_end). */
+/* One timestep (the generated loop body); timesteps [ts_begin, ts_end).
+   Both follow the resize protocol above (a step is its own chunk). */
 int pairs_hip_dem_step(pairs_hip_dem_ctx *ctx, int ts);
 int pairs_hip_dem_run(pairs_hip_dem_ctx *ctx, int ts_begin, int ts_end);
+/* Steps per checkpoint and counter read of pairs_hip_dem_run (default 100). */
+int pairs_hip_dem_set_check_every(pairs_hip_dem_ctx *ctx, int steps);
 int pairs_hip_dem_set_graphs(pairs_hip_dem_ctx *ctx, int enable);
 int pairs_hip_dem_sync(pairs_hip_dem_ctx *ctx);
 
 int pairs_hip_dem_count(pairs_hip_dem_ctx *ctx, int what);
+/* Copy one array (PAIRS_HIP_DEM_POSITION ...) to the host: count elements
+   of its leading dimension are written, at most what exists (locals and
+   ghosts; locals only for force, torque, rotation and contact counts).
+   Returns the count written. */
 int pairs_hip_dem_download(pairs_hip_dem_ctx *ctx, int what, void *host, int count);
-/* Contact history of the locals, row-major [nlocal][contact_capacity]:
+/* Contact history of the locals, row-major [nlocal][contact capacity]
+   (the current one: pairs_hip_dem_count(PAIRS_HIP_DEM_CONTACT_CAPACITY)):
    uid of the partner, is_sticking, tangential spring displacement [3],
    impact velocity magnitude (src/pairs/sim/contact_history.py). */
 int pairs_hip_dem_contacts(pairs_hip_dem_ctx *ctx, int *num_contacts, int *partner_uid, int *sticking,

@@ -42,6 +42,15 @@ struct pairs_hip_dem_ctx {
     double t_sum = 0.0;
     int t_count = 0;
     std::mt19937 generator;     // runtime/dem_sc_grid.hpp:13-20: default seeded, shared by all grids
+    // resize-and-retry: the locals' state at the start of a chunk of steps
+    int check_every = 100;      // steps per chunk (one counter read per chunk)
+    char *snap = nullptr;       // device checkpoint
+    size_t snap_bytes = 0;
+    int snap_n = 0, snap_k = 0;
+    int snap_counters[D_COUNT] = {};
+    CopyDesc *h_copy = nullptr; // pinned copy tables (checkpoint, restore)
+    CopyDesc *d_copy = nullptr;
+    int resizes = 0;            // capacity growths so far
 };
 
 #define HIP_OK(expr)                                                                                   \
@@ -75,16 +84,166 @@ static void drop_graph(pairs_hip_dem_ctx *ctx) {
     ctx->graph = nullptr;
 }
 
-static void free_all(pairs_hip_dem_ctx *ctx) {
-    DemPtrs &p = ctx->p;
+static void free_arrays(DemPtrs &p) {
     void *ptrs[] = {p.x, p.y, p.z, p.vx, p.vy, p.vz, p.wx, p.wy, p.wz, p.fx, p.fy, p.fz, p.tx, p.ty, p.tz, p.r, p.m,
                     p.nx, p.ny, p.nz, p.inv_i, p.rot_m, p.rot_q, p.uid, p.flags, p.type, p.shape, p.num_contacts,
                     p.c_uid, p.c_used, p.c_stick, p.c_tsd, p.c_ivm, p.cell_count, p.cell_start, p.cell_fill,
                     p.cell_particles, p.particle_cell, p.nshapes, p.send_map, p.send_mult, p.exchg_flag, p.copy_to,
-                    p.blk, p.scan_tmp, p.buf, p.hbuf, p.cand_n, p.cand, p.cslot, p.cf, ctx->counters};
+                    p.blk, p.scan_tmp, p.buf, p.hbuf, p.cand_n, p.cand, p.cslot, p.cf};
     for (void *q : ptrs)
         if (q) hipFree(q);
+    const int cap = p.cap, ccap = p.ccap, send_cap = p.send_cap;
+    p = DemPtrs{};
+    p.cap = cap; p.ccap = ccap; p.send_cap = send_cap;
+}
+
+static void free_all(pairs_hip_dem_ctx *ctx) {
+    free_arrays(ctx->p);
+    if (ctx->counters) hipFree(ctx->counters);
     if (ctx->h_counters) hipHostFree(ctx->h_counters);
+    if (ctx->snap) hipFree(ctx->snap);
+    if (ctx->d_copy) hipFree(ctx->d_copy);
+    if (ctx->h_copy) hipHostFree(ctx->h_copy);
+}
+
+// every array of the particle, contact and send capacities (zeroed)
+static int alloc_arrays(pairs_hip_dem_ctx *ctx, DemPtrs &p, size_t C, size_t K, size_t S) {
+    p.cap = (int) C; p.ccap = (int) K; p.send_cap = (int) S;
+    int r = 0;
+    for (double **q : {&p.x, &p.y, &p.z, &p.vx, &p.vy, &p.vz, &p.wx, &p.wy, &p.wz, &p.fx, &p.fy, &p.fz, &p.tx, &p.ty,
+                       &p.tz, &p.r, &p.m, &p.nx, &p.ny, &p.nz})
+        r |= dalloc(ctx, q, C);
+    r |= dalloc(ctx, &p.inv_i, 9 * C);
+    r |= dalloc(ctx, &p.rot_m, 9 * C);
+    r |= dalloc(ctx, &p.rot_q, 4 * C);
+    for (int **q : {&p.uid, &p.flags, &p.type, &p.shape, &p.num_contacts, &p.particle_cell, &p.exchg_flag})
+        r |= dalloc(ctx, q, C);
+    r |= dalloc(ctx, &p.c_uid, K * C);
+    r |= dalloc(ctx, &p.c_used, K * C);
+    r |= dalloc(ctx, &p.c_stick, K * C);
+    r |= dalloc(ctx, &p.c_tsd, 3 * K * C);
+    r |= dalloc(ctx, &p.c_ivm, K * C);
+    const size_t NC = (size_t) ctx->ncells_cap;
+    r |= dalloc(ctx, &p.cell_count, NC + 1);
+    r |= dalloc(ctx, &p.cell_start, NC + 1);
+    r |= dalloc(ctx, &p.cell_fill, NC + 1);
+    r |= dalloc(ctx, &p.nshapes, 2 * NC);
+    r |= dalloc(ctx, &p.cell_particles, C);
+    r |= dalloc(ctx, &p.scan_tmp, NC / 1024 + 8);
+    r |= dalloc(ctx, &p.send_map, S);
+    r |= dalloc(ctx, &p.send_mult, 3 * S);
+    r |= dalloc(ctx, &p.copy_to, S);
+    r |= dalloc(ctx, &p.blk, 2 * (C / 256 + 2));
+    r |= dalloc(ctx, &p.buf, 40 * S);
+    r |= dalloc(ctx, &p.hbuf, (2 + 5 * K) * S);
+    r |= dalloc(ctx, &p.cand_n, C);
+    r |= dalloc(ctx, &p.cand, DCAND * C);
+    r |= dalloc(ctx, &p.cslot, DCAND * C);
+    r |= dalloc(ctx, &p.cf, 6 * DCAND * C);
+    return r ? -1 : 0;
+}
+
+// The state a chunk of steps starts from: the locals' rows of every array a
+// step reads before writing (positions, velocities, angular velocities,
+// radius, mass, normals, inertia, rotation, uid, flags, type, shape, contact
+// counts and the contact history), packed at pitch n, and the counters.
+// Ghosts, cells and forces are rebuilt by every step.
+template <typename F>
+static void state_fields(DemPtrs &p, F &&f) {
+    for (double *a : {p.x, p.y, p.z, p.vx, p.vy, p.vz, p.wx, p.wy, p.wz, p.r, p.m, p.nx, p.ny, p.nz})
+        f((void *) a, 8, 1);
+    f((void *) p.inv_i, 8, 9);
+    f((void *) p.rot_m, 8, 9);
+    f((void *) p.rot_q, 8, 4);
+    for (int *a : {p.uid, p.flags, p.type, p.shape, p.num_contacts}) f((void *) a, 4, 1);
+    f((void *) p.c_uid, 4, -1);
+    f((void *) p.c_used, 4, -1);
+    f((void *) p.c_stick, 4, -1);
+    f((void *) p.c_tsd, 8, -3);
+    f((void *) p.c_ivm, 8, -1);
+}
+constexpr int MAX_COPY = 32;
+
+// checkpoint (to_snap) or restore the locals' state; rows of the contact
+// arrays: min(k, capacity) (restore: rows past the checkpoint stay zero)
+static int snap_copy(pairs_hip_dem_ctx *ctx, bool to_snap) {
+    DemPtrs &p = ctx->p;
+    const int n = to_snap ? ctx->h_counters[D_NLOCAL] : ctx->snap_n;
+    const int k = to_snap ? p.ccap : ctx->snap_k;
+    if (to_snap) {
+        size_t bytes = 0;
+        state_fields(p, [&](void *, int el, int rows) { bytes += (size_t) el * n * (rows > 0 ? rows : -rows * k); });
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes > ctx->snap_bytes) {
+            if (ctx->snap) hipFree(ctx->snap);
+            ctx->snap = nullptr;
+            ctx->snap_bytes = 0;
+            HIP_OK(hipMalloc((void **) &ctx->snap, bytes + bytes / 4));
+            ctx->snap_bytes = bytes + bytes / 4;
+        }
+        ctx->snap_n = n;
+        ctx->snap_k = k;
+        std::memcpy(ctx->snap_counters, ctx->h_counters, sizeof(ctx->snap_counters));
+    }
+    if (!ctx->h_copy) {
+        HIP_OK(hipHostMalloc((void **) &ctx->h_copy, 2 * MAX_COPY * sizeof(CopyDesc)));
+        HIP_OK(hipMalloc((void **) &ctx->d_copy, 2 * MAX_COPY * sizeof(CopyDesc)));
+    }
+    CopyDesc *tab = ctx->h_copy + (to_snap ? 0 : MAX_COPY);
+    int nt = 0;
+    size_t off = 0;
+    state_fields(p, [&](void *a, int el, int rows) {
+        const int nrows = rows > 0 ? rows : -rows * std::min(k, p.ccap);
+        const int srows = rows > 0 ? rows : -rows * k;  // rows held in the checkpoint
+        char *dev = (char *) a, *sn = ctx->snap + off;
+        CopyDesc &c = tab[nt++];
+        c.src = to_snap ? dev : sn;
+        c.dst = to_snap ? sn : dev;
+        c.words = (int) ((size_t) el * n / 4);
+        c.rows = nrows;
+        c.src_pitch = to_snap ? (size_t) el * p.cap / 4 : (size_t) el * n / 4;
+        c.dst_pitch = to_snap ? (size_t) el * n / 4 : (size_t) el * p.cap / 4;
+        off += (size_t) el * n * srows;
+    });
+    HIP_OK(hipMemcpyAsync(ctx->d_copy + (to_snap ? 0 : MAX_COPY), tab, nt * sizeof(CopyDesc), hipMemcpyHostToDevice,
+                          ctx->stream));
+    launch_copy_rows(ctx->stream, ctx->d_copy + (to_snap ? 0 : MAX_COPY), nt, n);
+    CHECK_LAUNCH();
+    if (!to_snap) {
+        // the counters of the checkpoint, overflow flags cleared
+        std::memcpy(ctx->h_counters, ctx->snap_counters, sizeof(ctx->snap_counters));
+        ctx->h_counters[D_OVF] = 0;
+        ctx->h_counters[D_REQ_SEND] = ctx->h_counters[D_REQ_CONTACT] = ctx->h_counters[D_REQ_PART] = 0;
+        HIP_OK(hipMemcpyAsync(ctx->counters, ctx->h_counters, D_COUNT * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
+    // the pinned table and counters must not change before the copies ran
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// after an overflow: every array re-allocated with the overflowing capacity
+// at twice the required size (the reference's grow rule,
+// src/pairs/transformations/modules.py:159-205), the checkpoint restored into
+// them and the step graph dropped
+static int grow_and_restore(pairs_hip_dem_ctx *ctx, int code) {
+    DemPtrs &p = ctx->p;
+    size_t C = p.cap, K = p.ccap, S = p.send_cap;
+    const int need = ctx->req[code];
+    if (code == PAIRS_HIP_DEM_OVERFLOW_PARTICLES) C = std::max(C + 1, 2 * (size_t) need);
+    else if (code == PAIRS_HIP_DEM_OVERFLOW_CONTACTS) K = std::max(K + 1, 2 * (size_t) need);
+    else if (code == PAIRS_HIP_DEM_OVERFLOW_SEND) S = std::max(S + 1, 2 * (size_t) need);
+    else return code;  // the cell grid is fixed by the domain
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    free_arrays(p);
+    if (alloc_arrays(ctx, p, C, K, S)) {
+        ctx->err = "re-allocation after a capacity overflow failed";
+        return -1;
+    }
+    if (snap_copy(ctx, false)) return -1;
+    ctx->resizes++;
+    return 0;
 }
 
 // build_cell_lists_stencil (src/pairs/sim/cell_lists.py:46-87)
@@ -146,40 +305,8 @@ int pairs_hip_dem_create(pairs_hip_dem_ctx **out, const pairs_hip_dem_config *cf
     prm.grav = cfg->gravity;
     prm.ntypes = cfg->ntypes;
     for (int k = 0; k < 16; k++) { prm.mu_s[k] = cfg->friction_static[k]; prm.mu_d[k] = cfg->friction_dynamic[k]; }
-    DemPtrs &p = ctx->p;
-    const size_t C = (size_t) cfg->particle_capacity, K = (size_t) cfg->contact_capacity, S = (size_t) cfg->send_capacity;
-    p.cap = (int) C; p.ccap = (int) K; p.send_cap = (int) S;
-    int r = 0;
-    for (double **q : {&p.x, &p.y, &p.z, &p.vx, &p.vy, &p.vz, &p.wx, &p.wy, &p.wz, &p.fx, &p.fy, &p.fz, &p.tx, &p.ty,
-                       &p.tz, &p.r, &p.m, &p.nx, &p.ny, &p.nz})
-        r |= dalloc(ctx, q, C);
-    r |= dalloc(ctx, &p.inv_i, 9 * C);
-    r |= dalloc(ctx, &p.rot_m, 9 * C);
-    r |= dalloc(ctx, &p.rot_q, 4 * C);
-    for (int **q : {&p.uid, &p.flags, &p.type, &p.shape, &p.num_contacts, &p.particle_cell, &p.exchg_flag})
-        r |= dalloc(ctx, q, C);
-    r |= dalloc(ctx, &p.c_uid, K * C);
-    r |= dalloc(ctx, &p.c_used, K * C);
-    r |= dalloc(ctx, &p.c_stick, K * C);
-    r |= dalloc(ctx, &p.c_tsd, 3 * K * C);
-    r |= dalloc(ctx, &p.c_ivm, K * C);
-    const size_t NC = (size_t) cfg->ncells_capacity;
-    r |= dalloc(ctx, &p.cell_count, NC + 1);
-    r |= dalloc(ctx, &p.cell_start, NC + 1);
-    r |= dalloc(ctx, &p.cell_fill, NC + 1);
-    r |= dalloc(ctx, &p.nshapes, 2 * NC);
-    r |= dalloc(ctx, &p.cell_particles, C);
-    r |= dalloc(ctx, &p.scan_tmp, NC / 1024 + 8);
-    r |= dalloc(ctx, &p.send_map, S);
-    r |= dalloc(ctx, &p.send_mult, 3 * S);
-    r |= dalloc(ctx, &p.copy_to, S);
-    r |= dalloc(ctx, &p.blk, 2 * (C / 256 + 2));
-    r |= dalloc(ctx, &p.buf, 40 * S);
-    r |= dalloc(ctx, &p.hbuf, (2 + 5 * K) * S);
-    r |= dalloc(ctx, &p.cand_n, C);
-    r |= dalloc(ctx, &p.cand, DCAND * C);
-    r |= dalloc(ctx, &p.cslot, DCAND * C);
-    r |= dalloc(ctx, &p.cf, 6 * DCAND * C);
+    int r = alloc_arrays(ctx, ctx->p, (size_t) cfg->particle_capacity, (size_t) cfg->contact_capacity,
+                         (size_t) cfg->send_capacity);
     r |= dalloc(ctx, &ctx->counters, D_COUNT);
     if (r) return fail(-1);
     if (hipHostMalloc((void **) &ctx->h_counters, D_COUNT * sizeof(int)) != hipSuccess) return fail(-1);
@@ -393,31 +520,24 @@ static void enqueue_step(pairs_hip_dem_ctx *ctx, bool timed) {
     if (timed) hipEventRecord(ctx->ev[6], ctx->stream);
 }
 
-int pairs_hip_dem_step(pairs_hip_dem_ctx *ctx, int ts) {
-    if (!ctx) return -2;
-    (void) ts;
+static int ensure_graph(pairs_hip_dem_ctx *ctx) {
+    if (!ctx->use_graphs || ctx->graph) return 0;
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(ctx, false);
-    CHECK_LAUNCH();
-    if (read_counters(ctx)) return -1;
-    return overflow_code(ctx);
+    HIP_OK(hipStreamEndCapture(ctx->stream, &g));
+    const hipError_t e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) { ctx->graph = nullptr; ctx->err = "graph instantiation failed"; return -1; }
+    return 0;
 }
 
-int pairs_hip_dem_run(pairs_hip_dem_ctx *ctx, int ts_begin, int ts_end) {
-    if (!ctx || ts_end < ts_begin) return -2;
-    if (ctx->use_graphs && !ctx->graph) {
-        hipGraph_t g;
-        HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(ctx, false);
-        HIP_OK(hipStreamEndCapture(ctx->stream, &g));
-        const hipError_t e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        if (e != hipSuccess) { ctx->graph = nullptr; ctx->err = "graph instantiation failed"; return -1; }
-    }
-    ctx->t_sum = 0.0;
-    ctx->t_count = 0;
-    for (double &t : ctx->t_phase) t = 0.0;
-    for (int ts = ts_begin; ts < ts_end; ts++) {
-        const bool timed = ctx->time_every > 0 && (ts - ts_begin) % ctx->time_every == ctx->time_every - 1;
+// steps [a, b) without a counter read; t0: the first timestep of the run
+// (the timing cadence)
+static int enqueue_steps(pairs_hip_dem_ctx *ctx, int a, int b, int t0) {
+    if (int e = ensure_graph(ctx)) return e;
+    for (int ts = a; ts < b; ts++) {
+        const bool timed = ctx->time_every > 0 && (ts - t0) % ctx->time_every == ctx->time_every - 1;
         if (timed) {
             enqueue_step(ctx, true);
             CHECK_LAUNCH();
@@ -438,8 +558,64 @@ int pairs_hip_dem_run(pairs_hip_dem_ctx *ctx, int ts_begin, int ts_end) {
             CHECK_LAUNCH();
         }
     }
+    return 0;
+}
+
+// Steps [a, b) in chunks of check_every: the locals' state is checkpointed
+// at the start of each chunk and the counters are read at its end; a chunk
+// that overflowed a capacity (every later kernel of it then did nothing) is
+// run again from its checkpoint with that capacity grown to twice the
+// required size, as the reference re-runs a module after a resize
+// (src/pairs/transformations/modules.py:159-205).  The results do not depend
+// on the capacities, so the trajectory is that of a run that never overflowed.
+static int run_checked(pairs_hip_dem_ctx *ctx, int a, int b, int t0) {
     if (read_counters(ctx)) return -1;
-    return overflow_code(ctx);
+    if (int code = overflow_code(ctx)) return code;  // a previous call left an overflow
+    for (int c0 = a; c0 < b;) {
+        const int c1 = std::min(b, c0 + std::max(ctx->check_every, 1));
+        if (snap_copy(ctx, true)) return -1;
+        const double ts_sum = ctx->t_sum;
+        const int ts_count = ctx->t_count;
+        double tp[6];
+        std::memcpy(tp, ctx->t_phase, sizeof(tp));
+        int code = 0;
+        for (int attempt = 0; attempt < 16; attempt++) {
+            if (int e = enqueue_steps(ctx, c0, c1, t0)) return e;
+            if (read_counters(ctx)) return -1;
+            code = overflow_code(ctx);
+            if (code == 0) break;
+            ctx->t_sum = ts_sum;
+            ctx->t_count = ts_count;
+            std::memcpy(ctx->t_phase, tp, sizeof(tp));
+            if (int e = grow_and_restore(ctx, code)) return e;
+        }
+        if (code) return code;
+        c0 = c1;
+    }
+    return 0;
+}
+
+int pairs_hip_dem_step(pairs_hip_dem_ctx *ctx, int ts) {
+    if (!ctx) return -2;
+    const bool g = ctx->use_graphs;
+    ctx->use_graphs = false;
+    const int e = run_checked(ctx, ts, ts + 1, ts);
+    ctx->use_graphs = g;
+    return e;
+}
+
+int pairs_hip_dem_run(pairs_hip_dem_ctx *ctx, int ts_begin, int ts_end) {
+    if (!ctx || ts_end < ts_begin) return -2;
+    ctx->t_sum = 0.0;
+    ctx->t_count = 0;
+    for (double &t : ctx->t_phase) t = 0.0;
+    return run_checked(ctx, ts_begin, ts_end, ts_begin);
+}
+
+int pairs_hip_dem_set_check_every(pairs_hip_dem_ctx *ctx, int steps) {
+    if (!ctx || steps < 1) return -2;
+    ctx->check_every = steps;
+    return 0;
 }
 
 int pairs_hip_dem_set_graphs(pairs_hip_dem_ctx *ctx, int enable) {
@@ -462,6 +638,10 @@ int pairs_hip_dem_count(pairs_hip_dem_ctx *ctx, int what) {
     case PAIRS_HIP_DEM_NLOCAL: return ctx->h_counters[D_NLOCAL];
     case PAIRS_HIP_DEM_NGHOST: return ctx->h_counters[D_NGHOST];
     case PAIRS_HIP_DEM_NCELLS: return ctx->geom.ncells;
+    case PAIRS_HIP_DEM_PARTICLE_CAPACITY: return ctx->p.cap;
+    case PAIRS_HIP_DEM_CONTACT_CAPACITY: return ctx->p.ccap;
+    case PAIRS_HIP_DEM_SEND_CAPACITY: return ctx->p.send_cap;
+    case PAIRS_HIP_DEM_RESIZES: return ctx->resizes;
     case PAIRS_HIP_DEM_CONTACTS: {
         const int n = ctx->h_counters[D_NLOCAL];
         std::vector<int> nc(std::max(n, 1));
@@ -478,10 +658,15 @@ int pairs_hip_dem_download(pairs_hip_dem_ctx *ctx, int what, void *host, int cou
     if (!ctx || !host || count < 0) return -2;
     if (read_counters(ctx)) return -1;
     const int nl = ctx->h_counters[D_NLOCAL], nall = nl + ctx->h_counters[D_NGHOST];
-    if (count > ctx->p.cap) return -2;
     const DemPtrs &p = ctx->p;
     const size_t C = p.cap;
+    // at most what exists: locals and ghosts, or the locals of local-only arrays
+    const int n_all = std::min(count, nall), n_loc = std::min(count, nl);
     auto get_d = [&](const double *d, double *h, int n) { return hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost); };
+    auto get_i = [&](const int *d, int *h, int n) -> int {
+        HIP_OK(hipMemcpy(h, d, n * sizeof(int), hipMemcpyDeviceToHost));
+        return n;
+    };
     auto interleave = [&](std::initializer_list<const double *> cols, double *out, int n) -> int {
         const int nc = (int) cols.size();
         std::vector<double> tmp(std::max(n, 1));
@@ -491,31 +676,28 @@ int pairs_hip_dem_download(pairs_hip_dem_ctx *ctx, int what, void *host, int cou
             for (int i = 0; i < n; i++) out[(size_t) i * nc + c] = tmp[i];
             c++;
         }
-        return 0;
+        return n;
     };
     double *hd = (double *) host;
     int *hi = (int *) host;
-    (void) nall;
     switch (what) {
-    case PAIRS_HIP_DEM_POSITION: return interleave({p.x, p.y, p.z}, hd, count);
-    case PAIRS_HIP_DEM_LINEAR_VELOCITY: return interleave({p.vx, p.vy, p.vz}, hd, count);
-    case PAIRS_HIP_DEM_ANGULAR_VELOCITY: return interleave({p.wx, p.wy, p.wz}, hd, count);
-    case PAIRS_HIP_DEM_FORCE: return interleave({p.fx, p.fy, p.fz}, hd, std::min(count, nl));
-    case PAIRS_HIP_DEM_TORQUE: return interleave({p.tx, p.ty, p.tz}, hd, std::min(count, nl));
-    case PAIRS_HIP_DEM_RADIUS: HIP_OK(get_d(p.r, hd, count)); return 0;
-    case PAIRS_HIP_DEM_MASS: HIP_OK(get_d(p.m, hd, count)); return 0;
+    case PAIRS_HIP_DEM_POSITION: return interleave({p.x, p.y, p.z}, hd, n_all);
+    case PAIRS_HIP_DEM_LINEAR_VELOCITY: return interleave({p.vx, p.vy, p.vz}, hd, n_all);
+    case PAIRS_HIP_DEM_ANGULAR_VELOCITY: return interleave({p.wx, p.wy, p.wz}, hd, n_all);
+    case PAIRS_HIP_DEM_FORCE: return interleave({p.fx, p.fy, p.fz}, hd, n_loc);
+    case PAIRS_HIP_DEM_TORQUE: return interleave({p.tx, p.ty, p.tz}, hd, n_loc);
+    case PAIRS_HIP_DEM_RADIUS: HIP_OK(get_d(p.r, hd, n_all)); return n_all;
+    case PAIRS_HIP_DEM_MASS: HIP_OK(get_d(p.m, hd, n_all)); return n_all;
     case PAIRS_HIP_DEM_ROTATION_QUAT:
-        return interleave({p.rot_q, p.rot_q + C, p.rot_q + 2 * C, p.rot_q + 3 * C}, hd, count);
+        return interleave({p.rot_q, p.rot_q + C, p.rot_q + 2 * C, p.rot_q + 3 * C}, hd, n_loc);
     case PAIRS_HIP_DEM_ROTATION_MATRIX:
         return interleave({p.rot_m, p.rot_m + C, p.rot_m + 2 * C, p.rot_m + 3 * C, p.rot_m + 4 * C, p.rot_m + 5 * C,
-                           p.rot_m + 6 * C, p.rot_m + 7 * C, p.rot_m + 8 * C}, hd, count);
-    case PAIRS_HIP_DEM_UID: HIP_OK(hipMemcpy(hi, p.uid, count * sizeof(int), hipMemcpyDeviceToHost)); return 0;
-    case PAIRS_HIP_DEM_FLAGS: HIP_OK(hipMemcpy(hi, p.flags, count * sizeof(int), hipMemcpyDeviceToHost)); return 0;
-    case PAIRS_HIP_DEM_TYPE: HIP_OK(hipMemcpy(hi, p.type, count * sizeof(int), hipMemcpyDeviceToHost)); return 0;
-    case PAIRS_HIP_DEM_SHAPE: HIP_OK(hipMemcpy(hi, p.shape, count * sizeof(int), hipMemcpyDeviceToHost)); return 0;
-    case PAIRS_HIP_DEM_NUM_CONTACTS:
-        HIP_OK(hipMemcpy(hi, p.num_contacts, std::min(count, nl) * sizeof(int), hipMemcpyDeviceToHost));
-        return 0;
+                           p.rot_m + 6 * C, p.rot_m + 7 * C, p.rot_m + 8 * C}, hd, n_loc);
+    case PAIRS_HIP_DEM_UID: return get_i(p.uid, hi, n_all);
+    case PAIRS_HIP_DEM_FLAGS: return get_i(p.flags, hi, n_all);
+    case PAIRS_HIP_DEM_TYPE: return get_i(p.type, hi, n_all);
+    case PAIRS_HIP_DEM_SHAPE: return get_i(p.shape, hi, n_all);
+    case PAIRS_HIP_DEM_NUM_CONTACTS: return get_i(p.num_contacts, hi, n_loc);
     default: return -2;
     }
 }

@@ -19,6 +19,26 @@
 
 namespace dk {
 
+// copy table entry blockIdx.y, grid-strided over its rows x words
+__global__ __launch_bounds__(256) void k_copy_rows(const CopyDesc *__restrict__ table) {
+    const CopyDesc c = table[blockIdx.y];
+    const uint32_t *__restrict__ src = (const uint32_t *) c.src;
+    uint32_t *__restrict__ dst = (uint32_t *) c.dst;
+    const size_t total = (size_t) c.words * (size_t) c.rows;
+    for (size_t e = (size_t) blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t) gridDim.x * 256) {
+        const size_t r = e / (size_t) c.words, w = e - r * (size_t) c.words;
+        dst[r * c.dst_pitch + w] = src[r * c.src_pitch + w];
+    }
+}
+
+void launch_copy_rows(hipStream_t s, const CopyDesc *table, int ncopies, int n) {
+    if (ncopies <= 0 || n <= 0) return;
+    // the widest entry: a double array of 9 rows of n particles
+    const size_t most = (size_t) 2 * n * 9;
+    const int gx = (int) std::min<size_t>((most + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_copy_rows, dim3(gx, ncopies), dim3(256), 0, s, table);
+}
+
 __global__ void k_fill_i32(int *__restrict__ p, int v, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }

@@ -69,6 +69,17 @@ struct DemPtrs {
     double *cf;          // their force and torque terms [6 t + d][cap]
 };
 
+// one strided copy of 32-bit words: rows x words, row r from src + r * src_pitch
+// to dst + r * dst_pitch (pitches in words)
+struct CopyDesc {
+    const void *src;
+    void *dst;
+    size_t src_pitch, dst_pitch;
+    int words, rows;
+};
+// every copy of a table in one launch (n: the largest row width in particles)
+void launch_copy_rows(hipStream_t s, const CopyDesc *table, int ncopies, int n);
+
 void launch_fill(hipStream_t s, int *p, int v, int n);
 void launch_mass_inertia(hipStream_t s, const DemPtrs &p, int nl);
 // one step = prepare (exchange, borders, cells) + contacts (+ gravity and

@@ -24,7 +24,7 @@ OVERFLOW_PARTICLES, OVERFLOW_CONTACTS, OVERFLOW_SEND, OVERFLOW_CELLS = 1, 2, 3, 
 SPHERE, HALFSPACE = 0, 1
 (POSITION, LINEAR_VELOCITY, ANGULAR_VELOCITY, FORCE, TORQUE, RADIUS, MASS, ROTATION_QUAT, ROTATION_MATRIX, UID,
  FLAGS, TYPE, SHAPE, NUM_CONTACTS) = range(14)
-NLOCAL, NGHOST, NCELLS, CONTACTS = range(4)
+NLOCAL, NGHOST, NCELLS, CONTACTS, PARTICLE_CAPACITY, CONTACT_CAPACITY, SEND_CAPACITY, RESIZES = range(8)
 
 
 class DEMConfig(ctypes.Structure):
@@ -50,6 +50,7 @@ native.register({
     "pairs_hip_dem_step": (_I, [_P, _I]),
     "pairs_hip_dem_run": (_I, [_P, _I, _I]),
     "pairs_hip_dem_set_graphs": (_I, [_P, _I]),
+    "pairs_hip_dem_set_check_every": (_I, [_P, _I]),
     "pairs_hip_dem_sync": (_I, [_P]),
     "pairs_hip_dem_count": (_I, [_P, _I]),
     "pairs_hip_dem_download": (_I, [_P, _I, _P, _I]),
@@ -227,6 +228,19 @@ class DEMSystem:
         self._check(self.L.pairs_hip_dem_sync(self.ctx), "sync")
 
     # ---- state
+    def set_check_every(self, steps):
+        """Steps per checkpoint and overflow check of run() (resize protocol)."""
+        self._check(self.L.pairs_hip_dem_set_check_every(self.ctx, steps), "set_check_every")
+
+    def count(self, what):
+        return self._check(self.L.pairs_hip_dem_count(self.ctx, what), "count")
+
+    @property
+    def capacities(self):
+        """(particle, contact, send) capacities now, and the growths so far"""
+        return (self.count(PARTICLE_CAPACITY), self.count(CONTACT_CAPACITY), self.count(SEND_CAPACITY),
+                self.count(RESIZES))
+
     @property
     def nlocal(self):
         return self._check(self.L.pairs_hip_dem_count(self.ctx, NLOCAL), "count")
@@ -251,7 +265,7 @@ class DEMSystem:
 
     def contacts(self):
         """num_contacts [n], partner uid / sticking [n][C], tangential spring [n][C][3], impact velocity [n][C]"""
-        n, c = self.nlocal, self.ccap
+        n, c = self.nlocal, self.count(CONTACT_CAPACITY)
         nc = np.zeros(n, np.int32)
         uid = np.zeros((n, c), np.int32)
         st = np.zeros((n, c), np.int32)

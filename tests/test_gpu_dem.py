@@ -95,6 +95,38 @@ def test_larger_domain_vs_oracle():
     assert int(nc.sum()) > 500
 
 
+def test_resize_and_retry_vs_oracle():
+    """The resize protocol: particle, send and contact capacities far too
+    small for the run (ghosts, periodic exchanges and piles of contacts), so
+    chunks of steps overflow mid-run; each is run again from its checkpoint
+    with the capacity grown to twice the required size, and the trajectory
+    equals the oracle's (which never overflows) at the same bar as above."""
+    dom = (0.12, 0.06, 0.05)
+    o = po.dem_reference_system(dom, cap=16384, send_cap=16384)
+    n = o.nlocal
+    g = _dem().reference_system(dom, particle_capacity=n + 64, send_capacity=64, contact_capacity=2)
+    g.set_check_every(37)
+    assert g.nlocal == n
+    for a, b in ((0, 250), (250, 400)):
+        g.run(a, b)
+        o.run(a, b)
+        assert g.nlocal == o.nlocal and g.nghost == o.nghost, b
+        assert np.array_equal(g.get("uid"), o.get("uid")), b
+        for k in ("position", "linear_velocity", "angular_velocity", "force", "torque"):
+            assert _rel(g.get(k), o.get(k)) <= REL, (b, k)
+        nc, cl, sk, tsd, iv = g.contacts()
+        onc, ocl, osk, otsd, oiv = o.contacts()
+        assert np.array_equal(nc, onc), b
+        gu, gs, gt, gi = _flat_contacts(nc, cl, sk, tsd, iv)
+        ou, os_, ot, oi = _flat_contacts(onc, ocl, osk, otsd, oiv)
+        assert np.array_equal(gu, ou) and np.array_equal(gs, os_), b
+        assert _rel(gt, ot) <= REL and _rel(gi, oi) <= REL, b
+    pc, cc, sc, resizes = g.capacities
+    print(f"capacities after the run: particles {pc}, contacts {cc}, send {sc}; {resizes} growths")
+    assert pc > n + 64 and cc > 2 and sc > 64 and resizes >= 3
+    assert int(nc.max()) > 2  # the contact capacity was really exceeded
+
+
 def test_planes_read_from_csv_equal_builtin_planes(tmp_path):
     """examples/dem.py reads its planes with read_particle_data: the CSV path
     gives the same system, step for step; the VTK writer sees every local"""
