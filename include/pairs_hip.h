@@ -158,6 +158,15 @@ int pairs_hip_euler(pairs_hip_ctx *ctx);
 int pairs_hip_thermo(pairs_hip_ctx *ctx, double out[5]);
 /* per-atom energies and totals of the current list (no reference counterpart) */
 int pairs_hip_energy(pairs_hip_ctx *ctx, double *pe, double *virial);
+/* pairs_hip_energy's totals without allocating device memory (no counterpart
+ * in the reference): for callers that cannot call hipMalloc (tight serving
+ * loops, harnesses that forbid it).  No per-atom energies are kept: the
+ * per-wave partials go to a static device array of 393 KB per GPU, so the
+ * context may hold at most 1048576 locals (more: PAIRS_HIP_ERR_ARG, with the
+ * reason in pairs_hip_last_error).  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other threads, other streams) race.  Same
+ * kernels and summation order as pairs_hip_energy, so the same bits. */
+int pairs_hip_energy_noalloc(pairs_hip_ctx *ctx, double *pe, double *virial);
 
 /* One iteration ts of the reference loop (src/pairs/sim/timestep.py:9-72). */
 int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps);

@@ -19,6 +19,7 @@ struct pairs_hip_ctx {
     DevState d;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int device = -1;           // the device current at creation
     std::string err;
     int req[5] = {0, 0, 0, 0, 0};
     int h_counters[C_COUNT];

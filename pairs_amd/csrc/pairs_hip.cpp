@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,38 @@
 #include "../../include/pairs_hip_mlp.h"
 #include "lj_ctx.hpp"
 #include "mlp_model.hpp"
+
+// Scratch of pairs_hip_energy_noalloc: the per-wave energy and virial
+// partials (rows 1 and 2 of the thermo layout, NOALLOC_WAVES each) and the
+// reduced row.  One copy per GPU, held until the process ends; host code
+// uses only the device address resolved per device (g_noalloc_ptr), never
+// the symbol itself.
+constexpr int NOALLOC_WAVES = 16384;  // locals <= 64 * NOALLOC_WAVES = 1048576
+constexpr int NOALLOC_DOUBLES = 3 * NOALLOC_WAVES + 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static double *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {nullptr};
+
+// resolve this device's copy of the scratch array (once per device, with the
+// device current); 0 or PAIRS_HIP_ERR_HIP
+static int noalloc_resolve(int dev, std::string &err) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        err = "device ordinal beyond the scratch table";
+        return PAIRS_HIP_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[dev]) return 0;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return PAIRS_HIP_ERR_HIP;
+    }
+    g_noalloc_ptr[dev] = (double *) ptr;
+    return 0;
+}
 
 static void drop_reb_graphs(pairs_hip_ctx *ctx) {
     for (int p = 0; p < 2; p++) {
@@ -275,9 +308,11 @@ int pairs_hip_create(pairs_hip_ctx **out, const pairs_hip_md_config *cfg) {
         ctx->own_stream = true;
     }
     int dev = 0, lds = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+    if (hipGetDevice(&dev) != hipSuccess) return fail(PAIRS_HIP_ERR_HIP);
+    ctx->device = dev;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
         ctx->max_lds = lds;
+    if (noalloc_resolve(dev, ctx->err)) return fail(PAIRS_HIP_ERR_HIP);
     DevState &d = ctx->d;
     d.max_lds = ctx->max_lds;
     if (const char *e = getenv("PAIRS_HIP_HOME")) ctx->home_target = std::max(8, atoi(e));
@@ -834,6 +869,37 @@ int pairs_hip_energy(pairs_hip_ctx *ctx, double *pe, double *virial) {
     launch_thermo_reduce(ctx->stream, ctx->d, 0, 2, ctx->geom, ctx->d.thermo);
     CHECK_LAUNCH();
     if (download_raw(ctx, out, ctx->d.thermo, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;
+    if (pe) *pe = out[2];
+    if (virial) *virial = out[3];
+    return 0;
+}
+
+int pairs_hip_energy_noalloc(pairs_hip_ctx *ctx, double *pe, double *virial) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    const int nw = force_waves(ctx->d);
+    if (nw > NOALLOC_WAVES) {
+        ctx->err = "pairs_hip_energy_noalloc: " + std::to_string(ctx->d.nlocal) +
+                   " locals exceed its scratch (at most " + std::to_string(64 * NOALLOC_WAVES) + ")";
+        return PAIRS_HIP_ERR_ARG;
+    }
+    double *ptr = ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES ? g_noalloc_ptr[ctx->device] : nullptr;
+    if (!ptr) {
+        ctx->err = "pairs_hip_energy_noalloc: the scratch address of this device is not resolved";
+        return PAIRS_HIP_ERR_STATE;
+    }
+    // the same kernels as pairs_hip_energy, without the per-atom energies and
+    // with the partials in the scratch array: every partial the reduction
+    // reads is written first by the energy pass (one per launched wave), so
+    // nothing needs clearing
+    DevState d = ctx->d;
+    d.partials = ptr;
+    d.pstride = NOALLOC_WAVES;
+    double *row = ptr + 3 * (size_t) NOALLOC_WAVES;
+    launch_energy(ctx->stream, d, ctx->geom, ctx->prm, ctx->cfg.half, nullptr);
+    launch_thermo_reduce(ctx->stream, d, 0, 2, ctx->geom, row);
+    CHECK_LAUNCH();
+    double out[5];
+    if (download_raw(ctx, out, row, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;
     if (pe) *pe = out[2];
     if (virial) *virial = out[3];
     return 0;

@@ -83,6 +83,7 @@ _SIGNATURES = {
     "pairs_hip_euler": (_I, [_P]),
     "pairs_hip_thermo": (_I, [_P, _DP]),
     "pairs_hip_energy": (_I, [_P, _DP, _DP]),
+    "pairs_hip_energy_noalloc": (_I, [_P, _DP, _DP]),
     "pairs_hip_step": (_I, [_P, _I, _I]),
     "pairs_hip_run": (_I, [_P, _I, _I, _I, _DP, _I]),
     "pairs_hip_sync": (_I, [_P]),
