@@ -145,8 +145,9 @@ def run_mlp(steps, warmup, train_steps, force_steps):
     temperatures in [0.8, 2.0], two snapshots 50 steps apart each), Adam on
     minibatches of 4096 rows; (2) configurations with forces -- five plain-LJ
     systems of 2048 atoms at the same temperatures each advance 50 steps
-    between Adam steps, and every step takes the gradient of the energy term
-    plus the relative squared force error of all five configurations
+    between Adam steps (velocities rescaled to their temperature every 500
+    steps), and every step takes the gradient of 0.1 x the energy term plus
+    the relative squared force error of all five configurations
     (force_steps x 5 snapshots sampled every 50 steps)."""
     from pairs_amd import md, mlp
 
@@ -171,12 +172,12 @@ def run_mlp(steps, warmup, train_steps, force_steps):
     if force_steps > 0:
         sims = []
         for T in MLP_TEMPS:
-            s = md.fcc_system(MLP_CONFIG_NX, temp=T)
+            s = md.fcc_system(MLP_CONFIG_NX, temp=T, particle_capacity=16384, send_capacity=16384)
             s.run(0, 200, 10 ** 9)
             sims.append(s)
-        _, hist = mlp.train_on_configurations(m, sims, force_steps, every=50, ts0=200, lr=1e-3, w_energy=1.0,
+        _, hist = mlp.train_on_configurations(m, sims, force_steps, every=50, ts0=200, lr=1e-3, w_energy=0.1,
                                               w_force=1.0, total_steps=10 ** 9,
-                                              record_every=max(force_steps // 10, 1))
+                                              record_every=max(force_steps // 10, 1), temps=MLP_TEMPS)
         for s in sims:
             s.close()
     t_force = time.perf_counter() - t0
@@ -425,9 +426,9 @@ def main():
             "measured_copy_GBps": copy_gbps,
             "frac_of_measured_copy": (achieved / copy_gbps) if (achieved and copy_gbps) else None,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, scaled by a "
-                              "512 MiB copy (profiles/lj_fcc32_force_pmc.md)",
-            "kernel": "k_lj<1,1,false,false,false,true> (LJ force + final/initial integrate + next packed positions, "
-                      "fused; one lane per local, 64 locals per wave)",
+                              "1 GiB device copy (profiles/lj_fcc32_force_pmc.md)",
+            "kernel": "k_lj2<1,1,false,false,true> (LJ force + final/initial integrate + next packed positions, "
+                      "fused; two lanes per local, 32 locals per wave)",
             "timing": "HIP events around every 5th force launch of a separate pass of 200 steps after the timed "
                       "steps (eager launches on the context stream)",
             "bytes_per_launch": fused_bytes,

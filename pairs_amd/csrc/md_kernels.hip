@@ -916,8 +916,175 @@ __global__ __launch_bounds__(64) void k_lj(LJArgs A) {
     lj_wave<NT, MODE, RED, HALF, OLDER, UNIT>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
+// The 64-bit pattern of lane 2p + 1 in lane 2p (DPP quad_perm 1,1,3,3 on
+// both 32-bit halves; the odd lanes receive lane 2p + 1's own value).
+__device__ __forceinline__ double odd_lane(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int) b, 0xF5, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), 0xF5, 0xF, 0xF, false);
+    return __longlong_as_double(((long long) hi << 32) | (unsigned int) lo);
+}
+
+// The force kernel of the timestep (MODE 0: forces; MODE 1: fused with
+// final/initial integrate and the next positions) on TWO lanes per local: a
+// wave takes 32 consecutive cell-sorted locals, lane 2p + h of local p
+// evaluates list entries 4s + h and 4s + 2 + h of step s (one 16-byte group
+// of four rows per step), and lane 2p adds the pair terms in list order --
+// its own entry, then lane 2p + 1's, received by DPP -- so every sum is the
+// reference's, bit for bit, as in lj_wave.  Twice the waves of lj_wave hide
+// the gather latency better; the positions of step s + 1 are gathered while
+// step s computes (two register sets, a loop body without branches: a last
+// odd step is computed fully masked, which adds +0).
+// HALF: the own part (rows [0, n)) into T and the mirror part (rows
+// [rev0, rev0 + nrev)) into R, force = R + T, as lj_wave.
+// RED: kinetic energy of the locals (lane 2p) and energy / virial of every
+// lane's pairs (full lists: weight 1/2; half lists: 1 for local pairs and 1/2
+// for local-ghost pairs, own part only) into one partial per wave; the sums
+// over a wave's lanes are taken in lane order, so these totals may differ
+// from lj_wave's in the last bits.
+template <int NT, int MODE, bool RED, bool HALF, bool UNIT>
+__device__ __forceinline__ void lj_wave2(const LJArgs &A, int wv) {
+    constexpr int U = 2;  // entries per lane per step (4 per local)
+    const int lane = threadIdx.x & 63, h = lane & 1;
+    const int *__restrict__ counters = A.counters;
+    const int nlocal = counters[C_NLOCAL];
+    const uint32_t jmax = (uint32_t) max(nlocal + counters[C_NGHOST] - 1, 0);
+    const double *__restrict__ X = A.X, *__restrict__ Y = A.Y, *__restrict__ Z = A.Z;
+    const int *__restrict__ type = A.type;
+    const char *__restrict__ P = (const char *) A.P;
+    const double rc2 = A.rc2, dt = A.dt, hdt = A.hdt;
+    const LJTab &tab = A.tab;
+    __shared__ double s_eps[16], s_sig6[16];
+    if (NT != 1) {
+        if (lane < 16) { s_eps[lane] = tab.eps[lane]; s_sig6[lane] = tab.sig6[lane]; }
+        __syncthreads();
+    }
+    const int i0 = wv * 32;
+    double ke = 0.0, pe = 0.0, vir = 0.0;
+    if (i0 < nlocal) {  // wave-uniform
+        const int i = i0 + (lane >> 1);
+        const bool live = i < nlocal;
+        const int ii = live ? i : i0;
+        const int fl = live ? A.flags[ii] : FLAG_FIXED;
+        const bool moves = !(fl & FLAG_FIXED);
+        const double xi = X[ii], yi = Y[ii], zi = Z[ii];
+        const int ti = NT != 1 ? type[ii] : 0;
+        const double eps0 = tab.eps[0], sig0 = tab.sig6[0];
+        const int nt = tab.nt;
+        const int stride = A.nl_stride;
+        double u = 0.0, w = 0.0;
+        struct Pos { double x[U], y[U], z[U]; int t[U]; uint32_t j[U]; };
+        auto gather = [&](const uint32_t (&j)[U], Pos &p) {
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const uint32_t off = j[q] << 5;
+                const double2 v = *(const double2 *) (P + off);
+                p.x[q] = v.x;
+                p.y[q] = v.y;
+                p.z[q] = *(const double *) (P + off + 16);
+                if (NT != 1) p.t[q] = type[j[q]];
+                p.j[q] = j[q];
+            }
+        };
+        // one list part, rows [row0, row0 + cnt), summed in order into (a0, a1, a2) of lane 2p
+        auto walk = [&](int row0, int cnt, bool mirror, double &a0, double &a1, double &a2) {
+            const int nmax = __builtin_amdgcn_readfirstlane(wave_max(cnt));
+            const int nst = (nmax + 2 * U - 1) / (2 * U);
+            if (nst <= 0) return;  // wave-uniform
+            const uint4 *lp = (const uint4 *) A.nl + (size_t) (row0 >> 2) * stride + ii;
+            const int gl = max((cnt + 3) / 4 - 1, 0);
+            auto ldl = [&](int st, uint32_t (&j)[U]) {
+                const uint4 g = lp[(size_t) min(st, gl) * stride];
+                j[0] = min(h ? g.y : g.x, jmax);
+                j[1] = min(h ? g.w : g.z, jmax);
+            };
+            auto compute = [&](int st, const Pos &p) {
+#pragma unroll
+                for (int q = 0; q < U; q++) {
+                    double sg6 = sig0, ep = eps0;
+                    if (NT != 1) { const int kt = ti * nt + p.t[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
+                    double g0, g1, g2, uu, ww;
+                    lj_pair<RED, false, UNIT>(xi, yi, zi, p.x[q], p.y[q], p.z[q], 4 * st + 2 * q + h < cnt, rc2, sg6,
+                                              ep, g0, g1, g2, uu, ww);
+                    // lane 2p: entry 4s + 2q (its own), then 4s + 2q + 1 (lane 2p + 1's)
+                    a0 = a0 + g0; a1 = a1 + g1; a2 = a2 + g2;
+                    a0 = a0 + odd_lane(g0); a1 = a1 + odd_lane(g1); a2 = a2 + odd_lane(g2);
+                    if (RED && !HALF) { u += uu; w += ww; }
+                    if (RED && HALF && !mirror) {
+                        const double wt = p.j[q] < (uint32_t) nlocal ? 2.0 : 1.0;
+                        u += wt * uu; w += wt * ww;
+                    }
+                }
+            };
+            uint32_t LA[U], LB[U];
+            Pos PA, PB;
+            ldl(0, LA);
+            ldl(1, LB);
+            gather(LA, PA);
+            ldl(2, LA);
+            for (int st = 0; st < nst; st += 2) {
+                gather(LB, PB);
+                ldl(st + 3, LB);
+                compute(st, PA);
+                gather(LA, PA);
+                ldl(st + 4, LA);
+                compute(st + 1, PB);
+            }
+        };
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0, r0 = 0.0, r1 = 0.0, r2 = 0.0;
+        // counts past the list rows (a build never writes them) are clamped
+        walk(0, moves ? min(A.numneighs[ii], A.ncap) : 0, false, t0, t1, t2);
+        if (HALF) walk(A.rev0, moves ? min(A.nrev[ii], A.ncap) : 0, true, r0, r1, r2);
+        if (RED) { pe = 0.5 * u; vir = 0.5 * w; }
+        if (live && h == 0) {
+            const double f0 = r0 + t0, f1 = r1 + t1, f2 = r2 + t2;
+            A.fx[i] = f0; A.fy[i] = f1; A.fz[i] = f2;
+            if (MODE == 1) {
+                const double m = A.mass[i];
+                double v0 = A.vx[i], v1 = A.vy[i], v2 = A.vz[i];
+                double px = xi, py = yi, pz = zi;
+                if (moves && A.do_vv2) {  // final_integrate(ts)
+                    double b0 = hdt * f0, b1 = hdt * f1, b2 = hdt * f2;
+                    b0 = b0 / m; b1 = b1 / m; b2 = b2 / m;
+                    v0 = v0 + b0; v1 = v1 + b1; v2 = v2 + b2;
+                }
+                if (RED) ke = m * (v0 * v0 + v1 * v1 + v2 * v2);
+                if (moves) {
+                    if (A.do_vv1) {  // initial_integrate(ts+1)
+                        double b0 = hdt * f0, b1 = hdt * f1, b2 = hdt * f2;
+                        b0 = b0 / m; b1 = b1 / m; b2 = b2 / m;
+                        v0 = v0 + b0; v1 = v1 + b1; v2 = v2 + b2;
+                        px = xi + dt * v0; py = yi + dt * v1; pz = zi + dt * v2;
+                    }
+                    A.vx[i] = v0; A.vy[i] = v1; A.vz[i] = v2;
+                }
+                if (A.do_vv1) {
+                    A.Xn[i] = px; A.Yn[i] = py; A.Zn[i] = pz;
+                    if (A.Pn) A.Pn[i] = make_double4(px, py, pz, 0.0);
+                    if (A.im.on) write_images(A.im, i, nlocal, px, py, pz, X, Y, Z, A.Xn, A.Yn, A.Zn, A.g, A.Pn);
+                }
+            }
+        }
+    }
+    if (RED) {
+        // one partial per wave (kinetic row from the fused kernel only)
+        const double a = wave_sum(ke), bb = wave_sum(pe), cc = wave_sum(vir);
+        if (lane == 0) {
+            if (MODE == 1) A.partials[wv] = a;
+            A.partials[(size_t) A.pstride + wv] = bb;
+            A.partials[(size_t) 2 * A.pstride + wv] = cc;
+        }
+    }
+}
+
+template <int NT, int MODE, bool RED, bool HALF, bool UNIT>
+__global__ __launch_bounds__(64) void k_lj2(LJArgs A) {
+    lj_wave2<NT, MODE, RED, HALF, UNIT>(A, xcd_block(blockIdx.x, gridDim.x));
+}
+
 // the waves of many independent systems in one launch: block b belongs to
-// system rep[b] and is its wave b - wave_base[rep[b]]
+// system rep[b] and is its wave b - wave_base[rep[b]] (one lane per local, 64
+// locals per wave)
 template <int NT, int MODE, bool RED, bool UNIT>
 __global__ __launch_bounds__(64) void k_lj_batch(const LJArgs *__restrict__ desc, const int *__restrict__ rep,
                                                  const int *__restrict__ wave_base) {
@@ -1094,7 +1261,7 @@ __global__ __launch_bounds__(256) void k_thermo_reduce(const double *__restrict_
 // (kinetic energy per segment): out[5 r + k], one workgroup per system
 __global__ __launch_bounds__(256) void k_thermo_reduce_batch(const RepDesc *__restrict__ D, double *__restrict__ out) {
     const RepDesc &R = D[blockIdx.x];
-    const int nw = (R.d.nlocal + 63) / 64;  // force waves of the system
+    const int nw = (R.d.nlocal + 63) / 64;  // waves of the batched force kernel (lj_wave)
     thermo_reduce_body(R.d.partials, R.d.pstride, nw, nw, 3, R.d.nlocal, R.g.len[0], R.g.len[1], R.g.len[2],
                        out + 5 * (size_t) blockIdx.x);
 }
@@ -1737,6 +1904,7 @@ static LJTab make_tab(const LJParams &prm) {
 }
 
 int force_waves(const DevState &d) { return (d.nlocal + 63) / 64; }
+int lj_waves(const DevState &d) { return (d.nlocal + 31) / 32; }
 
 // image refresh only when the kernel writes next positions into the other
 // buffer and every local's images fit the table
@@ -1772,9 +1940,15 @@ template <int NT, int MODE, bool RED, bool UNIT>
 static void lj_launch(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
                       int do_vv2, int do_vv1, double *eatom) {
     const LJArgs A = lj_args(d, g, prm, cur, next, do_vv2, do_vv1, MODE == 1, eatom);
-    const dim3 grid(force_waves(d));
-    if (d.rev0 > 0) hipLaunchKernelGGL((k_lj<NT, MODE, RED, true, false, UNIT>), grid, dim3(64), 0, s, A);
-    else hipLaunchKernelGGL((k_lj<NT, MODE, RED, false, false, UNIT>), grid, dim3(64), 0, s, A);
+    if (MODE == 2) {  // energy and per-atom energies: one lane per local
+        const dim3 grid(force_waves(d));
+        if (d.rev0 > 0) hipLaunchKernelGGL((k_lj<NT, MODE, RED, true, false, UNIT>), grid, dim3(64), 0, s, A);
+        else hipLaunchKernelGGL((k_lj<NT, MODE, RED, false, false, UNIT>), grid, dim3(64), 0, s, A);
+    } else {
+        const dim3 grid(lj_waves(d));
+        if (d.rev0 > 0) hipLaunchKernelGGL((k_lj2<NT, MODE, RED, true, UNIT>), grid, dim3(64), 0, s, A);
+        else hipLaunchKernelGGL((k_lj2<NT, MODE, RED, false, UNIT>), grid, dim3(64), 0, s, A);
+    }
 }
 
 void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *wave_base, int nblocks,
@@ -1866,8 +2040,9 @@ void launch_ke_partial(hipStream_t s, const DevState &d) {
                        d.nlocal, d.partials);
 }
 
-void launch_thermo_reduce(hipStream_t s, const DevState &d, int nb_ke, int which_mask, const Geom &g, double *out) {
-    hipLaunchKernelGGL(k_thermo_reduce, dim3(1), dim3(256), 0, s, d.partials, d.pstride, nb_ke, force_waves(d),
+void launch_thermo_reduce(hipStream_t s, const DevState &d, int nb_ke, int nb_f, int which_mask, const Geom &g,
+                          double *out) {
+    hipLaunchKernelGGL(k_thermo_reduce, dim3(1), dim3(256), 0, s, d.partials, d.pstride, nb_ke, nb_f,
                        which_mask, d.nlocal, g.len[0], g.len[1], g.len[2], out);
 }
 

@@ -138,6 +138,8 @@ void launch_fill_i32(hipStream_t s, int *p, int v, int n);
 void launch_pack_p(hipStream_t s, const DevState &d, int cur);
 // waves of the force kernels: 64 locals each
 int force_waves(const DevState &d);
+// waves of the fused LJ force kernel (two lanes per local: 32 locals per wave)
+int lj_waves(const DevState &d);
 // the older API's modules (examples/lj_onetype.py): forces with its pair
 // expression (full lists), and its Euler update in the current buffer
 void launch_lj_older(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm);
@@ -204,7 +206,10 @@ void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm);
 void launch_vv_post(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
                     int do_vv2, int do_vv1, int reduce);
 void launch_ke_partial(hipStream_t s, const DevState &d);
-void launch_thermo_reduce(hipStream_t s, const DevState &d, int nblocks, int which_mask, const Geom &g, double *out);
+// thermo row from the partials: kinetic row over nb_ke entries, energy and
+// virial rows over nb_f entries
+void launch_thermo_reduce(hipStream_t s, const DevState &d, int nb_ke, int nb_f, int which_mask, const Geom &g,
+                          double *out);
 
 // every argument of the full-list force kernel (one system)
 struct LJArgs {

@@ -87,7 +87,7 @@ static void dfree(void *p) {
 // the stand-alone integrator)
 static size_t pstride_of(const DevState &d, const Geom &g) {
     (void) g;
-    return (size_t) d.cap / 64 + 8;
+    return (size_t) d.cap / 32 + 8;
 }
 
 static void free_particle_arrays(DevState &d) {
@@ -866,7 +866,7 @@ int pairs_hip_energy(pairs_hip_ctx *ctx, double *pe, double *virial) {
     if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
     double out[5];
     launch_energy(ctx->stream, ctx->d, ctx->geom, ctx->prm, ctx->cfg.half, ctx->d_eatom);
-    launch_thermo_reduce(ctx->stream, ctx->d, 0, 2, ctx->geom, ctx->d.thermo);
+    launch_thermo_reduce(ctx->stream, ctx->d, 0, force_waves(ctx->d), 2, ctx->geom, ctx->d.thermo);
     CHECK_LAUNCH();
     if (download_raw(ctx, out, ctx->d.thermo, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;
     if (pe) *pe = out[2];
@@ -896,7 +896,7 @@ int pairs_hip_energy_noalloc(pairs_hip_ctx *ctx, double *pe, double *virial) {
     d.pstride = NOALLOC_WAVES;
     double *row = ptr + 3 * (size_t) NOALLOC_WAVES;
     launch_energy(ctx->stream, d, ctx->geom, ctx->prm, ctx->cfg.half, nullptr);
-    launch_thermo_reduce(ctx->stream, d, 0, 2, ctx->geom, row);
+    launch_thermo_reduce(ctx->stream, d, 0, nw, 2, ctx->geom, row);
     CHECK_LAUNCH();
     double out[5];
     if (download_raw(ctx, out, row, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;
@@ -910,7 +910,8 @@ int pairs_hip_thermo(pairs_hip_ctx *ctx, double out[5]) {
     if (ensure_eatom(ctx)) return PAIRS_HIP_ERR_HIP;
     launch_energy(ctx->stream, ctx->d, ctx->geom, ctx->prm, ctx->cfg.half, ctx->d_eatom);
     launch_ke_partial(ctx->stream, ctx->d);
-    launch_thermo_reduce(ctx->stream, ctx->d, (ctx->d.nlocal + 255) / 256, 3, ctx->geom, ctx->d.thermo);
+    launch_thermo_reduce(ctx->stream, ctx->d, (ctx->d.nlocal + 255) / 256, force_waves(ctx->d), 3, ctx->geom,
+                         ctx->d.thermo);
     CHECK_LAUNCH();
     return download_raw(ctx, out, ctx->d.thermo, 5 * sizeof(double));
 }
@@ -1012,6 +1013,7 @@ static bool overflow_before_permute(const int *h) { return h[C_OVF] == OVF_SEND 
 
 static int rebuild(pairs_hip_ctx *ctx) {
     if (ctx) ctx->xy_valid[0] = ctx->xy_valid[1] = false;
+    int last = 0;
     for (int attempt = 0; attempt < 8; attempt++) {
         // one counter read for the whole rebuild: after an overflow it is
         // repeated with grown capacities
@@ -1028,6 +1030,7 @@ static int rebuild(pairs_hip_ctx *ctx) {
             if (ctx->mlp_on) code = ensure_mlp_work(ctx);
             if (code <= 0) return code;
         }
+        last = code;
         int pc = 0, nc = 0, sc = 0;
         if (code == PAIRS_HIP_OVERFLOW_PARTICLES) pc = 2 * ctx->req[code];
         if (code == PAIRS_HIP_OVERFLOW_NEIGHBORS) nc = 2 * ctx->req[code];
@@ -1035,7 +1038,10 @@ static int rebuild(pairs_hip_ctx *ctx) {
         int r = pairs_hip_reserve(ctx, pc, nc, sc);
         if (r) return r;
     }
-    ctx->err = "capacity growth did not converge";
+    ctx->err = "capacity growth did not converge: overflow " + std::to_string(last) + " (required " +
+               std::to_string(last > 0 && last < 5 ? ctx->req[last] : 0) + "; capacities: particles " +
+               std::to_string(ctx->d.cap) + ", neighbours " + std::to_string(ctx->d.ncap) + ", send " +
+               std::to_string(ctx->d.send_cap) + ", tile " + std::to_string(ctx->d.tile_cap) + ")";
     return PAIRS_HIP_ERR_STATE;
 }
 
@@ -1218,8 +1224,11 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         }
         if (timed) ctx->ev_used++;
         if (T && nthermo < 4096) {
-            const int nb_ke = ctx->mlp_on ? (d.nlocal + 255) / 256 : force_waves(d);
-            launch_thermo_reduce(s, d, nb_ke, 3, ctx->geom, d.thermo + 5 * nthermo);
+            // partials of the fused LJ kernel (one per wave), or of the MLP
+            // force kernel (one per wave of 64) and the integrator (per 256)
+            const int nb_ke = ctx->mlp_on ? (d.nlocal + 255) / 256 : lj_waves(d);
+            const int nb_f = ctx->mlp_on ? force_waves(d) : lj_waves(d);
+            launch_thermo_reduce(s, d, nb_ke, nb_f, 3, ctx->geom, d.thermo + 5 * nthermo);
             nthermo++;
         }
         CHECK_LAUNCH();
@@ -1304,7 +1313,7 @@ int pairs_hip_mlp_forces(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double *energy, d
     if (int r = mlp_prepare(ctx, m)) return r;
     DevState &d = ctx->d;
     launch_mlp_forces(ctx->stream, d, ctx->geom, mlp_bind(ctx, m), d.cur, 1, ctx->ghost_grid);
-    launch_thermo_reduce(ctx->stream, d, 0, 2, ctx->geom, d.thermo);
+    launch_thermo_reduce(ctx->stream, d, 0, force_waves(d), 2, ctx->geom, d.thermo);
     CHECK_LAUNCH();
     double out[5];
     if (download_raw(ctx, out, d.thermo, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;

@@ -225,21 +225,26 @@ def collect_lj_snapshots(model, make_system, temps, *, equilibrate=200, every=50
 
 
 def train_on_configurations(model, sims, steps, *, every=50, ts0=0, lr=2e-3, lr_end_frac=0.05, w_energy=1.0,
-                            w_force=1.0, total_steps=None, record_every=0):
+                            w_force=1.0, total_steps=None, record_every=0, temps=None, rescale_every=10):
     """Adam on configurations of plain-LJ runs: before each Adam step every
     system advances `every` timesteps of its own LJ run, then the gradient of
     its configuration loss is accumulated; the step uses their mean.  The
-    learning rate decays linearly from lr to lr * lr_end_frac.  Returns the
-    next timestep of the systems and, every `record_every` steps (0: never),
-    (step, mean energy term, mean force term) over the systems."""
+    learning rate decays linearly from lr to lr * lr_end_frac.  With temps
+    (one per system), every `rescale_every` Adam steps each system's
+    velocities are rescaled to its temperature (adjust_thermo), which keeps
+    long NVE runs sampling at that temperature.  Returns the next timestep of
+    the systems and, every `record_every` steps (0: never), (step, mean energy
+    term, mean force term) over the systems."""
     ts = ts0
     total = total_steps if total_steps is not None else ts0 + every * steps
     hist = []
     for k in range(steps):
         rec = record_every > 0 and k % record_every == 0
         le = lf = 0.0
-        for sim in sims:
+        for i, sim in enumerate(sims):
             sim.run(ts, ts + every, total)
+            if temps is not None and k % rescale_every == rescale_every - 1:
+                sim.adjust_thermo(temps[i])
             _, (a, b) = model.config_grad(sim, w_energy, w_force, want_grad=False) if rec else \
                 (None, (0.0, 0.0))
             if not rec:
