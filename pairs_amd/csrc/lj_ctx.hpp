@@ -28,14 +28,16 @@ struct pairs_hip_ctx {
     int max_lds = 65536;       // shared memory a workgroup may allocate
     int zc_override = 0;
     int home_target = 40;      // home particles a segment aims at (sets zc; about 3 cells at rho 0.84)
-    // graphs of one non-rebuild step, by parity of the current buffer
+    // graphs of 2^k consecutive non-rebuild steps (k < STEP_CHAINS), by
+    // parity of the current buffer at their first step
+    static constexpr int STEP_CHAINS = 5;
     bool use_graphs = true;
-    hipGraphExec_t step_graph[2] = {nullptr, nullptr};
+    hipGraphExec_t step_graph[2][STEP_CHAINS] = {};
+    int graph_ghost_grid[2][STEP_CHAINS] = {};
+    int graph_img[2][STEP_CHAINS] = {};
     // a rebuild (exchange, borders, cell binning, neighbour build), by
     // position buffer
     hipGraphExec_t reb_graph[2] = {nullptr, nullptr};
-    int graph_ghost_grid[2] = {0, 0};
-    int graph_img[2] = {0, 0};
     // force-kernel timing
     bool time_force = false;
     int time_every = 10;

@@ -64,10 +64,11 @@ static void drop_reb_graphs(pairs_hip_ctx *ctx) {
 }
 
 static void drop_graphs(pairs_hip_ctx *ctx) {
-    for (int p = 0; p < 2; p++) {
-        if (ctx->step_graph[p]) hipGraphExecDestroy(ctx->step_graph[p]);
-        ctx->step_graph[p] = nullptr;
-    }
+    for (int p = 0; p < 2; p++)
+        for (int k = 0; k < pairs_hip_ctx::STEP_CHAINS; k++) {
+            if (ctx->step_graph[p][k]) hipGraphExecDestroy(ctx->step_graph[p][k]);
+            ctx->step_graph[p][k] = nullptr;
+        }
     drop_reb_graphs(ctx);
 }
 
@@ -1134,39 +1135,50 @@ int pairs_hip_force_kernel_time(pairs_hip_ctx *ctx, double *avg_ms, int *launche
     return 0;
 }
 
-static bool step_graph_stale(const pairs_hip_ctx *ctx, int cur) {
-    return !ctx->step_graph[cur] || ctx->graph_ghost_grid[cur] < ctx->ghost_grid || ctx->graph_img[cur] != ctx->d.img_on;
+static bool step_graph_stale(const pairs_hip_ctx *ctx, int cur, int k) {
+    return !ctx->step_graph[cur][k] || ctx->graph_ghost_grid[cur][k] < ctx->ghost_grid ||
+           ctx->graph_img[cur][k] != ctx->d.img_on;
 }
 
-// capture (without running) the steady step of buffer cur when missing or stale
-static int capture_step_graph(pairs_hip_ctx *ctx, int cur) {
-    if (step_graph_stale(ctx, cur)) {
-        if (ctx->step_graph[cur]) hipGraphExecDestroy(ctx->step_graph[cur]);
-        ctx->step_graph[cur] = nullptr;
+// capture (without running) 2^k steady steps from buffer cur when missing or
+// stale: the buffers alternate from step to step, so the chain is the single
+// step's kernels repeated with the parities swapped
+static int capture_step_graph(pairs_hip_ctx *ctx, int cur, int k) {
+    if (step_graph_stale(ctx, cur, k)) {
+        hipGraphExec_t &slot = ctx->step_graph[cur][k];
+        if (slot) hipGraphExecDestroy(slot);
+        slot = nullptr;
         // capture with headroom so that later rebuilds with a few more ghosts reuse it
         const int saved = ctx->ghost_grid;
         ctx->ghost_grid = saved + saved / 4 + 4;
         hipGraph_t g;
         HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        launch_fused_step(ctx, cur, 1, 1, 0, false);
+        for (int q = 0; q < (1 << k); q++) launch_fused_step(ctx, cur ^ (q & 1), 1, 1, 0, false);
         HIP_OK(hipStreamEndCapture(ctx->stream, &g));
-        hipError_t e = hipGraphInstantiate(&ctx->step_graph[cur], g, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
-        ctx->graph_ghost_grid[cur] = ctx->ghost_grid;
-        ctx->graph_img[cur] = ctx->d.img_on;
+        ctx->graph_ghost_grid[cur][k] = ctx->ghost_grid;
+        ctx->graph_img[cur][k] = ctx->d.img_on;
         ctx->ghost_grid = saved;
         if (e != hipSuccess) {
             ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
-            ctx->step_graph[cur] = nullptr;
+            slot = nullptr;
             return PAIRS_HIP_ERR_HIP;
         }
     }
     return 0;
 }
 
-static int graph_step(pairs_hip_ctx *ctx, int cur) {
-    if (int r = capture_step_graph(ctx, cur)) return r;
-    HIP_OK(hipGraphLaunch(ctx->step_graph[cur], ctx->stream));
+// n >= 1 steady steps from buffer cur, as chains of 2^k steps (longest first)
+static int graph_steps(pairs_hip_ctx *ctx, int cur, int n) {
+    while (n > 0) {
+        int k = pairs_hip_ctx::STEP_CHAINS - 1;
+        while ((1 << k) > n) k--;
+        if (int r = capture_step_graph(ctx, cur, k)) return r;
+        HIP_OK(hipGraphLaunch(ctx->step_graph[cur][k], ctx->stream));
+        cur ^= (1 << k) & 1;
+        n -= 1 << k;
+    }
     return 0;
 }
 
@@ -1177,7 +1189,8 @@ int pairs_hip_prepare_graphs(pairs_hip_ctx *ctx) {
     for (int b = 0; b < 2; b++) {
         if (int r = capture_rebuild_graph(ctx, b)) return r;
         if (ctx->h_counters[C_NLOCAL] > 0 && ctx->d.nl && !ctx->mlp_on)
-            if (int r = capture_step_graph(ctx, b)) return r;
+            for (int k = 0; k < pairs_hip_ctx::STEP_CHAINS; k++)
+                if (int r = capture_step_graph(ctx, b, k)) return r;
     }
     return 0;
 }
@@ -1198,12 +1211,23 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             ctx->xy_valid[d.cur] = false;
         }
         const int vv2 = ts > 0, vv1 = (ts + 1 < ts_end) && (ts + 1 <= nsteps);
-        const bool timed = ctx->time_force && !R && ((ts % ctx->time_every) == 0) && ctx->ev_used < 4096;
-        if (!R && !T && vv2 && vv1 && prev_in_other && ctx->use_graphs && !timed) {
-            int r = graph_step(ctx, d.cur);
-            if (r) return r;
-            d.cur = 1 - d.cur;
+        auto timed_at = [&](int t) {
+            return ctx->time_force && !every(t, ctx->cfg.reneighbor_every) && ((t % ctx->time_every) == 0) &&
+                   ctx->ev_used < 4096;
+        };
+        const bool timed = timed_at(ts);
+        auto steady = [&](int t) {
+            return !every(t, ctx->cfg.reneighbor_every) && !every(t, ctx->cfg.thermo_every) && t > 0 &&
+                   t + 1 < ts_end && t + 1 <= nsteps && !timed_at(t);
+        };
+        if (prev_in_other && ctx->use_graphs && steady(ts)) {
+            // this step and the steady ones after it replay as chains
+            int n = 1;
+            while (steady(ts + n)) n++;
+            if (int r = graph_steps(ctx, d.cur, n)) return r;
+            d.cur ^= n & 1;
             ctx->xy_valid[d.cur] = !ctx->mlp_on;  // the replayed LJ kernel wrote it
+            ts += n - 1;
             continue;
         }
         if (R) {
