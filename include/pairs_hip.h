@@ -76,7 +76,9 @@ enum {
     PAIRS_HIP_NLOCAL = 0, PAIRS_HIP_NGHOST = 1, PAIRS_HIP_NCELLS = 2, PAIRS_HIP_NSEND_ALL = 3,
     PAIRS_HIP_NEIGHBOR_CAPACITY = 4, PAIRS_HIP_PARTICLE_CAPACITY = 5, PAIRS_HIP_SEND_CAPACITY = 6,
     PAIRS_HIP_NEIGHBOR_TOTAL = 7,
-    PAIRS_HIP_TILE_CAPACITY = 8   /* cell-tile entries a force workgroup stages (grows at rebuilds) */
+    PAIRS_HIP_TILE_CAPACITY = 8,  /* cell-tile entries a force workgroup stages (grows at rebuilds) */
+    PAIRS_HIP_REBUILDS_DEFERRED = 9,  /* rebuilds of pairs_hip_run whose counters were checked later */
+    PAIRS_HIP_REBUILD_ROLLBACKS = 10  /* of those, repeated after an overflow (state restored)      */
 };
 
 typedef struct pairs_hip_md_config {
@@ -180,6 +182,15 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
 int pairs_hip_sync(pairs_hip_ctx *ctx);
 /* 1: replay hipGraphs in pairs_hip_run (default), 0: launch eagerly */
 int pairs_hip_set_graphs(pairs_hip_ctx *ctx, int enable);
+/* Counter checks of pairs_hip_run's rebuilds (1, default: deferred; 0: after
+ * every rebuild).  Replaces the reference's per-module copy of its resize
+ * counters to the host and the re-run on an overflow (src/pairs/
+ * transformations/devices.py:66-79, modules.py:159-205): with 1 a rebuild's
+ * counters are copied behind it into pinned memory and checked at the next
+ * rebuild or at the end of the run, so the host does not wait for the device
+ * at every rebuild; an overflow restores the state checkpointed before that
+ * rebuild and repeats it with grown capacities, so results are those of 0. */
+int pairs_hip_set_speculative_rebuilds(pairs_hip_ctx *ctx, int enable);
 /* Set-up step with no reference counterpart (the reference launches every
  * module eagerly): capture and instantiate, without running them, the graphs
  * pairs_hip_run replays -- the steady step and the rebuild of both position

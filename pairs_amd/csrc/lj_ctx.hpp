@@ -38,6 +38,20 @@ struct pairs_hip_ctx {
     // a rebuild (exchange, borders, cell binning, neighbour build), by
     // position buffer
     hipGraphExec_t reb_graph[2] = {nullptr, nullptr};
+    // Rebuilds of pairs_hip_run whose counters are checked later: the
+    // locals' state is checkpointed, the rebuild and the steps after it are
+    // enqueued without waiting for its counters, which are copied to pinned
+    // memory behind it and checked at the next rebuild or at the end of the
+    // run; an overflow restores the checkpoint and repeats the rebuild with
+    // the host loop (and capacity growth).
+    bool spec_enabled = true;
+    bool spec_pending = false;
+    int spec_ts = 0, spec_cur = 0, spec_nthermo = 0, spec_ev_used = 0;
+    int *h_spec = nullptr;          // pinned, C_COUNT
+    hipEvent_t spec_ev = nullptr;   // behind the counter copy
+    double *ckpt = nullptr;         // checkpoint arrays, ckpt_n locals
+    int ckpt_n = 0;
+    long long spec_count = 0, spec_rollbacks = 0;
     // force-kernel timing
     bool time_force = false;
     int time_every = 10;

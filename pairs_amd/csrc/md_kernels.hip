@@ -37,6 +37,22 @@ __global__ __launch_bounds__(256) void k_gather_counters(const int *const *__res
     if (t < n * C_COUNT) dst[t] = src[t / C_COUNT][t % C_COUNT];
 }
 
+// The per-particle state of the first n locals that a rebuild changes in
+// place (positions of the buffer it starts from, velocities, mass, type,
+// flags, uid, shape, reference index by slot and slot by reference index)
+// and the counters, copied from a to b: a checkpoint before a rebuild whose
+// counters are checked later, and the restore when they show an overflow.
+__global__ __launch_bounds__(256) void k_checkpoint(CkptArrays a, CkptArrays b, int n) {
+    if (blockIdx.x == 0 && threadIdx.x < C_COUNT) b.counters[threadIdx.x] = a.counters[threadIdx.x];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        b.X[i] = a.X[i]; b.Y[i] = a.Y[i]; b.Z[i] = a.Z[i];
+        b.vx[i] = a.vx[i]; b.vy[i] = a.vy[i]; b.vz[i] = a.vz[i];
+        b.mass[i] = a.mass[i];
+        b.type[i] = a.type[i]; b.flags[i] = a.flags[i]; b.uid[i] = a.uid[i]; b.shape[i] = a.shape[i];
+        b.ref[i] = a.ref[i]; b.phys[i] = a.phys[i];
+    }
+}
+
 // exclusive scan of one value per thread over a block of BS threads
 template <int BS>
 __device__ __forceinline__ int block_exclusive_scan(int v, int *lds_waves, int *total) {
@@ -1342,16 +1358,19 @@ __device__ __forceinline__ void k_sel_scan_body(int *__restrict__ counters, int 
     __syncthreads();
     if (threadIdx.x == 0) {
         const int n0 = carry[0], n1 = carry[1];
+        const int nall = base + n0 + n1;
+        if (nall > send_cap) {
+            // the counts stay within the maps (steps may still run on them
+            // before the host sees the overflow)
+            counters[C_OVF] = OVF_SEND;
+            counters[C_REQ_SEND] = max(counters[C_REQ_SEND], nall);
+            return;
+        }
         counters[C_BASE0] = base;
         counters[C_BASE1] = base + n0;
         counters[C_NSEND + 2 * dim] = n0;
         counters[C_NSEND + 2 * dim + 1] = n1;
-        const int nall = base + n0 + n1;
         counters[C_NSEND_ALL] = nall;
-        if (nall > send_cap) {
-            counters[C_OVF] = OVF_SEND;
-            counters[C_REQ_SEND] = max(counters[C_REQ_SEND], nall);
-        }
         if (!exch) {
             const int nghost = counters[C_NGHOST] + n0 + n1;
             if (counters[C_NLOCAL] + nghost > cap) {
@@ -1600,6 +1619,10 @@ void launch_identity(hipStream_t s, int *p, int n) {
 }
 void launch_set_counters(hipStream_t s, int *c, int first, int count, int v) {
     hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, s, c, first, count, v);
+}
+
+void launch_checkpoint(hipStream_t s, const CkptArrays &a, const CkptArrays &b, int n) {
+    hipLaunchKernelGGL(k_checkpoint, dim3(std::max(1, std::min((n + 255) / 256, 2048))), dim3(256), 0, s, a, b, n);
 }
 
 void launch_gather_counters(hipStream_t s, const int *const *src, int n, int *dst) {

@@ -148,6 +148,13 @@ void launch_fill_f64(hipStream_t s, double *p, double v, int n);
 void launch_set_counters(hipStream_t s, int *c, int first, int count, int v);
 // dst[r * C_COUNT + k] = src[r][k] for r < n (device array of counter pointers)
 void launch_gather_counters(hipStream_t s, const int *const *src, int n, int *dst);
+// the locals' state a rebuild changes in place, and the counters
+struct CkptArrays {
+    double *X, *Y, *Z, *vx, *vy, *vz, *mass;
+    int *type, *flags, *uid, *shape, *ref, *phys, *counters;
+};
+// copy the first n locals' entries (and all counters) from a to b
+void launch_checkpoint(hipStream_t s, const CkptArrays &a, const CkptArrays &b, int n);
 void launch_identity(hipStream_t s, int *p, int n);
 void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int *periodic);
 void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *periodic);

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -351,6 +352,9 @@ int pairs_hip_destroy(pairs_hip_ctx *ctx) {
     dfree(ctx->d_eatom); dfree(ctx->d_expand);
     dfree(ctx->mlp_G); dfree(ctx->mlp_D); dfree(ctx->mlp_E); dfree(ctx->mlp_f); dfree(ctx->mlp_red);
     dfree(ctx->mlp_rho); dfree(ctx->mlp_U);
+    dfree(ctx->ckpt);
+    if (ctx->h_spec) hipHostFree(ctx->h_spec);
+    if (ctx->spec_ev) hipEventDestroy(ctx->spec_ev);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -576,6 +580,8 @@ int pairs_hip_count(pairs_hip_ctx *ctx, int what) {
         case PAIRS_HIP_PARTICLE_CAPACITY: return ctx->d.cap;
         case PAIRS_HIP_SEND_CAPACITY: return ctx->d.send_cap;
         case PAIRS_HIP_TILE_CAPACITY: return ctx->d.tile_cap;
+        case PAIRS_HIP_REBUILDS_DEFERRED: return (int) std::min(ctx->spec_count, (long long) INT_MAX);
+        case PAIRS_HIP_REBUILD_ROLLBACKS: return (int) std::min(ctx->spec_rollbacks, (long long) INT_MAX);
         case PAIRS_HIP_NEIGHBOR_TOTAL: {
             const int n = ctx->d.nlocal;
             std::vector<int> nn(n);
@@ -1046,6 +1052,98 @@ static int rebuild(pairs_hip_ctx *ctx) {
     return PAIRS_HIP_ERR_STATE;
 }
 
+// ---- rebuilds checked later (pairs_hip_run)
+// The reference copies its resize counters to the host after every module
+// and re-runs the module on an overflow (src/pairs/transformations/
+// devices.py:66-79, modules.py:159-205).  Overflows are rare after the first
+// rebuild, so pairs_hip_run does not wait for them: it checkpoints the locals'
+// state, enqueues the rebuild, a copy of its counters into pinned memory and
+// the steps after it, and checks the copy at the next rebuild or at the end of
+// the run.  Every kernel stays in bounds after an overflow (counts are clamped
+// to capacities and later rebuild kernels return at once), so the steps run
+// on a faulty rebuild only compute values that are thrown away: the
+// checkpoint is restored and the rebuild repeated with the host loop.
+
+static CkptArrays ckpt_live(DevState &d, int cur) {
+    return CkptArrays{d.X[cur], d.Y[cur], d.Z[cur], d.vx, d.vy, d.vz, d.mass, d.type, d.flags,
+                      d.uid, d.shape, d.ref, d.phys, d.counters};
+}
+
+static CkptArrays ckpt_saved(pairs_hip_ctx *ctx) {
+    const size_t n = (size_t) ctx->ckpt_n;
+    double *c = ctx->ckpt;
+    int *k = (int *) (c + 7 * n);
+    return CkptArrays{c, c + n, c + 2 * n, c + 3 * n, c + 4 * n, c + 5 * n, c + 6 * n,
+                      k, k + n, k + 2 * n, k + 3 * n, k + 4 * n, k + 5 * n, k + 6 * n};
+}
+
+static int ensure_spec(pairs_hip_ctx *ctx) {
+    if (!ctx->h_spec) HIP_OK(hipHostMalloc((void **) &ctx->h_spec, sizeof(int) * C_COUNT));
+    if (!ctx->spec_ev) HIP_OK(hipEventCreateWithFlags(&ctx->spec_ev, hipEventDisableTiming));
+    const int n = ctx->d.nlocal;
+    if (ctx->ckpt_n < n) {
+        HIP_OK(hipStreamSynchronize(ctx->stream));
+        dfree(ctx->ckpt);
+        ctx->ckpt_n = 0;
+        // 7 doubles and 6 ints per local, then the counters
+        const size_t words = 7 * (size_t) n + (6 * (size_t) n + C_COUNT + 1) / 2;
+        if (dalloc(ctx, &ctx->ckpt, words)) return PAIRS_HIP_ERR_HIP;
+        ctx->ckpt_n = n;
+    }
+    return 0;
+}
+
+static int spec_rebuild(pairs_hip_ctx *ctx, int ts, int nthermo) {
+    if (int r = ensure_spec(ctx)) return r;
+    DevState &d = ctx->d;
+    ctx->xy_valid[0] = ctx->xy_valid[1] = false;
+    launch_checkpoint(ctx->stream, ckpt_live(d, d.cur), ckpt_saved(ctx), d.nlocal);
+    ctx->spec_cur = d.cur;
+    ctx->spec_ts = ts;
+    ctx->spec_nthermo = nthermo;
+    ctx->spec_ev_used = ctx->ev_used;
+    if (int r = launch_rebuild(ctx, ctx->stream)) return r;  // flips d.cur
+    HIP_OK(hipMemcpyAsync(ctx->h_spec, d.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(hipEventRecord(ctx->spec_ev, ctx->stream));
+    CHECK_LAUNCH();
+    ctx->spec_pending = true;
+    ctx->spec_count++;
+    return 0;
+}
+
+// 0: the pending rebuild was clean; 1: it overflowed (or left a local with
+// more images than the table holds) and must be repeated
+static int spec_check(pairs_hip_ctx *ctx) {
+    ctx->spec_pending = false;
+    HIP_OK(hipEventSynchronize(ctx->spec_ev));
+    std::memcpy(ctx->h_counters, ctx->h_spec, sizeof(int) * C_COUNT);
+    const int *h = ctx->h_counters;
+    if (h[C_OVF] != 0 || h[C_REQ_NEIGH] > 0 || h[C_IMG_OVF] != 0) return 1;
+    take_counters(ctx);
+    set_tile_cap(ctx);
+    return 0;
+}
+
+// back to the state before the pending rebuild: nothing enqueued after it
+// runs any more, the locals' state and the counters are restored
+static int spec_rollback(pairs_hip_ctx *ctx) {
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    DevState &d = ctx->d;
+    d.cur = ctx->spec_cur;
+    ctx->xy_valid[0] = ctx->xy_valid[1] = false;
+    launch_checkpoint(ctx->stream, ckpt_saved(ctx), ckpt_live(d, d.cur), d.nlocal);
+    CHECK_LAUNCH();
+    HIP_OK(hipStreamSynchronize(ctx->stream));
+    ctx->spec_rollbacks++;
+    return 0;
+}
+
+int pairs_hip_set_speculative_rebuilds(pairs_hip_ctx *ctx, int enable) {
+    if (!ctx) return PAIRS_HIP_ERR_ARG;
+    ctx->spec_enabled = enable != 0;
+    return 0;
+}
+
 int pairs_hip_step(pairs_hip_ctx *ctx, int ts, int nsteps) {
     if (!ctx || ts < 0 || ts > nsteps) return PAIRS_HIP_ERR_ARG;
     DevState &d = ctx->d;
@@ -1203,10 +1301,25 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
     bool prev_in_other = false;  // previous ghosts live in the other buffer
     ctx->ev_used = 0;
     if (ctx->time_force && ensure_events(ctx, 4096)) return PAIRS_HIP_ERR_HIP;
-    for (int ts = ts_begin; ts < ts_end; ts++) {
+    bool redo = false;  // repeating a rebuild step whose deferred check failed
+    for (int ts = ts_begin;; ts++) {
+        // a rebuild still unchecked is checked before the next one and at the
+        // end; when it overflowed, the run continues from its step again
+        if (ctx->spec_pending && (ts >= ts_end || every(ts, ctx->cfg.reneighbor_every))) {
+            const int c = spec_check(ctx);
+            if (c < 0) return c;
+            if (c > 0) {
+                if (int r = spec_rollback(ctx)) return r;
+                ts = ctx->spec_ts;
+                nthermo = ctx->spec_nthermo;
+                ctx->ev_used = ctx->spec_ev_used;
+                redo = true;
+            }
+        }
+        if (ts >= ts_end) break;
         const bool R = every(ts, ctx->cfg.reneighbor_every);
         const bool T = every(ts, ctx->cfg.thermo_every);
-        if (ts == ts_begin && ts > 0) {
+        if (ts == ts_begin && ts > 0 && !redo) {  // (a repeat starts from the checkpoint, taken after it)
             launch_vv_initial(s, d, ctx->prm);
             ctx->xy_valid[d.cur] = false;
         }
@@ -1231,8 +1344,10 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             continue;
         }
         if (R) {
-            int r = rebuild(ctx);
+            const bool spec = !redo && ctx->spec_enabled && ctx->use_graphs && !ctx->mlp_on && d.img_on;
+            int r = spec ? spec_rebuild(ctx, ts, nthermo) : rebuild(ctx);
             if (r) return r;
+            redo = false;
             prev_in_other = false;
         }
         const int cur = d.cur, other = 1 - cur;

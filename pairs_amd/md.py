@@ -206,6 +206,10 @@ class LJSystem:
     def set_graphs(self, enable):
         self._call("pairs_hip_set_graphs", int(bool(enable)))
 
+    def set_speculative_rebuilds(self, enable):
+        """Deferred counter checks of run()'s rebuilds (default on)."""
+        self._call("pairs_hip_set_speculative_rebuilds", int(bool(enable)))
+
     def prepare_graphs(self):
         """Capture the step and rebuild graphs of both buffers now (set-up)."""
         self._call("pairs_hip_prepare_graphs")

@@ -18,8 +18,8 @@ ERR_HIP, ERR_ARG, ERR_STATE = -1, -2, -3
 # download selectors
 POSITION, VELOCITY, FORCE, MASS, TYPE, FLAGS, UID, NUMNEIGHS, NEIGHBORLISTS, PARTICLE_CELL, \
     CELL_SIZES, CELL_PARTICLES, SEND_MAP, SEND_MULT, NSEND, ATOM_ENERGY = range(16)
-NLOCAL, NGHOST, NCELLS, NSEND_ALL, NEIGHBOR_CAPACITY, PARTICLE_CAPACITY, SEND_CAPACITY, NEIGHBOR_TOTAL, TILE_CAPACITY = \
-    range(9)
+NLOCAL, NGHOST, NCELLS, NSEND_ALL, NEIGHBOR_CAPACITY, PARTICLE_CAPACITY, SEND_CAPACITY, NEIGHBOR_TOTAL, TILE_CAPACITY, \
+    REBUILDS_DEFERRED, REBUILD_ROLLBACKS = range(11)
 
 FLAG_INFINITE, FLAG_GHOST, FLAG_FIXED, FLAG_GLOBAL = 1, 2, 4, 8
 
@@ -88,6 +88,7 @@ _SIGNATURES = {
     "pairs_hip_run": (_I, [_P, _I, _I, _I, _DP, _I]),
     "pairs_hip_sync": (_I, [_P]),
     "pairs_hip_set_graphs": (_I, [_P, _I]),
+    "pairs_hip_set_speculative_rebuilds": (_I, [_P, _I]),
     "pairs_hip_prepare_graphs": (_I, [_P]),
     "pairs_hip_set_force_timing": (_I, [_P, _I]),
     "pairs_hip_force_kernel_time": (_I, [_P, _DP, _IP]),

@@ -158,10 +158,35 @@ def test_overflow_at_a_later_rebuild(which):
     assert cap0 == (nsend0 if which == "send" else nall0)  # ts = 0 fitted
     g.run(1, 61, 60)
     assert g.count(kind) > cap0  # grown at a later rebuild
+    # run() checks a rebuild's counters only at the next rebuild or at the
+    # end: the overflowing rebuild ran ahead and was repeated from its
+    # checkpoint
+    assert g.count(nat.REBUILDS_DEFERRED) >= 2 and g.count(nat.REBUILD_ROLLBACKS) >= 1
     o = po.fcc_system(4, rho=0.55)
     o.run(0, 61)
     _compare_state(g, o, tag=f"overflow {which}")
     _compare_lists(g, o, tag=f"overflow {which}")
+
+
+@pytest.mark.parametrize("half", [False, True])
+def test_deferred_rebuild_checks_match_immediate_ones(half):
+    """Rebuild counters checked at the next rebuild (default) or right after
+    each rebuild: bit-identical trajectories, lists and maps over runs split
+    at arbitrary steps, with and without an overflow that forces a repeat."""
+    md = _md()
+    from pairs_amd import native as nat
+    res = []
+    for spec in (True, False):
+        g = md.fcc_system(6, half=half, neighbor_capacity=40)
+        g.set_speculative_rebuilds(spec)
+        for a, b in ((0, 7), (7, 45), (45, 46), (46, 130)):
+            g.run(a, b, 129)
+        res.append(g)
+    a, b = res
+    assert a.count(nat.REBUILDS_DEFERRED) > 0 and b.count(nat.REBUILDS_DEFERRED) == 0
+    assert a.count(nat.NEIGHBOR_CAPACITY) == b.count(nat.NEIGHBOR_CAPACITY) > 40
+    _compare_state(a, b, tag=f"deferred half={half}")
+    _compare_lists(a, b, tag=f"deferred half={half}")
 
 
 @pytest.mark.parametrize("half", [False, True])
