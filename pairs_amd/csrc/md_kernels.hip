@@ -127,11 +127,15 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ X,
     k_bin_count_body(X, Y, Z, flags, counters, g, cell, lcount, gcount, slot);
 }
 
-// exclusive scans of the local and ghost cell counts -> lstart, gstart [0..ncells]
-__device__ __forceinline__ void k_scan_cells_body(const int *__restrict__ counters, const int *__restrict__ lcount,
-                                                     const int *__restrict__ gcount, int *__restrict__ lstart,
+// exclusive scans of the local and ghost cell counts -> lstart, gstart
+// [0..ncells]; the counts are cleared behind the reads (nothing else uses
+// them, so the next binning starts from zeros).  The flags the rest of the
+// rebuild raises by atomics (images, largest tile) are cleared here too.
+__device__ __forceinline__ void k_scan_cells_body(int *__restrict__ counters, int *__restrict__ lcount,
+                                                     int *__restrict__ gcount, int *__restrict__ lstart,
                                                      int *__restrict__ gstart, int ncells) {
     __shared__ int lds[20];
+    if (threadIdx.x == 0) { counters[C_IMG_OVF] = 0; counters[C_TILE_MAX] = 0; }
     if (counters[C_OVF]) return;
     // each thread owns a contiguous run of cells (up to PT per round, all
     // loads issued together); one block scan of the run totals per round
@@ -148,6 +152,11 @@ __device__ __forceinline__ void k_scan_cells_body(const int *__restrict__ counte
             const int c = c0 + k;
             vl[k] = c < c1 ? lcount[c] : 0;
             vg[k] = c < c1 ? gcount[c] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int c = c0 + k;
+            if (c < c1) { lcount[c] = 0; gcount[c] = 0; }
         }
         int sl = 0, sg = 0;
 #pragma unroll
@@ -168,8 +177,8 @@ __device__ __forceinline__ void k_scan_cells_body(const int *__restrict__ counte
     if (threadIdx.x == 0) { lstart[ncells] = carry_l; gstart[ncells] = carry_g; }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_cells(const int *__restrict__ counters, const int *__restrict__ lcount,
-                                                     const int *__restrict__ gcount, int *__restrict__ lstart,
+__global__ __launch_bounds__(1024) void k_scan_cells(int *__restrict__ counters, int *__restrict__ lcount,
+                                                     int *__restrict__ gcount, int *__restrict__ lstart,
                                                      int *__restrict__ gstart, int ncells) {
     k_scan_cells_body(counters, lcount, gcount, lstart, gstart, ncells);
 }
@@ -284,51 +293,32 @@ __global__ __launch_bounds__(256) void k_permute(const int *__restrict__ counter
     k_permute_body(counters, perm, X, Y, Z, Xn, Yn, Zn, vx, vy, vz, mass, type, flags, uid, shape, ref, cell, avx, avy, avz, amass, atype, aflags, auid, ashape, aref, acell, phys);
 }
 
-__device__ __forceinline__ void k_copy_back_body(const int *__restrict__ counters, const double *__restrict__ avx,
-                                                   const double *__restrict__ avy, const double *__restrict__ avz,
-                                                   const double *__restrict__ amass, const int *__restrict__ atype,
-                                                   const int *__restrict__ aflags, const int *__restrict__ auid,
-                                                   const int *__restrict__ ashape, const int *__restrict__ aref,
-                                                   const int *__restrict__ acell, double *__restrict__ vx,
-                                                   double *__restrict__ vy, double *__restrict__ vz,
-                                                   double *__restrict__ mass, int *__restrict__ type,
-                                                   int *__restrict__ flags, int *__restrict__ uid,
-                                                   int *__restrict__ shape, int *__restrict__ ref,
-                                                   int *__restrict__ cell) {
+// Copy the permuted arrays back (array addresses stay fixed, so captured
+// graphs remain valid across rebuilds), clear the locals' image counts, and
+// build the ghost refresh maps in physical order: the physical slot of each
+// ghost's source and its image multipliers (send_map / send_mult are in
+// reference ghost order: ghost g = reference index nlocal + g, read from the
+// permuted copy of ref).
+__device__ __forceinline__ void k_copy_back_maps_body(
+    const int *__restrict__ counters, const double *__restrict__ avx, const double *__restrict__ avy,
+    const double *__restrict__ avz, const double *__restrict__ amass, const int *__restrict__ atype,
+    const int *__restrict__ aflags, const int *__restrict__ auid, const int *__restrict__ ashape,
+    const int *__restrict__ aref, const int *__restrict__ acell, double *__restrict__ vx, double *__restrict__ vy,
+    double *__restrict__ vz, double *__restrict__ mass, int *__restrict__ type, int *__restrict__ flags,
+    int *__restrict__ uid, int *__restrict__ shape, int *__restrict__ ref, int *__restrict__ cell,
+    const int *__restrict__ phys, const int *__restrict__ send_map, const int *__restrict__ send_mult,
+    int *__restrict__ sync_src, int *__restrict__ sync_mult, int *__restrict__ img_cnt) {
     if (counters[C_OVF]) return;
-    const int n = counters[C_NLOCAL] + counters[C_NGHOST];
+    const int nlocal = counters[C_NLOCAL], ng = counters[C_NGHOST];
+    const int n = nlocal + ng;
     for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
         vx[p] = avx[p]; vy[p] = avy[p]; vz[p] = avz[p]; mass[p] = amass[p];
         type[p] = atype[p]; flags[p] = aflags[p]; uid[p] = auid[p]; shape[p] = ashape[p];
         ref[p] = aref[p]; cell[p] = acell[p];
+        if (p < nlocal) img_cnt[p] = 0;
     }
-}
-
-__global__ __launch_bounds__(256) void k_copy_back(const int *__restrict__ counters, const double *__restrict__ avx,
-                                                   const double *__restrict__ avy, const double *__restrict__ avz,
-                                                   const double *__restrict__ amass, const int *__restrict__ atype,
-                                                   const int *__restrict__ aflags, const int *__restrict__ auid,
-                                                   const int *__restrict__ ashape, const int *__restrict__ aref,
-                                                   const int *__restrict__ acell, double *__restrict__ vx,
-                                                   double *__restrict__ vy, double *__restrict__ vz,
-                                                   double *__restrict__ mass, int *__restrict__ type,
-                                                   int *__restrict__ flags, int *__restrict__ uid,
-                                                   int *__restrict__ shape, int *__restrict__ ref,
-                                                   int *__restrict__ cell) {
-    k_copy_back_body(counters, avx, avy, avz, amass, atype, aflags, auid, ashape, aref, acell, vx, vy, vz, mass, type, flags, uid, shape, ref, cell);
-}
-
-// Ghost refresh maps in physical order: the physical slot of each ghost's
-// source and its image multipliers (send_map / send_mult are in reference
-// ghost order: ghost g = reference index nlocal + g).
-__device__ __forceinline__ void k_sync_maps_body(const int *__restrict__ counters, const int *__restrict__ ref,
-                                                   const int *__restrict__ phys, const int *__restrict__ send_map,
-                                                   const int *__restrict__ send_mult, int *__restrict__ sync_src,
-                                                   int *__restrict__ sync_mult) {
-    if (counters[C_OVF]) return;
-    const int nlocal = counters[C_NLOCAL], ng = counters[C_NGHOST];
     for (int q = blockIdx.x * 256 + threadIdx.x; q < ng; q += gridDim.x * 256) {
-        const int g = ref[nlocal + q] - nlocal;
+        const int g = aref[nlocal + q] - nlocal;
         sync_src[q] = phys[send_map[g]];
         sync_mult[3 * q + 0] = send_mult[3 * g + 0];
         sync_mult[3 * q + 1] = send_mult[3 * g + 1];
@@ -336,13 +326,18 @@ __device__ __forceinline__ void k_sync_maps_body(const int *__restrict__ counter
     }
 }
 
-__global__ __launch_bounds__(256) void k_sync_maps(const int *__restrict__ counters, const int *__restrict__ ref,
-                                                   const int *__restrict__ phys, const int *__restrict__ send_map,
-                                                   const int *__restrict__ send_mult, int *__restrict__ sync_src,
-                                                   int *__restrict__ sync_mult) {
-    k_sync_maps_body(counters, ref, phys, send_map, send_mult, sync_src, sync_mult);
+__global__ __launch_bounds__(256) void k_copy_back_maps(
+    const int *__restrict__ counters, const double *__restrict__ avx, const double *__restrict__ avy,
+    const double *__restrict__ avz, const double *__restrict__ amass, const int *__restrict__ atype,
+    const int *__restrict__ aflags, const int *__restrict__ auid, const int *__restrict__ ashape,
+    const int *__restrict__ aref, const int *__restrict__ acell, double *__restrict__ vx, double *__restrict__ vy,
+    double *__restrict__ vz, double *__restrict__ mass, int *__restrict__ type, int *__restrict__ flags,
+    int *__restrict__ uid, int *__restrict__ shape, int *__restrict__ ref, int *__restrict__ cell,
+    const int *__restrict__ phys, const int *__restrict__ send_map, const int *__restrict__ send_mult,
+    int *__restrict__ sync_src, int *__restrict__ sync_mult, int *__restrict__ img_cnt) {
+    k_copy_back_maps_body(counters, avx, avy, avz, amass, atype, aflags, auid, ashape, aref, acell, vx, vy, vz, mass,
+                          type, flags, uid, shape, ref, cell, phys, send_map, send_mult, sync_src, sync_mult, img_cnt);
 }
-
 
 // A segment's tile as physical slots: entry p of the tile is particle
 // tslot[p] (the force kernels gather positions from HBM through it, so the
@@ -1299,15 +1294,24 @@ __device__ __forceinline__ void ghost_sel(double x, int fl, int dim, const Geom 
 }
 
 // pass 1 of the stable two-sided compaction over reference indices: per-tile counts
+// The exchange of a dimension selects among the locals and appends at send
+// map entry 0; the borders of the first periodic dimension start with no
+// ghosts (comm.py:105-118), those of a later one select among locals and the
+// ghosts so far and append behind them.  The range and the base are kept for
+// pass 2 (nothing writes them while pass 1 runs).
 __device__ __forceinline__ void k_sel_count_body(const double *__restrict__ Xd, const int *__restrict__ flags,
                                                    const int *__restrict__ phys, int *__restrict__ counters, Geom g,
-                                                   int dim, int exch, int *__restrict__ tile_cnt,
+                                                   int dim, int exch, int first, int *__restrict__ tile_cnt,
                                                    int *__restrict__ exchg_flag) {
     __shared__ int wc[2][4];
     if (counters[C_OVF]) return;
-    const int n = exch ? counters[C_NLOCAL] : counters[C_NLOCAL] + counters[C_NGHOST];
+    const bool fresh = exch || first;
+    const int n = counters[C_NLOCAL] + (fresh ? 0 : counters[C_NGHOST]);
     const int ntiles = (n + 255) / 256;
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[C_RANGE] = n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[C_RANGE] = n;
+        counters[C_BASE0] = fresh ? 0 : counters[C_NSEND_ALL];
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int r = t * 256 + threadIdx.x;
@@ -1330,87 +1334,82 @@ __device__ __forceinline__ void k_sel_count_body(const double *__restrict__ Xd, 
 
 __global__ __launch_bounds__(256) void k_sel_count(const double *__restrict__ Xd, const int *__restrict__ flags,
                                                    const int *__restrict__ phys, int *__restrict__ counters, Geom g,
-                                                   int dim, int exch, int *__restrict__ tile_cnt,
+                                                   int dim, int exch, int first, int *__restrict__ tile_cnt,
                                                    int *__restrict__ exchg_flag) {
-    k_sel_count_body(Xd, flags, phys, counters, g, dim, exch, tile_cnt, exchg_flag);
+    k_sel_count_body(Xd, flags, phys, counters, g, dim, exch, first, tile_cnt, exchg_flag);
 }
 
-// pass 2: exclusive scan of the tile counts (side 0 entries precede side 1),
-// counts per direction, capacity checks.
-__device__ __forceinline__ void k_sel_scan_body(int *__restrict__ counters, int *__restrict__ tile_cnt, int dim,
-                                                   int exch, int send_cap, int cap) {
-    __shared__ int lds[20];
-    if (counters[C_OVF]) return;
-    const int n = counters[C_RANGE];
-    const int ntiles = (n + 255) / 256;
-    const int base = exch ? 0 : counters[C_NSEND_ALL];
-    int carry[2] = {0, 0};
-    for (int side = 0; side < 2; side++) {
-        for (int t0 = 0; t0 < ntiles; t0 += 1024) {
-            const int t = t0 + threadIdx.x;
-            const int v = t < ntiles ? tile_cnt[2 * t + side] : 0;
-            int total;
-            const int ex = block_exclusive_scan<1024>(v, lds, &total);
-            if (t < ntiles) tile_cnt[2 * t + side] = carry[side] + ex;
-            carry[side] += total;
-        }
+// sums of two per-thread values over a workgroup of 256 threads
+__device__ __forceinline__ void block_sum2_256(int a, int b, int *lds, int &sa, int &sb) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const int n0 = carry[0], n1 = carry[1];
-        const int nall = base + n0 + n1;
-        if (nall > send_cap) {
-            // the counts stay within the maps (steps may still run on them
-            // before the host sees the overflow)
-            counters[C_OVF] = OVF_SEND;
-            counters[C_REQ_SEND] = max(counters[C_REQ_SEND], nall);
-            return;
-        }
-        counters[C_BASE0] = base;
-        counters[C_BASE1] = base + n0;
-        counters[C_NSEND + 2 * dim] = n0;
-        counters[C_NSEND + 2 * dim + 1] = n1;
-        counters[C_NSEND_ALL] = nall;
-        if (!exch) {
-            const int nghost = counters[C_NGHOST] + n0 + n1;
-            if (counters[C_NLOCAL] + nghost > cap) {
-                counters[C_OVF] = OVF_PART;
-                counters[C_REQ_PART] = max(counters[C_REQ_PART], counters[C_NLOCAL] + nghost);
-            } else {
-                counters[C_NGHOST] = nghost;
-            }
-        }
-    }
+    if (lane == 0) { lds[2 * wave] = a; lds[2 * wave + 1] = b; }
+    __syncthreads();
+    sa = lds[0] + lds[2] + lds[4] + lds[6];
+    sb = lds[1] + lds[3] + lds[5] + lds[7];
 }
 
-__global__ __launch_bounds__(1024) void k_sel_scan(int *__restrict__ counters, int *__restrict__ tile_cnt, int dim,
-                                                   int exch, int send_cap, int cap) {
-    k_sel_scan_body(counters, tile_cnt, dim, exch, send_cap, cap);
-}
-
-// pass 3: stable scatter into the send map (reference indices); for borders
+// pass 2: stable scatter into the send map (reference indices); for borders
 // the ghost is created in the same pass (pack + communicateData self-copy +
 // unpack of comm.py:291-366: uid, type, mass, position + shift, velocity,
 // shape) at physical slot = reference index nlocal + g.  Ghost flags are not
 // sent by the reference; a ghost slot's flags read 0 (zeroed memory).
+// Side 0 entries precede side 1.  Every workgroup sums the tile counts of
+// pass 1 itself (totals, and its tiles' offsets: the counts of the tiles
+// before them), so the offsets need no pass of their own; all workgroups
+// see the same totals and take the same capacity decision, and workgroup 0
+// updates the counts, which no other workgroup of this pass reads.
 __device__ __forceinline__ void k_sel_scatter_body(double *__restrict__ X, double *__restrict__ Y,
                                                      double *__restrict__ Z, double *__restrict__ vx,
                                                      double *__restrict__ vy, double *__restrict__ vz,
                                                      double *__restrict__ mass, int *__restrict__ type,
                                                      int *__restrict__ uid, int *__restrict__ shape,
                                                      int *__restrict__ flags, int *__restrict__ ref,
-                                                     int *__restrict__ phys, const int *__restrict__ counters, Geom g,
-                                                     int dim, int exch, const int *__restrict__ tile_off,
-                                                     int *__restrict__ send_map, int *__restrict__ send_mult) {
+                                                     int *__restrict__ phys, int *__restrict__ counters, Geom g,
+                                                     int dim, int exch, const int *__restrict__ tile_cnt,
+                                                     int *__restrict__ send_map, int *__restrict__ send_mult,
+                                                     int send_cap, int cap) {
     __shared__ int wc[2][5];
+    __shared__ int red[8];
     if (counters[C_OVF]) return;
     const int n = counters[C_RANGE];
     const int ntiles = (n + 255) / 256;
+    if (blockIdx.x > 0 && (int) blockIdx.x >= ntiles) return;  // workgroup 0 always writes the counts
     const int nlocal = counters[C_NLOCAL];
-    const int base1 = counters[C_BASE1] - counters[C_BASE0];
+    const int base = counters[C_BASE0];
+    int t0 = 0, t1 = 0;
+    for (int u = threadIdx.x; u < ntiles; u += 256) { t0 += tile_cnt[2 * u]; t1 += tile_cnt[2 * u + 1]; }
+    int T0, T1;
+    block_sum2_256(t0, t1, red, T0, T1);
+    const int nall = base + T0 + T1;
+    const bool ovf_send = nall > send_cap, ovf_part = !exch && nlocal + nall > cap;
+    if (ovf_send || ovf_part) {
+        // the counts stay within the maps (steps may still run on them
+        // before the host sees the overflow)
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (ovf_send) {
+                counters[C_OVF] = OVF_SEND;
+                counters[C_REQ_SEND] = max(counters[C_REQ_SEND], nall);
+            }
+            if (ovf_part) {
+                counters[C_OVF] = OVF_PART;
+                counters[C_REQ_PART] = max(counters[C_REQ_PART], nlocal + nall);
+            }
+        }
+        return;
+    }
     const double *Xd = dim == 0 ? X : (dim == 1 ? Y : Z);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int q0 = 0, q1 = 0;
+        for (int u = threadIdx.x; u < t; u += 256) { q0 += tile_cnt[2 * u]; q1 += tile_cnt[2 * u + 1]; }
+        int P0, P1;
+        block_sum2_256(q0, q1, red, P0, P1);
         const int r = t * 256 + threadIdx.x;
         bool s0 = false, s1 = false;
         int p = 0;
@@ -1426,8 +1425,7 @@ __device__ __forceinline__ void k_sel_scatter_body(double *__restrict__ X, doubl
         if (s0 || s1) {
             const int side = s0 ? 0 : 1;
             const int rk = side == 0 ? lane_rank(m0) : lane_rank(m1);
-            const int pos = counters[C_BASE0] + (side == 0 ? tile_off[2 * t] : base1 + tile_off[2 * t + 1]) +
-                            wc[side][wave] + rk;
+            const int pos = base + (side == 0 ? P0 : T0 + P1) + wc[side][wave] + rk;
             const int mult = side == 0 ? 1 : -1;  // pbc_prev = +1, pbc_next = -1 on one rank
             const int mx = dim == 0 ? mult : 0, my = dim == 1 ? mult : 0, mz = dim == 2 ? mult : 0;
             send_map[pos] = r;
@@ -1448,6 +1446,12 @@ __device__ __forceinline__ void k_sel_scatter_body(double *__restrict__ X, doubl
         }
         __syncthreads();
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[C_NSEND + 2 * dim] = T0;
+        counters[C_NSEND + 2 * dim + 1] = T1;
+        counters[C_NSEND_ALL] = nall;
+        if (!exch) counters[C_NGHOST] = nall;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sel_scatter(double *__restrict__ X, double *__restrict__ Y,
@@ -1456,10 +1460,12 @@ __global__ __launch_bounds__(256) void k_sel_scatter(double *__restrict__ X, dou
                                                      double *__restrict__ mass, int *__restrict__ type,
                                                      int *__restrict__ uid, int *__restrict__ shape,
                                                      int *__restrict__ flags, int *__restrict__ ref,
-                                                     int *__restrict__ phys, const int *__restrict__ counters, Geom g,
-                                                     int dim, int exch, const int *__restrict__ tile_off,
-                                                     int *__restrict__ send_map, int *__restrict__ send_mult) {
-    k_sel_scatter_body(X, Y, Z, vx, vy, vz, mass, type, uid, shape, flags, ref, phys, counters, g, dim, exch, tile_off, send_map, send_mult);
+                                                     int *__restrict__ phys, int *__restrict__ counters, Geom g,
+                                                     int dim, int exch, const int *__restrict__ tile_cnt,
+                                                     int *__restrict__ send_map, int *__restrict__ send_mult,
+                                                     int send_cap, int cap) {
+    k_sel_scatter_body(X, Y, Z, vx, vy, vz, mass, type, uid, shape, flags, ref, phys, counters, g, dim, exch,
+                       tile_cnt, send_map, send_mult, send_cap, cap);
 }
 
 // Exchange of one dimension after its send map is built, as a relabeling:
@@ -1632,9 +1638,8 @@ void launch_gather_counters(hipStream_t s, const int *const *src, int n, int *ds
 
 static int tile_grid(int cap) { return max(1, min((cap + 255) / 256, 2048)); }
 
+// (the cell counts are zero on entry: allocated zeroed, cleared by every scan)
 void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g) {
-    launch_fill_i32(s, d.lcount, 0, g.ncells + 1);
-    launch_fill_i32(s, d.gcount, 0, g.ncells + 1);
     const int grid = tile_grid(d.cap);
     const int c = d.cur, o = 1 - d.cur;
     hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.flags, d.counters, g,
@@ -1650,14 +1655,10 @@ void launch_bin_sort(hipStream_t s, DevState &d, const Geom &g) {
                        d.particle_cell, d.alt_vx, d.alt_vy, d.alt_vz, d.alt_mass, d.alt_type, d.alt_flags,
                        d.alt_uid, d.alt_shape, d.alt_ref, d.alt_cell, d.phys);
     d.cur = o;
-    // copy the permuted arrays back: array addresses stay fixed, so captured
-    // step graphs remain valid across rebuilds
-    hipLaunchKernelGGL(k_copy_back, dim3(grid), dim3(256), 0, s, d.counters, d.alt_vx, d.alt_vy, d.alt_vz,
+    hipLaunchKernelGGL(k_copy_back_maps, dim3(grid), dim3(256), 0, s, d.counters, d.alt_vx, d.alt_vy, d.alt_vz,
                        d.alt_mass, d.alt_type, d.alt_flags, d.alt_uid, d.alt_shape, d.alt_ref, d.alt_cell, d.vx,
-                       d.vy, d.vz, d.mass, d.type, d.flags, d.uid, d.shape, d.ref, d.particle_cell);
-    hipLaunchKernelGGL(k_sync_maps, dim3(grid), dim3(256), 0, s, d.counters, d.ref, d.phys, d.send_map, d.send_mult,
-                       d.sync_src, d.sync_mult);
-    launch_fill_i32(s, d.img_cnt, 0, d.cap);
+                       d.vy, d.vz, d.mass, d.type, d.flags, d.uid, d.shape, d.ref, d.particle_cell, d.phys,
+                       d.send_map, d.send_mult, d.sync_src, d.sync_mult, d.img_cnt);
     hipLaunchKernelGGL(k_img_build, dim3(grid), dim3(256), 0, s, d.counters, d.sync_src, d.img_cnt, d.img_slot, d.cap);
 }
 
@@ -1683,24 +1684,26 @@ __global__ void k_set_counters_batch(const RepDesc *__restrict__ D, int dim, int
     if ((int) threadIdx.x < count) d.counters[first + threadIdx.x] = v;
 }
 
+// dim is the system's first periodic dimension
+__device__ __forceinline__ bool first_periodic(const RepDesc &R, int dim) {
+    for (int k = 0; k < dim; k++)
+        if (R.periodic[k]) return false;
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_sel_count_batch(const RepDesc *__restrict__ D, int dim, int exch) {
     REP_ARGS;
     if (!R.periodic[dim]) return;
     const double *Xd = dim == 0 ? d.X[c] : (dim == 1 ? d.Y[c] : d.Z[c]);
-    k_sel_count_body(Xd, d.flags, d.phys, d.counters, g, dim, exch, d.tile_cnt, d.exchg_flag);
-}
-
-__global__ __launch_bounds__(1024) void k_sel_scan_batch(const RepDesc *__restrict__ D, int dim, int exch) {
-    REP_ARGS;
-    if (!R.periodic[dim]) return;
-    k_sel_scan_body(d.counters, d.tile_cnt, dim, exch, d.send_cap, d.cap);
+    k_sel_count_body(Xd, d.flags, d.phys, d.counters, g, dim, exch, first_periodic(R, dim) ? 1 : 0, d.tile_cnt,
+                     d.exchg_flag);
 }
 
 __global__ __launch_bounds__(256) void k_sel_scatter_batch(const RepDesc *__restrict__ D, int dim, int exch) {
     REP_ARGS;
     if (!R.periodic[dim]) return;
     k_sel_scatter_body(d.X[c], d.Y[c], d.Z[c], d.vx, d.vy, d.vz, d.mass, d.type, d.uid, d.shape, d.flags, d.ref,
-                       d.phys, d.counters, g, dim, exch, d.tile_cnt, d.send_map, d.send_mult);
+                       d.phys, d.counters, g, dim, exch, d.tile_cnt, d.send_map, d.send_mult, d.send_cap, d.cap);
 }
 
 __global__ __launch_bounds__(1024) void k_exchange_relabel_batch(const RepDesc *__restrict__ D, int dim) {
@@ -1748,16 +1751,12 @@ __global__ __launch_bounds__(256) void k_permute_batch(const RepDesc *__restrict
                    d.alt_type, d.alt_flags, d.alt_uid, d.alt_shape, d.alt_ref, d.alt_cell, d.phys);
 }
 
-__global__ __launch_bounds__(256) void k_copy_back_batch(const RepDesc *__restrict__ D) {
+__global__ __launch_bounds__(256) void k_copy_back_maps_batch(const RepDesc *__restrict__ D) {
     REP_ARGS;
-    k_copy_back_body(d.counters, d.alt_vx, d.alt_vy, d.alt_vz, d.alt_mass, d.alt_type, d.alt_flags, d.alt_uid,
-                     d.alt_shape, d.alt_ref, d.alt_cell, d.vx, d.vy, d.vz, d.mass, d.type, d.flags, d.uid, d.shape,
-                     d.ref, d.particle_cell);
-}
-
-__global__ __launch_bounds__(256) void k_sync_maps_batch(const RepDesc *__restrict__ D) {
-    REP_ARGS;
-    k_sync_maps_body(d.counters, d.ref, d.phys, d.send_map, d.send_mult, d.sync_src, d.sync_mult);
+    k_copy_back_maps_body(d.counters, d.alt_vx, d.alt_vy, d.alt_vz, d.alt_mass, d.alt_type, d.alt_flags, d.alt_uid,
+                          d.alt_shape, d.alt_ref, d.alt_cell, d.vx, d.vy, d.vz, d.mass, d.type, d.flags, d.uid,
+                          d.shape, d.ref, d.particle_cell, d.phys, d.send_map, d.send_mult, d.sync_src, d.sync_mult,
+                          d.img_cnt);
 }
 
 __global__ __launch_bounds__(256) void k_img_build_batch(const RepDesc *__restrict__ D) {
@@ -1774,37 +1773,27 @@ void launch_cells_batch(hipStream_t s, const RepDesc *D, int n, int max_cap, int
     const int per = std::max(1, std::min(tile_grid(max_cap), 32));
     const dim3 one(1, n), tiles(per, n), cells(per, n), fills(per, n);
     (void) max_ncells;
-    hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, -1, C_IMG_OVF, 1, 0);
     // exchange (comm.py:105-151), one dimension at a time
-    static_assert(C_NSEND_ALL == C_NGHOST + 1 && C_NSEND == C_NSEND_ALL + 1, "counter layout");
     for (int dim = 0; dim < 3; dim++) {
-        hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, dim, C_NGHOST, 2, 0);
         hipLaunchKernelGGL(k_sel_count_batch, tiles, dim3(256), 0, s, D, dim, 1);
-        hipLaunchKernelGGL(k_sel_scan_batch, one, dim3(1024), 0, s, D, dim, 1);
         hipLaunchKernelGGL(k_sel_scatter_batch, tiles, dim3(256), 0, s, D, dim, 1);
         hipLaunchKernelGGL(k_exchange_relabel_batch, one, dim3(1024), 0, s, D, dim);
     }
     // borders
-    hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, -1, C_NGHOST, 8, 0);
     for (int dim = 0; dim < 3; dim++) {
         hipLaunchKernelGGL(k_sel_count_batch, tiles, dim3(256), 0, s, D, dim, 0);
-        hipLaunchKernelGGL(k_sel_scan_batch, one, dim3(1024), 0, s, D, dim, 0);
         hipLaunchKernelGGL(k_sel_scatter_batch, tiles, dim3(256), 0, s, D, dim, 0);
     }
     // cell binning and the permutation into cell order (flips each system's
-    // position buffer: the caller flips d.cur on the host)
-    hipLaunchKernelGGL(k_fill_batch, fills, dim3(256), 0, s, D, 0, 0);
-    hipLaunchKernelGGL(k_fill_batch, fills, dim3(256), 0, s, D, 1, 0);
+    // position buffer: the caller flips d.cur on the host); the cell counts
+    // are zero on entry, the scan clears the image and tile flags
     hipLaunchKernelGGL(k_bin_count_batch, tiles, dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_scan_cells_batch, one, dim3(1024), 0, s, D);
     hipLaunchKernelGGL(k_bin_scatter_batch, tiles, dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_sort_cells_batch, cells, dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_permute_batch, tiles, dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_copy_back_batch, tiles, dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_sync_maps_batch, tiles, dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_fill_batch, fills, dim3(256), 0, s, D, 2, 0);
+    hipLaunchKernelGGL(k_copy_back_maps_batch, tiles, dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_img_build_batch, tiles, dim3(256), 0, s, D);
-    hipLaunchKernelGGL(k_set_counters_batch, one, dim3(64), 0, s, D, -1, C_TILE_MAX, 1, 0);
 }
 
 int build_zt(const Geom &g) { return g.nchunks == 1 ? g.dim_cells[2] + 2 : g.zc + 3; }
@@ -1865,26 +1854,22 @@ void launch_expand_neighbors(hipStream_t s, const DevState &d, const Geom &g, in
                        s, d.counters, d.ref, d.nl, d.nl_stride, d.numneighs, out_ref, ncap_out);
 }
 
-static void launch_select_dim(hipStream_t s, const DevState &d, const Geom &g, int dim, int exch) {
+static void launch_select_dim(hipStream_t s, const DevState &d, const Geom &g, int dim, int exch, int first) {
     const int grid = tile_grid(d.cap);
     const int c = d.cur;
     const double *Xd = dim == 0 ? d.X[c] : (dim == 1 ? d.Y[c] : d.Z[c]);
-    hipLaunchKernelGGL(k_sel_count, dim3(grid), dim3(256), 0, s, Xd, d.flags, d.phys, d.counters, g, dim, exch,
+    hipLaunchKernelGGL(k_sel_count, dim3(grid), dim3(256), 0, s, Xd, d.flags, d.phys, d.counters, g, dim, exch, first,
                        d.tile_cnt, d.exchg_flag);
-    hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, s, d.counters, d.tile_cnt, dim, exch, d.send_cap, d.cap);
     hipLaunchKernelGGL(k_sel_scatter, dim3(grid), dim3(256), 0, s, d.X[c], d.Y[c], d.Z[c], d.vx, d.vy, d.vz, d.mass,
                        d.type, d.uid, d.shape, d.flags, d.ref, d.phys, d.counters, g, dim, exch, d.tile_cnt,
-                       d.send_map, d.send_mult);
+                       d.send_map, d.send_mult, d.send_cap, d.cap);
 }
 
 void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int *periodic) {
     const int c = d.cur;
     for (int dim = 0; dim < 3; dim++) {
         if (!periodic[dim]) continue;
-        // nsend_all = 0; nghost = 0 (comm.py:105-118; adjacent counters)
-        static_assert(C_NSEND_ALL == C_NGHOST + 1, "counter layout");
-        launch_set_counters(s, d.counters, C_NGHOST, 2, 0);
-        launch_select_dim(s, d, g, dim, 1);
+        launch_select_dim(s, d, g, dim, 1, 0);
         int *sc = d.scratch_i;
         hipLaunchKernelGGL(k_exchange_relabel, dim3(1), dim3(1024), 0, s, d.X[c], d.Y[c], d.Z[c], d.ref, d.phys,
                            d.counters, g, d.send_map, d.send_mult, d.exchg_flag, sc, sc + d.send_cap,
@@ -1893,13 +1878,16 @@ void launch_exchange(hipStream_t s, const DevState &d, const Geom &g, const int 
 }
 
 void launch_borders(hipStream_t s, const DevState &d, const Geom &g, const int *periodic) {
-    // nghost, nsend_all and the six per-side send counts (adjacent counters)
-    static_assert(C_NSEND_ALL == C_NGHOST + 1 && C_NSEND == C_NSEND_ALL + 1, "counter layout");
-    launch_set_counters(s, d.counters, C_NGHOST, 8, 0);
+    // the first periodic dimension starts from no ghosts and an empty send map
+    bool first = true;
     for (int dim = 0; dim < 3; dim++) {
         if (!periodic[dim]) continue;
-        launch_select_dim(s, d, g, dim, 0);
+        launch_select_dim(s, d, g, dim, 0, first ? 1 : 0);
+        first = false;
     }
+    // no periodic dimension: no ghosts, nothing sent (adjacent counters)
+    static_assert(C_NSEND_ALL == C_NGHOST + 1 && C_NSEND == C_NSEND_ALL + 1, "counter layout");
+    if (first) launch_set_counters(s, d.counters, C_NGHOST, 8, 0);
 }
 
 void launch_sync_pp(hipStream_t s, const DevState &d, const Geom &g, int cur, int prev, int grid) {

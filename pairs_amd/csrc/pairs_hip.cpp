@@ -719,6 +719,7 @@ int pairs_hip_exchange(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     drop_graphs(ctx);
     launch_exchange(ctx->stream, ctx->d, ctx->geom, ctx->cfg.periodic);
+    launch_set_counters(ctx->stream, ctx->d.counters, C_NGHOST, 1, 0);  // the ghosts are gone (comm.py:105-118)
     return finish_module(ctx);
 }
 
@@ -760,9 +761,9 @@ static void set_tile_cap(pairs_hip_ctx *ctx) {
     ctx->d.tile_cap = grown;
 }
 
-static void enqueue_build(pairs_hip_ctx *ctx, hipStream_t s) {
+static void enqueue_build(pairs_hip_ctx *ctx, hipStream_t s, bool reset_tile = true) {
     const double rl = ctx->cfg.cell_spacing;
-    launch_set_counters(s, ctx->d.counters, C_TILE_MAX, 1, 0);
+    if (reset_tile) launch_set_counters(s, ctx->d.counters, C_TILE_MAX, 1, 0);
     launch_build_neighbors(s, ctx->d, ctx->geom, ctx->cfg.half, rl * rl);
 }
 
@@ -958,13 +959,12 @@ static inline bool every(int ts, int n) { return n > 0 && (((ts + 1) % n) == 0 |
 // on overflow like the generated resize loops (transformations/modules.py:159-205)
 // exchange, borders, cell binning and the neighbour build of a rebuild
 static void enqueue_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
-    launch_set_counters(s, ctx->d.counters, C_IMG_OVF, 1, 0);
     launch_exchange(s, ctx->d, ctx->geom, ctx->cfg.periodic);
     launch_borders(s, ctx->d, ctx->geom, ctx->cfg.periodic);
-    launch_bin_sort(s, ctx->d, ctx->geom);  // flips d.cur
+    launch_bin_sort(s, ctx->d, ctx->geom);  // flips d.cur; clears the image and tile flags
     // the neighbour build runs right behind the cell phase; every rebuild kernel
     // returns at once when an earlier one has set an overflow
-    enqueue_build(ctx, s);
+    enqueue_build(ctx, s, false);
 }
 
 // The rebuild's ~37 small kernels have grids that depend only on
