@@ -11,12 +11,15 @@ value = atoms x steps / wall time of the timed steps (inputs resident in HBM,
 set-up excluded), full neighbour list; the half-list form, the force kernel's
 HBM roofline and the CPU baseline are reported beside it.
 
-Beside it: the batched ensemble (512 uneven replicas), the MLP potential at
-131072 atoms (training, force check, MD throughput) and the DEM path on a
-larger domain, each as its own object of the same JSON line.
+Beside it: the MLP potential at 131072 atoms (training, force check, MD
+throughput) and the DEM path on a larger domain, each as its own object of the
+same JSON line; with --batched also the batched ensemble (512 uneven
+replicas).  The batched line is off by default: one run of it ended in a GPU
+memory fault whose cause has not been found (the small batched ensembles of
+the GPU tests run).
 
   python bench.py [--gpus 1] [--steps 1000] [--warmup 100] [--no-half] [--no-cpu]
-                  [--no-batched] [--no-mlp] [--no-dem]
+                  [--batched] [--no-mlp] [--no-dem]
 """
 import argparse
 import json
@@ -365,7 +368,8 @@ def main():
     ap.add_argument("--time-every", type=int, default=5,
                     help="in the force-timing pass after the timed steps, time every n-th force launch")
     ap.add_argument("--time-steps", type=int, default=200, help="steps of the force-timing pass")
-    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched", action="store_true", help="add the 512-replica batched ensemble line")
+    ap.add_argument("--no-batched", action="store_true", help="(default)")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--no-dem", action="store_true")
     ap.add_argument("--replicas", type=int, default=512)
@@ -448,7 +452,7 @@ def main():
             "force_kernel_GBps": hb / (half["force_ms"] * 1e-3) / 1e9 if half["force_ms"] > 0 else None,
             "neighbour_entries": half["nneigh"],
         }
-    if not args.no_batched:
+    if args.batched and not args.no_batched:
         rec["batched"] = run_batched(args.side_steps, 40, args.replicas)
         progress(f"batched: {rec['batched']['ms_per_step']:.4f} ms/step")
     if not args.no_mlp:

@@ -80,7 +80,10 @@ static int dalloc(pairs_hip_dem_ctx *ctx, T **p, size_t n) {
 }
 
 static void drop_graph(pairs_hip_dem_ctx *ctx) {
-    if (ctx->graph) hipGraphExecDestroy(ctx->graph);
+    if (ctx->graph) {
+        if (ctx->stream) hipStreamSynchronize(ctx->stream);  // no replay pending
+        hipGraphExecDestroy(ctx->graph);
+    }
     ctx->graph = nullptr;
 }
 

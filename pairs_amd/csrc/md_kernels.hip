@@ -103,7 +103,12 @@ __device__ __forceinline__ int cell_of(double x, double y, double z, const Geom 
 // ---------------------------------------------------------------- cells (binning by counting sort)
 
 // BuildCellLists (src/pairs/sim/cell_lists.py:90-134), histogram half: cell
-// of every local and ghost, per-cell counts of locals and of ghosts.
+// of every local and ghost, per-cell counts of locals and of ghosts, and a
+// slot per particle within its cell's locals or ghosts.  The locals are in
+// cell order from the last rebuild, so the lanes of a wave share few cells:
+// each group of lanes with one key takes consecutive slots with one atomic
+// (up to 8 groups per wave, single atomics for the rest).  Slot order is
+// arrival order; the cell sort orders each cell by reference index after.
 __device__ __forceinline__ void k_bin_count_body(const double *__restrict__ X, const double *__restrict__ Y,
                                                    const double *__restrict__ Z, const int *__restrict__ flags,
                                                    const int *__restrict__ counters, Geom g,
@@ -112,10 +117,31 @@ __device__ __forceinline__ void k_bin_count_body(const double *__restrict__ X, c
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
     const int n = nlocal + counters[C_NGHOST];
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
-        const int c = (flags[p] & FLAG_INFINITE) ? 0 : cell_of(X[p], Y[p], Z[p], g);
-        cell[p] = c;
-        slot[p] = atomicAdd(p < nlocal ? &lcount[c] : &gcount[c], 1);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int p0 = blockIdx.x * 256 + (threadIdx.x & ~63); p0 < n; p0 += gridDim.x * 256) {
+        const int p = p0 + lane;
+        const bool live = p < n;
+        int c = 0;
+        if (live) {
+            c = (flags[p] & FLAG_INFINITE) ? 0 : cell_of(X[p], Y[p], Z[p], g);
+            cell[p] = c;
+        }
+        const int key = live ? 2 * c + (p < nlocal ? 0 : 1) : -1;
+        unsigned long long todo = __ballot(live);
+        int my = 0;
+        for (int it = 0; it < 8 && todo; it++) {
+            const int leader = __ffsll((long long) todo) - 1;
+            const int kl = __shfl(key, leader, 64);
+            const unsigned long long m = __ballot(key == kl) & todo;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(((kl & 1) ? gcount : lcount) + (kl >> 1), __popcll(m));
+            base = __shfl(base, leader, 64);
+            if ((m >> lane) & 1ull) my = base + __popcll(m & below);
+            todo &= ~m;
+        }
+        if ((todo >> lane) & 1ull) my = atomicAdd(((key & 1) ? gcount : lcount) + (key >> 1), 1);
+        if (live) slot[p] = my;
     }
 }
 
@@ -128,51 +154,72 @@ __global__ __launch_bounds__(256) void k_bin_count(const double *__restrict__ X,
 }
 
 // exclusive scans of the local and ghost cell counts -> lstart, gstart
-// [0..ncells]; the counts are cleared behind the reads (nothing else uses
-// them, so the next binning starts from zeros).  The flags the rest of the
-// rebuild raises by atomics (images, largest tile) are cleared here too.
+// [0..ncells], one workgroup of 16 waves: each wave takes a contiguous run
+// of rows of 64 cells (coalesced loads), the wave totals give each run's
+// base, and each row is scanned across the lanes.  The counts are cleared
+// behind the reads (nothing else uses them, so the next binning starts from
+// zeros).  The flags the rest of the rebuild raises by atomics (images,
+// largest tile) are cleared here too.
 __device__ __forceinline__ void k_scan_cells_body(int *__restrict__ counters, int *__restrict__ lcount,
                                                      int *__restrict__ gcount, int *__restrict__ lstart,
                                                      int *__restrict__ gstart, int ncells) {
-    __shared__ int lds[20];
+    __shared__ int tot[2][16];
     if (threadIdx.x == 0) { counters[C_IMG_OVF] = 0; counters[C_TILE_MAX] = 0; }
     if (counters[C_OVF]) return;
-    // each thread owns a contiguous run of cells (up to PT per round, all
-    // loads issued together); one block scan of the run totals per round
-    constexpr int PT = 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int RW = 16;
+    constexpr int ROUND = 16 * RW * 64;
     int carry_l = 0, carry_g = 0;
-    for (int r0 = 0; r0 < ncells; r0 += 1024 * PT) {
-        const int rn = min(ncells - r0, 1024 * PT);
-        const int per = (rn + 1023) / 1024;
-        const int c0 = r0 + min((int) threadIdx.x * per, rn);
-        const int c1 = r0 + min((int) threadIdx.x * per + per, rn);
-        int vl[PT], vg[PT];
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int c = c0 + k;
-            vl[k] = c < c1 ? lcount[c] : 0;
-            vg[k] = c < c1 ? gcount[c] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int c = c0 + k;
-            if (c < c1) { lcount[c] = 0; gcount[c] = 0; }
-        }
+    for (int r0 = 0; r0 < ncells; r0 += ROUND) {
+        const int rn = min(ncells - r0, ROUND);
+        const int rows = (rn + 63) / 64;
+        const int per = (rows + 15) / 16;
+        const int row0 = wave * per;
+        int vl[RW], vg[RW];
         int sl = 0, sg = 0;
 #pragma unroll
-        for (int k = 0; k < PT; k++) { sl += vl[k]; sg += vg[k]; }
-        int tl, tg;
-        int al = carry_l + block_exclusive_scan<1024>(sl, lds, &tl);
-        int ag = carry_g + block_exclusive_scan<1024>(sg, lds, &tg);
-#pragma unroll
-        for (int k = 0; k < PT; k++) {
-            const int c = c0 + k;
-            if (c < c1) { lstart[c] = al; gstart[c] = ag; }
-            al += vl[k];
-            ag += vg[k];
+        for (int k = 0; k < RW; k++) {
+            const int cr = (row0 + k) * 64 + lane;
+            const bool in = k < per && cr < rn;
+            vl[k] = in ? lcount[r0 + cr] : 0;
+            vg[k] = in ? gcount[r0 + cr] : 0;
+            sl += vl[k];
+            sg += vg[k];
         }
-        carry_l += tl;
-        carry_g += tg;
+#pragma unroll
+        for (int k = 0; k < RW; k++) {
+            const int cr = (row0 + k) * 64 + lane;
+            if (k < per && cr < rn) { lcount[r0 + cr] = 0; gcount[r0 + cr] = 0; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            sl += __shfl_xor(sl, off, 64);
+            sg += __shfl_xor(sg, off, 64);
+        }
+        if (lane == 0) { tot[0][wave] = sl; tot[1][wave] = sg; }
+        __syncthreads();
+        int bl = carry_l, bg = carry_g, al = carry_l, ag = carry_g;
+        for (int w = 0; w < 16; w++) {
+            const int a = tot[0][w], b = tot[1][w];
+            if (w < wave) { bl += a; bg += b; }
+            al += a; ag += b;
+        }
+#pragma unroll
+        for (int k = 0; k < RW; k++) {
+            int xl = vl[k], xg = vg[k];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int yl = __shfl_up(xl, off, 64), yg = __shfl_up(xg, off, 64);
+                if (lane >= off) { xl += yl; xg += yg; }
+            }
+            const int cr = (row0 + k) * 64 + lane;
+            if (k < per && cr < rn) { lstart[r0 + cr] = bl + xl - vl[k]; gstart[r0 + cr] = bg + xg - vg[k]; }
+            bl += __shfl(xl, 63, 64);
+            bg += __shfl(xg, 63, 64);
+        }
+        carry_l = al;
+        carry_g = ag;
+        __syncthreads();
     }
     if (threadIdx.x == 0) { lstart[ncells] = carry_l; gstart[ncells] = carry_g; }
 }
@@ -406,16 +453,12 @@ __device__ __forceinline__ void write_images(const Images &im, int i, int nlocal
 // tile offsets in LDS; then each chunk of 64 candidates is held one per lane
 // while the home particles of the cell are taken one at a time (broadcast
 // from the lane that holds them): a ballot over the chunk and a lane prefix
-// count append the accepted candidates in candidate order.  Entries go to an
-// LDS row buffer (one column per home particle) and leave as packed groups of
-// four rows; rows past the buffer, or home particles past its columns, are
-// written to the list directly.
+// count append the accepted candidates in candidate order, written to the
+// list directly (the four entries of a group share 16 bytes, so the L2
+// merges the appends of a cell's home particles before they reach HBM; an
+// LDS row buffer that wrote whole groups cost more in occupancy than it saved).
 constexpr int BB = 128;        // threads per build workgroup
 constexpr int BW = BB / 64;    // waves per build workgroup
-constexpr int LR_MAX = 96;     // list rows buffered in LDS (a multiple of 4)
-// home particles with an LDS buffer column: g.hcap (a multiple of 4, set
-// from the segment size); rows are hcap + 2 entries long (an odd number of
-// 4-byte words, so consecutive rows start on different banks)
 constexpr int NB_CAND = 2048;  // candidates of one home cell
 // half lists: mirror entries of one home particle collected in the top
 // RMAX slots of the wave's candidate buffer before they are sorted
@@ -438,19 +481,13 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
     const int *__restrict__ ref = A.ref, *__restrict__ flags = A.flags;
     const double rc2 = A.rc2;
     uint32_t *__restrict__ nl = A.nl;
-    const int nl_stride = A.nl_stride, ncap = A.ncap, lrows = A.lrows, tcap = A.tcap, zt = A.zt, rev0 = A.rev0;
+    const int nl_stride = A.nl_stride, ncap = A.ncap, tcap = A.tcap, zt = A.zt, rev0 = A.rev0;
     int *__restrict__ numneighs = A.numneighs, *__restrict__ ctr = A.counters_rw, *__restrict__ nrev = A.nrev;
     extern __shared__ char smem[];
-    const int HCAP = g.hcap, LBS = g.hcap + 2;
     int *tslot = (int *) smem;                                  // [tcap] physical slot of each tile entry
-    int *s_cnt = tslot + tcap;                                  // [hcap] list length per buffered column
-                                                                // (+ [hcap] mirror lengths, half lists)
-    int *ctab = s_cnt + (HALF ? 2 : 1) * HCAP;                  // [9][zt][4]: local st, len, ghost st, len
+    int *ctab = tslot + tcap;                                   // [9][zt][4]: local st, len, ghost st, len
     int *tref = ctab + 9 * zt * 4;                              // [tcap] half lists only
-    uint16_t *lbuf = (uint16_t *) (tref + (HALF ? tcap : 0));   // [lrows][LBS]
-    uint16_t *cand_all = lbuf + (size_t) lrows * LBS;           // [BW][NB_CAND]
-    // half lists: the row buffer's upper half holds the mirror rows
-    const int lr_own = HALF ? (lrows / 8) * 4 : lrows, lr_mir = lrows - lr_own;
+    uint16_t *cand_all = (uint16_t *) (tref + (HALF ? tcap : 0));  // [BW][NB_CAND]
     __shared__ int r_start[NRUNS], r_len[NRUNS], r_off[NRUNS + 1], col_lo[9];
     if (counters[C_OVF]) return;
     const int nlocal = counters[C_NLOCAL];
@@ -573,10 +610,7 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
                         const bool acc = qv && keep && rsq < rc2;
                         const unsigned long long m = __ballot(acc);
                         const int k = cnt + lane_rank(m);
-                        if (acc && k < ncap) {
-                            if (k < lr_own && idx < HCAP) lbuf[k * LBS + idx] = (uint16_t) o;
-                            else nl[nl_at(k, h0 + idx, nl_stride)] = (uint32_t) tslot[o];
-                        }
+                        if (acc && k < ncap) nl[nl_at(k, h0 + idx, nl_stride)] = (uint32_t) tslot[o];
                         cnt += __popcll(m);
                         if (HALF) {
                             // a mirror entry: an earlier-index local, not fixed
@@ -620,26 +654,17 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
                         const int re = lane < cntr ? (tref[oe] & REF_MASK) : 0;
                         int rank = 0;
                         for (int q = 0; q < cntr; q++) rank += __builtin_amdgcn_readlane(re, q) < re ? 1 : 0;
-                        if (lane < cntr && rank < KR) {
-                            if (rank < lr_mir && idx < HCAP) lbuf[(lr_own + rank) * LBS + idx] = (uint16_t) oe;
-                            else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint32_t) tslot[oe];
-                        }
+                        if (lane < cntr && rank < KR) nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint32_t) tslot[oe];
                     } else {
                         for (int e = lane; e < cntr; e += 64) {
                             const int oe = rbuf[e];
                             const int re = tref[oe] & REF_MASK;
                             int rank = 0;
                             for (int q = 0; q < cntr; q++) rank += (tref[rbuf[q]] & REF_MASK) < re ? 1 : 0;
-                            if (rank < KR) {
-                                if (rank < lr_mir && idx < HCAP) lbuf[(lr_own + rank) * LBS + idx] = (uint16_t) oe;
-                                else nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint32_t) tslot[oe];
-                            }
+                            if (rank < KR) nl[nl_at(rev0 + rank, h0 + idx, nl_stride)] = (uint32_t) tslot[oe];
                         }
                     }
-                    if (lane == 0) {
-                        nrev[h0 + idx] = KR;
-                        if (idx < HCAP) s_cnt[HCAP + idx] = KR;
-                    }
+                    if (lane == 0) nrev[h0 + idx] = KR;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -652,28 +677,9 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
                         K = ncap - 2;
                     }
                     numneighs[h0 + idx] = K;
-                    if (idx < HCAP) s_cnt[idx] = K;
                 }
             }
         }
-    }
-    __syncthreads();
-    // buffered rows leave as packed groups of four physical slots (lane
-    // order = physical order); rows of the last group past a particle's count
-    // hold earlier particles' entries or nothing: clamped into the tile
-    const int nhome = min(h1 - h0, HCAP);
-    uint4 *nl4 = (uint4 *) nl;
-    auto slot_of = [&](int r, int a) { return (uint32_t) tslot[min((int) lbuf[r * LBS + a], T - 1)]; };
-    auto flush = [&](int a, int row0, int nrows, int grp0) {
-        for (int gq = 0; gq < (nrows + 3) >> 2; gq++) {
-            const int r = row0 + 4 * gq;
-            nl4[(size_t) (grp0 + gq) * nl_stride + h0 + a] =
-                make_uint4(slot_of(r, a), slot_of(r + 1, a), slot_of(r + 2, a), slot_of(r + 3, a));
-        }
-    };
-    for (int a = threadIdx.x; a < nhome; a += BB) {
-        flush(a, 0, min(s_cnt[a], lr_own), 0);
-        if (HALF) flush(a, lr_own, min(s_cnt[HCAP + a], lr_mir), rev0 >> 2);
     }
 }
 
@@ -1798,11 +1804,9 @@ void launch_cells_batch(hipStream_t s, const RepDesc *D, int n, int max_cap, int
 
 int build_zt(const Geom &g) { return g.nchunks == 1 ? g.dim_cells[2] + 2 : g.zc + 3; }
 
-static int build_lrows(const DevState &d) { return min((d.ncap + 3) / 4 * 4, LR_MAX); }
-
 static size_t build_lds(const DevState &d, const Geom &g, int half, int tcap) {
-    return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) + (size_t) g.hcap * sizeof(int) * (half ? 2 : 1) +
-           (size_t) build_lrows(d) * (g.hcap + 2) * sizeof(uint16_t) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
+    (void) d;
+    return (size_t) tcap * (sizeof(int) + (half ? sizeof(int) : 0)) + (size_t) BW * NB_CAND * sizeof(uint16_t) +
            (size_t) 9 * build_zt(g) * 4 * sizeof(int);
 }
 
@@ -1823,7 +1827,6 @@ BuildArgs build_args(const DevState &d, const Geom &g, int half, double rc2) {
     A.ref = d.ref; A.flags = d.flags;
     A.rc2 = rc2;
     A.nl = d.nl; A.nl_stride = d.nl_stride; A.ncap = d.ncap;
-    A.lrows = build_lrows(d);
     A.tcap = min(d.tile_cap, build_tile_limit(d, g, half));
     A.zt = build_zt(g);
     A.numneighs = d.numneighs; A.nrev = d.nrev; A.rev0 = d.rev0;

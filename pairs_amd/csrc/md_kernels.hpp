@@ -62,7 +62,6 @@ struct Geom {
     int zc;        // cells per segment along z
     int nchunks;   // segments per z-column
     int nsegs;     // dim_cells[0] * dim_cells[1] * nchunks
-    int hcap;      // home particles of a segment whose lists the build buffers in LDS
 };
 
 // Neighbour lists: entry k of physical local i.  Rows come in groups of four
@@ -188,7 +187,7 @@ struct BuildArgs {
     const int *ref, *flags;
     double rc2;
     uint32_t *nl;
-    int nl_stride, ncap, lrows, tcap, zt, rev0;
+    int nl_stride, ncap, tcap, zt, rev0;
     int *numneighs, *nrev;
 };
 BuildArgs build_args(const DevState &d, const Geom &g, int half, double rc2);

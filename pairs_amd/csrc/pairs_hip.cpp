@@ -57,17 +57,24 @@ static int noalloc_resolve(int dev, std::string &err) {
     return 0;
 }
 
+// An executable graph is destroyed only once the stream has drained: runs
+// enqueue work ahead of the host (rebuild counters are checked a rebuild
+// later), so a replay of it may still be pending.
+static void destroy_exec(pairs_hip_ctx *ctx, hipGraphExec_t &g) {
+    if (!g) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    hipGraphExecDestroy(g);
+    g = nullptr;
+}
+
 static void drop_reb_graphs(pairs_hip_ctx *ctx) {
-    for (int p = 0; p < 2; p++) {
-        if (ctx->reb_graph[p]) hipGraphExecDestroy(ctx->reb_graph[p]);
-        ctx->reb_graph[p] = nullptr;
-    }
+    for (int p = 0; p < 2; p++) destroy_exec(ctx, ctx->reb_graph[p]);
 }
 
 static void drop_graphs(pairs_hip_ctx *ctx) {
     for (int p = 0; p < 2; p++)
         for (int k = 0; k < pairs_hip_ctx::STEP_CHAINS; k++) {
-            if (ctx->step_graph[p][k]) hipGraphExecDestroy(ctx->step_graph[p][k]);
+            destroy_exec(ctx, ctx->step_graph[p][k]);
             ctx->step_graph[p][k] = nullptr;
         }
     drop_reb_graphs(ctx);
@@ -183,15 +190,6 @@ static void setup_segments(pairs_hip_ctx *ctx, int zc) {
     g.zc = zc;
     g.nchunks = std::max(1, (g.dim_cells[2] - 1 + zc - 1) / zc);
     g.nsegs = g.dim_cells[0] * g.dim_cells[1] * g.nchunks;
-    // LDS list columns for twice the mean home count (denser segments write
-    // their extra particles' lists to memory directly), a multiple of 4
-    {
-        const double vol = g.len[0] * g.len[1] * g.len[2];
-        const double s3 = g.spacing * g.spacing * g.spacing;
-        const double per_cell = ctx->d.nlocal > 0 && vol > 0 ? ctx->d.nlocal * s3 / vol : 16.0;
-        const int want = (int) std::ceil(2.0 * per_cell * zc) + 8;
-        g.hcap = std::max(32, std::min(192, (want + 3) / 4 * 4));
-    }
 }
 
 // build_cell_lists_stencil (src/pairs/sim/cell_lists.py:46-87)
@@ -1244,8 +1242,7 @@ static bool step_graph_stale(const pairs_hip_ctx *ctx, int cur, int k) {
 static int capture_step_graph(pairs_hip_ctx *ctx, int cur, int k) {
     if (step_graph_stale(ctx, cur, k)) {
         hipGraphExec_t &slot = ctx->step_graph[cur][k];
-        if (slot) hipGraphExecDestroy(slot);
-        slot = nullptr;
+        destroy_exec(ctx, slot);
         // capture with headroom so that later rebuilds with a few more ghosts reuse it
         const int saved = ctx->ghost_grid;
         ctx->ghost_grid = saved + saved / 4 + 4;
@@ -1620,6 +1617,7 @@ struct pairs_hip_batch {
 };
 
 static void batch_drop_graphs(pairs_hip_batch *b) {
+    if (b->stream && (b->graph[0] || b->graph[1])) hipStreamSynchronize(b->stream);  // no replay pending
     for (int p = 0; p < 2; p++) {
         if (b->graph[p]) hipGraphExecDestroy(b->graph[p]);
         b->graph[p] = nullptr;
@@ -1789,7 +1787,10 @@ static int batch_rebuild(pairs_hip_batch *b) {
 
 static int batch_graph_launch(pairs_hip_batch *b, int phase) {
     if (!b->graph[phase] || b->graph_blocks[phase] != b->nblocks) {
-        if (b->graph[phase]) hipGraphExecDestroy(b->graph[phase]);
+        if (b->graph[phase]) {
+            BATCH_OK(hipStreamSynchronize(b->stream));  // no replay pending
+            hipGraphExecDestroy(b->graph[phase]);
+        }
         b->graph[phase] = nullptr;
         hipGraph_t g;
         BATCH_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
