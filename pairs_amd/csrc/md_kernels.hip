@@ -934,11 +934,13 @@ __global__ __launch_bounds__(64) void k_lj(LJArgs A) {
 }
 
 // The 64-bit pattern of lane 2p + 1 in lane 2p (DPP quad_perm 1,1,3,3 on
-// both 32-bit halves; the odd lanes receive lane 2p + 1's own value).
+// both 32-bit halves; the odd lanes receive lane 2p + 1's own value).  Every
+// lane is written by a quad permutation, so the move needs no old value (no
+// register to clear first).
 __device__ __forceinline__ double odd_lane(double v) {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int) b, 0xF5, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), 0xF5, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int) b, 0xF5, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int) (b >> 32), 0xF5, 0xF, 0xF, true);
     return __longlong_as_double(((long long) hi << 32) | (unsigned int) lo);
 }
 
