@@ -32,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, vendor figure
+FP64_OPS_PER_ENTRY = 21
 NX = 32
 NATOMS = 4 * NX ** 3
 
@@ -439,6 +441,19 @@ def main():
             "lj_only_bytes_per_launch": lj_bytes,
             "avg_launch_ms": full["force_ms"],
             "launches_timed": full["force_launches"],
+            # the same kernel against the FP64 vector peak: every list entry
+            # is evaluated (branch-free, masked past the cutoff), 21 FP64
+            # operations as the reference writes the pair (3 differences, 3
+            # multiplies + 2 adds for rsq, 1 division, 2 multiplies for sr6,
+            # 4 for the scalar factor, 3 multiplies + 3 adds for the force)
+            "fp64": {
+                "flops_per_launch": FP64_OPS_PER_ENTRY * full["nneigh"],
+                "achieved_TFLOPs": (FP64_OPS_PER_ENTRY * full["nneigh"] / (full["force_ms"] * 1e-3) / 1e12
+                                    if full["force_ms"] > 0 else None),
+                "peak_TFLOPs": FP64_VECTOR_PEAK_TFLOPS,
+                "frac": (FP64_OPS_PER_ENTRY * full["nneigh"] / (full["force_ms"] * 1e-3) / 1e12
+                         / FP64_VECTOR_PEAK_TFLOPS if full["force_ms"] > 0 else None),
+            },
         },
     }
     if not args.no_half:
