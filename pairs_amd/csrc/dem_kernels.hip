@@ -79,15 +79,25 @@ __device__ __forceinline__ void dsel(double xd, int fl, double lo, double hi, do
     s1 = ok && xd > hi - off;
 }
 
-// per-block counts of side 0 and side 1 over [0, nlocal + nghost); the
-// range is latched for the scatter; an exchange clears exchg_flag of locals
+// per-block counts of side 0 and side 1 over [0, nlocal + nghost); an
+// exchange, and the borders of the first periodic dimension, start from no
+// ghosts and an empty send map (comm.py:105-118: the counts are reset before
+// them), a later dimension selects among the locals and the ghosts so far and
+// appends behind them; workgroup 0 latches the range and the base for the
+// scatter (nothing writes them while this pass runs); an exchange clears
+// exchg_flag of locals
 __global__ __launch_bounds__(256) void k_sel_count(const double *__restrict__ Xd, const int *__restrict__ flags,
                                                    int *__restrict__ c, double lo, double hi, double off, int exch,
-                                                   int *__restrict__ blk, int *__restrict__ exchg_flag) {
+                                                   int first, int *__restrict__ blk, int *__restrict__ exchg_flag) {
     __shared__ int s0c, s1c;
+    const bool fresh = exch || first;
     const int nl = c[D_NLOCAL];
-    const int n = nl + c[D_NGHOST];
+    const int n = nl + (fresh ? 0 : c[D_NGHOST]);
     if (threadIdx.x == 0) { s0c = 0; s1c = 0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c[D_RANGE] = n;
+        c[D_BASE] = fresh ? 0 : c[D_NSEND_ALL];
+    }
     __syncthreads();
     const int i = blockIdx.x * 256 + threadIdx.x;
     bool a = false, b = false;
@@ -99,49 +109,63 @@ __global__ __launch_bounds__(256) void k_sel_count(const double *__restrict__ Xd
     if (threadIdx.x == 0) { blk[2 * blockIdx.x] = s0c; blk[2 * blockIdx.x + 1] = s1c; }
 }
 
-// exclusive offsets of the block counts (side 0 of all blocks, then side 1);
-// appends to the send map after the entries already there
-__global__ __launch_bounds__(1024) void k_sel_scan(int *__restrict__ c, int *__restrict__ blk, int nblocks, int dim,
-                                                   int send_cap) {
-    __shared__ int lds[20];
-    const int base = c[D_NSEND_ALL];
-    int carry0 = 0, carry1 = 0;
-    for (int b0 = 0; b0 < nblocks; b0 += 1024) {
-        const int b = b0 + threadIdx.x;
-        int t;
-        const int e0 = block_scan_excl<1024>(b < nblocks ? blk[2 * b] : 0, lds, &t);
-        const int c0 = t;
-        const int e1 = block_scan_excl<1024>(b < nblocks ? blk[2 * b + 1] : 0, lds, &t);
-        if (b < nblocks) { blk[2 * b] = carry0 + e0; blk[2 * b + 1] = carry1 + e1; }
-        carry0 += c0;
-        carry1 += t;
+// sums of two per-thread values over a workgroup of 256 threads
+__device__ __forceinline__ void block_sum2_256(int a, int b, int *lds, int &sa, int &sb) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
     }
-    if (threadIdx.x == 0) {
-        c[D_RANGE] = c[D_NLOCAL] + c[D_NGHOST];
-        c[D_BASE] = base;
-        c[D_SIDE0] = carry0;
-        c[D_NSEND + 2 * dim] = carry0;
-        c[D_NSEND + 2 * dim + 1] = carry1;
-        const int need = base + carry0 + carry1;
-        if (need > send_cap) {
-            atomicMax(&c[D_REQ_SEND], need);
-            atomicMax(&c[D_OVF], DOVF_SEND);
-        }
-        c[D_NSEND_ALL] = need;
-    }
+    __syncthreads();
+    if (lane == 0) { lds[2 * wave] = a; lds[2 * wave + 1] = b; }
+    __syncthreads();
+    sa = lds[0] + lds[2] + lds[4] + lds[6];
+    sb = lds[1] + lds[3] + lds[5] + lds[7];
 }
 
-// ordered scatter into the send map (index order within each side); borders
-// also create the ghosts at nlocal + send index (uid, type, mass, radius,
-// position + shift, linear and angular velocity, shape; flags are not sent,
-// src/pairs/sim/comm.py:61-72)
+// ordered scatter into the send map (index order within each side, side 0
+// entries before side 1); borders also create the ghosts at nlocal + send
+// index (uid, type, mass, radius, position + shift, linear and angular
+// velocity, shape; flags are not sent, src/pairs/sim/comm.py:61-72).  Every
+// workgroup sums the per-block counts of the count pass itself (totals, and
+// the counts of the blocks before its own), so the offsets need no pass of
+// their own; all workgroups take the same capacity decision, and workgroup 0
+// updates the counts (a border dimension's ghosts join the ghost range), which
+// no other workgroup of this pass reads.
 __global__ __launch_bounds__(256) void k_sel_scatter(DemPtrs p, int *__restrict__ c, int dim, double lo, double hi,
                                                      double off, double len, int exch, const int *__restrict__ blk,
                                                      int pbc0, int pbc1, int send_cap, int cap) {
     __shared__ int wbase[2][4];
+    __shared__ int red[8];
     if (c[D_OVF]) return;
     const int n = c[D_RANGE], nl = c[D_NLOCAL];
-    const int base = c[D_BASE], side0 = c[D_SIDE0];
+    const int nblk = (n + 255) / 256;
+    if (blockIdx.x > 0 && (int) blockIdx.x >= nblk) return;  // workgroup 0 always writes the counts
+    const int base = c[D_BASE];
+    int t0 = 0, t1 = 0, q0 = 0, q1 = 0;
+    for (int u = threadIdx.x; u < nblk; u += 256) {
+        const int x = blk[2 * u], y = blk[2 * u + 1];
+        t0 += x; t1 += y;
+        if (u < (int) blockIdx.x) { q0 += x; q1 += y; }
+    }
+    int T0, T1, P0, P1;
+    block_sum2_256(t0, t1, red, T0, T1);
+    block_sum2_256(q0, q1, red, P0, P1);
+    const int need = base + T0 + T1;
+    if (need > send_cap || (!exch && nl + need > cap)) {
+        // the counts stay within the maps and arrays
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (need > send_cap) {
+                atomicMax(&c[D_REQ_SEND], need);
+                atomicMax(&c[D_OVF], DOVF_SEND);
+            } else {
+                atomicMax(&c[D_REQ_PART], nl + need);
+                atomicMax(&c[D_OVF], DOVF_PART);
+            }
+        }
+        return;
+    }
     const double *Xd = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
     const int i = blockIdx.x * 256 + threadIdx.x;
     bool a = false, b = false;
@@ -155,9 +179,7 @@ __global__ __launch_bounds__(256) void k_sel_scatter(DemPtrs p, int *__restrict_
     for (int s = 0; s < 2; s++) {
         const bool sel = s == 0 ? a : b;
         if (!sel) continue;
-        const int idx = base + (s == 0 ? blk[2 * blockIdx.x] + pa + lane_rank(ma)
-                                       : side0 + blk[2 * blockIdx.x + 1] + pb + lane_rank(mb));
-        if (idx >= send_cap) continue;
+        const int idx = base + (s == 0 ? P0 + pa + lane_rank(ma) : T0 + P1 + pb + lane_rank(mb));
         const int mult = s == 0 ? pbc0 : pbc1;
         p.send_map[idx] = i;
         p.send_mult[3 * idx + 0] = dim == 0 ? mult : 0;
@@ -167,7 +189,6 @@ __global__ __launch_bounds__(256) void k_sel_scatter(DemPtrs p, int *__restrict_
             p.exchg_flag[i] = 1;
         } else {
             const int g = nl + idx;
-            if (g >= cap) { atomicMax(&c[D_REQ_PART], g + 1); atomicMax(&c[D_OVF], DOVF_PART); continue; }
             // pack: src += send_mult * length (comm.py:312-318)
             const double sh = (double) mult * len;
             p.uid[g] = p.uid[i]; p.type[g] = p.type[i]; p.m[g] = p.m[i]; p.r[g] = p.r[i];
@@ -179,11 +200,12 @@ __global__ __launch_bounds__(256) void k_sel_scatter(DemPtrs p, int *__restrict_
             p.shape[g] = p.shape[i];
         }
     }
-}
-
-// ghosts of this dimension join the ghost range
-__global__ void k_borders_commit(int *__restrict__ c) {
-    if (threadIdx.x == 0 && !c[D_OVF]) c[D_NGHOST] = c[D_NSEND_ALL];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c[D_NSEND + 2 * dim] = T0;
+        c[D_NSEND + 2 * dim + 1] = T1;
+        c[D_NSEND_ALL] = need;
+        if (!exch) c[D_NGHOST] = need;
+    }
 }
 
 __global__ void k_reset_comm(int *__restrict__ c) {
@@ -198,9 +220,8 @@ __global__ void k_reset_comm(int *__restrict__ c) {
 
 // remove_exchanged_particles_pt1 (src/pairs/sim/comm.py:445-470): holes of
 // leavers below nlocal - n are filled from the tail, skipping leavers
-__global__ void k_exch_pt1(const int *__restrict__ c, const int *__restrict__ send_map,
-                           const int *__restrict__ exchg_flag, int *__restrict__ copy_to) {
-    if (threadIdx.x != 0 || c[D_OVF]) return;
+__device__ __forceinline__ void exch_pt1(const int *__restrict__ c, const int *__restrict__ send_map,
+                                         const int *__restrict__ exchg_flag, int *__restrict__ copy_to) {
     const int n = c[D_NSEND_ALL], nl = c[D_NLOCAL];
     int pos = nl - 1;
     for (int i = 0; i < n; i++) {
@@ -233,11 +254,9 @@ __device__ __forceinline__ void copy_particle(DemPtrs &p, int dst, int src) {
 }
 
 // pack the leavers (all non-volatile properties, position shifted)
-__global__ __launch_bounds__(256) void k_exch_pack(DemPtrs p, const int *__restrict__ c, double lx, double ly,
-                                                   double lz) {
-    if (c[D_OVF]) return;
+__device__ __forceinline__ void exch_pack(DemPtrs &p, const int *__restrict__ c, double lx, double ly, double lz) {
     const int n = c[D_NSEND_ALL];
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int m = p.send_map[i];
         double *b = p.buf + (size_t) EXB * i;
         const int *mu = p.send_mult + 3 * i;
@@ -256,10 +275,9 @@ __global__ __launch_bounds__(256) void k_exch_pack(DemPtrs p, const int *__restr
 
 // remove_exchanged_particles_pt2 (comm.py:473-506): copy guard "src > 0",
 // properties and contact history move from the tail into the hole
-__global__ __launch_bounds__(256) void k_exch_pt2(DemPtrs p, const int *__restrict__ c) {
-    if (c[D_OVF]) return;
+__device__ __forceinline__ void exch_pt2(DemPtrs &p, const int *__restrict__ c) {
     const int n = c[D_NSEND_ALL];
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int src = p.copy_to[i];
         if (src <= 0) continue;
         const int dst = p.send_map[i];
@@ -283,10 +301,9 @@ __global__ __launch_bounds__(256) void k_exch_pt2(DemPtrs p, const int *__restri
 // the leaver's slot as it is after the hole filling (the reference packs the
 // history after pt2); every contact's uid lands in one buffer word, so the
 // last contact's uid is the one that travels
-__global__ __launch_bounds__(256) void k_exch_unpack(DemPtrs p, const int *__restrict__ c) {
-    if (c[D_OVF]) return;
+__device__ __forceinline__ void exch_unpack(DemPtrs &p, const int *__restrict__ c) {
     const int n = c[D_NSEND_ALL], base = c[D_NLOCAL] - n;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const double *b = p.buf + (size_t) EXB * i;
         const int g = base + i;
         p.uid[g] = (int) b[0]; p.shape[g] = (int) b[1]; p.flags[g] = (int) b[2];
@@ -302,11 +319,10 @@ __global__ __launch_bounds__(256) void k_exch_unpack(DemPtrs p, const int *__res
     }
 }
 
-__global__ __launch_bounds__(256) void k_hist_pack(DemPtrs p, const int *__restrict__ c) {
-    if (c[D_OVF]) return;
+__device__ __forceinline__ void hist_pack(DemPtrs &p, const int *__restrict__ c) {
     const int n = c[D_NSEND_ALL];
     const int hb = 2 + 5 * p.ccap;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const int m = p.send_map[i];
         double *b = p.hbuf + (size_t) hb * i;
         const int nc = p.num_contacts[m];
@@ -325,11 +341,10 @@ __global__ __launch_bounds__(256) void k_hist_pack(DemPtrs p, const int *__restr
     }
 }
 
-__global__ __launch_bounds__(256) void k_hist_unpack(DemPtrs p, const int *__restrict__ c) {
-    if (c[D_OVF]) return;
+__device__ __forceinline__ void hist_unpack(DemPtrs &p, const int *__restrict__ c) {
     const int n = c[D_NSEND_ALL], base = c[D_NLOCAL] - n;
     const int hb = 2 + 5 * p.ccap;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
         const double *b = p.hbuf + (size_t) hb * i;
         const int g = base + i;
         const int nc = (int) b[0];
@@ -345,6 +360,27 @@ __global__ __launch_bounds__(256) void k_hist_unpack(DemPtrs p, const int *__res
             p.c_ivm[a] = b[6 + 5 * k];
         }
     }
+}
+
+// The exchange of one dimension after its send map is built, in one
+// workgroup (a step moves few particles across a face): the tail fill plan
+// (pt1, sequential as the reference's), the leavers packed, the holes filled
+// from the tail with their contact history (pt2), the leavers unpacked at
+// nlocal - n + i, their history packed after the hole filling and unpacked --
+// the reference's order, each phase finished before the next begins.
+__global__ __launch_bounds__(1024) void k_exchange(DemPtrs p, const int *__restrict__ c, double lx, double ly,
+                                                   double lz) {
+    if (c[D_OVF] || c[D_NSEND_ALL] == 0) return;
+    if (threadIdx.x == 0) exch_pt1(c, p.send_map, p.exchg_flag, p.copy_to);
+    exch_pack(p, c, lx, ly, lz);
+    __syncthreads();
+    exch_pt2(p, c);
+    __syncthreads();
+    exch_unpack(p, c);
+    __syncthreads();
+    hist_pack(p, c);
+    __syncthreads();
+    hist_unpack(p, c);
 }
 
 // ---------------------------------------------------------------- cells
@@ -417,6 +453,9 @@ __global__ __launch_bounds__(256) void k_cell_order(DemPtrs p, int ncells) {
     const int cell = blockIdx.x * 256 + threadIdx.x;
     if (cell >= ncells) return;
     const int s0 = p.cell_start[cell], cnt = p.cell_count[cell];
+    // the counts are not used after this pass: cleared for the next binning
+    p.cell_count[cell] = 0;
+    p.cell_fill[cell] = 0;
     int *cp = p.cell_particles + s0;
     for (int a = 1; a < cnt; a++) {
         const int v = cp[a];
@@ -928,43 +967,35 @@ void launch_mass_inertia(hipStream_t s, const DemPtrs &p, int nl) {
     hipLaunchKernelGGL(k_mass_inertia, dim3(grid_of(nl, 256)), dim3(256), 0, s, p, nl);
 }
 
-static void select_dim(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, int dim, int exch) {
+static void select_dim(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g, int dim, int exch, int first) {
     const double *Xd = dim == 0 ? p.x : (dim == 1 ? p.y : p.z);
     const int nb = grid_of(p.cap, 256);
     const double off = exch ? 0.0 : g.width;
-    hipLaunchKernelGGL(k_sel_count, dim3(nb), dim3(256), 0, s, Xd, p.flags, c, g.lo[dim], g.hi[dim], off, exch,
+    hipLaunchKernelGGL(k_sel_count, dim3(nb), dim3(256), 0, s, Xd, p.flags, c, g.lo[dim], g.hi[dim], off, exch, first,
                        p.blk, p.exchg_flag);
-    hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, s, c, p.blk, nb, dim, p.send_cap);
     hipLaunchKernelGGL(k_sel_scatter, dim3(nb), dim3(256), 0, s, p, c, dim, g.lo[dim], g.hi[dim], off,
                        g.hi[dim] - g.lo[dim], exch, p.blk, 1, -1, p.send_cap, p.cap);
 }
 
-// exchange, borders, cells of one step
+// exchange, borders, cells of one step (the cell counts are zero on entry:
+// allocated zeroed, cleared by every cell ordering pass)
 void launch_prepare(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g) {
-    const int sgrid = std::max(1, std::min(grid_of(p.send_cap, 256), 1024));
     // exchange, dimension by dimension (comm.py:100-151); one rank: only the
     // periodic dimensions send
     for (int dim = 0; dim < 3; dim++) {
         if (!g.periodic[dim]) continue;
-        hipLaunchKernelGGL(k_reset_comm, dim3(1), dim3(64), 0, s, c);
-        select_dim(s, p, c, g, dim, 1);
-        hipLaunchKernelGGL(k_exch_pt1, dim3(1), dim3(64), 0, s, c, p.send_map, p.exchg_flag, p.copy_to);
-        hipLaunchKernelGGL(k_exch_pack, dim3(sgrid), dim3(256), 0, s, p, c, g.len[0], g.len[1], g.len[2]);
-        hipLaunchKernelGGL(k_exch_pt2, dim3(sgrid), dim3(256), 0, s, p, c);
-        hipLaunchKernelGGL(k_exch_unpack, dim3(sgrid), dim3(256), 0, s, p, c);
-        hipLaunchKernelGGL(k_hist_pack, dim3(sgrid), dim3(256), 0, s, p, c);
-        hipLaunchKernelGGL(k_hist_unpack, dim3(sgrid), dim3(256), 0, s, p, c);
+        select_dim(s, p, c, g, dim, 1, 0);
+        hipLaunchKernelGGL(k_exchange, dim3(1), dim3(1024), 0, s, p, c, g.len[0], g.len[1], g.len[2]);
     }
     // borders (comm.py:56-98)
-    hipLaunchKernelGGL(k_reset_comm, dim3(1), dim3(64), 0, s, c);
+    bool first = true;
     for (int dim = 0; dim < 3; dim++) {
         if (!g.periodic[dim]) continue;
-        select_dim(s, p, c, g, dim, 0);
-        hipLaunchKernelGGL(k_borders_commit, dim3(1), dim3(64), 0, s, c);
+        select_dim(s, p, c, g, dim, 0, first ? 1 : 0);
+        first = false;
     }
+    if (first) hipLaunchKernelGGL(k_reset_comm, dim3(1), dim3(64), 0, s, c);  // no periodic dimension: no ghosts
     // cells
-    launch_fill(s, p.cell_count, 0, g.ncells + 1);
-    launch_fill(s, p.cell_fill, 0, g.ncells + 1);
     const int pg = grid_of(p.cap, 256);
     hipLaunchKernelGGL(k_cell_count, dim3(pg), dim3(256), 0, s, p, c, g);
     const int nb = grid_of(g.ncells, 1024);

@@ -18,7 +18,7 @@ enum {
     D_NSEND = 3,   // 6 entries
     D_RANGE = 9,   // nlocal + nghost latched by the selection of a dimension
     D_BASE = 10,   // send entries before this dimension
-    D_SIDE0 = 11,  // side-0 entries of this dimension
+    D_UNUSED = 11,
     D_OVF = 12,
     D_REQ_SEND = 13,
     D_REQ_CONTACT = 14,
