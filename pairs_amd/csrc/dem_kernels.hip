@@ -408,7 +408,7 @@ __global__ __launch_bounds__(256) void k_cell_count(DemPtrs p, const int *__rest
     atomicAdd(&p.cell_count[cell], 1);
 }
 
-// exclusive scan of cell counts: block partials, block offsets, add
+// exclusive scan of cell counts: block partials, then each block adds its offset
 __global__ __launch_bounds__(1024) void k_scan_partial(const int *__restrict__ in, int *__restrict__ out,
                                                        int *__restrict__ bsum, int n) {
     __shared__ int lds[20];
@@ -419,22 +419,21 @@ __global__ __launch_bounds__(1024) void k_scan_partial(const int *__restrict__ i
     if (threadIdx.x == 0) bsum[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_blocks(int *__restrict__ bsum, int nb, int *__restrict__ total) {
-    __shared__ int lds[20];
-    int carry = 0;
-    for (int b0 = 0; b0 < nb; b0 += 1024) {
-        const int b = b0 + threadIdx.x;
-        int t;
-        const int e = block_scan_excl<1024>(b < nb ? bsum[b] : 0, lds, &t);
-        if (b < nb) bsum[b] = carry + e;
-        carry += t;
-    }
-    if (threadIdx.x == 0) *total = carry;
-}
-
+// each block adds the totals of the blocks before it (it sums them itself:
+// a few hundred words, so the block offsets need no pass of their own)
 __global__ __launch_bounds__(1024) void k_scan_add(int *__restrict__ out, const int *__restrict__ bsum, int n) {
+    __shared__ int red[16];
+    int q = 0;
+    for (int b = threadIdx.x; b < (int) blockIdx.x; b += 1024) q += bsum[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) q += __shfl_xor(q, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = q;
+    __syncthreads();
+    int off = 0;
+#pragma unroll
+    for (int w = 0; w < 16; w++) off += red[w];
     const int i = blockIdx.x * 1024 + threadIdx.x;
-    if (i < n) out[i] += bsum[blockIdx.x];
+    if (i < n) out[i] += off;
 }
 
 __global__ __launch_bounds__(256) void k_cell_fill(DemPtrs p, const int *__restrict__ c) {
@@ -1000,7 +999,6 @@ void launch_prepare(hipStream_t s, const DemPtrs &p, int *c, const DemGeom &g) {
     hipLaunchKernelGGL(k_cell_count, dim3(pg), dim3(256), 0, s, p, c, g);
     const int nb = grid_of(g.ncells, 1024);
     hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(1024), 0, s, p.cell_count, p.cell_start, p.scan_tmp, g.ncells);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, p.scan_tmp, nb, p.scan_tmp + nb);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, p.cell_start, p.scan_tmp, g.ncells);
     hipLaunchKernelGGL(k_cell_fill, dim3(pg), dim3(256), 0, s, p, c);
     hipLaunchKernelGGL(k_cell_order, dim3(grid_of(g.ncells, 256)), dim3(256), 0, s, p, g.ncells);
