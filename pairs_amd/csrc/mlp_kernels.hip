@@ -246,150 +246,168 @@ struct MLPSegArgs {
     float *U;            // MODE 2: [cap][16] back-projected directions
 };
 
-// G of the 64 locals of one wave (MODE 0), or their force, energy and
+// G of the 32 locals of one wave (MODE 0), or their force, energy and
 // virial (MODE 1), or (MODE 2) the back-projection of force residuals onto
 // the descriptor gradients: with rho_i = dL/dF_i and F_i = -sum_j w_ij .
 // (D_i + D_j), w_ij,k = dphi_k(r_ij) (x_i - x_j) / r_ij,
 //   dL/dD_a,k = -sum_{j in list(a)} w_aj,k . (rho_a - rho_j),
 // written as U_a,k = e_std g_istd,k dL/dD_a,k (the direction the network's
-// input gradient is differentiated along).  One lane per local, the full list
-// walked four entries per step as in the LJ kernel; pairs at r >= 2.5 and
-// entries past a lane's own count are masked.
+// input gradient is differentiated along).  Two lanes per local: lane 2p + h
+// walks entries 4s + h and 4s + 2 + h of local p's list (one 16-byte group of
+// four per step), the pair's partial sums are added across the two lanes at
+// the end (FP32 / FP64 sums in any order: the potential is not bit-matched to
+// a reference), so twice as many waves hide the gathers' latency.  Pairs at
+// r >= 2.5 and entries past a lane's count are masked.
 template <int MODE, bool RED>
 __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
     const int wv = xcd_block(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, h = lane & 1;
     const int nlocal = A.counters[C_NLOCAL];
     const uint32_t jmax = (uint32_t) max(nlocal + A.counters[C_NGHOST] - 1, 0);
     const char *__restrict__ P = (const char *) A.P;
     const double rc2 = MLP_RC * MLP_RC;
     double pe = 0.0, vir = 0.0;
-    const int i0 = wv * 64;
+    const int i0 = wv * MLP_WAVE_LOCALS;
     if (i0 < nlocal) {  // wave-uniform
-        {
-            const int i = i0 + lane;
-            const bool live = i < nlocal;
-            const int ii = live ? i : i0;
-            const int n = live ? min(A.numneighs[ii], A.ncap) : 0;
-            const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
-            const double xi = A.X[ii], yi = A.Y[ii], zi = A.Z[ii];
-            float acc[MLP_ND], di[MLP_ND];
+        const int i = i0 + (lane >> 1);
+        const bool live = i < nlocal;
+        const int ii = live ? i : i0;
+        const int n = live ? min(A.numneighs[ii], A.ncap) : 0;
+        const int nmax = __builtin_amdgcn_readfirstlane(wave_max(n));
+        const double xi = A.X[ii], yi = A.Y[ii], zi = A.Z[ii];
+        float acc[MLP_ND], di[MLP_ND];
 #pragma unroll
-            for (int k = 0; k < MLP_ND; k++) acc[k] = 0.f;
-            if (MODE == 1) {
-                const float4 *dp = (const float4 *) (A.D + (size_t) ii * MLP_ND);
+        for (int k = 0; k < MLP_ND; k++) acc[k] = 0.f;
+        if (MODE == 1) {
+            const float4 *dp = (const float4 *) (A.D + (size_t) ii * MLP_ND);
 #pragma unroll
-                for (int k4 = 0; k4 < 4; k4++) {
-                    const float4 d = dp[k4];
-                    di[4 * k4] = d.x; di[4 * k4 + 1] = d.y; di[4 * k4 + 2] = d.z; di[4 * k4 + 3] = d.w;
-                }
+            for (int k4 = 0; k4 < 4; k4++) {
+                const float4 d = dp[k4];
+                di[4 * k4] = d.x; di[4 * k4 + 1] = d.y; di[4 * k4 + 2] = d.z; di[4 * k4 + 3] = d.w;
             }
-            const float4 rhoi = MODE == 2 ? A.rho[ii] : make_float4(0.f, 0.f, 0.f, 0.f);
-            double f0 = 0.0, f1 = 0.0, f2 = 0.0, w = 0.0;
-            const float gb = exp2f(-2.0f * MLP_DMU * MLP_DMU * MLP_C);  // a_{k+1} / a_k
-            list_walk32<4>(A.nl, 0, ii, A.nl_stride, n, nmax, jmax, [&](int k, const uint32_t *jp) {
-                double xj[4], yj[4], zj[4];
-                float4 dj[4][4], rj[4];
+        }
+        const float4 rhoi = MODE == 2 ? A.rho[ii] : make_float4(0.f, 0.f, 0.f, 0.f);
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0, w = 0.0;
+        const float gb = exp2f(-2.0f * MLP_DMU * MLP_DMU * MLP_C);  // a_{k+1} / a_k
+        const uint4 *lp = (const uint4 *) A.nl + ii;
+        const int gl = max((n + 3) / 4 - 1, 0);
+        const int nst = (nmax + 3) / 4;
+        for (int st = 0; st < nst; st++) {
+            // groups past the lane's list re-read its last group; entries are
+            // clamped to the last particle slot
+            const uint4 grp = lp[(size_t) min(st, gl) * A.nl_stride];
+            const uint32_t jq[2] = {min(h ? grp.y : grp.x, jmax), min(h ? grp.w : grp.z, jmax)};
+            double xj[2], yj[2], zj[2];
+            float4 dj[2][4], rj[2];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t off = jp[q] << 5;
-                    const double2 pxy = *(const double2 *) (P + off);
-                    xj[q] = pxy.x; yj[q] = pxy.y; zj[q] = *(const double *) (P + off + 16);
-                    if (MODE == 1) {
-                        const float4 *dp = (const float4 *) (A.D + (size_t) jp[q] * MLP_ND);
+            for (int q = 0; q < 2; q++) {
+                const uint32_t off = jq[q] << 5;
+                const double2 pxy = *(const double2 *) (P + off);
+                xj[q] = pxy.x; yj[q] = pxy.y; zj[q] = *(const double *) (P + off + 16);
+                if (MODE == 1) {
+                    const float4 *dp = (const float4 *) (A.D + (size_t) jq[q] * MLP_ND);
 #pragma unroll
-                        for (int k4 = 0; k4 < 4; k4++) dj[q][k4] = dp[k4];
-                    }
-                    if (MODE == 2) rj[q] = A.rho[jp[q]];
+                    for (int k4 = 0; k4 < 4; k4++) dj[q][k4] = dp[k4];
                 }
+                if (MODE == 2) rj[q] = A.rho[jq[q]];
+            }
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const double dx = xi - xj[q], dy = yi - yj[q], dz = zi - zj[q];
-                    double rsq = dx * dx + dy * dy;
-                    rsq = rsq + dz * dz;
-                    const bool in = (k + q < n) && rsq < rc2;
-                    // masked entries take an in-range stand-in distance, so
-                    // the Gaussian recurrence below stays finite
-                    const float r = in ? sqrtf((float) rsq) : MLP_MU0;
-                    // g_k by recurrence over the uniform centres,
-                    // g_{k+1} = g_k a_k, a_{k+1} = a_k b, from two directly
-                    // computed anchors (k = 0 and k = 8): four exponentials per
-                    // pair instead of sixteen, relative error below 1e-5
-                    const float t0 = r - MLP_MU0;
-                    auto anchor = [&](int k0, float &g, float &a) {
-                        const float t = t0 - (float) k0 * MLP_DMU;
-                        g = exp2f(-(t * t) * MLP_C);
-                        a = exp2f((2.0f * t - MLP_DMU) * MLP_DMU * MLP_C);
-                    };
-                    float gk, ak;
-                    if (MODE == 0) {
-                        const float fc = in ? 0.5f * (cosf(MLP_PIRC * r) + 1.0f) : 0.0f;
-#pragma unroll
-                        for (int kk = 0; kk < MLP_ND; kk++) {
-                            if (kk % 8 == 0) anchor(kk, gk, ak);
-                            acc[kk] = fmaf(gk, fc, acc[kk]);
-                            gk *= ak;
-                            ak *= gb;
-                        }
-                    } else if (MODE == 2) {
-                        float sn, cs;
-                        sincosf(MLP_PIRC * r, &sn, &cs);
-                        const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
-                        // (x_a - x_j) / r . (rho_a - rho_j), zero for a masked entry
-                        float proj = (float) dx * (rhoi.x - rj[q].x) + (float) dy * (rhoi.y - rj[q].y);
-                        proj = proj + (float) dz * (rhoi.z - rj[q].z);
-                        proj = in ? -proj / r : 0.0f;
-#pragma unroll
-                        for (int kk = 0; kk < MLP_ND; kk++) {
-                            if (kk % 8 == 0) anchor(kk, gk, ak);
-                            const float t = t0 - (float) kk * MLP_DMU;
-                            const float dphi = gk * fmaf(-t * MLP_IW2, fc, dfc);
-                            acc[kk] = fmaf(dphi, proj, acc[kk]);
-                            gk *= ak;
-                            ak *= gb;
-                        }
-                    } else {
-                        float sn, cs;
-                        sincosf(MLP_PIRC * r, &sn, &cs);
-                        const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
-                        float ci = 0.f, cj = 0.f;
-                        const float *djq = (const float *) dj[q];
-#pragma unroll
-                        for (int kk = 0; kk < MLP_ND; kk++) {
-                            if (kk % 8 == 0) anchor(kk, gk, ak);
-                            const float t = t0 - (float) kk * MLP_DMU;
-                            const float dphi = gk * fmaf(-t * MLP_IW2, fc, dfc);
-                            ci = fmaf(di[kk], dphi, ci);
-                            cj = fmaf(djq[kk], dphi, cj);
-                            gk *= ak;
-                            ak *= gb;
-                        }
-                        const float cc = ci + cj;
-                        const double coef = in ? (double) (-cc / r) : 0.0;
-                        f0 += coef * dx; f1 += coef * dy; f2 += coef * dz;
-                        if (RED) w += in ? (double) (-cc * r) : 0.0;
-                    }
-                }
-            });
-            if (live) {
+            for (int q = 0; q < 2; q++) {
+                const double dx = xi - xj[q], dy = yi - yj[q], dz = zi - zj[q];
+                double rsq = dx * dx + dy * dy;
+                rsq = rsq + dz * dz;
+                const bool in = (4 * st + 2 * q + h < n) && rsq < rc2;
+                // masked entries take an in-range stand-in distance, so the
+                // Gaussian recurrence below stays finite
+                const float r = in ? sqrtf((float) rsq) : MLP_MU0;
+                // g_k by recurrence over the uniform centres,
+                // g_{k+1} = g_k a_k, a_{k+1} = a_k b, from two directly
+                // computed anchors (k = 0 and k = 8): four exponentials per
+                // pair instead of sixteen, relative error below 1e-5
+                const float t0 = r - MLP_MU0;
+                auto anchor = [&](int k0, float &g, float &a) {
+                    const float t = t0 - (float) k0 * MLP_DMU;
+                    g = exp2f(-(t * t) * MLP_C);
+                    a = exp2f((2.0f * t - MLP_DMU) * MLP_DMU * MLP_C);
+                };
+                float gk, ak;
                 if (MODE == 0) {
-                    float4 *gp = (float4 *) (A.G + (size_t) i * MLP_ND);
+                    const float fc = in ? 0.5f * (cosf(MLP_PIRC * r) + 1.0f) : 0.0f;
 #pragma unroll
-                    for (int k4 = 0; k4 < 4; k4++)
-                        gp[k4] = make_float4(acc[4 * k4], acc[4 * k4 + 1], acc[4 * k4 + 2], acc[4 * k4 + 3]);
+                    for (int kk = 0; kk < MLP_ND; kk++) {
+                        if (kk % 8 == 0) anchor(kk, gk, ak);
+                        acc[kk] = fmaf(gk, fc, acc[kk]);
+                        gk *= ak;
+                        ak *= gb;
+                    }
                 } else if (MODE == 2) {
-                    const float es = A.sc[MLP_ESTD];
-                    float4 *up = (float4 *) (A.U + (size_t) i * MLP_ND);
+                    float sn, cs;
+                    sincosf(MLP_PIRC * r, &sn, &cs);
+                    const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
+                    // (x_a - x_j) / r . (rho_a - rho_j), zero for a masked entry
+                    float proj = (float) dx * (rhoi.x - rj[q].x) + (float) dy * (rhoi.y - rj[q].y);
+                    proj = proj + (float) dz * (rhoi.z - rj[q].z);
+                    proj = in ? -proj / r : 0.0f;
 #pragma unroll
-                    for (int k4 = 0; k4 < 4; k4++) {
-                        const float *gi = A.sc + MLP_GISTD + 4 * k4;
-                        up[k4] = make_float4(es * gi[0] * acc[4 * k4], es * gi[1] * acc[4 * k4 + 1],
-                                             es * gi[2] * acc[4 * k4 + 2], es * gi[3] * acc[4 * k4 + 3]);
+                    for (int kk = 0; kk < MLP_ND; kk++) {
+                        if (kk % 8 == 0) anchor(kk, gk, ak);
+                        const float t = t0 - (float) kk * MLP_DMU;
+                        const float dphi = gk * fmaf(-t * MLP_IW2, fc, dfc);
+                        acc[kk] = fmaf(dphi, proj, acc[kk]);
+                        gk *= ak;
+                        ak *= gb;
                     }
                 } else {
-                    A.fx[i] = f0; A.fy[i] = f1; A.fz[i] = f2;
-                    if (RED) { pe += (double) A.E[i]; vir += 0.5 * w; }
+                    float sn, cs;
+                    sincosf(MLP_PIRC * r, &sn, &cs);
+                    const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
+                    float ci = 0.f, cj = 0.f;
+                    const float *djq = (const float *) dj[q];
+#pragma unroll
+                    for (int kk = 0; kk < MLP_ND; kk++) {
+                        if (kk % 8 == 0) anchor(kk, gk, ak);
+                        const float t = t0 - (float) kk * MLP_DMU;
+                        const float dphi = gk * fmaf(-t * MLP_IW2, fc, dfc);
+                        ci = fmaf(di[kk], dphi, ci);
+                        cj = fmaf(djq[kk], dphi, cj);
+                        gk *= ak;
+                        ak *= gb;
+                    }
+                    const float cc = ci + cj;
+                    const double coef = in ? (double) (-cc / r) : 0.0;
+                    f0 += coef * dx; f1 += coef * dy; f2 += coef * dz;
+                    if (RED) w += in ? (double) (-cc * r) : 0.0;
                 }
+            }
+        }
+        // the two lanes of a local: partial sums added
+        if (MODE == 1) {
+            f0 += __shfl_xor(f0, 1, 64);
+            f1 += __shfl_xor(f1, 1, 64);
+            f2 += __shfl_xor(f2, 1, 64);
+            if (RED) w += __shfl_xor(w, 1, 64);
+        } else {
+#pragma unroll
+            for (int k = 0; k < MLP_ND; k++) acc[k] += __shfl_xor(acc[k], 1, 64);
+        }
+        if (live && h == 0) {
+            if (MODE == 0) {
+                float4 *gp = (float4 *) (A.G + (size_t) i * MLP_ND);
+#pragma unroll
+                for (int k4 = 0; k4 < 4; k4++)
+                    gp[k4] = make_float4(acc[4 * k4], acc[4 * k4 + 1], acc[4 * k4 + 2], acc[4 * k4 + 3]);
+            } else if (MODE == 2) {
+                const float es = A.sc[MLP_ESTD];
+                float4 *up = (float4 *) (A.U + (size_t) i * MLP_ND);
+#pragma unroll
+                for (int k4 = 0; k4 < 4; k4++) {
+                    const float *gi = A.sc + MLP_GISTD + 4 * k4;
+                    up[k4] = make_float4(es * gi[0] * acc[4 * k4], es * gi[1] * acc[4 * k4 + 1],
+                                         es * gi[2] * acc[4 * k4 + 2], es * gi[3] * acc[4 * k4 + 3]);
+                }
+            } else {
+                A.fx[i] = f0; A.fy[i] = f1; A.fz[i] = f2;
+                if (RED) { pe += (double) A.E[i]; vir += 0.5 * w; }
             }
         }
     }
@@ -402,6 +420,8 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
         }
     }
 }
+
+int mlp_waves(const DevState &d) { return (d.nlocal + MLP_WAVE_LOCALS - 1) / MLP_WAVE_LOCALS; }
 
 // dE/dG of every ghost = that of the local at the end of its source chain
 __global__ __launch_bounds__(256) void k_mlp_ghost_d(const int *__restrict__ counters, const int *__restrict__ sync_src,
@@ -908,7 +928,7 @@ void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int
     launch_pack_p(s, d, cur);  // both passes gather packed positions
     MLPSegArgs A = seg_args(d, cur);
     A.G = G;
-    hipLaunchKernelGGL((k_mlp_wave<0, false>), dim3(force_waves(d)), dim3(64), 0, s, A);
+    hipLaunchKernelGGL((k_mlp_wave<0, false>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
 }
 
 static int eval_grid(int n) {
@@ -933,8 +953,8 @@ void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const ML
     A.E = b.E;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz;
     A.partials = d.partials;
-    if (reduce) hipLaunchKernelGGL((k_mlp_wave<1, true>), dim3(force_waves(d)), dim3(64), 0, s, A);
-    else hipLaunchKernelGGL((k_mlp_wave<1, false>), dim3(force_waves(d)), dim3(64), 0, s, A);
+    if (reduce) hipLaunchKernelGGL((k_mlp_wave<1, true>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
+    else hipLaunchKernelGGL((k_mlp_wave<1, false>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
 }
 
 void launch_mlp_grad(hipStream_t s, const float *p, const float *sc, const MLPTrain &t, uint64_t seed, int step) {
@@ -965,7 +985,7 @@ void launch_mlp_config_grad(hipStream_t s, const DevState &d, const Geom &g, con
     A.rho = rho;
     A.sc = b.sc;
     A.U = U;
-    hipLaunchKernelGGL((k_mlp_wave<2, false>), dim3(force_waves(d)), dim3(64), 0, s, A);
+    hipLaunchKernelGGL((k_mlp_wave<2, false>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
     MLPCfgGrad tt = t;
     tt.G = b.G;
     tt.label = eatom;

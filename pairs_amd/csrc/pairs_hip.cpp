@@ -1363,7 +1363,7 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             // partials of the fused LJ kernel (one per wave), or of the MLP
             // force kernel (one per wave of 64) and the integrator (per 256)
             const int nb_ke = ctx->mlp_on ? (d.nlocal + 255) / 256 : lj_waves(d);
-            const int nb_f = ctx->mlp_on ? force_waves(d) : lj_waves(d);
+            const int nb_f = ctx->mlp_on ? mlp_waves(d) : lj_waves(d);
             launch_thermo_reduce(s, d, nb_ke, nb_f, 3, ctx->geom, d.thermo + 5 * nthermo);
             nthermo++;
         }
@@ -1449,7 +1449,7 @@ int pairs_hip_mlp_forces(pairs_hip_ctx *ctx, pairs_hip_mlp *m, double *energy, d
     if (int r = mlp_prepare(ctx, m)) return r;
     DevState &d = ctx->d;
     launch_mlp_forces(ctx->stream, d, ctx->geom, mlp_bind(ctx, m), d.cur, 1, ctx->ghost_grid);
-    launch_thermo_reduce(ctx->stream, d, 0, force_waves(d), 2, ctx->geom, d.thermo);
+    launch_thermo_reduce(ctx->stream, d, 0, mlp_waves(d), 2, ctx->geom, d.thermo);
     CHECK_LAUNCH();
     double out[5];
     if (download_raw(ctx, out, d.thermo, 5 * sizeof(double))) return PAIRS_HIP_ERR_HIP;

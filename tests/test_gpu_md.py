@@ -189,6 +189,34 @@ def test_deferred_rebuild_checks_match_immediate_ones(half):
     _compare_lists(a, b, tag=f"deferred half={half}")
 
 
+def test_deferred_check_repeat_keeps_thermo_rows():
+    """A send-map overflow at a later rebuild, found a rebuild later: the run
+    repeats from the checkpoint, and the thermo rows it returns (every 10
+    steps, some of them computed on the faulty rebuild before the repeat) are
+    those of checks after every rebuild; the state equals the oracle's."""
+    md = _md()
+    from pairs_amd import native as nat
+    o = po.fcc_system(4, rho=0.55)
+    o.step(0)
+    nsend0 = o.nsend_all
+    rows = []
+    for spec in (True, False):
+        g = md.fcc_system(4, rho=0.55, send_capacity=nsend0, thermo_every=10)
+        g.set_speculative_rebuilds(spec)
+        th = [g.run(0, 1, 80)]
+        th.append(g.run(1, 81, 80))
+        rows.append((g, th))
+    (a, ta), (b, tb) = rows
+    assert a.count(nat.REBUILD_ROLLBACKS) >= 1 and b.count(nat.REBUILD_ROLLBACKS) == 0
+    for x, y in zip(ta, tb):
+        assert x.shape == y.shape and np.array_equal(x, y)
+    assert len(tb[1]) == 8
+    o = po.fcc_system(4, rho=0.55)
+    o.run(0, 81)
+    _compare_state(a, o, tag="deferred repeat")
+    _compare_lists(a, o, tag="deferred repeat")
+
+
 @pytest.mark.parametrize("half", [False, True])
 def test_energy_and_virial_vs_oracle(half):
     md = _md()
