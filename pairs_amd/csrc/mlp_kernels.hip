@@ -252,24 +252,25 @@ struct MLPSegArgs {
 // (D_i + D_j), w_ij,k = dphi_k(r_ij) (x_i - x_j) / r_ij,
 //   dL/dD_a,k = -sum_{j in list(a)} w_aj,k . (rho_a - rho_j),
 // written as U_a,k = e_std g_istd,k dL/dD_a,k (the direction the network's
-// input gradient is differentiated along).  Two lanes per local: lane 2p + h
-// walks entries 4s + h and 4s + 2 + h of local p's list (one 16-byte group of
-// four per step), the pair's partial sums are added across the two lanes at
-// the end (FP32 / FP64 sums in any order: the potential is not bit-matched to
-// a reference), so twice as many waves hide the gathers' latency.  Pairs at
-// r >= 2.5 and entries past a lane's count are masked.
-template <int MODE, bool RED>
+// input gradient is differentiated along).  L lanes per local (2, or 4 for
+// small systems): lane L p + h walks entries 4s + L q + h of local p's list
+// (one 16-byte group of four per step), the L partial sums are added across
+// the lanes at the end (FP32 / FP64 sums in any order: the potential is not
+// bit-matched to a reference), so L times as many waves hide the gathers'
+// latency.  Pairs at r >= 2.5 and entries past a lane's count are masked.
+template <int MODE, bool RED, int L>
 __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
+    constexpr int U = 4 / L;  // entries per lane per step
     const int wv = xcd_block(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x & 63, h = lane & 1;
+    const int lane = threadIdx.x & 63, h = lane & (L - 1);
     const int nlocal = A.counters[C_NLOCAL];
     const uint32_t jmax = (uint32_t) max(nlocal + A.counters[C_NGHOST] - 1, 0);
     const char *__restrict__ P = (const char *) A.P;
     const double rc2 = MLP_RC * MLP_RC;
     double pe = 0.0, vir = 0.0;
-    const int i0 = wv * MLP_WAVE_LOCALS;
+    const int i0 = wv * (64 / L);
     if (i0 < nlocal) {  // wave-uniform
-        const int i = i0 + (lane >> 1);
+        const int i = i0 + lane / L;
         const bool live = i < nlocal;
         const int ii = live ? i : i0;
         const int n = live ? min(A.numneighs[ii], A.ncap) : 0;
@@ -296,11 +297,17 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
             // groups past the lane's list re-read its last group; entries are
             // clamped to the last particle slot
             const uint4 grp = lp[(size_t) min(st, gl) * A.nl_stride];
-            const uint32_t jq[2] = {min(h ? grp.y : grp.x, jmax), min(h ? grp.w : grp.z, jmax)};
-            double xj[2], yj[2], zj[2];
-            float4 dj[2][4], rj[2];
+            uint32_t jq[U];
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
+            for (int q = 0; q < U; q++) {
+                const int c = L * q + h;
+                const uint32_t e = (c & 2) ? ((c & 1) ? grp.w : grp.z) : ((c & 1) ? grp.y : grp.x);
+                jq[q] = min(e, jmax);
+            }
+            double xj[U], yj[U], zj[U];
+            float4 dj[U][4], rj[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
                 const uint32_t off = jq[q] << 5;
                 const double2 pxy = *(const double2 *) (P + off);
                 xj[q] = pxy.x; yj[q] = pxy.y; zj[q] = *(const double *) (P + off + 16);
@@ -312,11 +319,11 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                 if (MODE == 2) rj[q] = A.rho[jq[q]];
             }
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
+            for (int q = 0; q < U; q++) {
                 const double dx = xi - xj[q], dy = yi - yj[q], dz = zi - zj[q];
                 double rsq = dx * dx + dy * dy;
                 rsq = rsq + dz * dz;
-                const bool in = (4 * st + 2 * q + h < n) && rsq < rc2;
+                const bool in = (4 * st + L * q + h < n) && rsq < rc2;
                 // masked entries take an in-range stand-in distance, so the
                 // Gaussian recurrence below stays finite
                 const float r = in ? sqrtf((float) rsq) : MLP_MU0;
@@ -380,15 +387,18 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                 }
             }
         }
-        // the two lanes of a local: partial sums added
-        if (MODE == 1) {
-            f0 += __shfl_xor(f0, 1, 64);
-            f1 += __shfl_xor(f1, 1, 64);
-            f2 += __shfl_xor(f2, 1, 64);
-            if (RED) w += __shfl_xor(w, 1, 64);
-        } else {
+        // the L lanes of a local: partial sums added
 #pragma unroll
-            for (int k = 0; k < MLP_ND; k++) acc[k] += __shfl_xor(acc[k], 1, 64);
+        for (int o = 1; o < L; o <<= 1) {
+            if (MODE == 1) {
+                f0 += __shfl_xor(f0, o, 64);
+                f1 += __shfl_xor(f1, o, 64);
+                f2 += __shfl_xor(f2, o, 64);
+                if (RED) w += __shfl_xor(w, o, 64);
+            } else {
+#pragma unroll
+                for (int k = 0; k < MLP_ND; k++) acc[k] += __shfl_xor(acc[k], o, 64);
+            }
         }
         if (live && h == 0) {
             if (MODE == 0) {
@@ -421,7 +431,18 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
     }
 }
 
-int mlp_waves(const DevState &d) { return (d.nlocal + MLP_WAVE_LOCALS - 1) / MLP_WAVE_LOCALS; }
+// lanes per local of the list passes: 4 below 32768 locals (a small system
+// fills few waves otherwise), 2 above
+static int mlp_lanes(const DevState &d) { return d.nlocal < 32768 ? 4 : 2; }
+int mlp_waves(const DevState &d) {
+    const int per = 64 / mlp_lanes(d);
+    return (d.nlocal + per - 1) / per;
+}
+template <int MODE, bool RED>
+static void launch_mlp_wave(hipStream_t s, const DevState &d, const MLPSegArgs &A) {
+    if (mlp_lanes(d) == 4) hipLaunchKernelGGL((k_mlp_wave<MODE, RED, 4>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
+    else hipLaunchKernelGGL((k_mlp_wave<MODE, RED, 2>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
+}
 
 // dE/dG of every ghost = that of the local at the end of its source chain
 __global__ __launch_bounds__(256) void k_mlp_ghost_d(const int *__restrict__ counters, const int *__restrict__ sync_src,
@@ -928,12 +949,15 @@ void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int
     launch_pack_p(s, d, cur);  // both passes gather packed positions
     MLPSegArgs A = seg_args(d, cur);
     A.G = G;
-    hipLaunchKernelGGL((k_mlp_wave<0, false>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
+    launch_mlp_wave<0, false>(s, d, A);
 }
 
+// each wave converts the weights to BF16 registers once (about 21 KB read)
+// and then strides over 16-row tiles: 512 workgroups (two waves per SIMD)
+// amortise that over several tiles at 131072 rows
 static int eval_grid(int n) {
     const int ntiles = (n + 15) / 16;
-    return std::max(1, std::min((ntiles + 3) / 4, 2048));
+    return std::max(1, std::min((ntiles + 3) / 4, 512));
 }
 
 void launch_mlp_eval(hipStream_t s, const float *p, const float *sc, const float *G, int n, float *E, float *D) {
@@ -953,8 +977,8 @@ void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const ML
     A.E = b.E;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz;
     A.partials = d.partials;
-    if (reduce) hipLaunchKernelGGL((k_mlp_wave<1, true>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
-    else hipLaunchKernelGGL((k_mlp_wave<1, false>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
+    if (reduce) launch_mlp_wave<1, true>(s, d, A);
+    else launch_mlp_wave<1, false>(s, d, A);
 }
 
 void launch_mlp_grad(hipStream_t s, const float *p, const float *sc, const MLPTrain &t, uint64_t seed, int step) {
@@ -985,7 +1009,7 @@ void launch_mlp_config_grad(hipStream_t s, const DevState &d, const Geom &g, con
     A.rho = rho;
     A.sc = b.sc;
     A.U = U;
-    hipLaunchKernelGGL((k_mlp_wave<2, false>), dim3(mlp_waves(d)), dim3(64), 0, s, A);
+    launch_mlp_wave<2, false>(s, d, A);
     MLPCfgGrad tt = t;
     tt.G = b.G;
     tt.label = eatom;

@@ -33,9 +33,8 @@ constexpr int MLP_NP = MLP_B3 + 1;                // 5313 parameters
 // standardisation block
 constexpr int MLP_GMEAN = 0, MLP_GISTD = 16, MLP_EMEAN = 32, MLP_ESTD = 33, MLP_NS = 34;
 
-constexpr int MLP_WAVE_LOCALS = 32;  // locals per wave of the list passes (two lanes per local)
-// waves of the list passes over the locals (their energy / virial partials
-// are one per wave)
+// waves of the list passes over the locals (2 or 4 lanes per local; their
+// energy / virial partials are one per wave)
 int mlp_waves(const DevState &d);
 
 constexpr double MLP_RC = 2.5;

@@ -94,9 +94,11 @@ static void dfree(void *p) {
 
 // rows of the per-wave partials (force waves of 64 locals; blocks of 256 of
 // the stand-alone integrator)
+// one partial per wave of the reduction kernels: the fewest locals per wave
+// is 16 (the MLP list passes of a small system, four lanes per local)
 static size_t pstride_of(const DevState &d, const Geom &g) {
     (void) g;
-    return (size_t) d.cap / 32 + 8;
+    return (size_t) d.cap / 16 + 8;
 }
 
 static void free_particle_arrays(DevState &d) {
