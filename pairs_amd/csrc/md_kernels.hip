@@ -565,7 +565,9 @@ __device__ __forceinline__ void build_segment(const BuildArgs &A, int seg) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         // the cell's candidates, chunk c on lane c * 64 + lane; the first
         // CREG chunks stay in registers while every home particle tests them
-        constexpr int CREG = 8;
+        // (half lists: 6, which leaves room for the mirror bookkeeping at the
+        // same occupancy; 272 against 289 us at 131072 atoms)
+        constexpr int CREG = HALF ? 6 : 8;
         const int nchunk = (ncand + 63) >> 6;
         double rcx[CREG], rcy[CREG], rcz[CREG];
         int rco[CREG], rcr[CREG];
@@ -944,6 +946,14 @@ __device__ __forceinline__ double odd_lane(double v) {
     return __longlong_as_double(((long long) hi << 32) | (unsigned int) lo);
 }
 
+// the 64-bit value of the other lane of each lane pair (DPP quad_perm 1,0,3,2)
+__device__ __forceinline__ double pair_swap(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int) b, 0xB1, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int) (b >> 32), 0xB1, 0xF, 0xF, true);
+    return __longlong_as_double(((long long) hi << 32) | (unsigned int) lo);
+}
+
 // The force kernel of the timestep (MODE 0: forces; MODE 1: fused with
 // final/initial integrate and the next positions) on TWO lanes per local: a
 // wave takes 32 consecutive cell-sorted locals, lane 2p + h of local p
@@ -953,7 +963,12 @@ __device__ __forceinline__ double odd_lane(double v) {
 // reference's, bit for bit, as in lj_wave.  Twice the waves of lj_wave hide
 // the gather latency better; the positions of step s + 1 are gathered while
 // step s computes (two register sets, a loop body without branches: a last
-// odd step is computed fully masked, which adds +0).
+// odd step is computed fully masked, which adds +0).  The two lanes of a
+// local load each neighbour record together: for the entry pair (e, o) one
+// 16-byte load has lane 2p read record e's (x, y) and lane 2p + 1 record e's
+// (z, 0), the next has lane 2p read record o's (z, 0) and lane 2p + 1 record
+// o's (x, y), and the z halves are swapped by DPP, so a record is touched by
+// one load instruction (one cache-line lookup) instead of two.
 // HALF: the own part (rows [0, n)) into T and the mirror part (rows
 // [rev0, rev0 + nrev)) into R, force = R + T, as lj_wave.
 // RED: kinetic energy of the locals (lane 2p) and energy / virial of every
@@ -992,17 +1007,16 @@ __device__ __forceinline__ void lj_wave2(const LJArgs &A, int wv) {
         const int nt = tab.nt;
         const int stride = A.nl_stride;
         double u = 0.0, w = 0.0;
-        struct Pos { double x[U], y[U], z[U]; int t[U]; uint32_t j[U]; };
-        auto gather = [&](const uint32_t (&j)[U], Pos &p) {
+        struct Pos { double2 a[U], b[U]; int t[U]; uint32_t j[U]; };
+        // j[2q] / j[2q + 1]: the even / odd entry of pair q (both lanes)
+        auto gather = [&](const uint32_t (&j)[2 * U], Pos &p) {
 #pragma unroll
             for (int q = 0; q < U; q++) {
-                const uint32_t off = j[q] << 5;
-                const double2 v = *(const double2 *) (P + off);
-                p.x[q] = v.x;
-                p.y[q] = v.y;
-                p.z[q] = *(const double *) (P + off + 16);
-                if (NT != 1) p.t[q] = type[j[q]];
-                p.j[q] = j[q];
+                p.a[q] = *(const double2 *) (P + (j[2 * q] << 5) + (h << 4));
+                p.b[q] = *(const double2 *) (P + (j[2 * q + 1] << 5) + ((h ^ 1) << 4));
+                const uint32_t jo = h ? j[2 * q + 1] : j[2 * q];
+                if (NT != 1) p.t[q] = type[jo];
+                p.j[q] = jo;
             }
         };
         // one list part, rows [row0, row0 + cnt), summed in order into (a0, a1, a2) of lane 2p
@@ -1012,10 +1026,12 @@ __device__ __forceinline__ void lj_wave2(const LJArgs &A, int wv) {
             if (nst <= 0) return;  // wave-uniform
             const uint4 *lp = (const uint4 *) A.nl + (size_t) (row0 >> 2) * stride + ii;
             const int gl = max((cnt + 3) / 4 - 1, 0);
-            auto ldl = [&](int st, uint32_t (&j)[U]) {
+            auto ldl = [&](int st, uint32_t (&j)[2 * U]) {
                 const uint4 g = lp[(size_t) min(st, gl) * stride];
-                j[0] = min(h ? g.y : g.x, jmax);
-                j[1] = min(h ? g.w : g.z, jmax);
+                j[0] = min(g.x, jmax);
+                j[1] = min(g.y, jmax);
+                j[2] = min(g.z, jmax);
+                j[3] = min(g.w, jmax);
             };
             auto compute = [&](int st, const Pos &p) {
 #pragma unroll
@@ -1023,7 +1039,9 @@ __device__ __forceinline__ void lj_wave2(const LJArgs &A, int wv) {
                     double sg6 = sig0, ep = eps0;
                     if (NT != 1) { const int kt = ti * nt + p.t[q]; sg6 = s_sig6[kt]; ep = s_eps[kt]; }
                     double g0, g1, g2, uu, ww;
-                    lj_pair<RED, false, UNIT>(xi, yi, zi, p.x[q], p.y[q], p.z[q], 4 * st + 2 * q + h < cnt, rc2, sg6,
+                    const double2 xy = h ? p.b[q] : p.a[q];
+                    const double zj = pair_swap(h ? p.a[q].x : p.b[q].x);
+                    lj_pair<RED, false, UNIT>(xi, yi, zi, xy.x, xy.y, zj, 4 * st + 2 * q + h < cnt, rc2, sg6,
                                               ep, g0, g1, g2, uu, ww);
                     // lane 2p: entry 4s + 2q (its own), then 4s + 2q + 1 (lane 2p + 1's)
                     a0 = a0 + g0; a1 = a1 + g1; a2 = a2 + g2;
@@ -1035,7 +1053,7 @@ __device__ __forceinline__ void lj_wave2(const LJArgs &A, int wv) {
                     }
                 }
             };
-            uint32_t LA[U], LB[U];
+            uint32_t LA[2 * U], LB[2 * U];
             Pos PA, PB;
             ldl(0, LA);
             ldl(1, LB);
