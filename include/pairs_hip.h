@@ -193,10 +193,12 @@ int pairs_hip_set_graphs(pairs_hip_ctx *ctx, int enable);
 int pairs_hip_set_speculative_rebuilds(pairs_hip_ctx *ctx, int enable);
 /* Set-up step with no reference counterpart (the reference launches every
  * module eagerly): capture and instantiate, without running them, the graphs
- * pairs_hip_run replays -- the steady step and the rebuild of both position
- * buffers -- so that no capture falls inside a timed loop.  pairs_hip_run
- * otherwise captures each at its first use (and again when capacities or
- * ghost counts outgrow it).  Call after the first rebuild. */
+ * pairs_hip_run replays -- the first step of a run, the chains of steady
+ * steps between two rebuilds and the rebuild, for both position buffers --
+ * and allocate what its deferred rebuild checks use, so that no capture or
+ * allocation falls inside a timed loop.  pairs_hip_run otherwise captures
+ * each at its first use (and again when capacities or ghost counts outgrow
+ * it).  Call after the first rebuild. */
 int pairs_hip_prepare_graphs(pairs_hip_ctx *ctx);
 /* Time the force kernel with HIP events on the context stream on every
  * every_n-th non-rebuild step of pairs_hip_run (those steps launch eagerly);

@@ -28,13 +28,18 @@ struct pairs_hip_ctx {
     int max_lds = 65536;       // shared memory a workgroup may allocate
     int zc_override = 0;
     int home_target = 40;      // home particles a segment aims at (sets zc; about 3 cells at rho 0.84)
-    // graphs of 2^k consecutive non-rebuild steps (k < STEP_CHAINS), by
-    // parity of the current buffer at their first step
-    static constexpr int STEP_CHAINS = 5;
+    // graphs of n consecutive non-rebuild steps (slot n - 1, n <=
+    // CHAIN_MAX), by parity of the current buffer at their first step: a run
+    // of steady steps between two rebuilds is one launch
+    static constexpr int CHAIN_MAX = 24;
     bool use_graphs = true;
-    hipGraphExec_t step_graph[2][STEP_CHAINS] = {};
-    int graph_ghost_grid[2][STEP_CHAINS] = {};
-    int graph_img[2][STEP_CHAINS] = {};
+    hipGraphExec_t step_graph[2][CHAIN_MAX] = {};
+    int graph_ghost_grid[2][CHAIN_MAX] = {};
+    int graph_img[2][CHAIN_MAX] = {};
+    // the first step of a run (initial integrate, in-place ghost refresh,
+    // packed copy, fused force step), by parity
+    hipGraphExec_t start_graph[2] = {nullptr, nullptr};
+    int start_ghost_grid[2] = {0, 0}, start_img[2] = {0, 0};
     // a rebuild (exchange, borders, cell binning, neighbour build), by
     // position buffer
     hipGraphExec_t reb_graph[2] = {nullptr, nullptr};
@@ -49,6 +54,10 @@ struct pairs_hip_ctx {
     int spec_ts = 0, spec_cur = 0, spec_nthermo = 0, spec_ev_used = 0;
     int *h_spec = nullptr;          // pinned, C_COUNT
     hipEvent_t spec_ev = nullptr;   // behind the counter copy
+    // the counter copy runs on a second stream behind the rebuild, so the
+    // steps after the rebuild do not queue behind it
+    hipStream_t side = nullptr;
+    hipEvent_t reb_ev = nullptr;    // behind the rebuild
     double *ckpt = nullptr;         // checkpoint arrays, ckpt_n locals
     int ckpt_n = 0;
     long long spec_count = 0, spec_rollbacks = 0;

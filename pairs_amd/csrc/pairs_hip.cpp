@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -72,11 +73,10 @@ static void drop_reb_graphs(pairs_hip_ctx *ctx) {
 }
 
 static void drop_graphs(pairs_hip_ctx *ctx) {
-    for (int p = 0; p < 2; p++)
-        for (int k = 0; k < pairs_hip_ctx::STEP_CHAINS; k++) {
-            destroy_exec(ctx, ctx->step_graph[p][k]);
-            ctx->step_graph[p][k] = nullptr;
-        }
+    for (int p = 0; p < 2; p++) {
+        for (int k = 0; k < pairs_hip_ctx::CHAIN_MAX; k++) destroy_exec(ctx, ctx->step_graph[p][k]);
+        destroy_exec(ctx, ctx->start_graph[p]);
+    }
     drop_reb_graphs(ctx);
 }
 
@@ -355,6 +355,8 @@ int pairs_hip_destroy(pairs_hip_ctx *ctx) {
     dfree(ctx->ckpt);
     if (ctx->h_spec) hipHostFree(ctx->h_spec);
     if (ctx->spec_ev) hipEventDestroy(ctx->spec_ev);
+    if (ctx->reb_ev) hipEventDestroy(ctx->reb_ev);
+    if (ctx->side) { hipStreamSynchronize(ctx->side); hipStreamDestroy(ctx->side); }
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -965,6 +967,8 @@ static void enqueue_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
     // the neighbour build runs right behind the cell phase; every rebuild kernel
     // returns at once when an earlier one has set an overflow
     enqueue_build(ctx, s, false);
+    // the packed (x, y, z) copy of the new buffer, for the force step after it
+    launch_pack_p(s, ctx->d, ctx->d.cur);
 }
 
 // The rebuild's ~37 small kernels have grids that depend only on
@@ -1034,6 +1038,7 @@ static int rebuild(pairs_hip_ctx *ctx) {
         int code = ctx->h_counters[C_OVF] == OVF_TILE ? finish_build(ctx) : take_overflow(ctx);
         if (code == 0) {
             set_tile_cap(ctx);
+            ctx->xy_valid[ctx->d.cur] = true;  // packed at the end of the rebuild
             if (ctx->mlp_on) code = ensure_mlp_work(ctx);
             if (code <= 0) return code;
         }
@@ -1080,6 +1085,8 @@ static CkptArrays ckpt_saved(pairs_hip_ctx *ctx) {
 static int ensure_spec(pairs_hip_ctx *ctx) {
     if (!ctx->h_spec) HIP_OK(hipHostMalloc((void **) &ctx->h_spec, sizeof(int) * C_COUNT));
     if (!ctx->spec_ev) HIP_OK(hipEventCreateWithFlags(&ctx->spec_ev, hipEventDisableTiming));
+    if (!ctx->reb_ev) HIP_OK(hipEventCreateWithFlags(&ctx->reb_ev, hipEventDisableTiming));
+    if (!ctx->side) HIP_OK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     const int n = ctx->d.nlocal;
     if (ctx->ckpt_n < n) {
         HIP_OK(hipStreamSynchronize(ctx->stream));
@@ -1103,8 +1110,14 @@ static int spec_rebuild(pairs_hip_ctx *ctx, int ts, int nthermo) {
     ctx->spec_nthermo = nthermo;
     ctx->spec_ev_used = ctx->ev_used;
     if (int r = launch_rebuild(ctx, ctx->stream)) return r;  // flips d.cur
-    HIP_OK(hipMemcpyAsync(ctx->h_spec, d.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OK(hipEventRecord(ctx->spec_ev, ctx->stream));
+    ctx->xy_valid[d.cur] = true;                               // packed at its end
+    // the steps after the rebuild only read the counters; the next writer
+    // (the next rebuild, or any host path after the run) comes after the
+    // check, which waits for the copy
+    HIP_OK(hipEventRecord(ctx->reb_ev, ctx->stream));
+    HIP_OK(hipStreamWaitEvent(ctx->side, ctx->reb_ev, 0));
+    HIP_OK(hipMemcpyAsync(ctx->h_spec, d.counters, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, ctx->side));
+    HIP_OK(hipEventRecord(ctx->spec_ev, ctx->side));
     CHECK_LAUNCH();
     ctx->spec_pending = true;
     ctx->spec_count++;
@@ -1233,61 +1246,106 @@ int pairs_hip_force_kernel_time(pairs_hip_ctx *ctx, double *avg_ms, int *launche
     return 0;
 }
 
-static bool step_graph_stale(const pairs_hip_ctx *ctx, int cur, int k) {
-    return !ctx->step_graph[cur][k] || ctx->graph_ghost_grid[cur][k] < ctx->ghost_grid ||
-           ctx->graph_img[cur][k] != ctx->d.img_on;
+static bool step_graph_stale(const pairs_hip_ctx *ctx, int cur, int n) {
+    return !ctx->step_graph[cur][n - 1] || ctx->graph_ghost_grid[cur][n - 1] < ctx->ghost_grid ||
+           ctx->graph_img[cur][n - 1] != ctx->d.img_on;
 }
 
-// capture (without running) 2^k steady steps from buffer cur when missing or
-// stale: the buffers alternate from step to step, so the chain is the single
-// step's kernels repeated with the parities swapped
-static int capture_step_graph(pairs_hip_ctx *ctx, int cur, int k) {
-    if (step_graph_stale(ctx, cur, k)) {
-        hipGraphExec_t &slot = ctx->step_graph[cur][k];
-        destroy_exec(ctx, slot);
-        // capture with headroom so that later rebuilds with a few more ghosts reuse it
-        const int saved = ctx->ghost_grid;
-        ctx->ghost_grid = saved + saved / 4 + 4;
-        hipGraph_t g;
-        HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int q = 0; q < (1 << k); q++) launch_fused_step(ctx, cur ^ (q & 1), 1, 1, 0, false);
-        HIP_OK(hipStreamEndCapture(ctx->stream, &g));
-        hipError_t e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
+// Capture the kernels that enqueue(ctx) launches on the context's stream
+// into slot, without running them, with the ghost grid given headroom so
+// that later rebuilds with a few more ghosts reuse the graph.  The host's
+// record of which packed copies are current is left as it was.
+static int capture_graph(pairs_hip_ctx *ctx, hipGraphExec_t &slot, int &grid_out, int &img_out,
+                         const std::function<void()> &enqueue) {
+    destroy_exec(ctx, slot);
+    const int saved = ctx->ghost_grid;
+    const bool xy0 = ctx->xy_valid[0], xy1 = ctx->xy_valid[1];
+    ctx->ghost_grid = saved + saved / 4 + 4;
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    hipError_t e = ec;
+    if (ec == hipSuccess) {
+        e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
-        ctx->graph_ghost_grid[cur][k] = ctx->ghost_grid;
-        ctx->graph_img[cur][k] = ctx->d.img_on;
-        ctx->ghost_grid = saved;
-        if (e != hipSuccess) {
-            ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
-            slot = nullptr;
-            return PAIRS_HIP_ERR_HIP;
-        }
+    }
+    grid_out = ctx->ghost_grid;
+    img_out = ctx->d.img_on;
+    ctx->ghost_grid = saved;
+    ctx->xy_valid[0] = xy0;
+    ctx->xy_valid[1] = xy1;
+    if (e != hipSuccess) {
+        ctx->err = std::string("step graph capture: ") + hipGetErrorString(e);
+        slot = nullptr;
+        return PAIRS_HIP_ERR_HIP;
     }
     return 0;
 }
 
-// n >= 1 steady steps from buffer cur, as chains of 2^k steps (longest first)
+// capture (without running) n steady steps from buffer cur when missing or
+// stale: the buffers alternate from step to step, so the chain is the single
+// step's kernels repeated with the parities swapped.  A chain runs only after
+// a fused step, which left the packed copy of its first buffer current.
+static int capture_step_graph(pairs_hip_ctx *ctx, int cur, int n) {
+    if (!step_graph_stale(ctx, cur, n)) return 0;
+    return capture_graph(ctx, ctx->step_graph[cur][n - 1], ctx->graph_ghost_grid[cur][n - 1],
+                         ctx->graph_img[cur][n - 1], [&] {
+                             ctx->xy_valid[0] = ctx->xy_valid[1] = true;
+                             for (int q = 0; q < n; q++) launch_fused_step(ctx, cur ^ (q & 1), 1, 1, 0, false);
+                         });
+}
+
+// n >= 1 steady steps from buffer cur: one chain, or chains of CHAIN_MAX
+// steps and the rest
 static int graph_steps(pairs_hip_ctx *ctx, int cur, int n) {
     while (n > 0) {
-        int k = pairs_hip_ctx::STEP_CHAINS - 1;
-        while ((1 << k) > n) k--;
-        if (int r = capture_step_graph(ctx, cur, k)) return r;
-        HIP_OK(hipGraphLaunch(ctx->step_graph[cur][k], ctx->stream));
-        cur ^= (1 << k) & 1;
-        n -= 1 << k;
+        const int m = std::min(n, pairs_hip_ctx::CHAIN_MAX);
+        if (int r = capture_step_graph(ctx, cur, m)) return r;
+        HIP_OK(hipGraphLaunch(ctx->step_graph[cur][m - 1], ctx->stream));
+        cur ^= m & 1;
+        n -= m;
     }
     return 0;
+}
+
+// The first step of a run that is neither a rebuild nor a thermo or timed
+// step, from buffer cur: initial_integrate(ts) in place, the ghosts
+// refreshed in place, the packed copy, and the fused force step into the
+// other buffer -- one launch instead of five.
+static int capture_start_graph(pairs_hip_ctx *ctx, int cur) {
+    if (ctx->start_graph[cur] && ctx->start_ghost_grid[cur] >= ctx->ghost_grid && ctx->start_img[cur] == ctx->d.img_on)
+        return 0;
+    const int cur0 = ctx->d.cur;
+    ctx->d.cur = cur;
+    const int r = capture_graph(ctx, ctx->start_graph[cur], ctx->start_ghost_grid[cur], ctx->start_img[cur], [&] {
+        DevState &d = ctx->d;
+        launch_vv_initial(ctx->stream, d, ctx->prm);
+        launch_sync_inplace(ctx->stream, d, ctx->geom, ctx->ghost_grid, 0);
+        ctx->xy_valid[cur] = false;
+        launch_force_post(ctx, cur, 1 - cur, 1, 1, 0, false);
+    });
+    ctx->d.cur = cur0;
+    return r;
 }
 
 int pairs_hip_prepare_graphs(pairs_hip_ctx *ctx) {
     if (!ctx) return PAIRS_HIP_ERR_ARG;
     if (!ctx->use_graphs) return 0;
     if (ctx->time_force && ensure_events(ctx, 4096)) return PAIRS_HIP_ERR_HIP;
+    // the deferred checks' pinned copy, events, second stream and checkpoint
+    if (ctx->spec_enabled && !ctx->mlp_on)
+        if (int r = ensure_spec(ctx)) return r;
+    // the chains a run between two rebuilds uses (longer or other lengths,
+    // cut by thermo or timed steps, are captured when first needed)
+    const int nmax = std::max(1, std::min(pairs_hip_ctx::CHAIN_MAX, ctx->cfg.reneighbor_every - 1));
     for (int b = 0; b < 2; b++) {
         if (int r = capture_rebuild_graph(ctx, b)) return r;
-        if (ctx->h_counters[C_NLOCAL] > 0 && ctx->d.nl && !ctx->mlp_on)
-            for (int k = 0; k < pairs_hip_ctx::STEP_CHAINS; k++)
-                if (int r = capture_step_graph(ctx, b, k)) return r;
+        if (ctx->h_counters[C_NLOCAL] > 0 && ctx->d.nl && !ctx->mlp_on) {
+            if (int r = capture_start_graph(ctx, b)) return r;
+            for (int n = 1; n <= nmax; n++)
+                if (int r = capture_step_graph(ctx, b, n)) return r;
+        }
     }
     return 0;
 }
@@ -1318,16 +1376,24 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
         if (ts >= ts_end) break;
         const bool R = every(ts, ctx->cfg.reneighbor_every);
         const bool T = every(ts, ctx->cfg.thermo_every);
-        if (ts == ts_begin && ts > 0 && !redo) {  // (a repeat starts from the checkpoint, taken after it)
-            launch_vv_initial(s, d, ctx->prm);
-            ctx->xy_valid[d.cur] = false;
-        }
         const int vv2 = ts > 0, vv1 = (ts + 1 < ts_end) && (ts + 1 <= nsteps);
         auto timed_at = [&](int t) {
             return ctx->time_force && !every(t, ctx->cfg.reneighbor_every) && ((t % ctx->time_every) == 0) &&
                    ctx->ev_used < 4096;
         };
         const bool timed = timed_at(ts);
+        if (ts == ts_begin && ts > 0 && !redo) {  // (a repeat starts from the checkpoint, taken after it)
+            if (ctx->use_graphs && !ctx->mlp_on && !R && !T && !timed && vv1) {
+                if (int r = capture_start_graph(ctx, d.cur)) return r;
+                HIP_OK(hipGraphLaunch(ctx->start_graph[d.cur], s));
+                ctx->xy_valid[0] = ctx->xy_valid[1] = true;  // packed, and written by the fused kernel
+                d.cur ^= 1;
+                prev_in_other = true;
+                continue;
+            }
+            launch_vv_initial(s, d, ctx->prm);
+            ctx->xy_valid[d.cur] = false;
+        }
         auto steady = [&](int t) {
             return !every(t, ctx->cfg.reneighbor_every) && !every(t, ctx->cfg.thermo_every) && t > 0 &&
                    t + 1 < ts_end && t + 1 <= nsteps && !timed_at(t);
