@@ -5,11 +5,13 @@
  * is the build's batched ensemble: R independent periodic LJ replicas of
  * uneven sizes, each a full pairs_hip_ctx (include/pairs_hip.h, same modules,
  * same parity with the reference), whose non-rebuild timesteps run as ONE
- * kernel launch for all replicas (fused force + final/initial integrate +
- * ghost-image refresh; block b is segment b - base[r] of replica r), replayed
- * from a captured hipGraph.  Rebuild steps launch every replica's rebuild
- * kernels back to back and synchronise once.  The segment size (cells per
- * workgroup, hence the neighbour-tile shape) is picked by an autotuner that
+ * persistent kernel launch for all replicas (fused force + final/initial
+ * integrate + ghost-image refresh; as many workgroups as the device holds
+ * resident, each walking a contiguous run of a table whose entry b is wave
+ * b - base[r] of replica r), replayed from a captured hipGraph.  Rebuild
+ * steps launch each rebuild kernel once over all replicas (a segment table
+ * for the neighbour build).  The segment size (cells per workgroup of the
+ * build, hence the neighbour-tile shape) is picked by an autotuner that
  * times the batched step over a set of candidates.
  *
  * All replicas share reneighbor_every, thermo_every, dt, cutoff, the list
@@ -46,6 +48,10 @@ int pairs_hip_batch_run_each(pairs_hip_batch *batch, int ts_begin, int ts_end, i
 /* Home particles per segment (the knob that sets cells per workgroup along z
    and thus the tile shape) for every replica; takes effect at the next rebuild. */
 int pairs_hip_batch_set_home(pairs_hip_batch *batch, int home_per_segment);
+/* At most `workgroups` workgroups in the persistent force launch (0, the
+   default: as many as the device holds resident); each walks a contiguous run
+   of the wave table.  Results do not depend on it. */
+int pairs_hip_batch_set_workgroups(pairs_hip_batch *batch, int workgroups);
 /* For each candidate home size: run the replicas to the next rebuild step,
    rebuild with that size and time `steps` batched steps (the replicas advance:
    trajectories do not depend on the segment size); keep the fastest.  Writes

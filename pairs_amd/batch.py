@@ -31,6 +31,7 @@ nat.register({
     "pairs_hip_batch_run": (_I, [_P, _I, _I, _I]),
     "pairs_hip_batch_run_each": (_I, [_P, _I, _I, _I]),
     "pairs_hip_batch_set_home": (_I, [_P, _I]),
+    "pairs_hip_batch_set_workgroups": (_I, [_P, _I]),
     "pairs_hip_batch_autotune": (_I, [_P, _I, _I, _I, _I, _IP, _DP, _IP, _IP]),
     "pairs_hip_batch_thermo": (_I, [_P, _DP]),
 })
@@ -151,6 +152,11 @@ class BatchEnsemble:
 
     def set_home(self, home):
         self._check(self.L.pairs_hip_batch_set_home(self.h, home), "batch_set_home")
+
+    def set_workgroups(self, workgroups):
+        """at most this many workgroups in the persistent force launch (0: all
+        the device holds resident)"""
+        self._check(self.L.pairs_hip_batch_set_workgroups(self.h, workgroups), "batch_set_workgroups")
 
     def autotune(self, ts_begin, steps, nsteps, candidates=(32, 48, 64, 96, 160)):
         """-> (chosen home size, {candidate: ms per step}, next timestep)"""

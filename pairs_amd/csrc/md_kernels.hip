@@ -1119,15 +1119,22 @@ __global__ __launch_bounds__(64) void k_lj2(LJArgs A) {
     lj_wave2<NT, MODE, RED, HALF, UNIT>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
-// the waves of many independent systems in one launch: block b belongs to
+// the waves of many independent systems in one persistent launch: the grid
+// is what the device holds resident at once, and workgroup w takes the
+// consecutive table entries [w * per, (w + 1) * per); entry b belongs to
 // system rep[b] and is its wave b - wave_base[rep[b]] (one lane per local, 64
-// locals per wave)
+// locals per wave).  The XCD-aware order of w gives each XCD one contiguous
+// eighth of the table.
 template <int NT, int MODE, bool RED, bool UNIT>
 __global__ __launch_bounds__(64) void k_lj_batch(const LJArgs *__restrict__ desc, const int *__restrict__ rep,
-                                                 const int *__restrict__ wave_base) {
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int r = rep[b];
-    lj_wave<NT, MODE, RED, false, false, UNIT>(desc[r], b - wave_base[r]);
+                                                 const int *__restrict__ wave_base, int nblocks) {
+    const int G = gridDim.x;
+    const int per = (nblocks + G - 1) / G;
+    const int b0 = xcd_block(blockIdx.x, G) * per, b1 = min(b0 + per, nblocks);
+    for (int b = b0; b < b1; b++) {
+        const int r = rep[b];
+        lj_wave<NT, MODE, RED, false, false, UNIT>(desc[r], b - wave_base[r]);
+    }
 }
 
 // euler of the older API (examples/lj_onetype.py:11-13) for the non-fixed
@@ -1985,11 +1992,26 @@ static void lj_launch(hipStream_t s, const DevState &d, const Geom &g, const LJP
     }
 }
 
+// workgroups of a kernel that the current device holds resident at once
+template <typename K>
+static int resident_grid(K kernel, int block) {
+    int dev = 0, occ = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 1024;
+    return std::max(1, occ * cus);
+}
+
 void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *wave_base, int nblocks,
-                          int ntypes1, int unit, int reduce) {
+                          int ntypes1, int unit, int reduce, int max_wg) {
     if (nblocks <= 0) return;
-    const dim3 grid(nblocks), blk(64);
-#define LJB(NT, RED, UNIT) hipLaunchKernelGGL((k_lj_batch<NT, 1, RED, UNIT>), grid, blk, 0, s, desc, rep, wave_base)
+    const dim3 blk(64);
+    // the resident workgroups, or at most max_wg of them
+    auto grid = [&](int resident) { return dim3(std::min(nblocks, max_wg > 0 ? std::min(max_wg, resident) : resident)); };
+#define LJB(NT, RED, UNIT)                                                                                     \
+    hipLaunchKernelGGL((k_lj_batch<NT, 1, RED, UNIT>), grid(resident_grid(k_lj_batch<NT, 1, RED, UNIT>, 64)), blk, 0, \
+                       s, desc, rep, wave_base, nblocks)
     if (ntypes1 && unit) { if (reduce) LJB(1, true, true); else LJB(1, false, true); }
     else if (ntypes1) { if (reduce) LJB(1, true, false); else LJB(1, false, false); }
     else { if (reduce) LJB(0, true, false); else LJB(0, false, false); }

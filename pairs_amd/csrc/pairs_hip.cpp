@@ -1669,7 +1669,8 @@ struct pairs_hip_batch {
     int *d_srep = nullptr, *d_sbase = nullptr;
     int nsblocks = 0, sblocks_cap = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_blocks[2] = {0, 0};
+    int graph_blocks[2] = {0, 0}, graph_wg[2] = {0, 0};
+    int max_wg = 0;                    // workgroups of the persistent force launch (0: all resident)
     std::vector<double> thermo;        // last thermo row per replica
     bool ntypes1 = true;
     bool unit = true;                  // every replica: one type, epsilon = sigma6 = 1
@@ -1854,7 +1855,7 @@ static int batch_rebuild(pairs_hip_batch *b) {
 }
 
 static int batch_graph_launch(pairs_hip_batch *b, int phase) {
-    if (!b->graph[phase] || b->graph_blocks[phase] != b->nblocks) {
+    if (!b->graph[phase] || b->graph_blocks[phase] != b->nblocks || b->graph_wg[phase] != b->max_wg) {
         if (b->graph[phase]) {
             BATCH_OK(hipStreamSynchronize(b->stream));  // no replay pending
             hipGraphExecDestroy(b->graph[phase]);
@@ -1869,12 +1870,14 @@ static int batch_graph_launch(pairs_hip_batch *b, int phase) {
                 launch_sync_pp(b->stream, c->d, c->geom, cur, 1 - cur, c->ghost_grid);
             }
         }
-        launch_lj_full_batch(b->stream, b->d_desc[phase], b->d_rep, b->d_base, b->nblocks, b->ntypes1, b->unit, 0);
+        launch_lj_full_batch(b->stream, b->d_desc[phase], b->d_rep, b->d_base, b->nblocks, b->ntypes1, b->unit, 0,
+                             b->max_wg);
         BATCH_OK(hipStreamEndCapture(b->stream, &g));
         const hipError_t e = hipGraphInstantiate(&b->graph[phase], g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) { b->graph[phase] = nullptr; b->err = "batch graph instantiation failed"; return PAIRS_HIP_ERR_HIP; }
         b->graph_blocks[phase] = b->nblocks;
+        b->graph_wg[phase] = b->max_wg;
     }
     BATCH_OK(hipGraphLaunch(b->graph[phase], b->stream));
     return 0;
@@ -2020,7 +2023,8 @@ int pairs_hip_batch_run(pairs_hip_batch *b, int ts_begin, int ts_end, int nsteps
             // an in-place ghost refresh or initial integrate): pack them;
             // this launch writes the next buffer's copy for the steps after
             launch_pack_p_batch(s, b->d_desc[2], R, max_cap);
-            launch_lj_full_batch(s, b->d_desc[2], b->d_rep, b->d_base, b->nblocks, b->ntypes1, b->unit, T ? 1 : 0);
+            launch_lj_full_batch(s, b->d_desc[2], b->d_rep, b->d_base, b->nblocks, b->ntypes1, b->unit, T ? 1 : 0,
+                                 b->max_wg);
             if (T) {
                 if (!b->rdesc_valid) {  // the replicas' arrays and sizes for the batched reduction
                     BATCH_OK(hipStreamSynchronize(s));
@@ -2072,6 +2076,12 @@ int pairs_hip_batch_set_home(pairs_hip_batch *b, int home) {
     if (!b || home < 8) return PAIRS_HIP_ERR_ARG;
     b->pending_home = home;
     b->rdesc_valid = false;
+    return 0;
+}
+
+int pairs_hip_batch_set_workgroups(pairs_hip_batch *b, int workgroups) {
+    if (!b || workgroups < 0) return PAIRS_HIP_ERR_ARG;
+    b->max_wg = workgroups;
     return 0;
 }
 

@@ -73,6 +73,26 @@ def test_batched_equals_oracle():
         assert np.array_equal(g.forces(), o.forces()), r
 
 
+def test_persistent_launch_grid_keeps_trajectories():
+    """The persistent force launch with one, seven or all resident workgroups
+    (each walking a contiguous run of the wave table): the same trajectories,
+    bit for bit, as the replicas' own single-system runs."""
+    bm = _batch()
+    ens = []
+    for wg in (1, 7, 0):
+        B = bm.BatchEnsemble(SIZES, RHO, TEMP)
+        B.set_workgroups(wg)
+        B.run(0, 1, 300)
+        B.run(1, 61, 300)
+        ens.append(B)
+    singles = [_single(ens[0], r) for r in range(len(SIZES))]
+    for s in singles:
+        s.run(0, 61, 300)
+    for B, wg in zip(ens, (1, 7, 0)):
+        for r, s in enumerate(singles):
+            _same(B.replicas[r], s, (wg, r))
+
+
 def test_run_each_and_autotune_keep_trajectories():
     bm = _batch()
     A = bm.BatchEnsemble(SIZES, RHO, TEMP)
