@@ -248,6 +248,10 @@ LJArgs lj_args(const DevState &d, const Geom &g, const LJParams &prm, int cur, i
 // the largest system's particle capacity)
 void launch_pack_p_batch(hipStream_t s, const LJArgs *desc, int n, int max_cap);
 // fused force + integrate for many systems: block b is wave b - wave_base[rep[b]] of system rep[b]
+// the ghost-image table of a kernel that advances positions (on only when
+// it writes the next positions into the other buffer and every local's
+// images fit the table)
+Images images_of(const DevState &d, bool writes_next);
 // one persistent launch over the wave table of many systems: the resident
 // workgroups, or at most max_wg of them when max_wg > 0
 void launch_lj_full_batch(hipStream_t s, const LJArgs *desc, const int *rep, const int *wave_base, int nblocks,

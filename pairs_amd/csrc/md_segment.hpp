@@ -149,4 +149,29 @@ __device__ __forceinline__ void list_walk32(const uint32_t *__restrict__ nl, int
     }
 }
 
+// Next positions of local i's images (the stale multi-hop refresh of
+// src/pairs/sim/comm.py:45-54,369-442 one step early): an image of i itself
+// is i's new position + shift, an image of a ghost is that ghost's current
+// position + shift -- exactly what the next step's synchronize packs.
+__device__ __forceinline__ void write_images(const Images &im, int i, int nlocal, double px, double py, double pz,
+                                             const double *__restrict__ X, const double *__restrict__ Y,
+                                             const double *__restrict__ Z, double *__restrict__ Xn,
+                                             double *__restrict__ Yn, double *__restrict__ Zn, const Geom &g,
+                                             double4 *__restrict__ Pn = nullptr) {
+    const int n = min(im.cnt[i], IMG_MAX);
+    for (int k = 0; k < n; k++) {
+        const int q = im.slot[(size_t) k * im.cap + i];
+        const int src = im.src[q];
+        const bool own = src == i;
+        const double bx = own ? px : X[src], by = own ? py : Y[src], bz = own ? pz : Z[src];
+        const double x = bx + (double) im.mult[3 * q + 0] * g.len[0];
+        const double y = by + (double) im.mult[3 * q + 1] * g.len[1];
+        const double z = bz + (double) im.mult[3 * q + 2] * g.len[2];
+        Xn[nlocal + q] = x;
+        Yn[nlocal + q] = y;
+        Zn[nlocal + q] = z;
+        if (Pn) Pn[nlocal + q] = make_double4(x, y, z, 0.0);
+    }
+}
+
 }  // namespace pk

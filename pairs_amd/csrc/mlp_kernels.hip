@@ -244,6 +244,17 @@ struct MLPSegArgs {
     const float4 *rho;   // MODE 2: dL/dF of every slot (a ghost holds its root's)
     const float *sc;     // MODE 2: standardisation (e_std, g_istd)
     float *U;            // MODE 2: [cap][16] back-projected directions
+    // MODE 1 without RED, vv set: the integration of k_vv_post fused behind
+    // the force (final_integrate(ts) if vv2, initial_integrate(ts+1) and the
+    // next positions and images if vv1)
+    int vv, vv2, vv1;
+    double dt, hdt;
+    const int *flags;
+    const double *mass;
+    double *vx, *vy, *vz;
+    double *Xn, *Yn, *Zn;
+    Images im;
+    Geom g;
 };
 
 // G of the 32 locals of one wave (MODE 0), or their force, energy and
@@ -418,6 +429,32 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
             } else {
                 A.fx[i] = f0; A.fy[i] = f1; A.fz[i] = f2;
                 if (RED) { pe += (double) A.E[i]; vir += 0.5 * w; }
+                if (!RED && A.vv) {  // k_vv_post's update, on the forces just summed
+                    const double m = A.mass[i];
+                    double v0 = A.vx[i], v1 = A.vy[i], v2 = A.vz[i];
+                    double px = A.X[i], py = A.Y[i], pz = A.Z[i];
+                    if (!(A.flags[i] & FLAG_FIXED)) {
+                        const double hdt = A.hdt, dt = A.dt;
+                        if (A.vv2) {
+                            double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
+                            a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
+                            v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
+                        }
+                        if (A.vv1) {
+                            double a0 = hdt * f0, a1 = hdt * f1, a2 = hdt * f2;
+                            a0 = a0 / m; a1 = a1 / m; a2 = a2 / m;
+                            v0 = v0 + a0; v1 = v1 + a1; v2 = v2 + a2;
+                            px = px + dt * v0; py = py + dt * v1; pz = pz + dt * v2;
+                        }
+                        A.vx[i] = v0; A.vy[i] = v1; A.vz[i] = v2;
+                    }
+                    if (A.vv1) {
+                        A.Xn[i] = px; A.Yn[i] = py; A.Zn[i] = pz;
+                        if (A.im.on)
+                            write_images(A.im, i, A.counters[C_NLOCAL], px, py, pz, A.X, A.Y, A.Z, A.Xn, A.Yn, A.Zn,
+                                         A.g);
+                    }
+                }
             }
         }
     }
@@ -967,7 +1004,7 @@ void launch_mlp_eval(hipStream_t s, const float *p, const float *sc, const float
 }
 
 void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const MLPBind &b, int cur, int reduce,
-                       int ghost_grid) {
+                       int ghost_grid, const MLPVV *vv) {
     if (d.nlocal <= 0) return;
     launch_mlp_descriptors(s, d, g, cur, b.G);
     launch_mlp_eval(s, b.p, b.sc, b.G, d.nlocal, b.E, b.D);
@@ -977,6 +1014,16 @@ void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const ML
     A.E = b.E;
     A.fx = d.fx; A.fy = d.fy; A.fz = d.fz;
     A.partials = d.partials;
+    if (vv && !reduce) {
+        A.vv = 1;
+        A.vv2 = vv->vv2; A.vv1 = vv->vv1;
+        A.dt = vv->prm->dt; A.hdt = vv->prm->hdt;
+        A.flags = d.flags; A.mass = d.mass;
+        A.vx = d.vx; A.vy = d.vy; A.vz = d.vz;
+        A.Xn = d.X[vv->next]; A.Yn = d.Y[vv->next]; A.Zn = d.Z[vv->next];
+        A.im = images_of(d, vv->vv1 && vv->next != cur);
+        A.g = g;
+    }
     if (reduce) launch_mlp_wave<1, true>(s, d, A);
     else launch_mlp_wave<1, false>(s, d, A);
 }

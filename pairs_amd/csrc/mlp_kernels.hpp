@@ -56,9 +56,15 @@ void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int
 void launch_mlp_eval(hipStream_t s, const float *p, const float *sc, const float *G, int n, float *E, float *D);
 // full MLP force evaluation on buffer `cur`: descriptors, network, ghost
 // gradients, forces (reset + written), and with `reduce` the energy and
-// virial partials of the thermo reduction
+// virial partials of the thermo reduction.  With `vv` (and no `reduce`) the
+// force pass also integrates, as launch_vv_post(cur, next, vv2, vv1) would:
+// final_integrate(ts), initial_integrate(ts+1), next positions and images.
+struct MLPVV {
+    const LJParams *prm;
+    int next, vv2, vv1;
+};
 void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const MLPBind &b, int cur, int reduce,
-                       int ghost_grid);
+                       int ghost_grid, const MLPVV *vv = nullptr);
 
 // training
 struct MLPTrain {

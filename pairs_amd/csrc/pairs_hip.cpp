@@ -1191,9 +1191,12 @@ static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, in
     hipStream_t s = ctx->stream;
     if (timed && ctx->mlp_on) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
     if (ctx->mlp_on) {
-        launch_mlp_forces(s, d, ctx->geom, ctx->mlp, cur, reduce, ctx->ghost_grid);
+        // the force pass integrates too, except on thermo steps (whose
+        // kinetic partials come from the stand-alone integrator)
+        const MLPVV vv{&ctx->prm, next, vv2, vv1};
+        launch_mlp_forces(s, d, ctx->geom, ctx->mlp, cur, reduce, ctx->ghost_grid, reduce ? nullptr : &vv);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-        launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
+        if (reduce) launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
     } else {
         // the timed region is the force kernel itself (after a pack of
         // (x, y) when the previous kernel did not leave one)
