@@ -1185,7 +1185,8 @@ __global__ __launch_bounds__(256) void k_vv_post(const double *__restrict__ X, c
                                                  const double *__restrict__ fy, const double *__restrict__ fz,
                                                  const double *__restrict__ mass, const int *__restrict__ flags,
                                                  int nlocal, double dt, double hdt, int do_vv2, int do_vv1,
-                                                 double *__restrict__ partials, Images im, Geom g) {
+                                                 double *__restrict__ partials, Images im, Geom g,
+                                                 double4 *__restrict__ Pn) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     double ke = 0.0;
     if (i < nlocal) {
@@ -1213,7 +1214,8 @@ __global__ __launch_bounds__(256) void k_vv_post(const double *__restrict__ X, c
         }
         if (do_vv1) {
             Xn[i] = px; Yn[i] = py; Zn[i] = pz;
-            if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, g);
+            if (Pn) Pn[i] = make_double4(px, py, pz, 0.0);
+            if (im.on) write_images(im, i, nlocal, px, py, pz, X, Y, Z, Xn, Yn, Zn, g, Pn);
         }
     }
     if (RED) {
@@ -2053,17 +2055,17 @@ void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm) {
 }
 
 void launch_vv_post(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
-                    int do_vv2, int do_vv1, int reduce) {
+                    int do_vv2, int do_vv1, int reduce, double4 *Pn) {
     if (d.nlocal <= 0) return;
     const int grid = (d.nlocal + 255) / 256;
     if (reduce)
         hipLaunchKernelGGL(k_vv_post<true>, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[next],
                            d.Y[next], d.Z[next], d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt,
-                           prm.hdt, do_vv2, do_vv1, d.partials, images_of(d, do_vv1 && next != cur), g);
+                           prm.hdt, do_vv2, do_vv1, d.partials, images_of(d, do_vv1 && next != cur), g, Pn);
     else
         hipLaunchKernelGGL(k_vv_post<false>, dim3(grid), dim3(256), 0, s, d.X[cur], d.Y[cur], d.Z[cur], d.X[next],
                            d.Y[next], d.Z[next], d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.mass, d.flags, d.nlocal, prm.dt,
-                           prm.hdt, do_vv2, do_vv1, d.partials, images_of(d, do_vv1 && next != cur), g);
+                           prm.hdt, do_vv2, do_vv1, d.partials, images_of(d, do_vv1 && next != cur), g, Pn);
 }
 
 void launch_ke_partial(hipStream_t s, const DevState &d) {

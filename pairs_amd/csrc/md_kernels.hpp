@@ -210,7 +210,7 @@ void launch_lj_full(hipStream_t s, const DevState &d, const Geom &g, const LJPar
 void launch_energy(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int half, double *eatom);
 void launch_vv_initial(hipStream_t s, const DevState &d, const LJParams &prm);
 void launch_vv_post(hipStream_t s, const DevState &d, const Geom &g, const LJParams &prm, int cur, int next,
-                    int do_vv2, int do_vv1, int reduce);
+                    int do_vv2, int do_vv1, int reduce, double4 *Pn = nullptr);
 void launch_ke_partial(hipStream_t s, const DevState &d);
 // thermo row from the partials: kinetic row over nb_ke entries, energy and
 // virial rows over nb_f entries

@@ -253,6 +253,7 @@ struct MLPSegArgs {
     const double *mass;
     double *vx, *vy, *vz;
     double *Xn, *Yn, *Zn;
+    double4 *Pn;         // packed copy of the next positions, or null
     Images im;
     Geom g;
 };
@@ -450,9 +451,10 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                     }
                     if (A.vv1) {
                         A.Xn[i] = px; A.Yn[i] = py; A.Zn[i] = pz;
+                        if (A.Pn) A.Pn[i] = make_double4(px, py, pz, 0.0);
                         if (A.im.on)
                             write_images(A.im, i, A.counters[C_NLOCAL], px, py, pz, A.X, A.Y, A.Z, A.Xn, A.Yn, A.Zn,
-                                         A.g);
+                                         A.g, A.Pn);
                     }
                 }
             }
@@ -980,10 +982,10 @@ static MLPSegArgs seg_args(const DevState &d, int cur) {
     return A;
 }
 
-void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int cur, float *G) {
+void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int cur, float *G, bool pack) {
     if (d.nlocal <= 0) return;
     (void) g;
-    launch_pack_p(s, d, cur);  // both passes gather packed positions
+    if (pack) launch_pack_p(s, d, cur);  // both passes gather packed positions
     MLPSegArgs A = seg_args(d, cur);
     A.G = G;
     launch_mlp_wave<0, false>(s, d, A);
@@ -1004,9 +1006,9 @@ void launch_mlp_eval(hipStream_t s, const float *p, const float *sc, const float
 }
 
 void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const MLPBind &b, int cur, int reduce,
-                       int ghost_grid, const MLPVV *vv) {
+                       int ghost_grid, const MLPVV *vv, bool pack) {
     if (d.nlocal <= 0) return;
-    launch_mlp_descriptors(s, d, g, cur, b.G);
+    launch_mlp_descriptors(s, d, g, cur, b.G, pack);
     launch_mlp_eval(s, b.p, b.sc, b.G, d.nlocal, b.E, b.D);
     hipLaunchKernelGGL(k_mlp_ghost_d, dim3(std::max(ghost_grid, 1)), dim3(256), 0, s, d.counters, d.sync_src, b.D);
     MLPSegArgs A = seg_args(d, cur);
@@ -1023,6 +1025,7 @@ void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const ML
         A.Xn = d.X[vv->next]; A.Yn = d.Y[vv->next]; A.Zn = d.Z[vv->next];
         A.im = images_of(d, vv->vv1 && vv->next != cur);
         A.g = g;
+        A.Pn = vv->Pn;
     }
     if (reduce) launch_mlp_wave<1, true>(s, d, A);
     else launch_mlp_wave<1, false>(s, d, A);

@@ -51,7 +51,7 @@ struct MLPBind {
 };
 
 // descriptors of every local of buffer `cur` into G (physical slots)
-void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int cur, float *G);
+void launch_mlp_descriptors(hipStream_t s, const DevState &d, const Geom &g, int cur, float *G, bool pack = true);
 // E and dE/dG of n descriptor rows (either output may be null)
 void launch_mlp_eval(hipStream_t s, const float *p, const float *sc, const float *G, int n, float *E, float *D);
 // full MLP force evaluation on buffer `cur`: descriptors, network, ghost
@@ -59,12 +59,16 @@ void launch_mlp_eval(hipStream_t s, const float *p, const float *sc, const float
 // virial partials of the thermo reduction.  With `vv` (and no `reduce`) the
 // force pass also integrates, as launch_vv_post(cur, next, vv2, vv1) would:
 // final_integrate(ts), initial_integrate(ts+1), next positions and images.
+// Pn, when set, receives the packed copy of the next positions.  `pack`:
+// the descriptor pass first packs buffer cur (not needed when the previous
+// step left its packed copy current).
 struct MLPVV {
     const LJParams *prm;
     int next, vv2, vv1;
+    double4 *Pn;
 };
 void launch_mlp_forces(hipStream_t s, const DevState &d, const Geom &g, const MLPBind &b, int cur, int reduce,
-                       int ghost_grid, const MLPVV *vv = nullptr);
+                       int ghost_grid, const MLPVV *vv = nullptr, bool pack = true);
 
 // training
 struct MLPTrain {

@@ -1192,11 +1192,18 @@ static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, in
     if (timed && ctx->mlp_on) hipEventRecord(ctx->ev_begin[ctx->ev_used], s);
     if (ctx->mlp_on) {
         // the force pass integrates too, except on thermo steps (whose
-        // kinetic partials come from the stand-alone integrator)
-        const MLPVV vv{&ctx->prm, next, vv2, vv1};
-        launch_mlp_forces(s, d, ctx->geom, ctx->mlp, cur, reduce, ctx->ghost_grid, reduce ? nullptr : &vv);
+        // kinetic partials come from the stand-alone integrator); either
+        // writes the next buffer's packed copy, so the descriptor pass packs
+        // only when the previous step did not leave one
+        double4 *Pn = (vv1 && next != cur) ? d.P[next] : nullptr;
+        const MLPVV vv{&ctx->prm, next, vv2, vv1, Pn};
+        launch_mlp_forces(s, d, ctx->geom, ctx->mlp, cur, reduce, ctx->ghost_grid, reduce ? nullptr : &vv,
+                          !ctx->xy_valid[cur]);
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
-        if (reduce) launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce);
+        if (reduce) launch_vv_post(s, d, ctx->geom, ctx->prm, cur, next, vv2, vv1, reduce, Pn);
+        ctx->xy_valid[cur] = true;
+        if (Pn) ctx->xy_valid[next] = true;
+        return;
     } else {
         // the timed region is the force kernel itself (after a pack of
         // (x, y) when the previous kernel did not leave one)
@@ -1205,9 +1212,7 @@ static void launch_force_post(pairs_hip_ctx *ctx, int cur, int next, int vv2, in
         if (timed) hipEventRecord(ctx->ev_end[ctx->ev_used], s);
         ctx->xy_valid[cur] = true;
         if (vv1 && next != cur) ctx->xy_valid[next] = true;  // written by the fused kernel
-        return;
     }
-    if (vv1 && next != cur) ctx->xy_valid[next] = false;  // moved without the packed copy
 }
 
 static void launch_fused_step(pairs_hip_ctx *ctx, int cur, int vv2, int vv1, int reduce, bool timed) {
@@ -1407,7 +1412,7 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
             while (steady(ts + n)) n++;
             if (int r = graph_steps(ctx, d.cur, n)) return r;
             d.cur ^= n & 1;
-            ctx->xy_valid[d.cur] = !ctx->mlp_on;  // the replayed LJ kernel wrote it
+            ctx->xy_valid[d.cur] = true;  // the replayed force kernel wrote it
             ts += n - 1;
             continue;
         }
