@@ -337,8 +337,11 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                 rsq = rsq + dz * dz;
                 const bool in = (4 * st + L * q + h < n) && rsq < rc2;
                 // masked entries take an in-range stand-in distance, so the
-                // Gaussian recurrence below stays finite
-                const float r = in ? sqrtf((float) rsq) : MLP_MU0;
+                // Gaussian recurrence below stays finite; 1/r from the
+                // hardware reciprocal square root, r = rsq / r
+                const float rsqf = (float) rsq;
+                const float rinv = in ? __builtin_amdgcn_rsqf(rsqf) : 1.0f / MLP_MU0;
+                const float r = in ? rsqf * rinv : MLP_MU0;
                 // g_k by recurrence over the uniform centres,
                 // g_{k+1} = g_k a_k, a_{k+1} = a_k b, from two directly
                 // computed anchors (k = 0 and k = 8): four exponentials per
@@ -360,13 +363,15 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                         ak *= gb;
                     }
                 } else if (MODE == 2) {
-                    float sn, cs;
-                    sincosf(MLP_PIRC * r, &sn, &cs);
+                    // the cutoff's sine and cosine from the hardware units
+                    // (argument in turns: pi r / rc = 2 pi (r / 2 rc), r < rc)
+                    const float ang = r * (0.5f / (float) MLP_RC);
+                    const float sn = __builtin_amdgcn_sinf(ang), cs = __builtin_amdgcn_cosf(ang);
                     const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
                     // (x_a - x_j) / r . (rho_a - rho_j), zero for a masked entry
                     float proj = (float) dx * (rhoi.x - rj[q].x) + (float) dy * (rhoi.y - rj[q].y);
                     proj = proj + (float) dz * (rhoi.z - rj[q].z);
-                    proj = in ? -proj / r : 0.0f;
+                    proj = in ? -proj * rinv : 0.0f;
 #pragma unroll
                     for (int kk = 0; kk < MLP_ND; kk++) {
                         if (kk % 8 == 0) anchor(kk, gk, ak);
@@ -377,8 +382,8 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                         ak *= gb;
                     }
                 } else {
-                    float sn, cs;
-                    sincosf(MLP_PIRC * r, &sn, &cs);
+                    const float ang = r * (0.5f / (float) MLP_RC);
+                    const float sn = __builtin_amdgcn_sinf(ang), cs = __builtin_amdgcn_cosf(ang);
                     const float fc = 0.5f * (cs + 1.0f), dfc = -0.5f * MLP_PIRC * sn;
                     float ci = 0.f, cj = 0.f;
                     const float *djq = (const float *) dj[q];
@@ -393,7 +398,7 @@ __global__ __launch_bounds__(64) void k_mlp_wave(MLPSegArgs A) {
                         ak *= gb;
                     }
                     const float cc = ci + cj;
-                    const double coef = in ? (double) (-cc / r) : 0.0;
+                    const double coef = in ? (double) (-cc * rinv) : 0.0;
                     f0 += coef * dx; f1 += coef * dy; f2 += coef * dz;
                     if (RED) w += in ? (double) (-cc * r) : 0.0;
                 }
