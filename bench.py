@@ -151,9 +151,10 @@ def run_mlp(steps, warmup, train_steps, force_steps):
     minibatches of 4096 rows; (2) configurations with forces -- five plain-LJ
     systems of 2048 atoms at the same temperatures each advance 50 steps
     between Adam steps (velocities rescaled to their temperature every 500
-    steps), and every step takes the gradient of 0.1 x the energy term plus
+    steps), and every step takes the gradient of 0.01 x the energy term plus
     the relative squared force error of all five configurations
-    (force_steps x 5 snapshots sampled every 50 steps)."""
+    (force_steps x 5 snapshots sampled every 50 steps; Adam from a fresh
+    state, learning rate 1e-2 decaying linearly to 5e-4)."""
     from pairs_amd import md, mlp
 
     m = mlp.MLPModel(seed=20241202, dataset_capacity=12 * NATOMS)
@@ -180,9 +181,16 @@ def run_mlp(steps, warmup, train_steps, force_steps):
             s = md.fcc_system(MLP_CONFIG_NX, temp=T, particle_capacity=16384, send_capacity=16384)
             s.run(0, 200, 10 ** 9)
             sims.append(s)
-        _, hist = mlp.train_on_configurations(m, sims, force_steps, every=50, ts0=200, lr=1e-3, w_energy=0.1,
+        # a fresh optimiser state for the new loss (the energy phase's Adam
+        # moments belong to another objective), the parameters copied back
+        p0, s0 = m.params()
+        mf = mlp.MLPModel(seed=1, dataset_capacity=16)
+        mf.set_params(p0, s0)
+        _, hist = mlp.train_on_configurations(mf, sims, force_steps, every=50, ts0=200, lr=1e-2, w_energy=0.01,
                                               w_force=1.0, total_steps=10 ** 9,
                                               record_every=max(force_steps // 10, 1), temps=MLP_TEMPS)
+        m.set_params(*mf.params())
+        mf.close()
         for s in sims:
             s.close()
     t_force = time.perf_counter() - t0
