@@ -135,6 +135,32 @@ def test_mlp_md_graph_and_module_paths_agree():
     assert np.all(np.isfinite(runs[0][0]))
 
 
+def test_mlp_md_fused_and_module_integration_agree():
+    """pairs_hip_run (the force pass integrates, writes the next packed
+    positions, and the descriptor pass skips its pack) == pairs_hip_step
+    (forces, then the stand-alone integrator, every position re-packed), bit
+    for bit, over rebuilds and a thermo step"""
+    md, mlp = _mods()
+    m = mlp.MLPModel(seed=4)
+    s0, _ = _state(n=5, steps=20)
+    G = mlp.descriptors(s0)
+    sc = np.zeros(mo.NSCALE, np.float32)
+    sc[0:16] = G.mean(0)
+    sc[16:32] = 1.0 / (G.std(0) + 1e-3)
+    sc[32], sc[33] = -6.0, 0.05
+    m.set_params(mo.xavier(5).astype(np.float32), sc)
+    a, _ = _state(n=5, steps=20)
+    b, _ = _state(n=5, steps=20)
+    for s in (a, b):
+        used, _ = mlp.attach(s, m, -1.0)
+        assert used
+    a.run(20, 121, 1000)
+    for ts in range(20, 121):
+        b.timestep(ts, 1000)
+    for u, v in ((a.positions(True), b.positions(True)), (a.velocities(), b.velocities()), (a.forces(), b.forces())):
+        assert np.array_equal(u, v)
+
+
 def _lj_set(md, mlp, temps, n, per_temp, seed_steps=120):
     """descriptors and per-atom LJ energies of snapshots (host copies)"""
     Gs, Es = [], []
