@@ -90,7 +90,7 @@ static void drop_graph(pairs_hip_dem_ctx *ctx) {
 static void free_arrays(DemPtrs &p) {
     void *ptrs[] = {p.x, p.y, p.z, p.vx, p.vy, p.vz, p.wx, p.wy, p.wz, p.fx, p.fy, p.fz, p.tx, p.ty, p.tz, p.r, p.m,
                     p.nx, p.ny, p.nz, p.inv_i, p.rot_m, p.rot_q, p.uid, p.flags, p.type, p.shape, p.num_contacts,
-                    p.c_uid, p.c_used, p.c_stick, p.c_tsd, p.c_ivm, p.cell_count, p.cell_start, p.cell_fill,
+                    p.c_uid, p.c_used, p.c_stick, p.c_tsd, p.c_ivm, p.cell_count, p.cell_start, p.cell_slot,
                     p.cell_particles, p.particle_cell, p.nshapes, p.send_map, p.send_mult, p.exchg_flag, p.copy_to,
                     p.blk, p.scan_tmp, p.buf, p.hbuf, p.cand_n, p.cand, p.cslot, p.cf};
     for (void *q : ptrs)
@@ -119,7 +119,8 @@ static int alloc_arrays(pairs_hip_dem_ctx *ctx, DemPtrs &p, size_t C, size_t K, 
     r |= dalloc(ctx, &p.inv_i, 9 * C);
     r |= dalloc(ctx, &p.rot_m, 9 * C);
     r |= dalloc(ctx, &p.rot_q, 4 * C);
-    for (int **q : {&p.uid, &p.flags, &p.type, &p.shape, &p.num_contacts, &p.particle_cell, &p.exchg_flag})
+    for (int **q : {&p.uid, &p.flags, &p.type, &p.shape, &p.num_contacts, &p.particle_cell, &p.exchg_flag,
+                    &p.cell_slot})
         r |= dalloc(ctx, q, C);
     r |= dalloc(ctx, &p.c_uid, K * C);
     r |= dalloc(ctx, &p.c_used, K * C);
@@ -129,7 +130,6 @@ static int alloc_arrays(pairs_hip_dem_ctx *ctx, DemPtrs &p, size_t C, size_t K, 
     const size_t NC = (size_t) ctx->ncells_cap;
     r |= dalloc(ctx, &p.cell_count, NC + 1);
     r |= dalloc(ctx, &p.cell_start, NC + 1);
-    r |= dalloc(ctx, &p.cell_fill, NC + 1);
     r |= dalloc(ctx, &p.nshapes, 2 * NC);
     r |= dalloc(ctx, &p.cell_particles, C);
     r |= dalloc(ctx, &p.scan_tmp, NC / 1024 + 8);

@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_cell_count(DemPtrs p, const int *__rest
     if (i >= n) return;
     const int cell = (p.flags[i] & DFLAG_INFINITE) ? 0 : dcell_of(p.x[i], p.y[i], p.z[i], g);
     p.particle_cell[i] = cell;
-    atomicAdd(&p.cell_count[cell], 1);
+    p.cell_slot[i] = atomicAdd(&p.cell_count[cell], 1);  // its place among the cell's entries
 }
 
 // exclusive scan of cell counts: block partials, then each block adds its offset
@@ -441,8 +441,7 @@ __global__ __launch_bounds__(256) void k_cell_fill(DemPtrs p, const int *__restr
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int cell = p.particle_cell[i];
-    const int slot = atomicAdd(&p.cell_fill[cell], 1);
-    p.cell_particles[p.cell_start[cell] + slot] = i;
+    p.cell_particles[p.cell_start[cell] + p.cell_slot[i]] = i;
 }
 
 // per cell: index order (the CPU's insertion order), then PartitionCellLists
@@ -454,7 +453,6 @@ __global__ __launch_bounds__(256) void k_cell_order(DemPtrs p, int ncells) {
     const int s0 = p.cell_start[cell], cnt = p.cell_count[cell];
     // the counts are not used after this pass: cleared for the next binning
     p.cell_count[cell] = 0;
-    p.cell_fill[cell] = 0;
     int *cp = p.cell_particles + s0;
     for (int a = 1; a < cnt; a++) {
         const int v = cp[a];

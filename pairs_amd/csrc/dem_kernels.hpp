@@ -61,7 +61,8 @@ struct DemPtrs {
     int *num_contacts, *c_uid, *c_used, *c_stick;  // [k][cap]
     double *c_tsd;                                 // [3k + d][cap]
     double *c_ivm;                                 // [k][cap]
-    int *cell_count, *cell_start, *cell_fill, *cell_particles, *particle_cell, *nshapes;
+    int *cell_count, *cell_start, *cell_particles, *particle_cell, *nshapes;
+    int *cell_slot;      // each particle's place in its cell, from the counting pass
     int *send_map, *send_mult, *exchg_flag, *copy_to, *blk, *scan_tmp;
     double *buf, *hbuf;
     int *cand_n, *cand;  // touching partners found by the contact search: [t][cap], t < DCAND
