@@ -1433,8 +1433,11 @@ __device__ __forceinline__ void k_sel_scatter_body(double *__restrict__ X, doubl
             for (int w = 0; w < 4; w++) { int x = wc[0][w]; wc[0][w] = a; a += x; int y = wc[1][w]; wc[1][w] = b; b += y; }
         }
         __syncthreads();
-        if (s0 || s1) {
-            const int side = s0 ? 0 : 1;
+        // each side on its own: in a box shorter than two cell spacings a
+        // particle near both faces of the dimension is sent to both sides
+        // (the reference's two selection loops, domain_partitioning.py:31-65)
+        for (int side = 0; side < 2; side++) {
+            if (!(side == 0 ? s0 : s1)) continue;
             const int rk = side == 0 ? lane_rank(m0) : lane_rank(m1);
             const int pos = base + (side == 0 ? P0 : T0 + P1) + wc[side][wave] + rk;
             const int mult = side == 0 ? 1 : -1;  // pbc_prev = +1, pbc_next = -1 on one rank

@@ -975,6 +975,15 @@ static void enqueue_rebuild(pairs_hip_ctx *ctx, hipStream_t s) {
 // capacities, and their counts live on the device: they are replayed from a
 // graph (one per position buffer, captured on the context's stream), so they
 // run back to back and cost one launch on the host.
+// The first launch of an executable graph prepares its dispatch packets on
+// the host (several times the cost of a later launch, with the device idle
+// when the graph opens a run); uploading it when it is instantiated moves
+// that work to set-up.  An upload the runtime refuses leaves it to the first
+// launch.
+static void upload_graph(pairs_hip_ctx *ctx, hipGraphExec_t exec) {
+    if (hipGraphUpload(exec, ctx->stream) != hipSuccess) (void) hipGetLastError();
+}
+
 // capture (without running) the rebuild that starts on buffer cur when missing
 static int capture_rebuild_graph(pairs_hip_ctx *ctx, int cur) {
     hipGraphExec_t *slot = &ctx->reb_graph[cur];
@@ -997,6 +1006,7 @@ static int capture_rebuild_graph(pairs_hip_ctx *ctx, int cur) {
             *slot = nullptr;
             return PAIRS_HIP_ERR_HIP;
         }
+        upload_graph(ctx, *slot);
     }
     return 0;
 }
@@ -1096,6 +1106,12 @@ static int ensure_spec(pairs_hip_ctx *ctx) {
         const size_t words = 7 * (size_t) n + (6 * (size_t) n + C_COUNT + 1) / 2;
         if (dalloc(ctx, &ctx->ckpt, words)) return PAIRS_HIP_ERR_HIP;
         ctx->ckpt_n = n;
+        // a checkpoint of no particles (it copies the counters only, which
+        // every checkpoint rewrites): the kernel's first launch, several
+        // times the host cost of a later one, happens here and not at the
+        // first rebuild of a run
+        launch_checkpoint(ctx->stream, ckpt_live(ctx->d, ctx->d.cur), ckpt_saved(ctx), 0);
+        CHECK_LAUNCH();
     }
     return 0;
 }
@@ -1277,6 +1293,7 @@ static int capture_graph(pairs_hip_ctx *ctx, hipGraphExec_t &slot, int &grid_out
     if (ec == hipSuccess) {
         e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
+        if (e == hipSuccess) upload_graph(ctx, slot);
     }
     grid_out = ctx->ghost_grid;
     img_out = ctx->d.img_on;
@@ -1318,9 +1335,15 @@ static int graph_steps(pairs_hip_ctx *ctx, int cur, int n) {
 }
 
 // The first step of a run that is neither a rebuild nor a thermo or timed
-// step, from buffer cur: initial_integrate(ts) in place, the ghosts
-// refreshed in place, the packed copy, and the fused force step into the
-// other buffer -- one launch instead of five.
+// step, from buffer cur, as one launch.  When every ghost is an image of a
+// local (img_on): initial_integrate(ts) of the locals into the other buffer
+// with their ghost images and the packed copy (an image of a ghost takes
+// that ghost's position before the refresh: the stale multi-hop
+// synchronize), then the fused force step from the other buffer back into
+// cur -- two kernels, and the run continues on buffer cur.  Otherwise:
+// initial_integrate in place, the ghosts refreshed in place, the packed copy
+// and the fused force step into the other buffer, which the run continues on.
+// start_img[cur] records which form the graph holds.
 static int capture_start_graph(pairs_hip_ctx *ctx, int cur) {
     if (ctx->start_graph[cur] && ctx->start_ghost_grid[cur] >= ctx->ghost_grid && ctx->start_img[cur] == ctx->d.img_on)
         return 0;
@@ -1328,10 +1351,17 @@ static int capture_start_graph(pairs_hip_ctx *ctx, int cur) {
     ctx->d.cur = cur;
     const int r = capture_graph(ctx, ctx->start_graph[cur], ctx->start_ghost_grid[cur], ctx->start_img[cur], [&] {
         DevState &d = ctx->d;
-        launch_vv_initial(ctx->stream, d, ctx->prm);
-        launch_sync_inplace(ctx->stream, d, ctx->geom, ctx->ghost_grid, 0);
-        ctx->xy_valid[cur] = false;
-        launch_force_post(ctx, cur, 1 - cur, 1, 1, 0, false);
+        const int other = 1 - cur;
+        if (d.img_on) {
+            launch_vv_post(ctx->stream, d, ctx->geom, ctx->prm, cur, other, 0, 1, 0, d.P[other]);
+            ctx->xy_valid[other] = true;
+            launch_force_post(ctx, other, cur, 1, 1, 0, false);
+        } else {
+            launch_vv_initial(ctx->stream, d, ctx->prm);
+            launch_sync_inplace(ctx->stream, d, ctx->geom, ctx->ghost_grid, 0);
+            ctx->xy_valid[cur] = false;
+            launch_force_post(ctx, cur, other, 1, 1, 0, false);
+        }
     });
     ctx->d.cur = cur0;
     return r;
@@ -1395,7 +1425,7 @@ int pairs_hip_run(pairs_hip_ctx *ctx, int ts_begin, int ts_end, int nsteps, doub
                 if (int r = capture_start_graph(ctx, d.cur)) return r;
                 HIP_OK(hipGraphLaunch(ctx->start_graph[d.cur], s));
                 ctx->xy_valid[0] = ctx->xy_valid[1] = true;  // packed, and written by the fused kernel
-                d.cur ^= 1;
+                if (!ctx->start_img[d.cur]) d.cur ^= 1;       // the image form ends on buffer cur
                 prev_in_other = true;
                 continue;
             }

@@ -189,23 +189,28 @@ def test_deferred_rebuild_checks_match_immediate_ones(half):
     _compare_lists(a, b, tag=f"deferred half={half}")
 
 
+@pytest.mark.parametrize("nx", [6, 3])
 @pytest.mark.parametrize("half", [False, True])
-def test_graph_chains_and_run_starts_vs_oracle(half):
+def test_graph_chains_and_run_starts_vs_oracle(half, nx):
     """Runs split at arbitrary steps, rebuilds every 60 steps: each run's
     first step replays from its own graph, each stretch of steady steps from
     one chain graph (stretches longer than the longest chain, 24 steps, are
     cut into several), one-step runs and last steps run eagerly.  The state
-    after 200 steps is the oracle's, bit for bit."""
+    after 200 steps is the oracle's, bit for bit.  FCC 6^3 (box 10.1): every
+    ghost is one of at most 8 images of a local, so a run starts by
+    integrating into the other buffer with the ghost images written there;
+    FCC 3^3 (box 5.04, under two cell spacings): locals have up to 17 images,
+    so a run starts with the in-place ghost refresh."""
     md = _md()
-    g = md.fcc_system(6, half=half, reneighbor_every=60, thermo_every=1000)
-    o = po.fcc_system(6, half=half, reneighbor_every=60, thermo_every=1000)
+    g = md.fcc_system(nx, half=half, reneighbor_every=60, thermo_every=1000)
+    o = po.fcc_system(nx, half=half, reneighbor_every=60, thermo_every=1000)
     g.run(0, 3, 200)
     g.prepare_graphs()
     for a, b in ((3, 31), (31, 32), (32, 90), (90, 91), (91, 157), (157, 201)):
         g.run(a, b, 200)
     o.run(0, 201)
-    _compare_state(g, o, tag=f"chains half={half}")
-    _compare_lists(g, o, tag=f"chains half={half}")
+    _compare_state(g, o, tag=f"chains half={half} nx={nx}")
+    _compare_lists(g, o, tag=f"chains half={half} nx={nx}")
 
 
 def test_deferred_check_repeat_keeps_thermo_rows():

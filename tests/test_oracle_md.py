@@ -215,6 +215,39 @@ def test_neighbor_sets_brute_force_exact_images():
     assert all(n == 78 for n in s.numneighs())  # FCC shells 12+6+24+12+24 inside 2.8
 
 
+def test_borders_small_box_both_faces():
+    """FCC 3^3 (box 5.04, shorter than two cell spacings of 2.8): the send map
+    and ghost positions of the first rebuild equal the borders stated from the
+    reference's selection rule (src/pairs/sim/comm.py:57-99,
+    domain_partitioning.py:31-65): per dimension, every particle so far (locals,
+    then the earlier dimensions' ghosts) with x_d < min_d + 2.8 in index order,
+    then every one with x_d > max_d - 2.8, each appended as a ghost shifted by
+    +L / -L -- a particle near both faces is sent to both sides."""
+    s = po.fcc_system(3)
+    x = s.positions().copy()
+    L = po.fcc_box_length(3, 0.8442)
+    pts, smap, mults = x.copy(), [], []
+    for d in range(3):
+        lo = [i for i in range(len(pts)) if pts[i, d] < 0.0 + 2.8]
+        hi = [i for i in range(len(pts)) if pts[i, d] > L - 2.8]
+        new = []
+        for side, sel in ((1, lo), (-1, hi)):
+            for i in sel:
+                q = pts[i].copy()
+                q[d] += side * L
+                new.append(q)
+                smap.append(i)
+                mults.append([side if k == d else 0 for k in range(3)])
+        pts = np.vstack([pts, np.array(new)])
+    both = sum(1 for i in range(len(x)) if 2.8 > x[i, 0] > L - 2.8)
+    assert both > 0  # the case under test: particles selected on both sides
+    s.step(0)
+    sm, mult = s.send_map()
+    assert s.nghost == len(pts) - len(x) == 990
+    assert np.array_equal(sm, smap) and np.array_equal(mult, mults)
+    assert np.array_equal(s.positions(True), pts)
+
+
 def test_full_and_half_forces_agree():
     a = po.fcc_system(4)
     b = po.fcc_system(4, half=True)
