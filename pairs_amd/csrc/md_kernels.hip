@@ -1500,10 +1500,39 @@ __device__ __forceinline__ void k_exchange_relabel_body(double *__restrict__ X, 
                                                            int *__restrict__ copy_to, int *__restrict__ p_leaver,
                                                            int *__restrict__ p_src) {
     __shared__ int lds[20];
+    __shared__ int s_tail[1024];
     if (counters[C_OVF]) return;
     const int n = counters[C_NSEND_ALL];
     if (n == 0) return;
     const int nlocal = counters[C_NLOCAL];
+    if (n <= 1024) {
+        // one leaver per thread (the usual case): the tail list in LDS and
+        // each leaver's entries in registers, the same steps as below
+        const int t = threadIdx.x;
+        const bool in = t < n;
+        const int keep = (in && exchg_flag[nlocal - 1 - t] == 0) ? 1 : 0;
+        int total;
+        const int ex = block_exclusive_scan<1024>(keep, lds, &total);
+        if (keep) s_tail[ex] = nlocal - 1 - t;
+        const int sm = in ? send_map[t] : 0;
+        const int q = (in && sm < nlocal - n) ? 1 : 0;
+        const int eq = block_exclusive_scan<1024>(q, lds, &total);  // (its barriers publish s_tail)
+        const int src = q ? s_tail[eq] : -1;
+        const int pl = in ? phys[sm] : 0;
+        const int ps = src > 0 ? phys[src] : -1;
+        __syncthreads();
+        if (ps >= 0) { ref[ps] = sm; phys[sm] = ps; }
+        __syncthreads();
+        if (in) {
+            X[pl] = X[pl] + (double) send_mult[3 * t + 0] * g.len[0];
+            Y[pl] = Y[pl] + (double) send_mult[3 * t + 1] * g.len[1];
+            Z[pl] = Z[pl] + (double) send_mult[3 * t + 2] * g.len[2];
+            const int nr = nlocal - n + t;
+            ref[pl] = nr;
+            phys[nr] = pl;
+        }
+        return;
+    }
     int carry = 0;
     for (int t0 = 0; t0 < n; t0 += 1024) {
         const int t = t0 + threadIdx.x;
